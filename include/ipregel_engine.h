@@ -1,0 +1,148 @@
+/*
+ * ipregel_engine.h - definitions shared by the C host driver, the HIP C-ABI
+ * library (libipregel_hip.so) and the kernels that iPregel.h instantiates in
+ * the application's translation unit. Plain C, no HIP types.
+ *
+ * Device layout (structure of arrays in HBM, V vertices, E arcs):
+ *   out_idx[V+1] u64, out_adj[E] u32, out_wgt[E] u32   out-edge CSR
+ *   in_idx[V+1]  u64, in_adj[E]  u32, in_wgt[E]  u32   in-edge CSC (aliases the
+ *                                                      CSR when directed=false,
+ *                                                      src/iPregel_postamble.h:368-396)
+ *   val[V]        application value                    (ip_vertex_t.value)
+ *   msg[V]        delivered combined message            (ip_vertex_t.message)
+ *   act[V], hm[V] active / has_message flags, one byte each
+ *   bcf[2][V/32]  broadcast bitmaps by superstep parity (pull variants)
+ *   bcm[2][V]     broadcast messages by parity (pull) / per broadcaster (push)
+ *   mbox[2][V]    combiner mailboxes by parity (push variants)
+ *   claim[2][V/32] first-delivery bitmaps by parity (push variants)
+ *   front[2][V]   frontier / target lists by parity (spread variants)
+ *   bins[3][V]    broadcasters this superstep, binned by out-degree
+ */
+#ifndef IPREGEL_ENGINE_H
+#define IPREGEL_ENGINE_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* engine variants: IP_USE_SPREAD x IP_USE_SINGLE_BROADCAST
+ * (src/iPregel_preamble.h:338-350) */
+enum {
+    IPR_VARIANT_PUSH = 0,        /* combiner_{pre,post}amble.h */
+    IPR_VARIANT_PULL = 1,        /* combiner_single_broadcast_{pre,post}amble.h */
+    IPR_VARIANT_SPREAD = 2,      /* combiner_spread_{pre,post}amble.h */
+    IPR_VARIANT_SPREAD_PULL = 3  /* combiner_spread_single_broadcast_{pre,post}amble.h */
+};
+
+/* combiner declared on the compile line (-DIP_COMBINER_MIN etc.) */
+enum {
+    IPR_COMBINER_GENERIC = 0,    /* user ip_combine in a CAS loop */
+    IPR_COMBINER_MIN = 1,
+    IPR_COMBINER_MAX = 2,
+    IPR_COMBINER_SUM = 3
+};
+
+/* degree bins of the push expansion and of the pull gather */
+enum { IPR_BIN_SMALL = 0, IPR_BIN_MEDIUM = 1, IPR_BIN_LARGE = 2, IPR_NBINS = 3 };
+
+/* device-resident control block: every kernel reads the superstep and the
+ * halting flag here, so a chunk of supersteps can be replayed as one HIP graph
+ * without the host deciding anything in between */
+typedef struct ipr_ctl {
+    uint64_t superstep;          /* current superstep */
+    uint32_t done;               /* set by the finalize kernel when the loop ends */
+    uint32_t error;              /* device-side API misuse flags */
+    uint64_t cnt[2];             /* ActiveVertexCount being accumulated, by parity */
+    uint64_t fsize[2];           /* frontier / target list sizes by parity */
+    uint64_t bsize[IPR_NBINS];   /* broadcaster bin sizes, current superstep */
+    uint64_t bcast[2];           /* broadcasts by superstep parity */
+    uint64_t step_edges;         /* W of this superstep */
+    uint64_t work_edges;         /* W accumulated */
+    uint64_t work_vertices;      /* vertices computed, accumulated */
+    uint64_t trace_cap;          /* entries in the trace arrays */
+    uint64_t executed;           /* supersteps executed so far */
+    uint64_t pull_mode;          /* push apps: 1 when this superstep gathers */
+    uint64_t last_active;        /* ActiveVertexCount of the last finalized superstep */
+    uint64_t gathered;           /* supersteps delivered by gather */
+    uint64_t pad[1];
+} ipr_ctl;
+
+typedef struct ipr_dev {
+    ipr_ctl* ctl;
+    uint64_t V, E;
+    uint64_t words;              /* V/32 rounded up */
+    const uint64_t* out_idx;
+    const uint32_t* out_adj;
+    const uint32_t* out_wgt;
+    const uint64_t* in_idx;
+    const uint32_t* in_adj;
+    const uint32_t* in_wgt;
+    void* val;
+    void* msg;
+    uint8_t* act;
+    uint8_t* hm;
+    uint32_t* bcf[2];
+    void* bcm[2];
+    void* mbox[2];
+    uint32_t* claim[2];
+    uint32_t* front[2];
+    uint32_t* bins[IPR_NBINS];
+    /* static pull bins over all V (by in-degree), built at load */
+    uint32_t* pbins[IPR_NBINS];
+    uint64_t pbin_size[IPR_NBINS];
+    uint64_t* trace_active;
+    uint64_t* trace_ts;
+    uint64_t* trace_edges;
+    /* tuning */
+    uint32_t bin_small_max;      /* deg <= this: lane-cooperative */
+    uint32_t bin_medium_max;     /* deg <= this: wavefront per vertex; above: grid-wide */
+    uint32_t grid_blocks;        /* persistent-style grids */
+    uint32_t block_threads;
+    uint32_t pull_ordered;       /* 1: fold in in-list order (bit-exact) */
+    uint32_t dir_opt;            /* push apps: allow gathering dense supersteps */
+    uint32_t dir_alpha;          /* gather when bcast out-arcs * alpha > E */
+    uint32_t pad2;
+} ipr_dev;
+
+/* kernels the application TU provides (typed by the app's message/value) */
+enum {
+    IPR_K_INIT = 0,      /* identity-initialise mailboxes, flags */
+    IPR_K_COMPUTE,       /* compute phase of the variant */
+    IPR_K_EXPAND,        /* push delivery of recorded broadcasts */
+    IPR_K_SWAP,          /* PUSH: mailbox -> message, reactivation */
+    IPR_K_PULL_STEP,     /* PULL: fetch(s-1) fused with compute(s) */
+    IPR_K_FETCH,         /* SPREAD_PULL: fetch for targets */
+    IPR_K_GATHER,        /* push apps, dense superstep: receivers gather */
+    IPR_K_COUNT
+};
+
+typedef struct ipr_app_desc {
+    int variant;
+    int combiner;
+    uint32_t id_size, val_size, msg_size, mbox_size;
+    int needs_in_lists;          /* pull variants */
+    int needs_weights;           /* IP_NEEDS_OUT_NEIGHBOUR_WEIGHTS */
+    int can_gather;              /* push app whose combiner makes gather exact */
+    const char* application;     /* IP_APPLICATION */
+    const char* version;         /* VERSION */
+    const char* machine;         /* IP_MACHINE */
+    const char* commits;         /* COMMITS */
+    const char* compilation_flags;
+    /* enqueue one app kernel on `stream` (a hipStream_t); 0 on success */
+    int (*launch)(int kernel, const ipr_dev* dev, void* stream);
+    /* make device-side globals point at this run's state */
+    int (*bind)(const ipr_dev* dev);
+    /* ip_dump: serialise vertices [first, first+count) from host copies */
+    void (*serialise)(void* file, const void* val, const uint64_t* out_idx,
+                      uint64_t first, uint64_t count);
+    /* host-side self-check of the declared combiner (0 = consistent) */
+    int (*check_combiner)(void);
+} ipr_app_desc;
+
+#ifdef __cplusplus
+}
+#endif
+#endif

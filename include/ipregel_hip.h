@@ -1,0 +1,82 @@
+/*
+ * ipregel_hip.h - the thin C-ABI between the C host driver and HIP
+ * (libipregel_hip.so). Plain pointers, sizes and opaque handles; no HIP or
+ * torch types. Every function returns 0 on success or a negative hipError_t;
+ * ipr_hip_last_error() gives the message of the last failure.
+ *
+ * The reference has no device boundary (it is one OpenMP process): these
+ * entry points replace what its host code does with memory, files and the
+ * OpenMP runtime on the superstep path:
+ *   ipr_hip_malloc/free/memcpy     malloc/realloc of vertex and engine state
+ *                                  (src/iPregel_postamble.h:117-146,520-533)
+ *   ipr_hip_build_csc              in-neighbour mirroring
+ *                                  (src/iPregel_postamble.h:429-487)
+ *   ipr_hip_begin / ipr_hip_finalize_*
+ *                                  the superstep loop control of ip_run: counter
+ *                                  reset, active count, halting test, superstep
+ *                                  increment (e.g. src/combiner_spread_postamble.h
+ *                                  :341-354, :410-414, :476-532)
+ *   ipr_hip_build_pull_bins        (new) in-degree bins of the pull gather
+ *   ipr_hip_graph_*                (new) replay of a chunk of supersteps
+ */
+#ifndef IPREGEL_HIP_H
+#define IPREGEL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include "ipregel_engine.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+const char* ipr_hip_last_error(void);
+
+int ipr_hip_init(int device, char* name, size_t name_len, uint64_t* hbm_bytes);
+int ipr_hip_wallclock_hz(uint64_t* hz);
+
+int ipr_hip_malloc(void** p, size_t bytes);
+int ipr_hip_free(void* p);
+int ipr_hip_host_alloc(void** p, size_t bytes);      /* pinned */
+int ipr_hip_host_free(void* p);
+int ipr_hip_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream);
+int ipr_hip_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream);
+int ipr_hip_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream);
+/* fill with a byte value by kernel (safe to record in a HIP graph) */
+int ipr_hip_fill(void* dst, int byte, size_t bytes, void* stream);
+
+int ipr_hip_stream_create(void** stream);
+int ipr_hip_stream_destroy(void* stream);
+int ipr_hip_stream_sync(void* stream);
+int ipr_hip_event_create(void** ev);
+int ipr_hip_event_destroy(void* ev);
+int ipr_hip_event_record(void* ev, void* stream);
+int ipr_hip_event_elapsed_ms(void* start, void* stop, float* ms);
+
+/* capture everything enqueued on `stream` between begin and end into an
+ * executable graph */
+int ipr_hip_graph_begin(void* stream);
+int ipr_hip_graph_end(void* stream, void** exec);
+int ipr_hip_graph_launch(void* exec, void* stream);
+int ipr_hip_graph_destroy(void* exec);
+
+/* CSC from a device CSR: in-rows ascending by source id, weights carried */
+int ipr_hip_build_csc(uint64_t V, uint64_t E, const uint64_t* out_idx,
+                      const uint32_t* out_adj, const uint32_t* out_wgt,
+                      uint64_t* in_idx, uint32_t* in_adj, uint32_t* in_wgt,
+                      void* stream);
+
+/* static in-degree bins for the pull gather; sizes returned in dev->pbin_size */
+int ipr_hip_build_pull_bins(ipr_dev* dev, uint32_t small_max, uint32_t medium_max,
+                            void* stream);
+
+/* superstep control (kernels on `stream`, graph-capturable) */
+int ipr_hip_begin(const ipr_dev* dev, void* stream);
+int ipr_hip_finalize(const ipr_dev* dev, int variant, void* stream);
+/* push applications: choose push or gather for the coming delivery */
+int ipr_hip_direction(const ipr_dev* dev, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,120 @@
+"""Build recipe for the MI355X iPregel engine (gfx950).
+
+Produces, in-tree (git-ignored, shipped with the tree):
+  ipregel_amd/lib/libipregel_graph.so   graph generators and file triple (C, OpenMP)
+  ipregel_amd/lib/libipregel_hip.so     thin HIP C-ABI + type-independent kernels
+  ipregel_amd/lib/ip_driver.o           C host driver (linked into every app)
+  ipregel_amd/lib/libipr_<app>_32.so    app TU + driver, loadable over ctypes
+  ipregel_amd/bin/<app>_32              the same app as a command-line program
+and the oracle's own library (oracle/build.py).
+
+Every HIP translation unit is compiled for gfx950 only, with packed-FP32
+instructions disabled (-Xclang -target-feature -Xclang -packed-fp32-ops).
+Application TUs use -ffp-contract=off so that FP64 arithmetic in ip_compute
+rounds like the reference's x86-64 build (no fused multiply-add).
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+INC = ROOT / "include"
+LIB = PKG / "lib"
+BIN = PKG / "bin"
+OBJ = PKG / "lib" / "obj"
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+]
+C_FLAGS = ["-std=c11", "-O3", "-fPIC", "-fopenmp", "-Wall", "-Wextra"]
+
+# (binary name, source, defines); the name follows the reference makefile's
+# <app>[_single_broadcast][_spread]_32 scheme (makefile:108-205).
+APPS = [
+    ("pagerank_single_broadcast", "pagerank.c", ["-DIP_USE_SINGLE_BROADCAST", "-DIP_COMBINER_SUM"]),
+    ("pagerank", "pagerank.c", ["-DIP_COMBINER_SUM"]),
+    ("cc_spread", "cc.c", ["-DIP_USE_SPREAD", "-DIP_COMBINER_MIN"]),
+    ("cc", "cc.c", ["-DIP_COMBINER_MIN"]),
+    ("cc_single_broadcast", "cc.c", ["-DIP_USE_SINGLE_BROADCAST", "-DIP_COMBINER_MIN"]),
+    ("cc_generic_spread", "cc.c", ["-DIP_USE_SPREAD"]),
+    ("sssp_spread", "sssp.c", ["-DIP_USE_SPREAD", "-DIP_COMBINER_MIN"]),
+    ("sssp", "sssp.c", ["-DIP_COMBINER_MIN"]),
+    ("sssp_single_broadcast", "sssp.c", ["-DIP_USE_SINGLE_BROADCAST", "-DIP_COMBINER_MIN"]),
+    ("sssp_weighted_spread", "sssp_weighted.c", ["-DIP_USE_SPREAD", "-DIP_COMBINER_MIN"]),
+    ("sssp_weighted", "sssp_weighted.c", ["-DIP_COMBINER_MIN"]),
+]
+
+
+def _run(cmd: list[str], cwd: Path | None = None) -> None:
+    r = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"build step failed ({r.returncode}): {' '.join(cmd)}\n{r.stdout}")
+
+
+def build_graph_lib() -> Path:
+    out = LIB / "libipregel_graph.so"
+    _run(["gcc", *C_FLAGS, "-shared", f"-I{INC}", str(PKG / "csrc/graph/ipr_graph.c"), "-o", str(out)])
+    return out
+
+
+def build_hip_lib() -> Path:
+    out = LIB / "libipregel_hip.so"
+    _run([HIPCC, *HIP_FLAGS, "-shared", f"-I{INC}", str(PKG / "csrc/hip/ipr_hip.hip"), "-o", str(out)])
+    return out
+
+
+def build_driver() -> Path:
+    out = OBJ / "ip_driver.o"
+    _run(["gcc", *C_FLAGS, f"-I{INC}", "-c", str(PKG / "csrc/driver/ip_driver.c"), "-o", str(out)])
+    return out
+
+
+def build_app(name: str, src: str, defines: list[str], driver_obj: Path) -> tuple[Path, Path]:
+    obj = OBJ / f"{name}_32.o"
+    app_label = name
+    _run([HIPCC, *HIP_FLAGS, "-x", "hip", "-ffp-contract=off", f"-I{INC}",
+          "-DIP_VERTEX_ID_TYPE=uint32_t", f'-DIP_APPLICATION="{app_label}"',
+          '-DIP_MACHINE="MI355X"', '-DVERSION="1.0.0-mi355x"',
+          *defines, "-c", str(PKG / "apps" / src), "-o", str(obj)])
+    so = LIB / f"libipr_{name}_32.so"
+    exe = BIN / f"{name}_32"
+    link = [str(obj), str(driver_obj), f"-L{LIB}", "-lipregel_hip", "-fopenmp"]
+    _run([HIPCC, "-shared", "-fPIC", "-Wl,-Bsymbolic", "-Wl,-rpath,$ORIGIN", *link, "-o", str(so)])
+    _run([HIPCC, "-Wl,-rpath,$ORIGIN/../lib", *link, "-o", str(exe)])
+    return so, exe
+
+
+def build_all(jobs: int | None = None, verbose: bool = True) -> None:
+    for d in (LIB, BIN, OBJ):
+        d.mkdir(parents=True, exist_ok=True)
+    jobs = jobs or min(8, os.cpu_count() or 4)
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        f_graph = ex.submit(build_graph_lib)
+        f_hip = ex.submit(build_hip_lib)
+        f_drv = ex.submit(build_driver)
+        drv = f_drv.result()
+        f_hip.result()
+        f_graph.result()
+        futs = {ex.submit(build_app, n, s, d, drv): n for (n, s, d) in APPS}
+        for f in cf.as_completed(futs):
+            f.result()
+            if verbose:
+                print(f"built {futs[f]}_32", flush=True)
+    # the oracle is test infrastructure with its own recipe
+    sys.path.insert(0, str(ROOT))
+    from oracle import build as oracle_build  # noqa: E402
+    oracle_build.build()
+
+
+if __name__ == "__main__":
+    build_all()

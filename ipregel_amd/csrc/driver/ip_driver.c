@@ -1,0 +1,644 @@
+/*
+ * ip_driver.c - C host driver of the MI355X iPregel engine.
+ *
+ * Owns what the reference's shared runtime (src/iPregel_postamble.h) and the
+ * superstep loops of its engine variants do on the host:
+ *   ip_init        schedule check, provenance banner, graph load   (:176-242)
+ *   loader         .config / .idx / .adj (+ .wgt)                    (:244-546)
+ *   ip_run         the superstep loop; per-superstep log lines       (e.g.
+ *                  src/combiner_spread_postamble.h:341-536)
+ *   ip_dump        values in id order through ip_serialise_vertex    (:103-115)
+ * All device work goes through libipregel_hip.so (include/ipregel_hip.h) and
+ * the kernel table of the application (ipr_app_desc.launch). No superstep
+ * needs a host decision: kernels read the superstep and the halting flag from
+ * the device control block, and the host replays chunks of supersteps (as
+ * HIP graphs) and reads the control block once per chunk.
+ */
+#define _GNU_SOURCE
+#include "ipregel_driver.h"
+#include "ipregel_hip.h"
+
+#include <errno.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#define IPR_TRACE_CAP (1u << 20)
+#define IPR_MAX_GRAPHS 8
+
+static struct {
+    ipr_app_desc app;
+    ipr_harness_cfg cfg;
+    int have_cfg;
+    int initialised;
+    int directed, weighted;
+    uint64_t V, E;
+    uint64_t* h_idx;         /* V+1 */
+    uint32_t* h_adj;
+    uint32_t* h_wgt;
+    int own_host;
+    ipr_dev dev;
+    void* stream;
+    int own_stream;
+    ipr_ctl* h_ctl;          /* pinned */
+    void* ev0;
+    void* ev1;
+    void* graphs[IPR_MAX_GRAPHS];
+    uint64_t wall_hz;
+    ipr_run_stats stats;
+    int use_graph;
+    uint32_t max_chunk;
+    int quiet;
+} D;
+
+/* ------------------------------------------------------------------------ */
+/* helpers                                                                   */
+/* ------------------------------------------------------------------------ */
+
+static double now_s(void)
+{
+    struct timespec t;
+    clock_gettime(CLOCK_MONOTONIC, &t);
+    return (double)t.tv_sec + 1e-9 * (double)t.tv_nsec;
+}
+
+static void die(const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vprintf(fmt, ap);
+    va_end(ap);
+    fflush(stdout);
+    exit(-1);
+}
+
+static void hip_check(int rc, const char* what)
+{
+    if (rc) die("HIP failure in %s: %s\n", what, ipr_hip_last_error());
+}
+
+static void* safe_malloc(size_t n)
+{
+    void* p = malloc(n ? n : 1);
+    if (!p) die("Failed to allocate %zu bytes.\n", n);
+    return p;
+}
+
+static FILE* safe_fopen(const char* path, const char* mode)
+{
+    FILE* f = fopen(path, mode);
+    if (!f) die("Cannot open the file \"%s\" in mode \"%s\".\n", path, mode);
+    return f;
+}
+
+static void safe_fread(void* p, size_t size, size_t count, FILE* f)
+{
+    if (count && fread(p, size, count, f) != count)
+        die("Failed to read %zu elements of %zu bytes each.\n", count, size);
+}
+
+static char* with_ext(const char* root, const char* ext)
+{
+    size_t n = strlen(root) + strlen(ext) + 1;
+    char* p = safe_malloc(n);
+    snprintf(p, n, "%s%s", root, ext);
+    return p;
+}
+
+static uint32_t env_u32(const char* name, uint32_t def)
+{
+    const char* v = getenv(name);
+    if (!v || !*v) return def;
+    return (uint32_t)strtoul(v, NULL, 10);
+}
+
+/* ------------------------------------------------------------------------ */
+/* device state                                                              */
+/* ------------------------------------------------------------------------ */
+
+static void dev_free_all(void)
+{
+    for (int i = 0; i < IPR_MAX_GRAPHS; i++) {
+        if (D.graphs[i]) ipr_hip_graph_destroy(D.graphs[i]);
+        D.graphs[i] = NULL;
+    }
+    ipr_dev* d = &D.dev;
+    if (d->in_idx && d->in_idx != d->out_idx) ipr_hip_free((void*)d->in_idx);
+    if (d->in_adj && d->in_adj != d->out_adj) ipr_hip_free((void*)d->in_adj);
+    if (d->in_wgt && d->in_wgt != d->out_wgt) ipr_hip_free((void*)d->in_wgt);
+    ipr_hip_free((void*)d->out_idx);
+    ipr_hip_free((void*)d->out_adj);
+    ipr_hip_free((void*)d->out_wgt);
+    ipr_hip_free(d->val);
+    ipr_hip_free(d->msg);
+    ipr_hip_free(d->act);
+    ipr_hip_free(d->hm);
+    for (int i = 0; i < 2; i++) {
+        ipr_hip_free(d->bcf[i]);
+        ipr_hip_free(d->bcm[i]);
+        ipr_hip_free(d->mbox[i]);
+        ipr_hip_free(d->claim[i]);
+        ipr_hip_free(d->front[i]);
+    }
+    for (int b = 0; b < IPR_NBINS; b++) {
+        ipr_hip_free(d->bins[b]);
+        ipr_hip_free(d->pbins[b]);
+    }
+    ipr_hip_free(d->trace_active);
+    ipr_hip_free(d->trace_ts);
+    ipr_hip_free(d->trace_edges);
+    ipr_hip_free(d->ctl);
+    memset(d, 0, sizeof(*d));
+    if (D.h_ctl) ipr_hip_host_free(D.h_ctl);
+    D.h_ctl = NULL;
+    if (D.ev0) ipr_hip_event_destroy(D.ev0);
+    if (D.ev1) ipr_hip_event_destroy(D.ev1);
+    D.ev0 = D.ev1 = NULL;
+    if (D.own_stream && D.stream) ipr_hip_stream_destroy(D.stream);
+    D.stream = NULL;
+    D.own_stream = 0;
+}
+
+static void* dalloc(size_t bytes, const char* what)
+{
+    void* p = NULL;
+    hip_check(ipr_hip_malloc(&p, bytes), what);
+    return p;
+}
+
+static void dev_setup(void)
+{
+    ipr_dev* d = &D.dev;
+    const ipr_app_desc* a = &D.app;
+    uint64_t V = D.V, E = D.E;
+    void* st = D.stream;
+    memset(d, 0, sizeof(*d));
+    d->V = V;
+    d->E = E;
+    d->words = (V + 31) / 32;
+    uint64_t vv = V ? V : 1;
+
+    d->out_idx = dalloc((V + 1) * 8, "out_idx");
+    d->out_adj = dalloc(E * 4, "out_adj");
+    hip_check(ipr_hip_memcpy_h2d((void*)d->out_idx, D.h_idx, (V + 1) * 8, st), "upload idx");
+    hip_check(ipr_hip_memcpy_h2d((void*)d->out_adj, D.h_adj, E * 4, st), "upload adj");
+    if (D.h_wgt) {
+        d->out_wgt = dalloc(E * 4, "out_wgt");
+        hip_check(ipr_hip_memcpy_h2d((void*)d->out_wgt, D.h_wgt, E * 4, st), "upload wgt");
+    }
+    /* in-lists: aliased when undirected (src/iPregel_postamble.h:368-396),
+     * mirrored in ascending source order when directed (:429-487) */
+    if (a->needs_in_lists && D.directed) {
+        d->in_idx = dalloc((V + 1) * 8, "in_idx");
+        d->in_adj = dalloc(E * 4, "in_adj");
+        if (d->out_wgt) d->in_wgt = dalloc(E * 4, "in_wgt");
+        hip_check(ipr_hip_build_csc(V, E, d->out_idx, d->out_adj, d->out_wgt, (uint64_t*)d->in_idx,
+                                    (uint32_t*)d->in_adj, (uint32_t*)d->in_wgt, st), "build CSC");
+    } else {
+        d->in_idx = d->out_idx;
+        d->in_adj = d->out_adj;
+        d->in_wgt = d->out_wgt;
+    }
+
+    d->val = dalloc(vv * a->val_size, "values");
+    d->msg = dalloc(vv * a->msg_size, "messages");
+    d->act = dalloc(vv, "active");
+    d->hm = dalloc(vv, "has_message");
+    int pull = a->variant == IPR_VARIANT_PULL || a->variant == IPR_VARIANT_SPREAD_PULL;
+    for (int i = 0; i < 2; i++) {
+        d->bcm[i] = dalloc(vv * a->msg_size, "broadcast messages");
+        if (pull) {
+            d->bcf[i] = dalloc(vv, "broadcast flags");
+        } else {
+            d->mbox[i] = dalloc(vv * a->mbox_size, "mailboxes");
+            d->claim[i] = dalloc(d->words * 4 + 4, "claim bits");
+        }
+        if (a->variant == IPR_VARIANT_SPREAD || a->variant == IPR_VARIANT_SPREAD_PULL)
+            d->front[i] = dalloc(vv * 4, "frontier");
+    }
+    if (!pull)
+        for (int b = 0; b < IPR_NBINS; b++) d->bins[b] = dalloc(vv * 4, "broadcaster bins");
+    d->ctl = dalloc(sizeof(ipr_ctl), "control block");
+    d->trace_active = dalloc(IPR_TRACE_CAP * 8, "trace");
+    d->trace_ts = dalloc((IPR_TRACE_CAP + 1) * 8, "trace");
+    d->trace_edges = dalloc(IPR_TRACE_CAP * 8, "trace");
+    hip_check(ipr_hip_fill(d->ctl, 0, sizeof(ipr_ctl), st), "ctl init");
+    uint64_t cap = IPR_TRACE_CAP;
+    hip_check(ipr_hip_memcpy_h2d(&d->ctl->trace_cap, &cap, 8, st), "ctl init");
+    hip_check(ipr_hip_stream_sync(st), "ctl init");
+
+    /* tuning */
+    const ipr_harness_cfg* c = D.have_cfg ? &D.cfg : NULL;
+    d->bin_small_max = c && c->bin_small_max ? c->bin_small_max : env_u32("IPR_BIN_SMALL", 64);
+    d->bin_medium_max = c && c->bin_medium_max ? c->bin_medium_max : env_u32("IPR_BIN_MEDIUM", 4096);
+    d->grid_blocks = c && c->grid_blocks ? c->grid_blocks : env_u32("IPR_GRID_BLOCKS", 2048);
+    d->block_threads = 256;
+    uint32_t po = c && c->pull_ordered ? (uint32_t)c->pull_ordered : env_u32("IPR_PULL_ORDERED", 1);
+    d->pull_ordered = po == 1;
+    uint32_t dopt = c && c->dir_opt ? (uint32_t)c->dir_opt : env_u32("IPR_DIR_OPT", 1);
+    d->dir_opt = dopt == 1 && a->can_gather;
+    d->dir_alpha = c && c->dir_alpha ? c->dir_alpha : env_u32("IPR_DIR_ALPHA", 4);
+    if (pull) {
+        uint32_t ps = c && c->pull_small_max ? c->pull_small_max : env_u32("IPR_PULL_SMALL", 32);
+        uint32_t pm = c && c->pull_medium_max ? c->pull_medium_max : env_u32("IPR_PULL_MEDIUM", 2048);
+        for (int b = 0; b < IPR_NBINS; b++) d->pbins[b] = dalloc(vv * 4, "pull bins");
+        hip_check(ipr_hip_build_pull_bins(d, ps, pm, st), "pull bins");
+    }
+    uint32_t ug = c && c->use_graph ? (uint32_t)c->use_graph : env_u32("IPR_GRAPH", 1);
+    D.use_graph = ug == 1;
+    D.max_chunk = c && c->max_chunk ? c->max_chunk : env_u32("IPR_MAX_CHUNK", 64);
+    if (D.max_chunk < 1) D.max_chunk = 1;
+    hip_check(ipr_hip_host_alloc((void**)&D.h_ctl, sizeof(ipr_ctl)), "pinned ctl");
+    hip_check(ipr_hip_event_create(&D.ev0), "event");
+    hip_check(ipr_hip_event_create(&D.ev1), "event");
+    int rc = a->bind(d);
+    if (rc) die("HIP failure in binding device state (%d)\n", rc);
+}
+
+/* ------------------------------------------------------------------------ */
+/* loader                                                                    */
+/* ------------------------------------------------------------------------ */
+
+static void load_from_files(const char* root, int want_weights)
+{
+    char* p = with_ext(root, ".config");
+    printf("\t- Loading configuration file from: \"%s\".\n", p);
+    FILE* f = safe_fopen(p, "r");
+    free(p);
+    unsigned long long V = 0, E = 0;
+    if (fscanf(f, "%llu %llu", &V, &E) != 2) {
+        printf("\t- Failure in reading the number of vertices and edges. Abort...\n");
+        fclose(f);
+        fflush(stdout);
+        exit(-1);
+    }
+    fclose(f);
+    printf("\t\t- %llu vertices\n\t\t- %llu edges\n", V, E);
+    D.V = V;
+    D.E = E;
+    D.h_idx = safe_malloc((V + 1) * sizeof(uint64_t));
+    D.h_adj = safe_malloc((E ? E : 1) * sizeof(uint32_t));
+    p = with_ext(root, ".idx");
+    printf("\t- Loading offset file from: \"%s\".\n", p);
+    f = safe_fopen(p, "rb");
+    free(p);
+    safe_fread(D.h_idx, sizeof(uint64_t), V, f);
+    fclose(f);
+    D.h_idx[V] = E;
+    p = with_ext(root, ".adj");
+    printf("\t- Loading adjacency file from: \"%s\".\n", p);
+    f = safe_fopen(p, "rb");
+    free(p);
+    safe_fread(D.h_adj, sizeof(uint32_t), E, f);
+    fclose(f);
+    if (want_weights) {
+        p = with_ext(root, ".wgt");
+        printf("\t- Loading weight file from: \"%s\".\n", p);
+        f = safe_fopen(p, "rb");
+        free(p);
+        D.h_wgt = safe_malloc((E ? E : 1) * sizeof(uint32_t));
+        safe_fread(D.h_wgt, sizeof(uint32_t), E, f);
+        fclose(f);
+    }
+    D.own_host = 1;
+}
+
+/* The reference trusts its input; kernels index with these values, so they
+ * are checked before anything reaches the device. */
+static void validate_graph(void)
+{
+    uint64_t V = D.V, E = D.E;
+    if (V > 0x7fffffffull) die("Graph has %llu vertices; this build supports ids below 2^31.\n",
+                               (unsigned long long)V);
+    if (V && D.h_idx[0] != 0) die("Invalid graph: first offset is %llu, expected 0.\n",
+                                  (unsigned long long)D.h_idx[0]);
+    int bad = 0;
+    for (uint64_t v = 0; v < V; v++)
+        if (D.h_idx[v + 1] < D.h_idx[v]) { bad = 1; break; }
+    if (bad || (V && D.h_idx[V] != E)) die("Invalid graph: offsets are not non-decreasing up to E.\n");
+    uint64_t badarc = 0;
+    #pragma omp parallel for reduction(|:badarc) schedule(static)
+    for (uint64_t e = 0; e < E; e++) badarc |= D.h_adj[e] >= V;
+    if (badarc) die("Invalid graph: an arc points outside [0, V).\n");
+}
+
+static void host_free(void)
+{
+    if (D.own_host) {
+        free(D.h_idx);
+        free(D.h_adj);
+        free(D.h_wgt);
+    }
+    D.h_idx = NULL;
+    D.h_adj = NULL;
+    D.h_wgt = NULL;
+    D.own_host = 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* ip_init                                                                   */
+/* ------------------------------------------------------------------------ */
+
+void ipr_drv_init(const ipr_app_desc* app, const char* file_path, int number_of_threads,
+                  const char* schedule, int chunk_size, bool directed, bool weighted)
+{
+    /* src/iPregel_postamble.h:176-193: only "static" and "dynamic" exist */
+    if (strcmp(schedule, "static") == 0)
+        printf("Runtime schedule set to static(%d).\n", chunk_size);
+    else if (strcmp(schedule, "dynamic") == 0)
+        printf("Runtime schedule set to dynamic(%d).\n", chunk_size);
+    else
+        die("The schedule %s is unknown.\n", schedule);
+
+    if (D.initialised) {
+        dev_free_all();
+        host_free();
+        D.initialised = 0;
+    }
+    D.app = *app;
+    printf("Version:%s\n", app->version);
+    printf("Software:iPregel\n");
+    printf("FileCommits:%s\n", app->commits);
+    printf("Application:%s\n", app->application);
+    printf("MpiProcessCount:0\n");
+    printf("Machine:%s\n", app->machine);
+    printf("CompilationFlags:%s\n", app->compilation_flags);
+    time_t t = time(NULL);
+    struct tm tm = *localtime(&t);
+    printf("SubmissionDate:%02d/%02d/%d\n", tm.tm_mday, tm.tm_mon + 1, tm.tm_year + 1900);
+    printf("SubmissionTime:%02d:%02d:%02d\n", tm.tm_hour, tm.tm_min, tm.tm_sec);
+    double t0 = now_s();
+    const char* name = file_path;
+    for (const char* q = strchr(name, '/'); q; q = strchr(name, '/')) name = q + 1;
+    printf("Graph:%s\n", name);
+    printf("OpenmpThreadCount:%d\n", number_of_threads);
+
+    D.directed = directed;
+    D.weighted = weighted;
+    double l0 = now_s();
+    printf("[INFO] The application indicates that the graph must be %sdirected and %sweighted, "
+           "so the graph passed is expected to be so.\n", directed ? "" : "un", weighted ? "" : "un");
+    printf("[INFO] Starting graph loading.\n");
+    int want_w = app->needs_weights;
+    if (D.have_cfg && D.cfg.idx) {
+        D.V = D.cfg.V;
+        D.E = D.cfg.E;
+        D.h_idx = (uint64_t*)D.cfg.idx;
+        D.h_adj = (uint32_t*)D.cfg.adj;
+        D.h_wgt = want_w ? (uint32_t*)D.cfg.wgt : NULL;
+        D.own_host = 0;
+        if (want_w && !D.h_wgt) die("The application needs edge weights but none were given.\n");
+        printf("\t\t- %llu vertices\n\t\t- %llu edges\n", (unsigned long long)D.V, (unsigned long long)D.E);
+    } else {
+        load_from_files(file_path, want_w);
+    }
+    validate_graph();
+
+    int device = D.have_cfg ? D.cfg.device : (int)env_u32("IPR_DEVICE", 0);
+    char devname[128] = {0};
+    uint64_t hbm = 0;
+    hip_check(ipr_hip_init(device, devname, sizeof(devname), &hbm), "device init");
+    printf("Device:%s\n", devname);
+    snprintf(D.stats.device_name, sizeof(D.stats.device_name), "%s", devname);
+    hip_check(ipr_hip_wallclock_hz(&D.wall_hz), "wall clock rate");
+    if (D.have_cfg && D.cfg.stream) {
+        D.stream = D.cfg.stream;
+        D.own_stream = 0;
+    } else {
+        hip_check(ipr_hip_stream_create(&D.stream), "stream");
+        D.own_stream = 1;
+    }
+    dev_setup();
+    int cc = app->check_combiner();
+    if (cc) die("The combiner declared on the compile line does not match ip_combine.\n");
+    double l1 = now_s();
+    D.stats.load_s = l1 - l0;
+    printf("LoadingTime:%f\n", l1 - l0);
+    D.initialised = 1;
+    D.quiet = D.have_cfg ? D.cfg.quiet : 0;
+    memset(&D.stats.supersteps, 0, sizeof(D.stats) - offsetof(ipr_run_stats, supersteps));
+    D.stats.load_s = l1 - l0;
+    snprintf(D.stats.device_name, sizeof(D.stats.device_name), "%s", devname);
+    D.stats.V = D.V;
+    D.stats.E = D.E;
+    printf("InitialisationTime:%f\n", now_s() - t0);
+    fflush(stdout);
+}
+
+uint64_t ipr_drv_vertices_count(void) { return D.V; }
+uint64_t ipr_drv_edges_count(void) { return D.E; }
+uint64_t ipr_drv_superstep(void) { return D.stats.supersteps; }
+
+/* ------------------------------------------------------------------------ */
+/* ip_run                                                                    */
+/* ------------------------------------------------------------------------ */
+
+static void launch_app(int k)
+{
+    int rc = D.app.launch(k, &D.dev, D.stream);
+    if (rc) die("HIP failure launching kernel %d (%d)\n", k, rc);
+}
+
+/* one superstep of the variant's BSP loop, in phase order */
+static void enqueue_superstep(void)
+{
+    switch (D.app.variant) {
+    case IPR_VARIANT_PULL:
+        launch_app(IPR_K_PULL_STEP);
+        break;
+    case IPR_VARIANT_SPREAD:
+        launch_app(IPR_K_COMPUTE);
+        launch_app(IPR_K_EXPAND);
+        break;
+    case IPR_VARIANT_PUSH:
+        launch_app(IPR_K_COMPUTE);
+        launch_app(IPR_K_EXPAND);
+        launch_app(IPR_K_SWAP);
+        break;
+    default:
+        die("Engine variant %d is not available in this build.\n", D.app.variant);
+    }
+    hip_check(ipr_hip_finalize(&D.dev, D.app.variant, D.stream), "finalize");
+}
+
+static void enqueue_chunk(uint32_t n)
+{
+    int slot = 0;
+    while ((1u << slot) < n) slot++;
+    if (D.use_graph && slot < IPR_MAX_GRAPHS) {
+        if (!D.graphs[slot]) {
+            hip_check(ipr_hip_graph_begin(D.stream), "graph capture");
+            for (uint32_t i = 0; i < n; i++) enqueue_superstep();
+            hip_check(ipr_hip_graph_end(D.stream, &D.graphs[slot]), "graph capture");
+        }
+        hip_check(ipr_hip_graph_launch(D.graphs[slot], D.stream), "graph launch");
+    } else {
+        for (uint32_t i = 0; i < n; i++) enqueue_superstep();
+    }
+}
+
+static void report_device_error(uint32_t err)
+{
+    if (err & 1u)
+        die("The function send_message should not be used in the SINGLE_BROADCAST "
+            "version; only broadcast() should be called, and once per superstep maximum.\n");
+    if (err & 2u)
+        die("ip_broadcast_weighted cannot be combined with another broadcast in one superstep.\n");
+    die("Device-side error flags 0x%x.\n", err);
+}
+
+static void run_once(void)
+{
+    void* st = D.stream;
+    hip_check(ipr_hip_begin(&D.dev, st), "begin");
+    launch_app(IPR_K_INIT);
+    hip_check(ipr_hip_event_record(D.ev0, st), "event");
+    uint32_t chunk = 1;
+    uint64_t launched = 0;
+    for (;;) {
+        enqueue_chunk(chunk);
+        launched += chunk;
+        hip_check(ipr_hip_memcpy_d2h(D.h_ctl, D.dev.ctl, sizeof(ipr_ctl), st), "ctl readback");
+        hip_check(ipr_hip_stream_sync(st), "superstep chunk");
+        if (D.h_ctl->error) report_device_error(D.h_ctl->error);
+        if (D.h_ctl->done) break;
+        if (chunk < D.max_chunk) chunk *= 2;
+        if (chunk > D.max_chunk) chunk = D.max_chunk;
+    }
+    hip_check(ipr_hip_event_record(D.ev1, st), "event");
+    float ms = 0;
+    hip_check(ipr_hip_event_elapsed_ms(D.ev0, D.ev1, &ms), "event");
+    D.stats.last_ms = ms;
+    D.stats.total_ms += ms;
+    if (D.stats.runs == 0 || ms < D.stats.min_ms) D.stats.min_ms = ms;
+    D.stats.runs++;
+    D.stats.launches = launched;
+    D.stats.supersteps = D.h_ctl->executed;
+    D.stats.work_edges = D.h_ctl->work_edges;
+    D.stats.work_vertices = D.h_ctl->work_vertices;
+    D.stats.gathered = D.h_ctl->gathered;
+}
+
+int ipr_drv_run(void)
+{
+    if (!D.initialised) die("ip_run called before ip_init.\n");
+    if (D.V == 0) {
+        printf("Total time of supersteps: %fs.\n", 0.0);
+        D.stats.supersteps = 0;
+        return 0;
+    }
+    int reps = D.have_cfg && D.cfg.repeat > 0 ? D.cfg.repeat : 1;
+    for (int r = 0; r < reps; r++) run_once();
+    uint64_t n = D.stats.supersteps;
+    uint64_t m = n < IPR_TRACE_CAP ? n : IPR_TRACE_CAP;
+    uint64_t* act = safe_malloc((m + 1) * 8);
+    uint64_t* ts = safe_malloc((m + 2) * 8);
+    hip_check(ipr_hip_memcpy_d2h(act, D.dev.trace_active, m * 8, D.stream), "trace");
+    hip_check(ipr_hip_memcpy_d2h(ts, D.dev.trace_ts, (m + 1) * 8, D.stream), "trace");
+    hip_check(ipr_hip_stream_sync(D.stream), "trace");
+    double tick = D.wall_hz ? 1.0 / (double)D.wall_hz : 1e-8;
+    if (!D.quiet) {
+        for (uint64_t s = 0; s < m; s++) {
+            printf("Superstep%zuDuration:%f\n", (size_t)s, (double)(ts[s + 1] - ts[s]) * tick);
+            printf("Superstep%zuActiveVertexCount:%zu\n", (size_t)s, (size_t)act[s]);
+        }
+    }
+    D.stats.device_ms = m ? (double)(ts[m] - ts[0]) * tick * 1e3 : 0.0;
+    double t = D.stats.last_ms * 1e-3;
+    printf("Total time of supersteps: %fs.\n", t);
+    printf("DeviceSuperstepTime:%f\n", D.stats.device_ms * 1e-3);
+    printf("WorkEdges:%llu\n", (unsigned long long)D.stats.work_edges);
+    printf("GTEPS:%f\n", t > 0 ? (double)D.stats.work_edges / t * 1e-9 : 0.0);
+    fflush(stdout);
+    free(act);
+    free(ts);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* ip_dump                                                                   */
+/* ------------------------------------------------------------------------ */
+
+void ipr_drv_dump(FILE* f)
+{
+    double t0 = now_s();
+    if (D.have_cfg && D.cfg.skip_dump) {
+        printf("DumpingTime:%f\n", now_s() - t0);
+        return;
+    }
+    if (!D.initialised) die("ip_dump called before ip_init.\n");
+    size_t bytes = (size_t)D.V * D.app.val_size;
+    void* host = safe_malloc(bytes);
+    hip_check(ipr_hip_memcpy_d2h(host, D.dev.val, bytes, D.stream), "value readback");
+    hip_check(ipr_hip_stream_sync(D.stream), "value readback");
+    D.app.serialise(f, host, D.h_idx, 0, D.V);
+    fflush(f);
+    free(host);
+    printf("DumpingTime:%f\n", now_s() - t0);
+}
+
+/* ------------------------------------------------------------------------ */
+/* harness                                                                   */
+/* ------------------------------------------------------------------------ */
+
+void ipr_harness_configure(const ipr_harness_cfg* cfg)
+{
+    if (cfg) {
+        D.cfg = *cfg;
+        D.have_cfg = 1;
+    } else {
+        memset(&D.cfg, 0, sizeof(D.cfg));
+        D.have_cfg = 0;
+    }
+}
+
+void ipr_harness_reset(void) { ipr_harness_configure(NULL); }
+
+int ipr_harness_stats(ipr_run_stats* out)
+{
+    if (!out) return -EINVAL;
+    *out = D.stats;
+    return 0;
+}
+
+int ipr_harness_values(void* out, uint64_t bytes)
+{
+    if (!D.initialised) return -EINVAL;
+    uint64_t need = D.V * D.app.val_size;
+    if (bytes < need) return -EINVAL;
+    if (ipr_hip_memcpy_d2h(out, D.dev.val, need, D.stream)) return -EIO;
+    if (ipr_hip_stream_sync(D.stream)) return -EIO;
+    return 0;
+}
+
+int ipr_harness_trace(uint64_t* active, uint64_t* edges, double* ns, uint64_t cap)
+{
+    if (!D.initialised) return -EINVAL;
+    uint64_t n = D.stats.supersteps;
+    if (n > IPR_TRACE_CAP) n = IPR_TRACE_CAP;
+    if (n > cap) n = cap;
+    uint64_t* ts = safe_malloc((n + 1) * 8);
+    int rc = 0;
+    if (active && ipr_hip_memcpy_d2h(active, D.dev.trace_active, n * 8, D.stream)) rc = -EIO;
+    if (edges && ipr_hip_memcpy_d2h(edges, D.dev.trace_edges, n * 8, D.stream)) rc = -EIO;
+    if (ipr_hip_memcpy_d2h(ts, D.dev.trace_ts, (n + 1) * 8, D.stream)) rc = -EIO;
+    if (ipr_hip_stream_sync(D.stream)) rc = -EIO;
+    if (!rc && ns) {
+        double tick = D.wall_hz ? 1e9 / (double)D.wall_hz : 10.0;
+        for (uint64_t s = 0; s < n; s++) ns[s] = (double)(ts[s + 1] - ts[s]) * tick;
+    }
+    free(ts);
+    return rc ? rc : (int)n;
+}
+
+void ipr_harness_release(void)
+{
+    if (D.initialised) {
+        ipr_hip_stream_sync(D.stream);
+        dev_free_all();
+        host_free();
+        D.initialised = 0;
+    }
+}

@@ -1,0 +1,419 @@
+/*
+ * ipr_hip.hip - libipregel_hip.so: the thin C-ABI between the C host driver
+ * and HIP on gfx950 (memory, streams, events, graphs), plus the kernels that
+ * do not depend on the application's types: superstep control, CSC build,
+ * in-degree binning.
+ */
+#include <cstring>
+#include <cstdio>
+
+#include <hip/hip_runtime.h>
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include "ipregel_hip.h"
+
+static thread_local char g_err[512];
+
+static int fail(hipError_t e, const char* what)
+{
+    snprintf(g_err, sizeof(g_err), "%s: %s", what, hipGetErrorString(e));
+    return -(int)e;
+}
+#define CK(call)                                            \
+    do {                                                    \
+        hipError_t e_ = (call);                             \
+        if (e_ != hipSuccess) return fail(e_, #call);       \
+    } while (0)
+
+extern "C" const char* ipr_hip_last_error(void) { return g_err; }
+
+extern "C" int ipr_hip_init(int device, char* name, size_t name_len, uint64_t* hbm_bytes)
+{
+    int n = 0;
+    CK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) {
+        snprintf(g_err, sizeof(g_err), "device %d not present (%d visible)", device, n);
+        return -(int)hipErrorInvalidDevice;
+    }
+    CK(hipSetDevice(device));
+    hipDeviceProp_t p;
+    CK(hipGetDeviceProperties(&p, device));
+    if (name && name_len) snprintf(name, name_len, "%s (%s)", p.name, p.gcnArchName);
+    if (hbm_bytes) *hbm_bytes = (uint64_t)p.totalGlobalMem;
+    return 0;
+}
+
+extern "C" int ipr_hip_wallclock_hz(uint64_t* hz)
+{
+    int dev = 0, khz = 0;
+    CK(hipGetDevice(&dev));
+    CK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    *hz = (uint64_t)khz * 1000ull;
+    return 0;
+}
+
+extern "C" int ipr_hip_malloc(void** p, size_t bytes)
+{
+    *p = nullptr;
+    CK(hipMalloc(p, bytes ? bytes : 16));
+    return 0;
+}
+extern "C" int ipr_hip_free(void* p)
+{
+    if (p) CK(hipFree(p));
+    return 0;
+}
+extern "C" int ipr_hip_host_alloc(void** p, size_t bytes)
+{
+    *p = nullptr;
+    CK(hipHostMalloc(p, bytes ? bytes : 16, hipHostMallocDefault));
+    return 0;
+}
+extern "C" int ipr_hip_host_free(void* p)
+{
+    if (p) CK(hipHostFree(p));
+    return 0;
+}
+extern "C" int ipr_hip_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream)
+{
+    if (!bytes) return 0;
+    CK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
+    return 0;
+}
+extern "C" int ipr_hip_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream)
+{
+    if (!bytes) return 0;
+    CK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
+    return 0;
+}
+extern "C" int ipr_hip_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream)
+{
+    if (!bytes) return 0;
+    CK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    return 0;
+}
+
+__global__ void k_fill(uint8_t* p, uint8_t b, size_t n)
+{
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t n16 = n / 16;
+    uint4 v;
+    v.x = v.y = v.z = v.w = 0x01010101u * b;
+    if (((uintptr_t)p & 15) == 0) {
+        for (size_t k = i; k < n16; k += stride) ((uint4*)p)[k] = v;
+        for (size_t k = n16 * 16 + i; k < n; k += stride) p[k] = b;
+    } else {
+        for (size_t k = i; k < n; k += stride) p[k] = b;
+    }
+}
+
+extern "C" int ipr_hip_fill(void* dst, int byte, size_t bytes, void* stream)
+{
+    if (!bytes) return 0;
+    size_t blocks = (bytes / 16 + 255) / 256;
+    if (blocks < 1) blocks = 1;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                       (uint8_t*)dst, (uint8_t)byte, bytes);
+    CK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int ipr_hip_stream_create(void** stream)
+{
+    hipStream_t s;
+    CK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = (void*)s;
+    return 0;
+}
+extern "C" int ipr_hip_stream_destroy(void* stream)
+{
+    if (stream) CK(hipStreamDestroy((hipStream_t)stream));
+    return 0;
+}
+extern "C" int ipr_hip_stream_sync(void* stream)
+{
+    CK(hipStreamSynchronize((hipStream_t)stream));
+    return 0;
+}
+extern "C" int ipr_hip_event_create(void** ev)
+{
+    hipEvent_t e;
+    CK(hipEventCreate(&e));
+    *ev = (void*)e;
+    return 0;
+}
+extern "C" int ipr_hip_event_destroy(void* ev)
+{
+    if (ev) CK(hipEventDestroy((hipEvent_t)ev));
+    return 0;
+}
+extern "C" int ipr_hip_event_record(void* ev, void* stream)
+{
+    CK(hipEventRecord((hipEvent_t)ev, (hipStream_t)stream));
+    return 0;
+}
+extern "C" int ipr_hip_event_elapsed_ms(void* start, void* stop, float* ms)
+{
+    CK(hipEventSynchronize((hipEvent_t)stop));
+    CK(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+    return 0;
+}
+
+extern "C" int ipr_hip_graph_begin(void* stream)
+{
+    CK(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal));
+    return 0;
+}
+extern "C" int ipr_hip_graph_end(void* stream, void** exec)
+{
+    hipGraph_t g;
+    hipGraphExec_t x;
+    CK(hipStreamEndCapture((hipStream_t)stream, &g));
+    hipError_t e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(e, "hipGraphInstantiate");
+    *exec = (void*)x;
+    return 0;
+}
+extern "C" int ipr_hip_graph_launch(void* exec, void* stream)
+{
+    CK(hipGraphLaunch((hipGraphExec_t)exec, (hipStream_t)stream));
+    return 0;
+}
+extern "C" int ipr_hip_graph_destroy(void* exec)
+{
+    if (exec) CK(hipGraphExecDestroy((hipGraphExec_t)exec));
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* CSC                                                                       */
+/* ------------------------------------------------------------------------ */
+
+/* (dst << 32 | src) for every arc, one wavefront per source row */
+__global__ void k_coo_keys(uint64_t V, const uint64_t* idx, const uint32_t* adj, uint64_t* keys)
+{
+    uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) / 64;
+    uint32_t lane = threadIdx.x & 63;
+    for (uint64_t u = wave; u < V; u += nw) {
+        uint64_t b = idx[u], e = idx[u + 1];
+        for (uint64_t k = b + lane; k < e; k += 64) keys[k] = ((uint64_t)adj[k] << 32) | u;
+    }
+}
+
+__global__ void k_iota_u32(uint32_t* p, uint64_t n)
+{
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = (uint32_t)i;
+}
+
+/* sorted keys -> in_adj, in_wgt and the row boundaries in_idx */
+__global__ void k_csc_finish(uint64_t V, uint64_t E, const uint64_t* keys, const uint32_t* perm,
+                             const uint32_t* out_wgt, uint64_t* in_idx, uint32_t* in_adj,
+                             uint32_t* in_wgt)
+{
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k <= E; k += stride) {
+        uint64_t dcur = k < E ? (keys[k] >> 32) : V;
+        uint64_t dprev = k > 0 ? (keys[k - 1] >> 32) : (uint64_t)-1;
+        if (k < E) {
+            in_adj[k] = (uint32_t)keys[k];
+            if (in_wgt) in_wgt[k] = out_wgt[perm[k]];
+        }
+        for (uint64_t v = dprev + 1; v <= dcur && v <= V; v++) in_idx[v] = k;
+    }
+}
+
+static int nbits(uint64_t x)
+{
+    int b = 0;
+    while (x) { b++; x >>= 1; }
+    return b ? b : 1;
+}
+
+extern "C" int ipr_hip_build_csc(uint64_t V, uint64_t E, const uint64_t* out_idx,
+                                 const uint32_t* out_adj, const uint32_t* out_wgt,
+                                 uint64_t* in_idx, uint32_t* in_adj, uint32_t* in_wgt,
+                                 void* stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    uint64_t *kin = nullptr, *kout = nullptr;
+    uint32_t *pin = nullptr, *pout = nullptr;
+    void* tmp = nullptr;
+    size_t tmp_bytes = 0;
+    int rc = 0;
+    const unsigned end_bit = 32 + nbits(V ? V - 1 : 0);
+    hipError_t e;
+    if (E == 0) {
+        hipLaunchKernelGGL(k_csc_finish, dim3(1), dim3(256), 0, st, V, E, (const uint64_t*)nullptr,
+                           (const uint32_t*)nullptr, out_wgt, in_idx, in_adj, (uint32_t*)nullptr);
+        CK(hipGetLastError());
+        return 0;
+    }
+    if ((e = hipMalloc(&kin, E * 8)) != hipSuccess) { rc = fail(e, "csc keys"); goto out; }
+    if ((e = hipMalloc(&kout, E * 8)) != hipSuccess) { rc = fail(e, "csc keys"); goto out; }
+    hipLaunchKernelGGL(k_coo_keys, dim3(4096), dim3(256), 0, st, V, out_idx, out_adj, kin);
+    if (out_wgt) {
+        if ((e = hipMalloc(&pin, E * 4)) != hipSuccess) { rc = fail(e, "csc perm"); goto out; }
+        if ((e = hipMalloc(&pout, E * 4)) != hipSuccess) { rc = fail(e, "csc perm"); goto out; }
+        hipLaunchKernelGGL(k_iota_u32, dim3(4096), dim3(256), 0, st, pin, E);
+        e = rocprim::radix_sort_pairs(nullptr, tmp_bytes, kin, kout, pin, pout, (size_t)E, 0, end_bit, st);
+    } else {
+        e = rocprim::radix_sort_keys(nullptr, tmp_bytes, kin, kout, (size_t)E, 0, end_bit, st);
+    }
+    if (e != hipSuccess) { rc = fail(e, "radix sort size"); goto out; }
+    if ((e = hipMalloc(&tmp, tmp_bytes ? tmp_bytes : 16)) != hipSuccess) { rc = fail(e, "csc temp"); goto out; }
+    if (out_wgt)
+        e = rocprim::radix_sort_pairs(tmp, tmp_bytes, kin, kout, pin, pout, (size_t)E, 0, end_bit, st);
+    else
+        e = rocprim::radix_sort_keys(tmp, tmp_bytes, kin, kout, (size_t)E, 0, end_bit, st);
+    if (e != hipSuccess) { rc = fail(e, "radix sort"); goto out; }
+    {
+        uint64_t blocks = (E + 1 + 255) / 256;
+        if (blocks > 65536) blocks = 65536;
+        hipLaunchKernelGGL(k_csc_finish, dim3((unsigned)blocks), dim3(256), 0, st, V, E, kout, pout,
+                           out_wgt, in_idx, in_adj, out_wgt ? in_wgt : (uint32_t*)nullptr);
+    }
+    if ((e = hipGetLastError()) != hipSuccess) { rc = fail(e, "csc finish"); goto out; }
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) { rc = fail(e, "csc sync"); goto out; }
+out:
+    (void)hipFree(kin); (void)hipFree(kout); (void)hipFree(pin); (void)hipFree(pout); (void)hipFree(tmp);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* in-degree bins                                                            */
+/* ------------------------------------------------------------------------ */
+
+__global__ void k_pull_bins(uint64_t V, const uint64_t* in_idx, uint32_t small_max, uint32_t medium_max,
+                            uint32_t* b0, uint32_t* b1, uint32_t* b2, unsigned long long* cnt)
+{
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += stride) {
+        uint64_t deg = in_idx[v + 1] - in_idx[v];
+        int b = deg <= small_max ? 0 : (deg <= medium_max ? 1 : 2);
+        uint32_t* dst = b == 0 ? b0 : (b == 1 ? b1 : b2);
+        unsigned long long k = atomicAdd(&cnt[b], 1ull);
+        dst[k] = (uint32_t)v;
+    }
+}
+
+extern "C" int ipr_hip_build_pull_bins(ipr_dev* dev, uint32_t small_max, uint32_t medium_max, void* stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    unsigned long long* cnt = nullptr;
+    unsigned long long h[3] = {0, 0, 0};
+    CK(hipMalloc(&cnt, sizeof(h)));
+    hipError_t e = hipMemsetAsync(cnt, 0, sizeof(h), st);
+    if (e == hipSuccess) {
+        uint64_t blocks = (dev->V + 255) / 256;
+        if (blocks < 1) blocks = 1;
+        if (blocks > 65536) blocks = 65536;
+        hipLaunchKernelGGL(k_pull_bins, dim3((unsigned)blocks), dim3(256), 0, st, dev->V, dev->in_idx,
+                           small_max, medium_max, dev->pbins[0], dev->pbins[1], dev->pbins[2], cnt);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(cnt);
+    if (e != hipSuccess) return fail(e, "pull bins");
+    for (int b = 0; b < 3; b++) dev->pbin_size[b] = h[b];
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* superstep control                                                         */
+/* ------------------------------------------------------------------------ */
+
+__global__ void k_begin(ipr_dev d)
+{
+    if (threadIdx.x != 0) return;
+    ipr_ctl* c = d.ctl;
+    uint64_t cap = c->trace_cap;
+    memset(c, 0, sizeof(ipr_ctl));
+    c->trace_cap = cap;
+    if (cap) d.trace_ts[0] = wall_clock64();
+}
+
+/* End of one superstep: active count, halting test, superstep increment.
+ * PULL finalizes superstep s-1 after the fused fetch(s-1)+compute(s) kernel
+ * (src/combiner_single_broadcast_postamble.h:267 loop test `active != 0`);
+ * SPREAD: src/combiner_spread_postamble.h:341 `first || active > 0` with
+ * active = |next frontier| (:410-411); PUSH: src/combiner_postamble.h:229. */
+__global__ void k_finalize(ipr_dev d, int variant)
+{
+    if (threadIdx.x != 0) return;
+    ipr_ctl* c = d.ctl;
+    if (c->done) return;
+    const uint64_t s = c->superstep;
+    uint64_t fin, active, edges;
+    if (variant == IPR_VARIANT_PULL) {
+        if (s == 0) { c->superstep = 1; return; }
+        fin = s - 1;
+        active = c->cnt[fin & 1];
+        c->cnt[fin & 1] = 0;
+        edges = c->bcast[fin & 1] ? d.in_idx[d.V] : 0;
+        c->bcast[fin & 1] = 0;
+        c->done = active == 0;
+        c->executed = fin + 1;
+        c->superstep = c->done ? s : s + 1;
+    } else {
+        fin = s;
+        if (variant == IPR_VARIANT_PUSH) {
+            active = c->cnt[s & 1];
+            c->cnt[s & 1] = 0;
+        } else {
+            active = c->fsize[(s + 1) & 1];
+            c->fsize[s & 1] = 0;
+        }
+        edges = c->step_edges;
+        c->done = active == 0;
+        c->executed = s + 1;
+        c->superstep = s + 1;
+    }
+    for (int b = 0; b < IPR_NBINS; b++) c->bsize[b] = 0;
+    c->step_edges = 0;
+    c->work_edges += edges;
+    if (c->pull_mode) c->gathered++;
+    c->pull_mode = 0;
+    c->last_active = active;
+    if (fin < c->trace_cap) {
+        d.trace_active[fin] = active;
+        d.trace_edges[fin] = edges;
+        d.trace_ts[fin + 1] = wall_clock64();
+    }
+}
+
+/* push applications with an order-independent combiner: deliver this
+ * superstep by gather when the broadcasters' out-arcs are a large share of E */
+__global__ void k_direction(ipr_dev d)
+{
+    if (threadIdx.x != 0) return;
+    ipr_ctl* c = d.ctl;
+    if (c->done) return;
+    c->pull_mode = (d.dir_opt && c->step_edges * (uint64_t)d.dir_alpha > d.E) ? 1 : 0;
+}
+
+extern "C" int ipr_hip_begin(const ipr_dev* dev, void* stream)
+{
+    hipLaunchKernelGGL(k_begin, dim3(1), dim3(64), 0, (hipStream_t)stream, *dev);
+    CK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int ipr_hip_finalize(const ipr_dev* dev, int variant, void* stream)
+{
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, (hipStream_t)stream, *dev, variant);
+    CK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int ipr_hip_direction(const ipr_dev* dev, void* stream)
+{
+    hipLaunchKernelGGL(k_direction, dim3(1), dim3(64), 0, (hipStream_t)stream, *dev);
+    CK(hipGetLastError());
+    return 0;
+}

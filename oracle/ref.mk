@@ -1,0 +1,30 @@
+# Compiles the reference applications from the reference tree where they lie
+# (never copied), with the reference makefile's flags (makefile:10-17), into
+# $(OUT)/bin. COMMITS is a literal because the tree has no git metadata.
+REF ?= /root/reference
+OUT ?= _ref
+CFLAGS := -std=c11 -O3 -fopenmp -w
+DEFS := -DIP_FORCE_DIRECT_MAPPING -DVERSION=\"1.0.0\" -DIP_MACHINE=\"ref\" -D_GNU_SOURCE \
+        -DCOMMITS=\"none\" -DCOMPILATION_FLAGS=\"ref\"
+
+APPS := cc pagerank sssp
+VARIANTS := plain spread single_broadcast single_broadcast_spread
+BINS := $(OUT)/bin/cc_32 $(OUT)/bin/cc_spread_32 $(OUT)/bin/cc_single_broadcast_32 \
+        $(OUT)/bin/cc_single_broadcast_spread_32 \
+        $(OUT)/bin/pagerank_32 $(OUT)/bin/pagerank_single_broadcast_32 \
+        $(OUT)/bin/sssp_32 $(OUT)/bin/sssp_spread_32 $(OUT)/bin/sssp_single_broadcast_32 \
+        $(OUT)/bin/sssp_single_broadcast_spread_32
+
+all: $(BINS)
+
+$(OUT)/bin:
+	mkdir -p $@
+
+define variant_flags
+$(if $(findstring single_broadcast,$(1)),-DIP_USE_SINGLE_BROADCAST) $(if $(findstring spread,$(1)),-DIP_USE_SPREAD)
+endef
+
+$(OUT)/bin/%_32: | $(OUT)/bin
+	gcc $(CFLAGS) $(DEFS) -DIP_APPLICATION=\"$*\" $(call variant_flags,$*) \
+	    -DIP_VERTEX_ID_TYPE=uint32_t -I$(REF)/src \
+	    $(REF)/benchmarks/$(firstword $(subst _, ,$*)).c -o $@
