@@ -1,0 +1,247 @@
+"""Benchmark of the MI355X iPregel engine on one GPU.
+
+A step is one complete superstep loop (ip_run: every superstep until the
+engine halts) of the application over its graph, which stays resident in HBM
+across steps. The headline line is hashmin connected components (cc_spread:
+push with an active frontier, min combiner) on a symmetrised R-MAT graph,
+scale 22, edge factor 16, a/b/c/d = .57/.19/.19/.05, seed 2.
+
+  value      GTEPS = W / t, W = sum over supersteps of the out-degrees of the
+             vertices that broadcast (deterministic under BSP), t = device time
+             of the K timed steps (HIP events on the engine's stream)
+  roofline   the dominant kernel (the message expansion), algorithmic bytes
+             8 B per relaxed arc (4 B destination id + 4 B mailbox update)
+             + 4 B per next-frontier entry, over its HIP-event-timed launch
+             durations, against the 8 TB/s HBM peak of MI355X
+  cpu_baseline  the reference's own OpenMP build (oracle/_ref, compiled from
+             the reference sources) on the same graph file, its
+             "Total time of supersteps", with its thread count
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--apps cc22,pr16,...]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+from ipregel_amd import engine as EN  # noqa: E402
+from ipregel_amd import graph as G  # noqa: E402
+
+METRIC = "GTEPS & % HBM peak per app; served p50/p99 + queries/s vs plain loop, 1 GPU"
+HBM_PEAK_GBS = 8000.0
+REF_BIN = ROOT / "oracle" / "_ref" / "bin"
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+# workload name -> (app, graph builder, description, reference binary, source)
+def wl_cc22():
+    g = G.rmat(22, 16, seed=2, symmetrise=True)
+    return ("cc_spread", g, "cc_spread hashmin CC, symmetrised R-MAT scale 22 ef 16 "
+            "a/b/c/d .57/.19/.19/.05 seed 2", "cc_spread_32", 0)
+
+
+def wl_pr16():
+    g = G.rmat(16, 16, seed=1)
+    return ("pagerank_single_broadcast", g, "pagerank_single_broadcast FP64 10 supersteps, "
+            "R-MAT scale 16 ef 16 seed 1 (pull over own list, directed=false)",
+            "pagerank_single_broadcast_32", 0)
+
+
+def wl_ss24a():
+    g = G.rmat(24, 16, seed=3)
+    return ("sssp_spread", g, "sssp_spread unit weights, R-MAT scale 24 ef 16 seed 3, "
+            "source = highest out-degree vertex", "sssp_spread_32", g.max_degree_vertex())
+
+
+def wl_ss24b():
+    g = G.rmat(24, 16, seed=3).with_weights(3)
+    return ("sssp_weighted_spread", g, "sssp_weighted_spread uint32 weights [1,255], R-MAT "
+            "scale 24 ef 16 seed 3", None, g.max_degree_vertex())
+
+
+def wl_grid():
+    g = G.grid(4096, 4096).with_weights(3)
+    return ("sssp_weighted_spread", g, "sssp_weighted_spread 4096x4096 grid, weights [1,255] "
+            "seed 3, source 4097", None, 4097)
+
+
+WORKLOADS = {"cc22": wl_cc22, "pr16": wl_pr16, "ss24a": wl_ss24a, "ss24b": wl_ss24b,
+             "grid": wl_grid}
+
+
+def model_bytes(app: str, g: G.Graph, W: int, F: int, supersteps_with_bcast: int) -> int:
+    """Algorithmic bytes of a whole run: push 8 B/relaxed arc (+4 B weight) +
+    28 B/computed vertex; pull PR 12 B/in-arc + 40 B/vertex per superstep."""
+    if app.startswith("pagerank"):
+        return supersteps_with_bcast * (12 * g.E + 40 * g.V)
+    per_arc = 12 if "weighted" in app else 8
+    return per_arc * W + 28 * F
+
+
+def kernel_bytes(app: str, g: G.Graph, W: int, F_next: int, supersteps_with_bcast: int) -> int:
+    """Algorithmic bytes moved by the dominant kernel over the run."""
+    if app.startswith("pagerank"):
+        # fused fetch + compute: 4 B in-index + 8 B gathered message per in-arc,
+        # 8 B offset + 8 B value read/write + 8 B slot write per vertex
+        return supersteps_with_bcast * (12 * g.E + 40 * g.V)
+    per_arc = 12 if "weighted" in app else 8
+    return per_arc * W + 4 * F_next
+
+
+def cpu_baseline(binname: str | None, g: G.Graph, W: int, source: int) -> dict | None:
+    if binname is None or not (REF_BIN / binname).exists():
+        return None
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    with tempfile.TemporaryDirectory(dir="/tmp") as td:
+        root = os.path.join(td, "g")
+        g.write(root)
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="TRUE",
+                   OMP_PLACES="cores")
+        args = [str(REF_BIN / binname), root, "/dev/null", str(threads), "static", "0"]
+        if binname.startswith("sssp"):
+            args.append(str(source))
+        t0 = time.time()
+        try:
+            p = subprocess.run(args, env=env, capture_output=True, text=True, timeout=150)
+        except subprocess.TimeoutExpired:
+            return {"value": None, "unit": "GTEPS", "cores": threads, "kind": "reference",
+                    "sample": f"{binname} did not finish within 150 s"}
+        wall = time.time() - t0
+        m = re.search(r"Total time of supersteps: ([0-9.]+)s", p.stdout)
+        if p.returncode != 0 or not m:
+            return {"value": None, "unit": "GTEPS", "cores": threads, "kind": "reference",
+                    "sample": f"{binname} failed rc={p.returncode}"}
+        t = float(m.group(1))
+        trace = [int(x.split(":")[1]) for x in p.stdout.splitlines() if "ActiveVertexCount" in x]
+        return {"value": W / t / 1e9, "unit": "GTEPS", "cores": threads, "kind": "reference",
+                "sample": f"one full run of the reference {binname} (OpenMP, static schedule) on "
+                          f"the same graph file, its 'Total time of supersteps' {t:.3f} s "
+                          f"(process wall {wall:.1f} s)",
+                "seconds": t, "trace": trace}
+
+
+def run_workload(key: str, steps: int, warmup: int, with_cpu: bool) -> dict:
+    t0 = time.time()
+    app, g, desc, refbin, source = WORKLOADS[key]()
+    log(f"{key}: graph V={g.V} E={g.E} built in {time.time() - t0:.1f}s")
+    sess = EN.Session(app, g, source=source)
+    log(f"{key}: resident on {sess.stats().device_name.decode()} (load {sess.stats().load_s:.2f}s)")
+    sess.run(warmup)
+    st = sess.stats()
+    before_ms = st.total_ms
+    # timed region: exactly `steps` runs, each bracketed by HIP events on the
+    # engine's stream and followed by a stream synchronisation
+    w0 = time.perf_counter()
+    sess.run(steps)
+    wall_ms = (time.perf_counter() - w0) * 1e3 / steps
+    st = sess.stats()
+    ms = (st.total_ms - before_ms) / steps
+    W = int(st.work_edges)
+    F = int(st.work_vertices)
+    active, sedges, sns = sess.trace()
+    n_super = int(st.supersteps)
+    n_bcast_steps = int(np.count_nonzero(sedges))
+    gteps = W / (ms * 1e-3) / 1e9
+    step_bytes = model_bytes(app, g, W, F, n_bcast_steps)
+    log(f"{key}: {n_super} supersteps, W={W}, {ms:.3f} ms/step, {gteps:.2f} GTEPS")
+    # the dominant kernel, timed by HIP events around each of its launches
+    sess.kernel_events(True)
+    k0_ms, k0_n = st.kernel_ms, st.kernel_launches
+    sess.run(steps)
+    st2 = sess.stats()
+    sess.kernel_events(False)
+    k_ms = (st2.kernel_ms - k0_ms) / steps
+    k_launch = (st2.kernel_launches - k0_n) / steps
+    k_bytes = kernel_bytes(app, g, W, int(active.sum()) if not app.startswith("pagerank") else 0,
+                           n_bcast_steps)
+    achieved = k_bytes / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
+    values_digest = int(np.bitwise_xor.reduce(sess.values().view(np.uint32))) if g.V else 0
+    sess.close()
+    out = {
+        "app": app, "workload": desc, "V": g.V, "E": g.E, "supersteps": n_super, "W": W,
+        "vertices_computed": F, "ms_per_step": ms, "wall_ms_per_step": wall_ms, "GTEPS": gteps,
+        "model_bytes_per_step": step_bytes,
+        "model_GBps": step_bytes / (ms * 1e-3) / 1e9,
+        "hbm_frac_model": step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "kernel": {"name": "ipr_k_pull_step" if app.startswith("pagerank") else "ipr_k_expand",
+                   "ms_per_step": k_ms, "launches_per_step": k_launch,
+                   "avg_launch_ms": k_ms / k_launch if k_launch else None,
+                   "algorithmic_bytes_per_step": k_bytes, "achieved_GBps": achieved,
+                   "ms_per_step_plain_launches": (st2.total_ms - st.total_ms) / steps},
+        "active_trace": [int(x) for x in active[:64]],
+        "values_xor": values_digest,
+    }
+    if with_cpu:
+        cb = cpu_baseline(refbin, g, W, source)
+        if cb:
+            if cb.get("trace") is not None:
+                cb["trace_matches_gpu"] = cb["trace"] == [int(x) for x in active]
+                cb.pop("trace")
+            out["cpu_baseline"] = cb
+            log(f"{key}: cpu baseline {cb}")
+    return out
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--apps", default="cc22,pr16")
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is a design "
+                                   "(see the design notes), --gpus must be 1"}))
+        return 2
+    keys = [k for k in a.apps.split(",") if k]
+    results = {}
+    for k in keys:
+        results[k] = run_workload(k, a.steps, a.warmup, not a.no_cpu)
+    head = results[keys[0]]
+    k = head["kernel"]
+    line = {
+        "metric": METRIC,
+        "value": head["GTEPS"],
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32" if not head["app"].startswith("pagerank") else "f64",
+        "data": "synthetic",
+        "config": {"workload": head["workload"], "V": head["V"], "E": head["E"],
+                   "supersteps": head["supersteps"], "W": head["W"]},
+        "roofline": {"bound": "hbm", "achieved": k["achieved_GBps"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": k["achieved_GBps"] / HBM_PEAK_GBS,
+                     "traffic": None, "kernel": k["name"],
+                     "avg_launch_ms": k["avg_launch_ms"]},
+        "cpu_baseline": head.get("cpu_baseline"),
+        "hbm_frac_model": head["hbm_frac_model"],
+        "apps": {kk: {x: v for x, v in r.items() if x not in ("active_trace",)}
+                 for kk, r in results.items()},
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

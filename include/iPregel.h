@@ -16,9 +16,13 @@
  *   - ip_init, ip_run, ip_dump (host, src/iPregel_preamble.h:317,328,336).
  * Messages are passed by value, as in the reference code (not its README).
  *
- * The only edits an application needs are device qualifiers: __device__ on
- * ip_compute and ip_combine, and __managed__ on globals that ip_compute reads
- * after main() sets them (e.g. benchmarks/pagerank.c:32-33).
+ * The edits an application needs are device qualifiers: __device__ on
+ * ip_compute and ip_combine, and __device__ on globals that ip_compute reads
+ * after main() sets them (e.g. benchmarks/pagerank.c:32-33), followed by one
+ * IP_PUBLISH(name) after the host assignment: host code writes the host copy
+ * of a __device__ variable and IP_PUBLISH copies it to the device.
+ * (__managed__ globals are not used: managed memory is HMM-backed, and GPU
+ * access to it relies on page-fault support that these GPUs run without.)
  *
  * Optional compile-line declarations:
  *   -DIP_COMBINER_MIN / _MAX / _SUM  the combiner is min / max / sum, so
@@ -151,6 +155,18 @@ void ip_dump(FILE* f);
 
 #include "ipregel_kernels.h"
 
+/* Copy the host copy of a __device__ application global to the device, e.g.
+ *   __device__ double ratio;  ...  ratio = 0.15 / n; IP_PUBLISH(ratio);
+ * Exits like the reference's safe functions on failure. */
+template <typename T> static inline void ipr_publish(const T& var, const char* name)
+{
+    if (hipMemcpyToSymbol(HIP_SYMBOL(var), &var, sizeof(T), 0, hipMemcpyHostToDevice) != hipSuccess) {
+        printf("Failed to publish the application global %s to the device.\n", name);
+        exit(-1);
+    }
+}
+#define IP_PUBLISH(var) ipr_publish((var), #var)
+
 extern "C" {
 
 __host__ __device__ inline size_t ip_get_superstep(void)
@@ -184,6 +200,7 @@ __host__ __device__ inline size_t ip_get_edges_count(void)
     return ip_edges_count;
 #endif
 }
+
 
 __device__ inline bool ip_has_message(struct ip_vertex_t* v)
 {

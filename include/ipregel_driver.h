@@ -50,6 +50,8 @@ typedef struct ipr_harness_cfg {
     uint32_t bin_small_max, bin_medium_max, grid_blocks;
     uint32_t pull_small_max, pull_medium_max;
     uint32_t max_chunk;
+    int32_t defer_run;          /* ip_run returns at once; runs go through ipr_harness_run */
+    int32_t kernel_events;      /* HIP events around every launch of the dominant kernel */
 } ipr_harness_cfg;
 
 typedef struct ipr_run_stats {
@@ -65,6 +67,8 @@ typedef struct ipr_run_stats {
     double min_ms;
     double device_ms;           /* device timestamps: first to last superstep */
     double load_s;              /* graph load + upload */
+    double kernel_ms;           /* dominant kernel: summed event time of live launches */
+    uint64_t kernel_launches;   /* dominant kernel: live launches timed */
     char device_name[128];
 } ipr_run_stats;
 
@@ -75,6 +79,13 @@ int ipr_harness_stats(ipr_run_stats* out);
 int ipr_harness_values(void* out, uint64_t bytes);
 /* per-superstep ActiveVertexCount, W and duration (ns) of the last run */
 int ipr_harness_trace(uint64_t* active, uint64_t* edges, double* ns, uint64_t cap);
+/* run the superstep loop `repeat` more times on the resident graph */
+int ipr_harness_run(int repeat);
+/* 1: HIP events around every launch of the dominant kernel (plain launches);
+ * 0: off (chunks replayed as HIP graphs) */
+int ipr_harness_kernel_events(int on);
+/* the hipStream_t the driver launches on */
+void* ipr_harness_stream(void);
 /* release all device state of the current graph */
 void ipr_harness_release(void);
 

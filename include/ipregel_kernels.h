@@ -438,7 +438,10 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_expand(ipr_dev d)
         bool wf = (e0 >> 31) != 0;
         uint64_t off = d.out_idx[v], deg = d.out_idx[v + 1] - off;
         IP_MESSAGE_TYPE m = bm[v];
-        for (uint64_t k = gtid; k < deg + (gthreads - 1) - ((deg + gthreads - 1) % gthreads) && k < ((deg + gthreads - 1) / gthreads) * gthreads; k += gthreads) {
+        /* rounded to whole wavefronts so every lane of a wavefront makes the
+         * same number of trips (the append below is wavefront-collective) */
+        const uint64_t deg_r = (deg + IPR_WAVE - 1) & ~(uint64_t)(IPR_WAVE - 1);
+        for (uint64_t k = gtid; k < deg_r; k += gthreads) {
             bool first = false;
             uint32_t u = 0;
             if (k < deg) {
@@ -524,18 +527,14 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_swap(ipr_dev d)
     uint64_t n_react = 0;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < d.V; v += stride) {
-        uint32_t word = cl[v >> 5];
-        if (word & (1u << (v & 31))) {
+        const uint32_t bit = 1u << (v & 31);
+        if (cl[v >> 5] & bit) {
             ((IP_MESSAGE_TYPE*)d.msg)[v] = ipr_policy::value(mb[v]);
             mb[v] = ipr_policy::empty();
+            atomicAnd(cl + (v >> 5), ~bit);   /* clean for superstep s+2 */
             d.hm[v] = 1;
             if (!d.act[v]) { d.act[v] = 1; n_react++; }
         }
-    }
-    __syncthreads();
-    /* bits are cleared after every thread of the block has read its word */
-    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w * 32 < d.V && w < d.words; w += stride) {
-        (void)w;
     }
     ipr_wave_add((unsigned long long*)&ctl->cnt[s & 1], n_react);
 }
@@ -615,7 +614,7 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_pull_step_ordered(ipr_dev d)
     if (s > 0) ipr_wave_add((unsigned long long*)&ctl->cnt[(s + 1) & 1], n_react);
     ipr_wave_add((unsigned long long*)&ctl->cnt[s & 1], n_active);
     ipr_wave_add((unsigned long long*)&ctl->work_vertices, n_computed);
-    ipr_wave_add((unsigned long long*)&ctl->bcast, n_bcast);
+    ipr_wave_add((unsigned long long*)&ctl->bcast[s & 1], n_bcast);
 }
 
 /* reassociating fold for declared combiners: vertices binned by in-degree
@@ -702,7 +701,7 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_pull_step_binned(ipr_dev d, u
     if (s > 0) ipr_wave_add((unsigned long long*)&ctl->cnt[(s + 1) & 1], n_react);
     ipr_wave_add((unsigned long long*)&ctl->cnt[s & 1], n_active);
     ipr_wave_add((unsigned long long*)&ctl->work_vertices, n_computed);
-    ipr_wave_add((unsigned long long*)&ctl->bcast, n_bcast);
+    ipr_wave_add((unsigned long long*)&ctl->bcast[s & 1], n_bcast);
 }
 
 /* ------------------------------------------------------------------------ */

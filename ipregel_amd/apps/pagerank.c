@@ -19,8 +19,8 @@ typedef IP_MESSAGE_TYPE IP_VALUE_TYPE;
 #define PR_ROUNDS 10u
 
 /* set by main after ip_init, read by ip_compute on the device */
-__managed__ double pr_teleport;
-__managed__ double pr_start;
+__device__ double pr_teleport;
+__device__ double pr_start;
 
 __device__ void ip_compute(struct ip_vertex_t* v)
 {
@@ -60,6 +60,8 @@ int main(int argc, char* argv[])
     ip_init(argv[1], atoi(argv[3]), argv[4], atoi(argv[5]), false, false);
     pr_teleport = 0.15 / ip_get_vertices_count();
     pr_start = 1.0 / ip_get_vertices_count();
+    IP_PUBLISH(pr_teleport);
+    IP_PUBLISH(pr_start);
     ip_run();
     FILE* out = fopen(argv[2], "w");
     if (!out) {

@@ -19,7 +19,7 @@ typedef IP_VERTEX_ID_TYPE IP_MESSAGE_TYPE;
 typedef IP_VERTEX_ID_TYPE IP_VALUE_TYPE;
 #include "iPregel.h"
 
-__managed__ IP_VERTEX_ID_TYPE sssp_source;
+__device__ IP_VERTEX_ID_TYPE sssp_source;
 
 __device__ void ip_compute(struct ip_vertex_t* v)
 {
@@ -63,6 +63,7 @@ int main(int argc, char* argv[])
     sssp_source = (IP_VERTEX_ID_TYPE)strtoul(argv[6], NULL, 10);
     printf("ApplicationConfiguration:startVertex=%u\n", (unsigned)sssp_source);
     ip_init(argv[1], atoi(argv[3]), argv[4], atoi(argv[5]), false, false);
+    IP_PUBLISH(sssp_source);
     ip_run();
     FILE* out = fopen(argv[2], "w");
     if (!out) {

@@ -51,6 +51,9 @@ static struct {
     int use_graph;
     uint32_t max_chunk;
     int quiet;
+    int kev_on;              /* events around the dominant kernel */
+    void* kev[2][64];        /* start/stop per superstep slot of a chunk */
+    uint32_t kev_slot;       /* slot of the superstep being enqueued */
 } D;
 
 /* ------------------------------------------------------------------------ */
@@ -153,6 +156,12 @@ static void dev_free_all(void)
     memset(d, 0, sizeof(*d));
     if (D.h_ctl) ipr_hip_host_free(D.h_ctl);
     D.h_ctl = NULL;
+    for (int i = 0; i < 64; i++) {
+        if (D.kev[0][i]) ipr_hip_event_destroy(D.kev[0][i]);
+        if (D.kev[1][i]) ipr_hip_event_destroy(D.kev[1][i]);
+        D.kev[0][i] = D.kev[1][i] = NULL;
+    }
+    D.kev_on = 0;
     if (D.ev0) ipr_hip_event_destroy(D.ev0);
     if (D.ev1) ipr_hip_event_destroy(D.ev1);
     D.ev0 = D.ev1 = NULL;
@@ -253,6 +262,9 @@ static void dev_setup(void)
     hip_check(ipr_hip_host_alloc((void**)&D.h_ctl, sizeof(ipr_ctl)), "pinned ctl");
     hip_check(ipr_hip_event_create(&D.ev0), "event");
     hip_check(ipr_hip_event_create(&D.ev1), "event");
+    if (D.max_chunk > 64) D.max_chunk = 64;
+    D.kev_on = 0;
+    if (c && c->kernel_events) ipr_harness_kernel_events(1);
     int rc = a->bind(d);
     if (rc) die("HIP failure in binding device state (%d)\n", rc);
 }
@@ -441,20 +453,38 @@ static void launch_app(int k)
     if (rc) die("HIP failure launching kernel %d (%d)\n", k, rc);
 }
 
+/* the kernel that moves the messages: expansion (push) or fused fetch (pull) */
+static int dominant_kernel(void)
+{
+    return D.app.variant == IPR_VARIANT_PULL ? IPR_K_PULL_STEP : IPR_K_EXPAND;
+}
+
+static void launch_timed(int k)
+{
+    if (D.kev_on && k == dominant_kernel() && D.kev_slot < 64) {
+        hip_check(ipr_hip_event_record(D.kev[0][D.kev_slot], D.stream), "kernel event");
+        launch_app(k);
+        hip_check(ipr_hip_event_record(D.kev[1][D.kev_slot], D.stream), "kernel event");
+        D.kev_slot++;
+    } else {
+        launch_app(k);
+    }
+}
+
 /* one superstep of the variant's BSP loop, in phase order */
 static void enqueue_superstep(void)
 {
     switch (D.app.variant) {
     case IPR_VARIANT_PULL:
-        launch_app(IPR_K_PULL_STEP);
+        launch_timed(IPR_K_PULL_STEP);
         break;
     case IPR_VARIANT_SPREAD:
         launch_app(IPR_K_COMPUTE);
-        launch_app(IPR_K_EXPAND);
+        launch_timed(IPR_K_EXPAND);
         break;
     case IPR_VARIANT_PUSH:
         launch_app(IPR_K_COMPUTE);
-        launch_app(IPR_K_EXPAND);
+        launch_timed(IPR_K_EXPAND);
         launch_app(IPR_K_SWAP);
         break;
     default:
@@ -467,7 +497,8 @@ static void enqueue_chunk(uint32_t n)
 {
     int slot = 0;
     while ((1u << slot) < n) slot++;
-    if (D.use_graph && slot < IPR_MAX_GRAPHS) {
+    D.kev_slot = 0;
+    if (D.use_graph && !D.kev_on && slot < IPR_MAX_GRAPHS) {
         if (!D.graphs[slot]) {
             hip_check(ipr_hip_graph_begin(D.stream), "graph capture");
             for (uint32_t i = 0; i < n; i++) enqueue_superstep();
@@ -496,13 +527,25 @@ static void run_once(void)
     launch_app(IPR_K_INIT);
     hip_check(ipr_hip_event_record(D.ev0, st), "event");
     uint32_t chunk = 1;
-    uint64_t launched = 0;
+    uint64_t launched = 0, prev_exec = 0;
     for (;;) {
         enqueue_chunk(chunk);
         launched += chunk;
         hip_check(ipr_hip_memcpy_d2h(D.h_ctl, D.dev.ctl, sizeof(ipr_ctl), st), "ctl readback");
         hip_check(ipr_hip_stream_sync(st), "superstep chunk");
         if (D.h_ctl->error) report_device_error(D.h_ctl->error);
+        if (D.kev_on) {
+            /* launches after the halting superstep exit at once: not counted */
+            uint64_t live = D.h_ctl->executed - prev_exec;
+            if (D.app.variant == IPR_VARIANT_PULL && prev_exec == 0 && live < chunk) live++;
+            for (uint32_t i = 0; i < D.kev_slot && i < live; i++) {
+                float kms = 0;
+                hip_check(ipr_hip_event_elapsed_ms(D.kev[0][i], D.kev[1][i], &kms), "kernel event");
+                D.stats.kernel_ms += kms;
+                D.stats.kernel_launches++;
+            }
+        }
+        prev_exec = D.h_ctl->executed;
         if (D.h_ctl->done) break;
         if (chunk < D.max_chunk) chunk *= 2;
         if (chunk > D.max_chunk) chunk = D.max_chunk;
@@ -521,15 +564,46 @@ static void run_once(void)
     D.stats.gathered = D.h_ctl->gathered;
 }
 
+static int run_and_report(int reps);
+
 int ipr_drv_run(void)
 {
     if (!D.initialised) die("ip_run called before ip_init.\n");
+    if (D.have_cfg && D.cfg.defer_run) return 0;
+    int reps = D.have_cfg && D.cfg.repeat > 0 ? D.cfg.repeat : 1;
+    return run_and_report(reps);
+}
+
+int ipr_harness_run(int repeat)
+{
+    if (!D.initialised) return -EINVAL;
+    int q = D.quiet;
+    D.quiet = 1;
+    int rc = run_and_report(repeat > 0 ? repeat : 1);
+    D.quiet = q;
+    return rc;
+}
+
+void* ipr_harness_stream(void) { return D.stream; }
+
+int ipr_harness_kernel_events(int on)
+{
+    if (on && !D.kev[0][0]) {
+        for (int i = 0; i < 64; i++) {
+            if (ipr_hip_event_create(&D.kev[0][i]) || ipr_hip_event_create(&D.kev[1][i])) return -EIO;
+        }
+    }
+    D.kev_on = on != 0;
+    return 0;
+}
+
+static int run_and_report(int reps)
+{
     if (D.V == 0) {
         printf("Total time of supersteps: %fs.\n", 0.0);
         D.stats.supersteps = 0;
         return 0;
     }
-    int reps = D.have_cfg && D.cfg.repeat > 0 ? D.cfg.repeat : 1;
     for (int r = 0; r < reps; r++) run_once();
     uint64_t n = D.stats.supersteps;
     uint64_t m = n < IPR_TRACE_CAP ? n : IPR_TRACE_CAP;

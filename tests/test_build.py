@@ -1,0 +1,50 @@
+"""CPU checks of the built libraries: they load, and every function that
+include/*.h declares is exported (no compute calls; no GPU needed)."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import pytest
+
+from ipregel_amd import engine as EN
+
+ROOT = Path(__file__).resolve().parents[1]
+INC = ROOT / "include"
+LIB = ROOT / "ipregel_amd" / "lib"
+
+
+def declared(header):
+    text = (INC / header).read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[\w\s\*]+?\b(ipr?_\w+)\s*\(", text, flags=re.M)
+    return sorted(set(n for n in names if not n.startswith("ipr_mix64") and n != "ipr_hash3"))
+
+
+def test_hip_layer_exports():
+    L = C.CDLL(str(LIB / "libipregel_hip.so"))
+    names = declared("ipregel_hip.h")
+    assert len(names) > 20
+    for n in names:
+        assert hasattr(L, n), n
+
+
+def test_graph_lib_exports():
+    L = C.CDLL(str(LIB / "libipregel_graph.so"))
+    for n in declared("ipregel_graph.h"):
+        assert hasattr(L, n), n
+
+
+@pytest.mark.parametrize("lib", sorted(p.name for p in LIB.glob("libipr_*_32.so")))
+def test_app_libs_export_driver_and_ipregel_api(lib):
+    L = C.CDLL(str(LIB / lib))
+    for n in declared("ipregel_driver.h"):
+        assert hasattr(L, n), n
+    for n in ("ip_init", "ip_run", "ip_dump", "main"):
+        assert hasattr(L, n), n
+
+
+def test_all_apps_built():
+    from ipregel_amd.build import APPS
+    for name, _, _ in APPS:
+        assert (LIB / f"libipr_{name}_32.so").exists(), name
+        assert (ROOT / "ipregel_amd" / "bin" / f"{name}_32").exists(), name

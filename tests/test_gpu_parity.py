@@ -1,0 +1,123 @@
+"""GPU parity: the HIP engine through the C-ABI application libraries against
+the reference binaries' golden outputs and against the oracle.
+Bit-exact for CC/SSSP (labels, distances, ip_dump text, active-count trace);
+FP64 PageRank bit-exact in ordered mode and within 1e-12 relative per vertex
+in the reassociating mode."""
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, golden_runs
+from ipregel_amd import engine as EN
+from ipregel_amd import graph as G
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+BUILT = {p.name[len("libipr_"):-len("_32.so")] for p in EN.LIB.glob("libipr_*_32.so")}
+PR_RTOL = 1e-12
+
+
+def ours(binname):
+    return binname[:-3]
+
+
+def engine_name_for(binname):
+    name = ours(binname)
+    if name.endswith("single_broadcast_spread"):
+        pytest.skip("spread + single_broadcast variant not built yet")
+    assert name in BUILT, f"{name} missing from {sorted(BUILT)}"
+    return name
+
+
+@pytest.mark.parametrize("case,binname", golden_runs())
+def test_golden_text_and_trace(case, binname, tmp_path):
+    name = engine_name_for(binname)
+    g = G.read(GOLDEN / case)
+    out = tmp_path / "out.txt"
+    print(f"case={case} app={name}", flush=True)
+    r = EN.run_app(name, g, source=0, out_path=str(out), pull_ordered=1)
+    assert out.read_text() == (GOLDEN / f"{case}__{binname}.out").read_text()
+    trace = [int(x) for x in (GOLDEN / f"{case}__{binname}.trace").read_text().split()]
+    assert list(r.active) == trace
+    assert r.supersteps == len(trace)
+
+
+@pytest.mark.parametrize("case", ["rmat10", "rmat8", "star65", "chain10"])
+def test_pagerank_reassociated_within_tolerance(case):
+    g = G.read(GOLDEN / case)
+    ref = O.run(g, O.PR, O.PULL).values
+    r = EN.run_app("pagerank_single_broadcast", g, pull_ordered=2, pull_small_max=2,
+                   pull_medium_max=16)
+    rel = np.abs(r.values - ref) / np.maximum(np.abs(ref), 1e-300)
+    assert rel.max() <= PR_RTOL, rel.max()
+
+
+def test_pagerank_scale16_config():
+    """PageRank, 10 supersteps, FP64, pull, R-MAT scale 16 ef 16 seed 1."""
+    g = G.rmat(16, 16, seed=1)
+    ref = O.run(g, O.PR, O.PULL)
+    r1 = EN.run_app("pagerank_single_broadcast", g, pull_ordered=1)
+    assert np.array_equal(r1.values.view(np.uint64), ref.values.view(np.uint64))
+    assert list(r1.active) == list(ref.active[: ref.supersteps])
+    r2 = EN.run_app("pagerank_single_broadcast", g, pull_ordered=2)
+    rel = np.abs(r2.values - ref.values) / np.abs(ref.values)
+    assert rel.max() <= PR_RTOL, rel.max()
+    assert r2.work_edges == ref.work_edges == 10 * g.E
+
+
+@pytest.mark.parametrize("name", ["cc_spread", "cc", "cc_single_broadcast", "cc_generic_spread"])
+def test_cc_medium(name):
+    g = G.rmat(15, 16, seed=2, symmetrise=True)
+    ref = O.run(g, O.CC, O.VARIANTS[name[2:].replace("_generic", "")])
+    r = EN.run_app(name, g)
+    assert np.array_equal(r.values, ref.values)
+    assert np.array_equal(r.values, O.min_ancestor(g))
+    assert list(r.active) == list(ref.active[: ref.supersteps])
+    if name != "cc_single_broadcast":
+        assert r.work_edges == ref.work_edges
+
+
+@pytest.mark.parametrize("name", ["sssp_spread", "sssp", "sssp_single_broadcast"])
+def test_sssp_medium(name):
+    g = G.rmat(15, 16, seed=3)
+    src = g.max_degree_vertex()
+    ref = O.run(g, O.SSSP, O.VARIANTS[name[4:]], source=src)
+    r = EN.run_app(name, g, source=src)
+    assert np.array_equal(r.values, ref.values)
+    assert list(r.active) == list(ref.active[: ref.supersteps])
+
+
+@pytest.mark.parametrize("name", ["sssp_weighted_spread", "sssp_weighted"])
+def test_weighted_sssp_medium(name):
+    g = G.rmat(14, 16, seed=3).with_weights(3)
+    ref = O.dijkstra(g, 0)
+    r = EN.run_app(name, g, source=0)
+    assert np.array_equal(r.values, ref)
+    gg = G.grid(64, 64).with_weights(3)
+    r = EN.run_app(name, gg, source=65)
+    assert np.array_equal(r.values, O.dijkstra(gg, 65))
+
+
+@pytest.mark.parametrize("tune", [
+    dict(use_graph=2), dict(max_chunk=1), dict(bin_small_max=1, bin_medium_max=2),
+    dict(bin_small_max=4096, bin_medium_max=8192), dict(grid_blocks=7), dict(dir_opt=2)])
+def test_cc_tuning_does_not_change_results(tune):
+    g = G.rmat(14, 16, seed=9, symmetrise=True)
+    ref = O.min_ancestor(g)
+    r = EN.run_app("cc_spread", g, **tune)
+    assert np.array_equal(r.values, ref)
+
+
+def test_repeat_runs_identical():
+    g = G.rmat(13, 16, seed=4)
+    r = EN.run_app("sssp_spread", g, source=0, repeat=3)
+    assert r.runs == 3
+    assert np.array_equal(r.values, O.bfs(g, 0))
+
+
+def test_empty_and_edgeless_graphs():
+    g = G.from_arcs(5, [], [])
+    r = EN.run_app("cc_spread", g)
+    assert r.values.tolist() == [0, 1, 2, 3, 4]
+    r = EN.run_app("sssp_spread", g, source=2)
+    assert r.values.tolist() == [2**32 - 1, 2**32 - 1, 0, 2**32 - 1, 2**32 - 1]
