@@ -64,15 +64,18 @@
 #define IP_NEEDS_IN_NEIGHBOUR_COUNT
 #endif
 
+/* numeric, so that the preprocessor can test them; the values are those of
+ * the IPR_VARIANT_* enumerators of ipregel_engine.h */
 #if defined(IP_USE_SPREAD) && defined(IP_USE_SINGLE_BROADCAST)
-#define IPR_VARIANT IPR_VARIANT_SPREAD_PULL
+#define IPR_VARIANT 3
 #elif defined(IP_USE_SPREAD)
-#define IPR_VARIANT IPR_VARIANT_SPREAD
+#define IPR_VARIANT 2
 #elif defined(IP_USE_SINGLE_BROADCAST)
-#define IPR_VARIANT IPR_VARIANT_PULL
+#define IPR_VARIANT 1
 #else
-#define IPR_VARIANT IPR_VARIANT_PUSH
+#define IPR_VARIANT 0
 #endif
+#define IPR_IS_PULL (IPR_VARIANT == 1 || IPR_VARIANT == 3)
 
 #if defined(IP_COMBINER_MIN)
 #define IPR_COMBINER IPR_COMBINER_MIN
@@ -225,7 +228,7 @@ __device__ inline bool ip_get_next_message(struct ip_vertex_t* v, IP_MESSAGE_TYP
  * src/combiner_single_broadcast_postamble.h:155-159. */
 __device__ inline void ip_broadcast(struct ip_vertex_t* v, IP_MESSAGE_TYPE message)
 {
-#if IPR_VARIANT == IPR_VARIANT_PULL || IPR_VARIANT == IPR_VARIANT_SPREAD_PULL
+#if IPR_IS_PULL
     v->ipr_bcast = true;
     v->ipr_bcast_msg = message;
 #else
@@ -244,7 +247,7 @@ __device__ inline void ip_broadcast(struct ip_vertex_t* v, IP_MESSAGE_TYPE messa
  * u receives message + w(v,u). Push variants only, once per superstep. */
 __device__ inline void ip_broadcast_weighted(struct ip_vertex_t* v, IP_MESSAGE_TYPE message)
 {
-#if IPR_VARIANT == IPR_VARIANT_PULL || IPR_VARIANT == IPR_VARIANT_SPREAD_PULL
+#if IPR_IS_PULL
     ipr_raise_error(IPR_ERR_SEND_IN_PULL);
 #else
     if (v->ipr_bcast) ipr_raise_error(IPR_ERR_MIXED_BROADCAST);
@@ -262,7 +265,7 @@ __device__ inline void ip_broadcast_weighted(struct ip_vertex_t* v, IP_MESSAGE_T
  * the error and the driver prints the same text and exits after the launch. */
 __device__ inline void ip_send_message(IP_VERTEX_ID_TYPE id, IP_MESSAGE_TYPE message)
 {
-#if IPR_VARIANT == IPR_VARIANT_PULL || IPR_VARIANT == IPR_VARIANT_SPREAD_PULL
+#if IPR_IS_PULL
     (void)id; (void)message;
     ipr_raise_error(IPR_ERR_SEND_IN_PULL);
 #else
@@ -309,8 +312,9 @@ void ip_init(const char* file_path, int number_of_threads, const char* schedule,
     d.id_size = sizeof(IP_VERTEX_ID_TYPE);
     d.val_size = sizeof(IP_VALUE_TYPE);
     d.msg_size = sizeof(IP_MESSAGE_TYPE);
-    d.mbox_size = sizeof(typename ipr_policy::slot_t);
-#if IPR_VARIANT == IPR_VARIANT_PULL || IPR_VARIANT == IPR_VARIANT_SPREAD_PULL
+    d.mbox_size = sizeof(ipr_policy::slot_t);
+    d.bslot_size = sizeof(ipr_bslot_t);
+#if IPR_IS_PULL
     d.needs_in_lists = 1;
 #endif
 #ifdef IP_NEEDS_OUT_NEIGHBOUR_WEIGHTS

@@ -11,12 +11,15 @@
  *   val[V]        application value                    (ip_vertex_t.value)
  *   msg[V]        delivered combined message            (ip_vertex_t.message)
  *   act[V], hm[V] active / has_message flags, one byte each
- *   bcf[2][V/32]  broadcast bitmaps by superstep parity (pull variants)
- *   bcm[2][V]     broadcast messages by parity (pull) / per broadcaster (push)
+ *   bcf[2][V]     broadcast flags (bytes) by superstep parity (pull variants)
+ *   bcm[2][V]     broadcast messages by superstep parity (pull variants)
+ *   bslot[V]      push variants: last broadcast {superstep+1 | weighted<<31, msg}
  *   mbox[2][V]    combiner mailboxes by parity (push variants)
- *   claim[2][V/32] first-delivery bitmaps by parity (push variants)
- *   front[2][V]   frontier / target lists by parity (spread variants)
+ *   claim[2][V/32] first-delivery bitmaps by parity (push variants); the next
+ *                 frontier is compacted from it in ascending id runs
+ *   front[2][V]   frontier lists by parity (spread variants)
  *   bins[3][V]    broadcasters this superstep, binned by out-degree
+ *   lbase[V]      first arc chunk of each LARGE-bin broadcaster
  */
 #ifndef IPREGEL_ENGINE_H
 #define IPREGEL_ENGINE_H
@@ -67,8 +70,11 @@ typedef struct ipr_ctl {
     uint64_t pull_mode;          /* push apps: 1 when this superstep gathers */
     uint64_t last_active;        /* ActiveVertexCount of the last finalized superstep */
     uint64_t gathered;           /* supersteps delivered by gather */
-    uint64_t pad[1];
+    uint64_t lpack;              /* LARGE bin: entries << 40 | arc chunks, one atomic */
 } ipr_ctl;
+
+#define IPR_LPACK_SHIFT 40
+#define IPR_LPACK_MASK ((1ull << IPR_LPACK_SHIFT) - 1)
 
 typedef struct ipr_dev {
     ipr_ctl* ctl;
@@ -80,6 +86,11 @@ typedef struct ipr_dev {
     const uint64_t* in_idx;
     const uint32_t* in_adj;
     const uint32_t* in_wgt;
+    /* push variants that gather dense supersteps: the exact transpose of the
+     * out-edge CSR (aliases it when the graph is symmetric) */
+    const uint64_t* t_idx;
+    const uint32_t* t_adj;
+    const uint32_t* t_wgt;
     void* val;
     void* msg;
     uint8_t* act;
@@ -90,7 +101,9 @@ typedef struct ipr_dev {
     uint32_t* claim[2];
     uint32_t* front[2];
     uint32_t* bins[IPR_NBINS];
-    /* static pull bins over all V (by in-degree), built at load */
+    uint64_t* lbase;             /* LARGE bin: first arc chunk of each entry */
+    void* bslot;                 /* push variants: {superstep+1 | weighted<<31, message} */
+    /* static pull / gather bins over all V (by in-degree), built at load */
     uint32_t* pbins[IPR_NBINS];
     uint64_t pbin_size[IPR_NBINS];
     uint64_t* trace_active;
@@ -104,7 +117,7 @@ typedef struct ipr_dev {
     uint32_t pull_ordered;       /* 1: fold in in-list order (bit-exact) */
     uint32_t dir_opt;            /* push apps: allow gathering dense supersteps */
     uint32_t dir_alpha;          /* gather when bcast out-arcs * alpha > E */
-    uint32_t pad2;
+    uint32_t large_chunk;        /* arcs per workgroup task in the LARGE bin */
 } ipr_dev;
 
 /* kernels the application TU provides (typed by the app's message/value) */
@@ -122,7 +135,7 @@ enum {
 typedef struct ipr_app_desc {
     int variant;
     int combiner;
-    uint32_t id_size, val_size, msg_size, mbox_size;
+    uint32_t id_size, val_size, msg_size, mbox_size, bslot_size;
     int needs_in_lists;          /* pull variants */
     int needs_weights;           /* IP_NEEDS_OUT_NEIGHBOUR_WEIGHTS */
     int can_gather;              /* push app whose combiner makes gather exact */

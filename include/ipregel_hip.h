@@ -70,6 +70,13 @@ int ipr_hip_build_csc(uint64_t V, uint64_t E, const uint64_t* out_idx,
 int ipr_hip_build_pull_bins(ipr_dev* dev, uint32_t small_max, uint32_t medium_max,
                             void* stream);
 
+/* byte-wise equality of two device buffers (multiple of 4 bytes) */
+int ipr_hip_equal(const void* a, const void* b, uint64_t bytes, int* equal, void* stream);
+
+/* push variants: the next frontier (claim bitmap of superstep s+1) compacted
+ * into front[(s+1)&1], ascending within runs; the bitmap is cleared */
+int ipr_hip_compact_frontier(const ipr_dev* dev, void* stream);
+
 /* superstep control (kernels on `stream`, graph-capturable) */
 int ipr_hip_begin(const ipr_dev* dev, void* stream);
 int ipr_hip_finalize(const ipr_dev* dev, int variant, void* stream);

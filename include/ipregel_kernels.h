@@ -225,6 +225,15 @@ template <typename T> struct ipr_comb<T, IPR_COMBINER_GENERIC> {
 typedef ipr_comb<IP_MESSAGE_TYPE, IPR_COMBINER> ipr_policy;
 typedef ipr_policy::slot_t ipr_slot_t;
 
+/* broadcast record of a push-variant vertex: the superstep it broadcast in
+ * (plus one, bit 31 set for ip_broadcast_weighted) and its message; one load
+ * gives a gathering receiver both */
+struct ipr_bslot_t {
+    uint32_t step;
+    IP_MESSAGE_TYPE msg;
+};
+#define IPR_BSTEP_WEIGHTED 0x80000000u
+
 /* set the claim bit of u; true for the one caller that set it */
 __device__ inline bool ipr_claim(uint32_t* bits, uint32_t u)
 {
@@ -234,25 +243,20 @@ __device__ inline bool ipr_claim(uint32_t* bits, uint32_t u)
     return (atomicOr(w, bit) & bit) == 0;
 }
 
-/* deliver one message to u in superstep s; true when u must join the next
- * frontier */
-__device__ inline bool ipr_deliver(const ipr_dev& d, uint64_t s, uint32_t u, IP_MESSAGE_TYPE m)
+/* deliver one message to u in superstep s: combine into the mailbox of
+ * superstep s+1 and mark u as having received (the claim bitmap, from which
+ * the next frontier is compacted and PUSH's swap reads has_message_next) */
+__device__ inline void ipr_deliver(const ipr_dev& d, uint64_t s, uint32_t u, IP_MESSAGE_TYPE m)
 {
     ipr_slot_t* mb = (ipr_slot_t*)d.mbox[(s + 1) & 1];
-    if (!ipr_policy::deliver(mb + u, m)) return false;
-    return ipr_claim(d.claim[(s + 1) & 1], u);
+    if (ipr_policy::deliver(mb + u, m)) ipr_claim(d.claim[(s + 1) & 1], u);
 }
 
-/* ip_send_message in push variants: single-lane delivery (the caller may be
- * divergent user code, so no wavefront aggregation here) */
+/* ip_send_message in push variants: immediate delivery */
 __device__ inline void ipr_send_immediate(IP_VERTEX_ID_TYPE id, IP_MESSAGE_TYPE m)
 {
     const ipr_dev& d = ipr_d_dev;
-    uint64_t s = d.ctl->superstep;
-    if (ipr_deliver(d, s, (uint32_t)id, m) && IPR_VARIANT == IPR_VARIANT_SPREAD) {
-        unsigned long long k = atomicAdd((unsigned long long*)&d.ctl->fsize[(s + 1) & 1], 1ull);
-        d.front[(s + 1) & 1][k] = (uint32_t)id;
-    }
+    ipr_deliver(d, d.ctl->superstep, (uint32_t)id, m);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -299,36 +303,33 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_init(ipr_dev d)
         ((IP_VALUE_TYPE*)d.val)[v] = IP_VALUE_TYPE();
         d.act[v] = 1;  /* ip_init_vertex_range, e.g. src/combiner_postamble.h:189 */
         d.hm[v] = 0;
-        if (IPR_VARIANT == IPR_VARIANT_PUSH || IPR_VARIANT == IPR_VARIANT_SPREAD) {
-            ((ipr_slot_t*)d.mbox[0])[v] = ipr_policy::empty();
-            ((ipr_slot_t*)d.mbox[1])[v] = ipr_policy::empty();
-        } else {
-            ((uint8_t*)d.bcf[0])[v] = 0;
-            ((uint8_t*)d.bcf[1])[v] = 0;
-        }
+#if IPR_IS_PULL
+        ((uint8_t*)d.bcf[0])[v] = 0;
+        ((uint8_t*)d.bcf[1])[v] = 0;
+#else
+        ((ipr_slot_t*)d.mbox[0])[v] = ipr_policy::empty();
+        ((ipr_slot_t*)d.mbox[1])[v] = ipr_policy::empty();
+        ((ipr_bslot_t*)d.bslot)[v].step = 0;
+#endif
     }
+#if !IPR_IS_PULL
     for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < d.words; w += stride) {
-        if (IPR_VARIANT == IPR_VARIANT_PUSH || IPR_VARIANT == IPR_VARIANT_SPREAD) {
-            d.claim[0][w] = 0;
-            d.claim[1][w] = 0;
-        }
+        d.claim[0][w] = 0;
+        d.claim[1][w] = 0;
     }
+#endif
 }
 
 /* ------------------------------------------------------------------------ */
 /* push variants: compute                                                    */
 /* ------------------------------------------------------------------------ */
 
-__device__ inline uint32_t ipr_bin_of(const ipr_dev& d, uint64_t deg)
-{
-    return deg <= d.bin_small_max ? IPR_BIN_SMALL : (deg <= d.bin_medium_max ? IPR_BIN_MEDIUM : IPR_BIN_LARGE);
-}
-
 /* SPREAD: src/combiner_spread_postamble.h:364-401 (superstep 0 over all V,
  * then the frontier) and :459-471 (the frontier's mailboxes become messages:
- * here read straight from the mailbox and reset for reuse two supersteps on).
- * PUSH: src/combiner_postamble.h:246-259 (every vertex with active or
- * has_message). */
+ * here read straight from the mailbox, which is reset for reuse two
+ * supersteps on). PUSH: src/combiner_postamble.h:246-259 (every vertex with
+ * active or has_message). A broadcast is recorded in bslot and the vertex is
+ * appended to its out-degree bin. */
 __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_compute_push(ipr_dev d)
 {
     ipr_ctl* ctl = d.ctl;
@@ -341,7 +342,7 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_compute_push(ipr_dev d)
         list = d.front[s & 1];
     }
     ipr_slot_t* mcur = (ipr_slot_t*)d.mbox[s & 1];
-    uint32_t* ccur = d.claim[s & 1];
+    ipr_bslot_t* bs = (ipr_bslot_t*)d.bslot;
     uint64_t n_active = 0, n_computed = 0, n_edges = 0;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
@@ -354,7 +355,6 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_compute_push(ipr_dev d)
             if (s > 0) {
                 x.message = ipr_policy::value(mcur[v]);
                 mcur[v] = ipr_policy::empty();
-                atomicAnd(ccur + (v >> 5), ~(1u << (v & 31)));
             }
         } else {
             x.active = d.act[v] != 0;
@@ -363,9 +363,8 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_compute_push(ipr_dev d)
             run = x.active || x.has_message;
             x.active = true;
         }
-        bool bc = false;
+        bool small = false, medium = false;
         uint32_t entry = 0;
-        uint32_t bin = 0;
         if (run) {
             ipr_fill_vertex(d, &x, v);
             const IP_VALUE_TYPE before = x.value;
@@ -382,15 +381,31 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_compute_push(ipr_dev d)
                 uint64_t deg = d.out_idx[v + 1] - d.out_idx[v];
                 n_edges += deg;
                 if (deg > 0) {
-                    bc = true;
-                    bin = ipr_bin_of(d, deg);
+                    ipr_bslot_t rec;
+                    rec.step = (uint32_t)(s + 1) | (x.ipr_bcast_weighted ? IPR_BSTEP_WEIGHTED : 0u);
+                    rec.msg = x.ipr_bcast_msg;
+                    bs[v] = rec;
                     entry = v | (x.ipr_bcast_weighted ? 0x80000000u : 0u);
-                    ((IP_MESSAGE_TYPE*)d.bcm[0])[v] = x.ipr_bcast_msg;
+                    if (deg <= d.bin_small_max) {
+                        small = true;
+                    } else if (deg <= d.bin_medium_max) {
+                        medium = true;
+                    } else {
+                        /* one atomic hands out both the list slot and the
+                         * vertex's first arc chunk, so lbase ascends with the
+                         * slot and a chunk finds its vertex by binary search */
+                        uint64_t chunks = (deg + d.large_chunk - 1) / d.large_chunk;
+                        unsigned long long old = atomicAdd((unsigned long long*)&ctl->lpack,
+                                                           (1ull << IPR_LPACK_SHIFT) | chunks);
+                        uint64_t slot = old >> IPR_LPACK_SHIFT;
+                        d.lbase[slot] = old & IPR_LPACK_MASK;
+                        d.bins[IPR_BIN_LARGE][slot] = entry;
+                    }
                 }
             }
         }
-        for (uint32_t b = 0; b < IPR_NBINS; b++)
-            ipr_wave_append((unsigned long long*)&ctl->bsize[b], d.bins[b], bc && bin == b, entry);
+        ipr_wave_append((unsigned long long*)&ctl->bsize[IPR_BIN_SMALL], d.bins[IPR_BIN_SMALL], small, entry);
+        ipr_wave_append((unsigned long long*)&ctl->bsize[IPR_BIN_MEDIUM], d.bins[IPR_BIN_MEDIUM], medium, entry);
     }
     ipr_wave_add((unsigned long long*)&ctl->cnt[s & 1], n_active);
     ipr_wave_add((unsigned long long*)&ctl->work_vertices, n_computed);
@@ -398,58 +413,50 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_compute_push(ipr_dev d)
 }
 
 /* ------------------------------------------------------------------------ */
-/* push variants: expansion                                                  */
+/* push variants: expansion (sparse supersteps)                              */
 /* ------------------------------------------------------------------------ */
 
-__device__ inline IP_MESSAGE_TYPE ipr_edge_msg(const ipr_dev& d, IP_MESSAGE_TYPE m, bool weighted, uint64_t e)
+__device__ inline IP_MESSAGE_TYPE ipr_edge_msg(const uint32_t* wgt, IP_MESSAGE_TYPE m, bool weighted, uint64_t e)
 {
 #ifdef IP_NEEDS_OUT_NEIGHBOUR_WEIGHTS
-    if (weighted) return (IP_MESSAGE_TYPE)(m + (IP_MESSAGE_TYPE)d.out_wgt[e]);
+    if (weighted) return (IP_MESSAGE_TYPE)(m + (IP_MESSAGE_TYPE)wgt[e]);
 #else
-    (void)d; (void)weighted; (void)e;
+    (void)wgt; (void)weighted; (void)e;
 #endif
     return m;
-}
-
-__device__ inline void ipr_after_deliver(const ipr_dev& d, uint64_t s, bool first, uint32_t u)
-{
-    if (IPR_VARIANT == IPR_VARIANT_SPREAD)
-        ipr_wave_append((unsigned long long*)&d.ctl->fsize[(s + 1) & 1], d.front[(s + 1) & 1], first, u);
 }
 
 __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_expand(ipr_dev d)
 {
     ipr_ctl* ctl = d.ctl;
     if (ctl->done) return;
-    const uint64_t s = ctl->superstep;
     if (ctl->pull_mode) return;
-    const IP_MESSAGE_TYPE* bm = (const IP_MESSAGE_TYPE*)d.bcm[0];
+    const uint64_t s = ctl->superstep;
+    const ipr_bslot_t* bs = (const ipr_bslot_t*)d.bslot;
     const uint32_t lane = ipr_lane();
     const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t gthreads = (uint64_t)gridDim.x * blockDim.x;
     const uint64_t gwave = gtid / IPR_WAVE;
     const uint64_t nwaves = gthreads / IPR_WAVE;
 
-    /* LARGE: the whole grid strides over each vertex's arcs */
-    const uint64_t nL = ctl->bsize[IPR_BIN_LARGE];
-    for (uint64_t j = 0; j < nL; j++) {
-        uint32_t e0 = d.bins[IPR_BIN_LARGE][j];
+    /* LARGE: one workgroup per chunk of large_chunk arcs */
+    const uint64_t lp = ctl->lpack;
+    const uint64_t nL = lp >> IPR_LPACK_SHIFT, nC = lp & IPR_LPACK_MASK;
+    for (uint64_t c = blockIdx.x; c < nC; c += gridDim.x) {
+        uint64_t lo = 0, hi = nL;            /* last j with lbase[j] <= c */
+        while (hi - lo > 1) {
+            uint64_t mid = (lo + hi) / 2;
+            if (d.lbase[mid] <= c) lo = mid; else hi = mid;
+        }
+        uint32_t e0 = d.bins[IPR_BIN_LARGE][lo];
         uint32_t v = e0 & 0x7fffffffu;
         bool wf = (e0 >> 31) != 0;
         uint64_t off = d.out_idx[v], deg = d.out_idx[v + 1] - off;
-        IP_MESSAGE_TYPE m = bm[v];
-        /* rounded to whole wavefronts so every lane of a wavefront makes the
-         * same number of trips (the append below is wavefront-collective) */
-        const uint64_t deg_r = (deg + IPR_WAVE - 1) & ~(uint64_t)(IPR_WAVE - 1);
-        for (uint64_t k = gtid; k < deg_r; k += gthreads) {
-            bool first = false;
-            uint32_t u = 0;
-            if (k < deg) {
-                u = d.out_adj[off + k];
-                first = ipr_deliver(d, s, u, ipr_edge_msg(d, m, wf, off + k));
-            }
-            ipr_after_deliver(d, s, first, u);
-        }
+        uint64_t beg = (c - d.lbase[lo]) * d.large_chunk;
+        uint64_t end = beg + d.large_chunk < deg ? beg + d.large_chunk : deg;
+        IP_MESSAGE_TYPE m = bs[v].msg;
+        for (uint64_t k = beg + threadIdx.x; k < end; k += blockDim.x)
+            ipr_deliver(d, s, d.out_adj[off + k], ipr_edge_msg(d.out_wgt, m, wf, off + k));
     }
 
     /* MEDIUM: one wavefront per vertex */
@@ -459,34 +466,27 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_expand(ipr_dev d)
         uint32_t v = e0 & 0x7fffffffu;
         bool wf = (e0 >> 31) != 0;
         uint64_t off = d.out_idx[v], deg = d.out_idx[v + 1] - off;
-        IP_MESSAGE_TYPE m = bm[v];
-        for (uint64_t k0 = 0; k0 < deg; k0 += IPR_WAVE) {
-            uint64_t k = k0 + lane;
-            bool first = false;
-            uint32_t u = 0;
-            if (k < deg) {
-                u = d.out_adj[off + k];
-                first = ipr_deliver(d, s, u, ipr_edge_msg(d, m, wf, off + k));
-            }
-            ipr_after_deliver(d, s, first, u);
-        }
+        IP_MESSAGE_TYPE m = bs[v].msg;
+        for (uint64_t k = lane; k < deg; k += IPR_WAVE)
+            ipr_deliver(d, s, d.out_adj[off + k], ipr_edge_msg(d.out_wgt, m, wf, off + k));
     }
 
-    /* SMALL: 64 vertices per wavefront, arcs spread over the lanes */
+    /* SMALL: 64 vertices per wavefront, their arcs spread over the lanes by a
+     * wavefront prefix sum of the degrees */
     const uint64_t nS = ctl->bsize[IPR_BIN_SMALL];
     for (uint64_t base = gwave * IPR_WAVE; base < nS; base += nwaves * IPR_WAVE) {
         uint64_t i = base + lane;
-        uint32_t v = 0, deg = 0;
+        uint32_t deg = 0;
         uint64_t off = 0;
         bool wf = false;
         IP_MESSAGE_TYPE m = IP_MESSAGE_TYPE();
         if (i < nS) {
             uint32_t e0 = d.bins[IPR_BIN_SMALL][i];
-            v = e0 & 0x7fffffffu;
+            uint32_t v = e0 & 0x7fffffffu;
             wf = (e0 >> 31) != 0;
             off = d.out_idx[v];
             deg = (uint32_t)(d.out_idx[v + 1] - off);
-            m = bm[v];
+            m = bs[v].msg;
         }
         uint32_t incl = ipr_wave_incl_scan(deg);
         uint32_t excl = incl - deg;
@@ -496,21 +496,107 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_expand(ipr_dev d)
             /* owner: the last lane whose exclusive start is <= t */
             uint32_t lo = 0;
             for (uint32_t step = 32; step > 0; step >>= 1) {
-                uint32_t probe = __shfl(excl, lo + step);
+                uint32_t probe = __shfl(excl, (lo + step) & (IPR_WAVE - 1));
                 if (lo + step < IPR_WAVE && probe <= t) lo += step;
             }
             uint64_t o_off = __shfl(off, lo);
             uint32_t o_ex = __shfl(excl, lo);
             IP_MESSAGE_TYPE o_m = __shfl(m, lo);
             bool o_wf = __shfl((int)wf, lo) != 0;
-            bool first = false;
-            uint32_t u = 0;
             if (t < total) {
                 uint64_t e = o_off + (t - o_ex);
-                u = d.out_adj[e];
-                first = ipr_deliver(d, s, u, ipr_edge_msg(d, o_m, o_wf, e));
+                ipr_deliver(d, s, d.out_adj[e], ipr_edge_msg(d.out_wgt, o_m, o_wf, e));
             }
-            ipr_after_deliver(d, s, first, u);
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* push variants: gather (dense supersteps)                                  */
+/* ------------------------------------------------------------------------ */
+
+/* fold of the messages broadcast in superstep s by the in-neighbours
+ * [beg, end) of one vertex, strided by `step` from `first` */
+__device__ inline bool ipr_gather_range(const ipr_dev& d, uint64_t s, uint64_t first, uint64_t end,
+                                        uint64_t step, IP_MESSAGE_TYPE* acc)
+{
+    const ipr_bslot_t* bs = (const ipr_bslot_t*)d.bslot;
+    const uint32_t want = (uint32_t)(s + 1);
+    bool has = false;
+    for (uint64_t k = first; k < end; k += step) {
+        ipr_bslot_t b = bs[d.t_adj[k]];
+        if ((b.step & ~IPR_BSTEP_WEIGHTED) == want) {
+            IP_MESSAGE_TYPE m = b.msg;
+#ifdef IP_NEEDS_OUT_NEIGHBOUR_WEIGHTS
+            if (b.step & IPR_BSTEP_WEIGHTED) m = (IP_MESSAGE_TYPE)(m + (IP_MESSAGE_TYPE)d.t_wgt[k]);
+#endif
+            ipr_policy::fold(acc, m);
+            has = true;
+        }
+    }
+    return has;
+}
+
+/* Receivers fold their broadcasting in-neighbours over the CSC: the same
+ * combined message and the same set of receivers as the push, without
+ * atomics on the arcs (exact for integer min / max / sum combiners).
+ * Vertices binned by in-degree; blocks [0, nb0) one thread per vertex,
+ * [nb0, nb0+nb1) one wavefront per vertex, the rest one workgroup. */
+__global__ void __launch_bounds__(IPR_BLOCK) ipr_k_gather(ipr_dev d, uint32_t nb0, uint32_t nb1)
+{
+    ipr_ctl* ctl = d.ctl;
+    if (ctl->done) return;
+    if (!ctl->pull_mode) return;
+    const uint64_t s = ctl->superstep;
+    const uint32_t lane = ipr_lane();
+    const uint32_t b = blockIdx.x;
+    if (b < nb0) {
+        const uint64_t n = d.pbin_size[0];
+        for (uint64_t i = (uint64_t)b * blockDim.x + threadIdx.x; i < n; i += (uint64_t)nb0 * blockDim.x) {
+            uint32_t v = d.pbins[0][i];
+            IP_MESSAGE_TYPE acc = ipr_policy::ident();
+            if (ipr_gather_range(d, s, d.t_idx[v], d.t_idx[v + 1], 1, &acc)) ipr_deliver(d, s, v, acc);
+        }
+    } else if (b < nb0 + nb1) {
+        const uint64_t n = d.pbin_size[1];
+        const uint64_t wave = (uint64_t)(b - nb0) * (IPR_BLOCK / IPR_WAVE) + threadIdx.x / IPR_WAVE;
+        const uint64_t nwaves = (uint64_t)nb1 * (IPR_BLOCK / IPR_WAVE);
+        for (uint64_t i = wave; i < n; i += nwaves) {
+            uint32_t v = d.pbins[1][i];
+            IP_MESSAGE_TYPE acc = ipr_policy::ident();
+            bool has = ipr_gather_range(d, s, d.t_idx[v] + lane, d.t_idx[v + 1], IPR_WAVE, &acc);
+            for (int o = 32; o > 0; o >>= 1) {
+                IP_MESSAGE_TYPE y = __shfl_xor(acc, o);
+                ipr_policy::fold(&acc, y);
+            }
+            has = __ballot(has) != 0;
+            if (lane == 0 && has) ipr_deliver(d, s, v, acc);
+        }
+    } else {
+        __shared__ IP_MESSAGE_TYPE part[IPR_BLOCK / IPR_WAVE];
+        __shared__ int parth[IPR_BLOCK / IPR_WAVE];
+        const uint64_t n = d.pbin_size[2];
+        const uint32_t nb2 = gridDim.x - nb0 - nb1;
+        for (uint64_t i = b - nb0 - nb1; i < n; i += nb2) {
+            uint32_t v = d.pbins[2][i];
+            IP_MESSAGE_TYPE acc = ipr_policy::ident();
+            bool has = ipr_gather_range(d, s, d.t_idx[v] + threadIdx.x, d.t_idx[v + 1], IPR_BLOCK, &acc);
+            for (int o = 32; o > 0; o >>= 1) {
+                IP_MESSAGE_TYPE y = __shfl_xor(acc, o);
+                ipr_policy::fold(&acc, y);
+            }
+            bool wh = __ballot(has) != 0;
+            if (lane == 0) { part[threadIdx.x / IPR_WAVE] = acc; parth[threadIdx.x / IPR_WAVE] = wh; }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                bool any = false;
+                for (int w = 0; w < IPR_BLOCK / IPR_WAVE; w++) {
+                    ipr_policy::fold(&acc, part[w]);
+                    any |= parth[w] != 0;
+                }
+                if (any) ipr_deliver(d, s, v, acc);
+            }
+            __syncthreads();
         }
     }
 }
@@ -708,6 +794,18 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_pull_step_binned(ipr_dev d, u
 /* launch table                                                              */
 /* ------------------------------------------------------------------------ */
 
+/* workgroups per in-degree bin: threads, wavefronts, workgroups per vertex */
+static void ipr_bin_blocks(const ipr_dev& d, uint32_t nb[3])
+{
+    uint64_t b0 = (d.pbin_size[0] + IPR_BLOCK - 1) / IPR_BLOCK;
+    uint64_t b1 = (d.pbin_size[1] + (IPR_BLOCK / IPR_WAVE) - 1) / (IPR_BLOCK / IPR_WAVE);
+    uint64_t b2 = d.pbin_size[2];
+    nb[0] = (uint32_t)(b0 > 8192 ? 8192 : b0);
+    nb[1] = (uint32_t)(b1 > 8192 ? 8192 : b1);
+    nb[2] = (uint32_t)(b2 > 8192 ? 8192 : b2);
+    if (nb[0] + nb[1] + nb[2] == 0) nb[0] = 1;
+}
+
 static int ipr_app_launch(int kernel, const ipr_dev* dp, void* stream)
 {
     hipStream_t st = (hipStream_t)stream;
@@ -728,19 +826,20 @@ static int ipr_app_launch(int kernel, const ipr_dev* dp, void* stream)
     case IPR_K_SWAP:
         hipLaunchKernelGGL(ipr_k_swap, dim3(vblocks), dim3(IPR_BLOCK), 0, st, d);
         break;
+    case IPR_K_GATHER: {
+        uint32_t nb[3];
+        ipr_bin_blocks(d, nb);
+        hipLaunchKernelGGL(ipr_k_gather, dim3(nb[0] + nb[1] + nb[2]), dim3(IPR_BLOCK), 0, st, d, nb[0], nb[1]);
+        break;
+    }
     case IPR_K_PULL_STEP:
         if (d.pull_ordered || IPR_COMBINER == IPR_COMBINER_GENERIC) {
             hipLaunchKernelGGL(ipr_k_pull_step_ordered, dim3(vblocks), dim3(IPR_BLOCK), 0, st, d);
         } else {
-            uint32_t nb0 = (uint32_t)((d.pbin_size[0] + IPR_BLOCK - 1) / IPR_BLOCK);
-            uint32_t nb1 = (uint32_t)((d.pbin_size[1] + (IPR_BLOCK / IPR_WAVE) - 1) / (IPR_BLOCK / IPR_WAVE));
-            uint32_t nb2 = (uint32_t)d.pbin_size[2];
-            if (nb0 > 8192) nb0 = 8192;
-            if (nb1 > 8192) nb1 = 8192;
-            if (nb2 > 8192) nb2 = 8192;
-            uint32_t total = nb0 + nb1 + nb2;
-            if (total == 0) total = 1;
-            hipLaunchKernelGGL(ipr_k_pull_step_binned, dim3(total), dim3(IPR_BLOCK), 0, st, d, nb0, nb1);
+            uint32_t nb[3];
+            ipr_bin_blocks(d, nb);
+            hipLaunchKernelGGL(ipr_k_pull_step_binned, dim3(nb[0] + nb[1] + nb[2]), dim3(IPR_BLOCK), 0, st,
+                               d, nb[0], nb[1]);
         }
         break;
     default:

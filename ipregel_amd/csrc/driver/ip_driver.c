@@ -128,6 +128,11 @@ static void dev_free_all(void)
         D.graphs[i] = NULL;
     }
     ipr_dev* d = &D.dev;
+    if (d->t_idx && d->t_idx != d->out_idx) ipr_hip_free((void*)d->t_idx);
+    if (d->t_adj && d->t_adj != d->out_adj) ipr_hip_free((void*)d->t_adj);
+    if (d->t_wgt && d->t_wgt != d->out_wgt) ipr_hip_free((void*)d->t_wgt);
+    ipr_hip_free(d->bslot);
+    ipr_hip_free(d->lbase);
     if (d->in_idx && d->in_idx != d->out_idx) ipr_hip_free((void*)d->in_idx);
     if (d->in_adj && d->in_adj != d->out_adj) ipr_hip_free((void*)d->in_adj);
     if (d->in_wgt && d->in_wgt != d->out_wgt) ipr_hip_free((void*)d->in_wgt);
@@ -227,8 +232,11 @@ static void dev_setup(void)
         if (a->variant == IPR_VARIANT_SPREAD || a->variant == IPR_VARIANT_SPREAD_PULL)
             d->front[i] = dalloc(vv * 4, "frontier");
     }
-    if (!pull)
+    if (!pull) {
         for (int b = 0; b < IPR_NBINS; b++) d->bins[b] = dalloc(vv * 4, "broadcaster bins");
+        d->bslot = dalloc(vv * a->bslot_size, "broadcast records");
+        d->lbase = dalloc(vv * 8, "large-bin chunk bases");
+    }
     d->ctl = dalloc(sizeof(ipr_ctl), "control block");
     d->trace_active = dalloc(IPR_TRACE_CAP * 8, "trace");
     d->trace_ts = dalloc((IPR_TRACE_CAP + 1) * 8, "trace");
@@ -248,8 +256,36 @@ static void dev_setup(void)
     d->pull_ordered = po == 1;
     uint32_t dopt = c && c->dir_opt ? (uint32_t)c->dir_opt : env_u32("IPR_DIR_OPT", 1);
     d->dir_opt = dopt == 1 && a->can_gather;
-    d->dir_alpha = c && c->dir_alpha ? c->dir_alpha : env_u32("IPR_DIR_ALPHA", 4);
-    if (pull) {
+    d->dir_alpha = c && c->dir_alpha ? c->dir_alpha : env_u32("IPR_DIR_ALPHA", 6);
+    d->large_chunk = env_u32("IPR_LARGE_CHUNK", 2048);
+    if (d->large_chunk < 64) d->large_chunk = 64;
+    if (!pull && d->dir_opt) {
+        /* dense supersteps gather over the exact transpose; a symmetric graph
+         * is its own transpose and keeps one copy */
+        uint64_t* t_idx = dalloc((V + 1) * 8, "transpose idx");
+        uint32_t* t_adj = dalloc(E * 4, "transpose adj");
+        uint32_t* t_wgt = d->out_wgt ? dalloc(E * 4, "transpose wgt") : NULL;
+        hip_check(ipr_hip_build_csc(V, E, d->out_idx, d->out_adj, d->out_wgt, t_idx, t_adj, t_wgt, st),
+                  "build transpose");
+        int same_idx = 0, same_adj = 0, same_w = 1;
+        hip_check(ipr_hip_equal(t_idx, d->out_idx, (V + 1) * 8, &same_idx, st), "transpose check");
+        if (same_idx) hip_check(ipr_hip_equal(t_adj, d->out_adj, E * 4, &same_adj, st), "transpose check");
+        if (same_idx && same_adj && t_wgt)
+            hip_check(ipr_hip_equal(t_wgt, d->out_wgt, E * 4, &same_w, st), "transpose check");
+        if (same_idx && same_adj && same_w) {
+            ipr_hip_free(t_idx);
+            ipr_hip_free(t_adj);
+            ipr_hip_free(t_wgt);
+            d->t_idx = d->out_idx;
+            d->t_adj = d->out_adj;
+            d->t_wgt = d->out_wgt;
+        } else {
+            d->t_idx = t_idx;
+            d->t_adj = t_adj;
+            d->t_wgt = t_wgt;
+        }
+    }
+    if (pull || d->dir_opt) {
         uint32_t ps = c && c->pull_small_max ? c->pull_small_max : env_u32("IPR_PULL_SMALL", 32);
         uint32_t pm = c && c->pull_medium_max ? c->pull_medium_max : env_u32("IPR_PULL_MEDIUM", 2048);
         for (int b = 0; b < IPR_NBINS; b++) d->pbins[b] = dalloc(vv * 4, "pull bins");
@@ -453,22 +489,13 @@ static void launch_app(int k)
     if (rc) die("HIP failure launching kernel %d (%d)\n", k, rc);
 }
 
-/* the kernel that moves the messages: expansion (push) or fused fetch (pull) */
-static int dominant_kernel(void)
+/* HIP events bracket the delivery phase of each superstep (gather and/or
+ * expansion, or the fused pull kernel) when kernel timing is on */
+static void kev_mark(int which)
 {
-    return D.app.variant == IPR_VARIANT_PULL ? IPR_K_PULL_STEP : IPR_K_EXPAND;
-}
-
-static void launch_timed(int k)
-{
-    if (D.kev_on && k == dominant_kernel() && D.kev_slot < 64) {
-        hip_check(ipr_hip_event_record(D.kev[0][D.kev_slot], D.stream), "kernel event");
-        launch_app(k);
-        hip_check(ipr_hip_event_record(D.kev[1][D.kev_slot], D.stream), "kernel event");
-        D.kev_slot++;
-    } else {
-        launch_app(k);
-    }
+    if (!D.kev_on || D.kev_slot >= 64) return;
+    hip_check(ipr_hip_event_record(D.kev[which][D.kev_slot], D.stream), "kernel event");
+    if (which == 1) D.kev_slot++;
 }
 
 /* one superstep of the variant's BSP loop, in phase order */
@@ -476,16 +503,22 @@ static void enqueue_superstep(void)
 {
     switch (D.app.variant) {
     case IPR_VARIANT_PULL:
-        launch_timed(IPR_K_PULL_STEP);
+        kev_mark(0);
+        launch_app(IPR_K_PULL_STEP);
+        kev_mark(1);
         break;
     case IPR_VARIANT_SPREAD:
-        launch_app(IPR_K_COMPUTE);
-        launch_timed(IPR_K_EXPAND);
-        break;
     case IPR_VARIANT_PUSH:
         launch_app(IPR_K_COMPUTE);
-        launch_timed(IPR_K_EXPAND);
-        launch_app(IPR_K_SWAP);
+        if (D.dev.dir_opt) hip_check(ipr_hip_direction(&D.dev, D.stream), "direction");
+        kev_mark(0);
+        if (D.dev.dir_opt) launch_app(IPR_K_GATHER);
+        launch_app(IPR_K_EXPAND);
+        kev_mark(1);
+        if (D.app.variant == IPR_VARIANT_SPREAD)
+            hip_check(ipr_hip_compact_frontier(&D.dev, D.stream), "compact");
+        else
+            launch_app(IPR_K_SWAP);
         break;
     default:
         die("Engine variant %d is not available in this build.\n", D.app.variant);

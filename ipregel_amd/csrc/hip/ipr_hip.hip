@@ -288,21 +288,40 @@ out:
 /* in-degree bins                                                            */
 /* ------------------------------------------------------------------------ */
 
+/* One wavefront per 64 consecutive vertices and one atomic per wavefront and
+ * bin, so every bin is a sequence of ascending-id runs (coalesced writes in
+ * the kernels that walk it). */
 __global__ void k_pull_bins(uint64_t V, const uint64_t* in_idx, uint32_t small_max, uint32_t medium_max,
                             uint32_t* b0, uint32_t* b1, uint32_t* b2, unsigned long long* cnt)
 {
-    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += stride) {
-        uint64_t deg = in_idx[v + 1] - in_idx[v];
-        int b = deg <= small_max ? 0 : (deg <= medium_max ? 1 : 2);
-        uint32_t* dst = b == 0 ? b0 : (b == 1 ? b1 : b2);
-        unsigned long long k = atomicAdd(&cnt[b], 1ull);
-        dst[k] = (uint32_t)v;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t Vr = (V + 63) & ~63ull;
+    const uint32_t lane = threadIdx.x & 63;
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < Vr; v += stride) {
+        int b = -1;
+        if (v < V) {
+            uint64_t deg = in_idx[v + 1] - in_idx[v];
+            b = deg <= small_max ? 0 : (deg <= medium_max ? 1 : 2);
+        }
+        for (int k = 0; k < 3; k++) {
+            uint64_t mask = __ballot(b == k);
+            if (!mask) continue;
+            uint32_t leader = __ffsll((unsigned long long)mask) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(&cnt[k], (unsigned long long)__popcll(mask));
+            base = __shfl(base, leader);
+            uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+            uint32_t* dst = k == 0 ? b0 : (k == 1 ? b1 : b2);
+            if (b == k) dst[base + below] = (uint32_t)v;
+        }
     }
 }
 
 extern "C" int ipr_hip_build_pull_bins(ipr_dev* dev, uint32_t small_max, uint32_t medium_max, void* stream)
 {
+    /* pull variants bin their in-lists, push variants their gather transpose */
+    const uint64_t* in_idx = dev->t_idx ? dev->t_idx : dev->in_idx;
     hipStream_t st = (hipStream_t)stream;
     unsigned long long* cnt = nullptr;
     unsigned long long h[3] = {0, 0, 0};
@@ -312,7 +331,7 @@ extern "C" int ipr_hip_build_pull_bins(ipr_dev* dev, uint32_t small_max, uint32_
         uint64_t blocks = (dev->V + 255) / 256;
         if (blocks < 1) blocks = 1;
         if (blocks > 65536) blocks = 65536;
-        hipLaunchKernelGGL(k_pull_bins, dim3((unsigned)blocks), dim3(256), 0, st, dev->V, dev->in_idx,
+        hipLaunchKernelGGL(k_pull_bins, dim3((unsigned)blocks), dim3(256), 0, st, dev->V, in_idx,
                            small_max, medium_max, dev->pbins[0], dev->pbins[1], dev->pbins[2], cnt);
         e = hipGetLastError();
     }
@@ -321,6 +340,91 @@ extern "C" int ipr_hip_build_pull_bins(ipr_dev* dev, uint32_t small_max, uint32_
     (void)hipFree(cnt);
     if (e != hipSuccess) return fail(e, "pull bins");
     for (int b = 0; b < 3; b++) dev->pbin_size[b] = h[b];
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* frontier compaction                                                       */
+/* ------------------------------------------------------------------------ */
+
+/* The claim bitmap of superstep s+1 becomes the frontier list: each thread
+ * owns one 32-bit word (and clears it for reuse two supersteps on), a
+ * wavefront prefix sum of the words' populations places the ids, one atomic
+ * per wavefront reserves its run. Ids ascend within each 2048-vertex run. */
+__global__ void k_compact(ipr_dev d)
+{
+    ipr_ctl* c = d.ctl;
+    if (c->done) return;
+    const uint64_t s = c->superstep;
+    uint32_t* bits = d.claim[(s + 1) & 1];
+    uint32_t* out = d.front[(s + 1) & 1];
+    unsigned long long* counter = (unsigned long long*)&c->fsize[(s + 1) & 1];
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t Wr = (d.words + 63) & ~63ull;
+    const uint32_t lane = threadIdx.x & 63;
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < Wr; w += stride) {
+        uint32_t word = 0;
+        if (w < d.words) {
+            word = bits[w];
+            if (word) bits[w] = 0;
+        }
+        uint32_t n = __popc(word);
+        uint32_t incl = n;
+        for (int o = 1; o < 64; o <<= 1) {
+            uint32_t y = __shfl_up(incl, o);
+            if (lane >= (uint32_t)o) incl += y;
+        }
+        uint32_t total = __shfl(incl, 63);
+        if (total == 0) continue;
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(counter, (unsigned long long)total);
+        base = __shfl(base, 0);
+        uint64_t pos = base + incl - n;
+        while (word) {
+            uint32_t bit = __ffs(word) - 1;
+            out[pos++] = (uint32_t)(w * 32 + bit);
+            word &= word - 1;
+        }
+    }
+}
+
+extern "C" int ipr_hip_compact_frontier(const ipr_dev* dev, void* stream)
+{
+    uint64_t blocks = (dev->words + 255) / 256;
+    if (blocks < 1) blocks = 1;
+    if (blocks > 16384) blocks = 16384;
+    hipLaunchKernelGGL(k_compact, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, *dev);
+    CK(hipGetLastError());
+    return 0;
+}
+
+__global__ void k_differ(const uint32_t* a, const uint32_t* b, uint64_t n, unsigned int* flag)
+{
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        if (a[i] != b[i]) { atomicOr(flag, 1u); return; }
+}
+
+extern "C" int ipr_hip_equal(const void* a, const void* b, uint64_t bytes, int* equal, void* stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    unsigned int* flag = nullptr;
+    unsigned int h = 0;
+    CK(hipMalloc(&flag, sizeof(unsigned int)));
+    hipError_t e = hipMemsetAsync(flag, 0, sizeof(unsigned int), st);
+    if (e == hipSuccess && bytes >= 4) {
+        uint64_t n = bytes / 4;
+        uint64_t blocks = (n + 255) / 256;
+        if (blocks > 16384) blocks = 16384;
+        hipLaunchKernelGGL(k_differ, dim3((unsigned)blocks), dim3(256), 0, st, (const uint32_t*)a,
+                           (const uint32_t*)b, n, flag);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, flag, sizeof(h), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(flag);
+    if (e != hipSuccess) return fail(e, "equality check");
+    *equal = h == 0;
     return 0;
 }
 
@@ -375,6 +479,7 @@ __global__ void k_finalize(ipr_dev d, int variant)
         c->superstep = s + 1;
     }
     for (int b = 0; b < IPR_NBINS; b++) c->bsize[b] = 0;
+    c->lpack = 0;
     c->step_edges = 0;
     c->work_edges += edges;
     if (c->pull_mode) c->gathered++;
