@@ -47,7 +47,9 @@ typedef struct ipr_harness_cfg {
     int32_t use_graph;          /* 1 replay supersteps as HIP graphs, 2 plain launches */
     int32_t dir_opt;            /* 1 on, 2 off */
     uint32_t dir_alpha;
-    uint32_t bin_small_max, bin_medium_max, grid_blocks;
+    uint32_t bin_small_max;     /* arcs per chunk of a large broadcaster */
+    uint32_t bin_medium_max;    /* out-degree above which a broadcaster is split in chunks */
+    uint32_t grid_blocks;
     uint32_t pull_small_max, pull_medium_max;
     uint32_t max_chunk;
     int32_t defer_run;          /* ip_run returns at once; runs go through ipr_harness_run */

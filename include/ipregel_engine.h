@@ -15,11 +15,10 @@
  *   bcm[2][V]     broadcast messages by superstep parity (pull variants)
  *   bslot[V]      push variants: last broadcast {superstep+1 | weighted<<31, msg}
  *   mbox[2][V]    combiner mailboxes by parity (push variants)
- *   claim[2][V/32] first-delivery bitmaps by parity (push variants); the next
- *                 frontier is compacted from it in ascending id runs
+ *   recv[2][V]    received-a-message flags (bytes) by parity (push variants);
+ *                 the next frontier is compacted from them in ascending runs
  *   front[2][V]   frontier lists by parity (spread variants)
- *   bins[3][V]    broadcasters this superstep, binned by out-degree
- *   lbase[V]      first arc chunk of each LARGE-bin broadcaster
+ *   lbins, lbase  broadcasters above large_min out-arcs and their first chunk
  */
 #ifndef IPREGEL_ENGINE_H
 #define IPREGEL_ENGINE_H
@@ -48,8 +47,22 @@ enum {
     IPR_COMBINER_SUM = 3
 };
 
-/* degree bins of the push expansion and of the pull gather */
+/* in-degree bins of the pull fetch and of the push gather */
 enum { IPR_BIN_SMALL = 0, IPR_BIN_MEDIUM = 1, IPR_BIN_LARGE = 2, IPR_NBINS = 3 };
+
+/* counters spread over IPR_CSLOTS slots (a workgroup adds its total to slot
+ * blockIdx % IPR_CSLOTS) so that no single address takes every atomic; the
+ * finalize kernel sums and clears them */
+#define IPR_CSLOTS 64
+enum {
+    IPR_C_ACT0 = 0,      /* ActiveVertexCount, superstep parity 0 */
+    IPR_C_ACT1 = 1,      /* ... parity 1 */
+    IPR_C_WORKV = 2,     /* vertices computed */
+    IPR_C_EDGES = 3,     /* out-arcs of this superstep's broadcasters (W) */
+    IPR_C_BCAST0 = 4,    /* broadcasts, superstep parity 0 */
+    IPR_C_BCAST1 = 5,    /* ... parity 1 */
+    IPR_NCNT = 6
+};
 
 /* device-resident control block: every kernel reads the superstep and the
  * halting flag here, so a chunk of supersteps can be replayed as one HIP graph
@@ -58,19 +71,17 @@ typedef struct ipr_ctl {
     uint64_t superstep;          /* current superstep */
     uint32_t done;               /* set by the finalize kernel when the loop ends */
     uint32_t error;              /* device-side API misuse flags */
-    uint64_t cnt[2];             /* ActiveVertexCount being accumulated, by parity */
-    uint64_t fsize[2];           /* frontier / target list sizes by parity */
-    uint64_t bsize[IPR_NBINS];   /* broadcaster bin sizes, current superstep */
-    uint64_t bcast[2];           /* broadcasts by superstep parity */
-    uint64_t step_edges;         /* W of this superstep */
+    uint64_t fsize[2];           /* frontier list sizes by parity */
+    uint64_t lpack;              /* large broadcasters: entries << 40 | arc chunks */
+    uint64_t pull_mode;          /* push apps: 1 when this superstep gathers */
+    uint64_t step_edges;         /* W of this superstep (set by the direction kernel) */
     uint64_t work_edges;         /* W accumulated */
     uint64_t work_vertices;      /* vertices computed, accumulated */
     uint64_t trace_cap;          /* entries in the trace arrays */
     uint64_t executed;           /* supersteps executed so far */
-    uint64_t pull_mode;          /* push apps: 1 when this superstep gathers */
     uint64_t last_active;        /* ActiveVertexCount of the last finalized superstep */
     uint64_t gathered;           /* supersteps delivered by gather */
-    uint64_t lpack;              /* LARGE bin: entries << 40 | arc chunks, one atomic */
+    uint64_t pad[4];
 } ipr_ctl;
 
 #define IPR_LPACK_SHIFT 40
@@ -78,8 +89,8 @@ typedef struct ipr_ctl {
 
 typedef struct ipr_dev {
     ipr_ctl* ctl;
+    uint64_t* cnt;               /* IPR_NCNT x IPR_CSLOTS counters */
     uint64_t V, E;
-    uint64_t words;              /* V/32 rounded up */
     const uint64_t* out_idx;
     const uint32_t* out_adj;
     const uint32_t* out_wgt;
@@ -95,13 +106,13 @@ typedef struct ipr_dev {
     void* msg;
     uint8_t* act;
     uint8_t* hm;
-    uint32_t* bcf[2];
+    uint8_t* bcf[2];
     void* bcm[2];
     void* mbox[2];
-    uint32_t* claim[2];
+    uint8_t* recv[2];
     uint32_t* front[2];
-    uint32_t* bins[IPR_NBINS];
-    uint64_t* lbase;             /* LARGE bin: first arc chunk of each entry */
+    uint32_t* lbins;             /* large broadcasters of this superstep */
+    uint64_t* lbase;             /* their first arc chunk */
     void* bslot;                 /* push variants: {superstep+1 | weighted<<31, message} */
     /* static pull / gather bins over all V (by in-degree), built at load */
     uint32_t* pbins[IPR_NBINS];
@@ -110,14 +121,12 @@ typedef struct ipr_dev {
     uint64_t* trace_ts;
     uint64_t* trace_edges;
     /* tuning */
-    uint32_t bin_small_max;      /* deg <= this: lane-cooperative */
-    uint32_t bin_medium_max;     /* deg <= this: wavefront per vertex; above: grid-wide */
+    uint32_t large_min;          /* out-degree above which a broadcaster is split into chunks */
+    uint32_t large_chunk;        /* arcs per workgroup task of a large broadcaster */
     uint32_t grid_blocks;        /* persistent-style grids */
-    uint32_t block_threads;
     uint32_t pull_ordered;       /* 1: fold in in-list order (bit-exact) */
     uint32_t dir_opt;            /* push apps: allow gathering dense supersteps */
     uint32_t dir_alpha;          /* gather when bcast out-arcs * alpha > E */
-    uint32_t large_chunk;        /* arcs per workgroup task in the LARGE bin */
 } ipr_dev;
 
 /* kernels the application TU provides (typed by the app's message/value) */

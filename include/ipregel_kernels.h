@@ -3,28 +3,32 @@
  * Included by iPregel.h, i.e. instantiated once, in the application's
  * translation unit, with the application's message and value types.
  *
- * gfx950: 64-lane wavefronts, 256-thread workgroups (4 waves).
+ * gfx950: 64-lane wavefronts, 256-thread workgroups (4 wavefronts).
  *
- * Push variants (PUSH, SPREAD):
+ * Push variants (PUSH, SPREAD), per superstep s:
  *   compute  every vertex of the work set (all V at superstep 0 and in PUSH,
  *            the frontier list in SPREAD) runs ip_compute; a broadcast is
- *            recorded (message in bcm[0], vertex appended to one of three
- *            out-degree bins with one atomic per wavefront).
- *   expand   the recorded broadcasts are delivered: bin SMALL by
- *            lane-cooperative expansion (64 vertices per wavefront, edges
- *            spread over the lanes by a wavefront prefix sum), bin MEDIUM one
- *            wavefront per vertex, bin LARGE the whole grid per vertex.
- *            A delivery combines into mbox[(s+1)&1] with a hardware atomic
- *            (declared combiner) or a CAS loop (ip_combine); the first
- *            delivery to a vertex sets its claim bit and, in SPREAD, appends
- *            it to the next frontier (src/combiner_spread_postamble.h:202-227).
- *   gather   dense supersteps of push applications whose combiner makes the
- *            result order-independent: each vertex folds the broadcasts of
- *            its in-neighbours (CSC) without atomics; same messages, same
- *            frontier.
+ *            recorded in bslot[v] = {s+1, message}; broadcasters with more
+ *            than large_min out-arcs are also listed, with their first arc
+ *            chunk, in lbins/lbase (one atomic per workgroup).
+ *   gather   (dense supersteps: broadcasters' out-arcs > E/dir_alpha) every
+ *            vertex folds the records of its in-neighbours over the exact
+ *            transpose: same combined message and same receivers as the push,
+ *            one writer per mailbox, no atomics.
+ *   expand   (sparse supersteps) large broadcasters one workgroup per chunk
+ *            of large_chunk arcs; the rest of the work set by a workgroup
+ *            prefix sum over the out-degrees of 256 vertices, so the
+ *            workgroup's threads take consecutive arcs (load balance for the
+ *            skewed R-MAT degrees). A delivery combines into mbox[(s+1)&1]
+ *            with a hardware atomic (declared combiner) or a CAS loop around
+ *            ip_combine and raises recv[(s+1)&1][u]
+ *            (src/combiner_spread_postamble.h:202-227).
+ *   The next frontier is compacted from recv (libipregel_hip.so) in
+ *   ascending runs; PUSH's swap turns recv into has_message instead.
  * Pull variant (PULL): one kernel per superstep fetches superstep s-1's
  *   broadcasts over the in-lists and runs ip_compute for superstep s on the
  *   same vertex (broadcast slots double-buffered by superstep parity).
+ * Counters are reduced per workgroup and added to one of IPR_CSLOTS slots.
  */
 #ifndef IPREGEL_AMD_KERNELS_H
 #define IPREGEL_AMD_KERNELS_H
@@ -36,6 +40,7 @@
 
 #define IPR_BLOCK 256
 #define IPR_WAVE 64
+#define IPR_WAVES (IPR_BLOCK / IPR_WAVE)
 
 /* device copy of the engine state, read by the device-side API functions */
 __device__ ipr_dev ipr_d_dev;
@@ -48,38 +53,15 @@ __device__ inline void ipr_raise_error(uint32_t code)
 }
 
 /* ------------------------------------------------------------------------ */
-/* wavefront helpers                                                         */
+/* wavefront and workgroup helpers                                           */
 /* ------------------------------------------------------------------------ */
 
 __device__ inline uint32_t ipr_lane(void) { return __lane_id(); }
 
-__device__ inline uint32_t ipr_lanes_below(uint64_t mask)
-{
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
-                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-}
-
-/* Append `item` to list[counter++] for every lane with pred; one atomic per
- * wavefront. Must be reached by all active lanes. */
-__device__ inline void ipr_wave_append(unsigned long long* counter, uint32_t* list,
-                                       bool pred, uint32_t item)
-{
-    uint64_t mask = __ballot(pred);
-    if (mask == 0) return;
-    uint32_t leader = __ffsll((unsigned long long)mask) - 1;
-    unsigned long long base = 0;
-    if (ipr_lane() == leader) base = atomicAdd(counter, (unsigned long long)__popcll(mask));
-    base = __shfl(base, leader);
-    if (pred) list[base + ipr_lanes_below(mask)] = item;
-}
-
-/* wavefront sum of a 64-bit counter, added once to *dst */
-__device__ inline void ipr_wave_add(unsigned long long* dst, uint64_t x)
+__device__ inline uint64_t ipr_wave_sum(uint64_t x)
 {
     for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
-    uint64_t mask = __ballot(1);
-    if (ipr_lane() == (uint32_t)(__ffsll((unsigned long long)mask) - 1) && x)
-        atomicAdd(dst, (unsigned long long)x);
+    return x;
 }
 
 __device__ inline uint32_t ipr_wave_incl_scan(uint32_t x)
@@ -90,6 +72,49 @@ __device__ inline uint32_t ipr_wave_incl_scan(uint32_t x)
         if (lane >= (uint32_t)o) x += y;
     }
     return x;
+}
+
+/* workgroup exclusive scan of x (all threads); *total gets the sum.
+ * `scratch` holds IPR_WAVES entries of LDS. Ends with a barrier. */
+__device__ inline uint32_t ipr_block_excl_scan(uint32_t x, uint32_t* scratch, uint32_t* total)
+{
+    const uint32_t lane = ipr_lane(), wave = threadIdx.x / IPR_WAVE;
+    uint32_t incl = ipr_wave_incl_scan(x);
+    if (lane == IPR_WAVE - 1) scratch[wave] = incl;
+    __syncthreads();
+    uint32_t before = 0, sum = 0;
+    for (uint32_t w = 0; w < IPR_WAVES; w++) {
+        uint32_t t = scratch[w];
+        if (w < wave) before += t;
+        sum += t;
+    }
+    __syncthreads();
+    *total = sum;
+    return before + incl - x;
+}
+
+/* add per-thread counts a, b, c to counters ia, ib, ic: one atomic per
+ * workgroup and counter, spread over IPR_CSLOTS slots. Every thread of the
+ * workgroup must call it. */
+__device__ inline void ipr_block_count3(const ipr_dev& d, int ia, uint64_t a, int ib, uint64_t b,
+                                        int ic, uint64_t c)
+{
+    __shared__ uint64_t red[3][IPR_WAVES];
+    const uint32_t lane = ipr_lane(), wave = threadIdx.x / IPR_WAVE;
+    a = ipr_wave_sum(a);
+    b = ipr_wave_sum(b);
+    c = ipr_wave_sum(c);
+    if (lane == 0) { red[0][wave] = a; red[1][wave] = b; red[2][wave] = c; }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        uint64_t t = 0;
+        for (int w = 0; w < IPR_WAVES; w++) t += red[threadIdx.x][w];
+        int id = threadIdx.x == 0 ? ia : (threadIdx.x == 1 ? ib : ic);
+        if (t && id >= 0)
+            atomicAdd((unsigned long long*)&d.cnt[id * IPR_CSLOTS + (blockIdx.x % IPR_CSLOTS)],
+                      (unsigned long long)t);
+    }
+    __syncthreads();
 }
 
 template <typename T> __host__ __device__ inline bool ipr_same_bits(const T& a, const T& b)
@@ -158,6 +183,8 @@ template <typename T> struct ipr_comb<T, IPR_COMBINER_MIN> {
         if (m < cur) return ipr_atomic_min(p, m) == ident();
         return cur == ident();            /* m == identity on an empty slot */
     }
+    /* single writer (gather): no atomics */
+    __device__ static void merge(slot_t* p, T m) { if (m < *p) *p = m; }
 };
 
 template <typename T> struct ipr_comb<T, IPR_COMBINER_MAX> {
@@ -173,6 +200,7 @@ template <typename T> struct ipr_comb<T, IPR_COMBINER_MAX> {
         if (m > cur) return ipr_atomic_max(p, m) == ident();
         return cur == ident();
     }
+    __device__ static void merge(slot_t* p, T m) { if (m > *p) *p = m; }
 };
 
 /* SUM: zero-initialised slot, every delivery adds; firstness from the claim
@@ -189,6 +217,7 @@ template <typename T> struct ipr_comb<T, IPR_COMBINER_SUM> {
         atomicAdd(p, m);
         return true;
     }
+    __device__ static void merge(slot_t* p, T m) { *p += m; }
 };
 
 /* GENERIC: 32-bit message packed with a presence flag in one 64-bit slot,
@@ -220,36 +249,41 @@ template <typename T> struct ipr_comb<T, IPR_COMBINER_GENERIC> {
             old = prev;
         }
     }
+    __device__ static void merge(slot_t* p, T m)
+    {
+        T a = m;
+        if (*p >> 32) { a = value(*p); ip_combine(&a, m); }
+        uint32_t bits;
+        memcpy(&bits, &a, 4);
+        *p = (1ull << 32) | bits;
+    }
 };
 
 typedef ipr_comb<IP_MESSAGE_TYPE, IPR_COMBINER> ipr_policy;
 typedef ipr_policy::slot_t ipr_slot_t;
 
 /* broadcast record of a push-variant vertex: the superstep it broadcast in
- * (plus one, bit 31 set for ip_broadcast_weighted) and its message; one load
- * gives a gathering receiver both */
+ * (plus one; bit 31 set for ip_broadcast_weighted) and its message; a
+ * gathering receiver gets both with one load */
 struct ipr_bslot_t {
     uint32_t step;
     IP_MESSAGE_TYPE msg;
 };
 #define IPR_BSTEP_WEIGHTED 0x80000000u
 
-/* set the claim bit of u; true for the one caller that set it */
-__device__ inline bool ipr_claim(uint32_t* bits, uint32_t u)
+/* raise the received flag of u by atomic OR on its 32-bit word (plain byte
+ * read first: most deliveries find it already raised) */
+__device__ inline void ipr_mark_recv(uint8_t* recv, uint32_t u)
 {
-    uint32_t bit = 1u << (u & 31);
-    uint32_t* w = bits + (u >> 5);
-    if (__atomic_load_n(w, __ATOMIC_RELAXED) & bit) return false;
-    return (atomicOr(w, bit) & bit) == 0;
+    if (__atomic_load_n(recv + u, __ATOMIC_RELAXED)) return;
+    atomicOr((uint32_t*)(recv + (u & ~3u)), 1u << (8 * (u & 3u)));
 }
 
-/* deliver one message to u in superstep s: combine into the mailbox of
- * superstep s+1 and mark u as having received (the claim bitmap, from which
- * the next frontier is compacted and PUSH's swap reads has_message_next) */
+/* deliver one message to u in superstep s (push: concurrent senders) */
 __device__ inline void ipr_deliver(const ipr_dev& d, uint64_t s, uint32_t u, IP_MESSAGE_TYPE m)
 {
     ipr_slot_t* mb = (ipr_slot_t*)d.mbox[(s + 1) & 1];
-    if (ipr_policy::deliver(mb + u, m)) ipr_claim(d.claim[(s + 1) & 1], u);
+    if (ipr_policy::deliver(mb + u, m)) ipr_mark_recv(d.recv[(s + 1) & 1], u);
 }
 
 /* ip_send_message in push variants: immediate delivery */
@@ -259,8 +293,18 @@ __device__ inline void ipr_send_immediate(IP_VERTEX_ID_TYPE id, IP_MESSAGE_TYPE 
     ipr_deliver(d, d.ctl->superstep, (uint32_t)id, m);
 }
 
+__device__ inline IP_MESSAGE_TYPE ipr_edge_msg(const uint32_t* wgt, IP_MESSAGE_TYPE m, bool weighted, uint64_t e)
+{
+#ifdef IP_NEEDS_OUT_NEIGHBOUR_WEIGHTS
+    if (weighted) return (IP_MESSAGE_TYPE)(m + (IP_MESSAGE_TYPE)wgt[e]);
+#else
+    (void)wgt; (void)weighted; (void)e;
+#endif
+    return m;
+}
+
 /* ------------------------------------------------------------------------ */
-/* vertex proxy fill                                                         */
+/* vertex proxy                                                              */
 /* ------------------------------------------------------------------------ */
 
 __device__ inline void ipr_fill_vertex(const ipr_dev& d, struct ip_vertex_t* x, uint32_t v)
@@ -304,20 +348,16 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_init(ipr_dev d)
         d.act[v] = 1;  /* ip_init_vertex_range, e.g. src/combiner_postamble.h:189 */
         d.hm[v] = 0;
 #if IPR_IS_PULL
-        ((uint8_t*)d.bcf[0])[v] = 0;
-        ((uint8_t*)d.bcf[1])[v] = 0;
+        d.bcf[0][v] = 0;
+        d.bcf[1][v] = 0;
 #else
         ((ipr_slot_t*)d.mbox[0])[v] = ipr_policy::empty();
         ((ipr_slot_t*)d.mbox[1])[v] = ipr_policy::empty();
         ((ipr_bslot_t*)d.bslot)[v].step = 0;
+        d.recv[0][v] = 0;
+        d.recv[1][v] = 0;
 #endif
     }
-#if !IPR_IS_PULL
-    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < d.words; w += stride) {
-        d.claim[0][w] = 0;
-        d.claim[1][w] = 0;
-    }
-#endif
 }
 
 /* ------------------------------------------------------------------------ */
@@ -326,12 +366,13 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_init(ipr_dev d)
 
 /* SPREAD: src/combiner_spread_postamble.h:364-401 (superstep 0 over all V,
  * then the frontier) and :459-471 (the frontier's mailboxes become messages:
- * here read straight from the mailbox, which is reset for reuse two
- * supersteps on). PUSH: src/combiner_postamble.h:246-259 (every vertex with
- * active or has_message). A broadcast is recorded in bslot and the vertex is
- * appended to its out-degree bin. */
+ * read straight from the mailbox, which is reset for reuse two supersteps
+ * on). PUSH: src/combiner_postamble.h:246-259 (every vertex with active or
+ * has_message). */
 __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_compute_push(ipr_dev d)
 {
+    __shared__ uint32_t scan_a[IPR_WAVES], scan_b[IPR_WAVES];
+    __shared__ unsigned long long lp_base;
     ipr_ctl* ctl = d.ctl;
     if (ctl->done) return;
     const uint64_t s = ctl->superstep;
@@ -344,102 +385,97 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_compute_push(ipr_dev d)
     ipr_slot_t* mcur = (ipr_slot_t*)d.mbox[s & 1];
     ipr_bslot_t* bs = (ipr_bslot_t*)d.bslot;
     uint64_t n_active = 0, n_computed = 0, n_edges = 0;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        uint32_t v = list ? list[i] : (uint32_t)i;
-        struct ip_vertex_t x;
-        bool run = true;
-        if (IPR_VARIANT == IPR_VARIANT_SPREAD) {
-            x.active = true;
-            x.has_message = s > 0;
-            if (s > 0) {
-                x.message = ipr_policy::value(mcur[v]);
-                mcur[v] = ipr_policy::empty();
+    const uint64_t tile_stride = (uint64_t)gridDim.x * IPR_BLOCK;
+    for (uint64_t base = (uint64_t)blockIdx.x * IPR_BLOCK; base < n; base += tile_stride) {
+        const uint64_t i = base + threadIdx.x;
+        uint32_t large = 0, chunks = 0, entry = 0;
+        if (i < n) {
+            uint32_t v = list ? list[i] : (uint32_t)i;
+            struct ip_vertex_t x;
+            bool run = true;
+            if (IPR_VARIANT == IPR_VARIANT_SPREAD) {
+                x.active = true;
+                x.has_message = s > 0;
+                if (s > 0) {
+                    x.message = ipr_policy::value(mcur[v]);
+                    mcur[v] = ipr_policy::empty();
+                }
+            } else {
+                x.active = d.act[v] != 0;
+                x.has_message = d.hm[v] != 0;
+                if (x.has_message) x.message = ((const IP_MESSAGE_TYPE*)d.msg)[v];
+                run = x.active || x.has_message;
+                x.active = true;
             }
-        } else {
-            x.active = d.act[v] != 0;
-            x.has_message = d.hm[v] != 0;
-            if (x.has_message) x.message = ((const IP_MESSAGE_TYPE*)d.msg)[v];
-            run = x.active || x.has_message;
-            x.active = true;
-        }
-        bool small = false, medium = false;
-        uint32_t entry = 0;
-        if (run) {
-            ipr_fill_vertex(d, &x, v);
-            const IP_VALUE_TYPE before = x.value;
-            const bool hm_before = x.has_message;
-            ip_compute(&x);
-            ipr_store_value(d, v, before, x.value);
-            n_computed++;
-            if (IPR_VARIANT == IPR_VARIANT_PUSH) {
-                d.act[v] = x.active;
-                if (hm_before != x.has_message) d.hm[v] = x.has_message;
-                n_active += x.active;
-            }
-            if (x.ipr_bcast) {
-                uint64_t deg = d.out_idx[v + 1] - d.out_idx[v];
-                n_edges += deg;
-                if (deg > 0) {
-                    ipr_bslot_t rec;
-                    rec.step = (uint32_t)(s + 1) | (x.ipr_bcast_weighted ? IPR_BSTEP_WEIGHTED : 0u);
-                    rec.msg = x.ipr_bcast_msg;
-                    bs[v] = rec;
-                    entry = v | (x.ipr_bcast_weighted ? 0x80000000u : 0u);
-                    if (deg <= d.bin_small_max) {
-                        small = true;
-                    } else if (deg <= d.bin_medium_max) {
-                        medium = true;
-                    } else {
-                        /* one atomic hands out both the list slot and the
-                         * vertex's first arc chunk, so lbase ascends with the
-                         * slot and a chunk finds its vertex by binary search */
-                        uint64_t chunks = (deg + d.large_chunk - 1) / d.large_chunk;
-                        unsigned long long old = atomicAdd((unsigned long long*)&ctl->lpack,
-                                                           (1ull << IPR_LPACK_SHIFT) | chunks);
-                        uint64_t slot = old >> IPR_LPACK_SHIFT;
-                        d.lbase[slot] = old & IPR_LPACK_MASK;
-                        d.bins[IPR_BIN_LARGE][slot] = entry;
+            if (run) {
+                ipr_fill_vertex(d, &x, v);
+                const IP_VALUE_TYPE before = x.value;
+                const bool hm_before = x.has_message;
+                ip_compute(&x);
+                ipr_store_value(d, v, before, x.value);
+                n_computed++;
+                if (IPR_VARIANT == IPR_VARIANT_PUSH) {
+                    if (!x.active) d.act[v] = 0;
+                    else if (!d.act[v]) d.act[v] = 1;
+                    if (hm_before != x.has_message) d.hm[v] = x.has_message;
+                    n_active += x.active;
+                }
+                if (x.ipr_bcast) {
+                    uint64_t deg = d.out_idx[v + 1] - d.out_idx[v];
+                    n_edges += deg;
+                    if (deg > 0) {
+                        ipr_bslot_t rec;
+                        rec.step = (uint32_t)(s + 1) | (x.ipr_bcast_weighted ? IPR_BSTEP_WEIGHTED : 0u);
+                        rec.msg = x.ipr_bcast_msg;
+                        bs[v] = rec;
+                        if (deg > d.large_min) {
+                            large = 1;
+                            chunks = (uint32_t)((deg + d.large_chunk - 1) / d.large_chunk);
+                            entry = v;
+                        }
                     }
                 }
             }
         }
-        ipr_wave_append((unsigned long long*)&ctl->bsize[IPR_BIN_SMALL], d.bins[IPR_BIN_SMALL], small, entry);
-        ipr_wave_append((unsigned long long*)&ctl->bsize[IPR_BIN_MEDIUM], d.bins[IPR_BIN_MEDIUM], medium, entry);
+        /* large broadcasters of this tile take consecutive list slots and
+         * consecutive chunk ranges with one atomic */
+        if (__syncthreads_or(large)) {
+            uint32_t tl = 0, tc = 0;
+            uint32_t el = ipr_block_excl_scan(large, scan_a, &tl);
+            uint32_t ec = ipr_block_excl_scan(chunks, scan_b, &tc);
+            if (threadIdx.x == 0)
+                lp_base = atomicAdd((unsigned long long*)&ctl->lpack,
+                                    ((unsigned long long)tl << IPR_LPACK_SHIFT) | tc);
+            __syncthreads();
+            if (large) {
+                uint64_t slot = (lp_base >> IPR_LPACK_SHIFT) + el;
+                d.lbins[slot] = entry;
+                d.lbase[slot] = (lp_base & IPR_LPACK_MASK) + ec;
+            }
+        }
     }
-    ipr_wave_add((unsigned long long*)&ctl->cnt[s & 1], n_active);
-    ipr_wave_add((unsigned long long*)&ctl->work_vertices, n_computed);
-    ipr_wave_add((unsigned long long*)&ctl->step_edges, n_edges);
+    ipr_block_count3(d, IPR_C_ACT0 + (int)(s & 1), n_active, IPR_C_WORKV, n_computed, IPR_C_EDGES, n_edges);
 }
 
 /* ------------------------------------------------------------------------ */
 /* push variants: expansion (sparse supersteps)                              */
 /* ------------------------------------------------------------------------ */
 
-__device__ inline IP_MESSAGE_TYPE ipr_edge_msg(const uint32_t* wgt, IP_MESSAGE_TYPE m, bool weighted, uint64_t e)
-{
-#ifdef IP_NEEDS_OUT_NEIGHBOUR_WEIGHTS
-    if (weighted) return (IP_MESSAGE_TYPE)(m + (IP_MESSAGE_TYPE)wgt[e]);
-#else
-    (void)wgt; (void)weighted; (void)e;
-#endif
-    return m;
-}
-
 __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_expand(ipr_dev d)
 {
+    __shared__ uint32_t s_ex[IPR_BLOCK];
+    __shared__ uint64_t s_off[IPR_BLOCK];
+    __shared__ IP_MESSAGE_TYPE s_msg[IPR_BLOCK];
+    __shared__ uint8_t s_wf[IPR_BLOCK];
+    __shared__ uint32_t scan[IPR_WAVES];
     ipr_ctl* ctl = d.ctl;
     if (ctl->done) return;
     if (ctl->pull_mode) return;
     const uint64_t s = ctl->superstep;
     const ipr_bslot_t* bs = (const ipr_bslot_t*)d.bslot;
-    const uint32_t lane = ipr_lane();
-    const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t gthreads = (uint64_t)gridDim.x * blockDim.x;
-    const uint64_t gwave = gtid / IPR_WAVE;
-    const uint64_t nwaves = gthreads / IPR_WAVE;
+    const uint32_t want = (uint32_t)(s + 1);
 
-    /* LARGE: one workgroup per chunk of large_chunk arcs */
+    /* large broadcasters: one workgroup per chunk of large_chunk arcs */
     const uint64_t lp = ctl->lpack;
     const uint64_t nL = lp >> IPR_LPACK_SHIFT, nC = lp & IPR_LPACK_MASK;
     for (uint64_t c = blockIdx.x; c < nC; c += gridDim.x) {
@@ -448,66 +484,57 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_expand(ipr_dev d)
             uint64_t mid = (lo + hi) / 2;
             if (d.lbase[mid] <= c) lo = mid; else hi = mid;
         }
-        uint32_t e0 = d.bins[IPR_BIN_LARGE][lo];
-        uint32_t v = e0 & 0x7fffffffu;
-        bool wf = (e0 >> 31) != 0;
+        uint32_t v = d.lbins[lo];
+        ipr_bslot_t b = bs[v];
+        bool wf = (b.step & IPR_BSTEP_WEIGHTED) != 0;
         uint64_t off = d.out_idx[v], deg = d.out_idx[v + 1] - off;
         uint64_t beg = (c - d.lbase[lo]) * d.large_chunk;
         uint64_t end = beg + d.large_chunk < deg ? beg + d.large_chunk : deg;
-        IP_MESSAGE_TYPE m = bs[v].msg;
-        for (uint64_t k = beg + threadIdx.x; k < end; k += blockDim.x)
-            ipr_deliver(d, s, d.out_adj[off + k], ipr_edge_msg(d.out_wgt, m, wf, off + k));
+        for (uint64_t k = beg + threadIdx.x; k < end; k += IPR_BLOCK)
+            ipr_deliver(d, s, d.out_adj[off + k], ipr_edge_msg(d.out_wgt, b.msg, wf, off + k));
     }
 
-    /* MEDIUM: one wavefront per vertex */
-    const uint64_t nM = ctl->bsize[IPR_BIN_MEDIUM];
-    for (uint64_t w = gwave; w < nM; w += nwaves) {
-        uint32_t e0 = d.bins[IPR_BIN_MEDIUM][w];
-        uint32_t v = e0 & 0x7fffffffu;
-        bool wf = (e0 >> 31) != 0;
-        uint64_t off = d.out_idx[v], deg = d.out_idx[v + 1] - off;
-        IP_MESSAGE_TYPE m = bs[v].msg;
-        for (uint64_t k = lane; k < deg; k += IPR_WAVE)
-            ipr_deliver(d, s, d.out_adj[off + k], ipr_edge_msg(d.out_wgt, m, wf, off + k));
+    /* the rest of the work set: 256 vertices per workgroup, their arcs laid
+     * end to end by a prefix sum and taken 256 at a time */
+    uint64_t n = d.V;
+    const uint32_t* list = nullptr;
+    if (IPR_VARIANT == IPR_VARIANT_SPREAD && s > 0) {
+        n = ctl->fsize[s & 1];
+        list = d.front[s & 1];
     }
-
-    /* SMALL: 64 vertices per wavefront, their arcs spread over the lanes by a
-     * wavefront prefix sum of the degrees */
-    const uint64_t nS = ctl->bsize[IPR_BIN_SMALL];
-    for (uint64_t base = gwave * IPR_WAVE; base < nS; base += nwaves * IPR_WAVE) {
-        uint64_t i = base + lane;
+    const uint64_t tile_stride = (uint64_t)gridDim.x * IPR_BLOCK;
+    for (uint64_t base = (uint64_t)blockIdx.x * IPR_BLOCK; base < n; base += tile_stride) {
+        const uint64_t i = base + threadIdx.x;
         uint32_t deg = 0;
-        uint64_t off = 0;
-        bool wf = false;
-        IP_MESSAGE_TYPE m = IP_MESSAGE_TYPE();
-        if (i < nS) {
-            uint32_t e0 = d.bins[IPR_BIN_SMALL][i];
-            uint32_t v = e0 & 0x7fffffffu;
-            wf = (e0 >> 31) != 0;
-            off = d.out_idx[v];
-            deg = (uint32_t)(d.out_idx[v + 1] - off);
-            m = bs[v].msg;
-        }
-        uint32_t incl = ipr_wave_incl_scan(deg);
-        uint32_t excl = incl - deg;
-        uint32_t total = __shfl(incl, IPR_WAVE - 1);
-        for (uint32_t t0 = 0; t0 < total; t0 += IPR_WAVE) {
-            uint32_t t = t0 + lane;
-            /* owner: the last lane whose exclusive start is <= t */
-            uint32_t lo = 0;
-            for (uint32_t step = 32; step > 0; step >>= 1) {
-                uint32_t probe = __shfl(excl, (lo + step) & (IPR_WAVE - 1));
-                if (lo + step < IPR_WAVE && probe <= t) lo += step;
+        if (i < n) {
+            uint32_t v = list ? list[i] : (uint32_t)i;
+            ipr_bslot_t b = bs[v];
+            if ((b.step & ~IPR_BSTEP_WEIGHTED) == want) {
+                uint64_t off = d.out_idx[v];
+                uint64_t dg = d.out_idx[v + 1] - off;
+                if (dg <= d.large_min) {
+                    deg = (uint32_t)dg;
+                    s_off[threadIdx.x] = off;
+                    s_msg[threadIdx.x] = b.msg;
+                    s_wf[threadIdx.x] = (b.step & IPR_BSTEP_WEIGHTED) != 0;
+                }
             }
-            uint64_t o_off = __shfl(off, lo);
-            uint32_t o_ex = __shfl(excl, lo);
-            IP_MESSAGE_TYPE o_m = __shfl(m, lo);
-            bool o_wf = __shfl((int)wf, lo) != 0;
+        }
+        uint32_t total = 0;
+        uint32_t ex = ipr_block_excl_scan(deg, scan, &total);
+        s_ex[threadIdx.x] = ex;
+        __syncthreads();
+        for (uint32_t t0 = 0; t0 < total; t0 += IPR_BLOCK) {
+            uint32_t t = t0 + threadIdx.x;
             if (t < total) {
-                uint64_t e = o_off + (t - o_ex);
-                ipr_deliver(d, s, d.out_adj[e], ipr_edge_msg(d.out_wgt, o_m, o_wf, e));
+                uint32_t lo = 0;                 /* last owner with start <= t */
+                for (uint32_t step = IPR_BLOCK / 2; step > 0; step >>= 1)
+                    if (s_ex[lo + step] <= t) lo += step;
+                uint64_t e = s_off[lo] + (t - s_ex[lo]);
+                ipr_deliver(d, s, d.out_adj[e], ipr_edge_msg(d.out_wgt, s_msg[lo], s_wf[lo] != 0, e));
             }
         }
+        __syncthreads();
     }
 }
 
@@ -515,8 +542,8 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_expand(ipr_dev d)
 /* push variants: gather (dense supersteps)                                  */
 /* ------------------------------------------------------------------------ */
 
-/* fold of the messages broadcast in superstep s by the in-neighbours
- * [beg, end) of one vertex, strided by `step` from `first` */
+/* fold of the messages broadcast in superstep s by the in-neighbours in
+ * transpose positions first, first+step, ... < end */
 __device__ inline bool ipr_gather_range(const ipr_dev& d, uint64_t s, uint64_t first, uint64_t end,
                                         uint64_t step, IP_MESSAGE_TYPE* acc)
 {
@@ -537,13 +564,20 @@ __device__ inline bool ipr_gather_range(const ipr_dev& d, uint64_t s, uint64_t f
     return has;
 }
 
-/* Receivers fold their broadcasting in-neighbours over the CSC: the same
- * combined message and the same set of receivers as the push, without
- * atomics on the arcs (exact for integer min / max / sum combiners).
- * Vertices binned by in-degree; blocks [0, nb0) one thread per vertex,
+/* the gathering vertex is the only writer of its mailbox and flag (messages
+ * from ip_send_message were delivered before, in the compute kernel) */
+__device__ inline void ipr_gather_store(const ipr_dev& d, uint64_t s, uint32_t v, IP_MESSAGE_TYPE acc)
+{
+    ipr_policy::merge((ipr_slot_t*)d.mbox[(s + 1) & 1] + v, acc);
+    d.recv[(s + 1) & 1][v] = 1;
+}
+
+/* Vertices binned by in-degree; blocks [0, nb0) one thread per vertex,
  * [nb0, nb0+nb1) one wavefront per vertex, the rest one workgroup. */
 __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_gather(ipr_dev d, uint32_t nb0, uint32_t nb1)
 {
+    __shared__ IP_MESSAGE_TYPE part[IPR_WAVES];
+    __shared__ int parth[IPR_WAVES];
     ipr_ctl* ctl = d.ctl;
     if (ctl->done) return;
     if (!ctl->pull_mode) return;
@@ -555,12 +589,12 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_gather(ipr_dev d, uint32_t nb
         for (uint64_t i = (uint64_t)b * blockDim.x + threadIdx.x; i < n; i += (uint64_t)nb0 * blockDim.x) {
             uint32_t v = d.pbins[0][i];
             IP_MESSAGE_TYPE acc = ipr_policy::ident();
-            if (ipr_gather_range(d, s, d.t_idx[v], d.t_idx[v + 1], 1, &acc)) ipr_deliver(d, s, v, acc);
+            if (ipr_gather_range(d, s, d.t_idx[v], d.t_idx[v + 1], 1, &acc)) ipr_gather_store(d, s, v, acc);
         }
     } else if (b < nb0 + nb1) {
         const uint64_t n = d.pbin_size[1];
-        const uint64_t wave = (uint64_t)(b - nb0) * (IPR_BLOCK / IPR_WAVE) + threadIdx.x / IPR_WAVE;
-        const uint64_t nwaves = (uint64_t)nb1 * (IPR_BLOCK / IPR_WAVE);
+        const uint64_t wave = (uint64_t)(b - nb0) * IPR_WAVES + threadIdx.x / IPR_WAVE;
+        const uint64_t nwaves = (uint64_t)nb1 * IPR_WAVES;
         for (uint64_t i = wave; i < n; i += nwaves) {
             uint32_t v = d.pbins[1][i];
             IP_MESSAGE_TYPE acc = ipr_policy::ident();
@@ -570,11 +604,9 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_gather(ipr_dev d, uint32_t nb
                 ipr_policy::fold(&acc, y);
             }
             has = __ballot(has) != 0;
-            if (lane == 0 && has) ipr_deliver(d, s, v, acc);
+            if (lane == 0 && has) ipr_gather_store(d, s, v, acc);
         }
     } else {
-        __shared__ IP_MESSAGE_TYPE part[IPR_BLOCK / IPR_WAVE];
-        __shared__ int parth[IPR_BLOCK / IPR_WAVE];
         const uint64_t n = d.pbin_size[2];
         const uint32_t nb2 = gridDim.x - nb0 - nb1;
         for (uint64_t i = b - nb0 - nb1; i < n; i += nb2) {
@@ -590,11 +622,11 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_gather(ipr_dev d, uint32_t nb
             __syncthreads();
             if (threadIdx.x == 0) {
                 bool any = false;
-                for (int w = 0; w < IPR_BLOCK / IPR_WAVE; w++) {
+                for (int w = 0; w < IPR_WAVES; w++) {
                     ipr_policy::fold(&acc, part[w]);
                     any |= parth[w] != 0;
                 }
-                if (any) ipr_deliver(d, s, v, acc);
+                if (any) ipr_gather_store(d, s, v, acc);
             }
             __syncthreads();
         }
@@ -609,20 +641,19 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_swap(ipr_dev d)
     if (ctl->done) return;
     const uint64_t s = ctl->superstep;
     ipr_slot_t* mb = (ipr_slot_t*)d.mbox[(s + 1) & 1];
-    uint32_t* cl = d.claim[(s + 1) & 1];
+    uint8_t* rv = d.recv[(s + 1) & 1];
     uint64_t n_react = 0;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < d.V; v += stride) {
-        const uint32_t bit = 1u << (v & 31);
-        if (cl[v >> 5] & bit) {
+        if (rv[v]) {
+            rv[v] = 0;   /* clean for superstep s+2 */
             ((IP_MESSAGE_TYPE*)d.msg)[v] = ipr_policy::value(mb[v]);
             mb[v] = ipr_policy::empty();
-            atomicAnd(cl + (v >> 5), ~bit);   /* clean for superstep s+2 */
             d.hm[v] = 1;
             if (!d.act[v]) { d.act[v] = 1; n_react++; }
         }
     }
-    ipr_wave_add((unsigned long long*)&ctl->cnt[s & 1], n_react);
+    ipr_block_count3(d, IPR_C_ACT0 + (int)(s & 1), n_react, -1, 0, -1, 0);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -652,12 +683,12 @@ __device__ inline bool ipr_fetch_ordered(const ipr_dev& d, uint32_t v, const uin
 /* one vertex: fetch(s-1) result -> activation -> compute(s) */
 __device__ inline void ipr_pull_vertex(const ipr_dev& d, uint64_t s, uint32_t v, bool hm,
                                        IP_MESSAGE_TYPE msg, uint64_t* n_react, uint64_t* n_active,
-                                       uint64_t* n_computed, uint64_t* n_bcast)
+                                       uint64_t* n_bcast)
 {
     bool act = d.act[v] != 0;
     const bool act0 = act;
     if (hm && !act) { act = true; (*n_react)++; }
-    uint8_t* bf_cur = (uint8_t*)d.bcf[s & 1];
+    uint8_t* bf_cur = d.bcf[s & 1];
     bf_cur[v] = 0;
     if (act) {
         struct ip_vertex_t x;
@@ -669,7 +700,6 @@ __device__ inline void ipr_pull_vertex(const ipr_dev& d, uint64_t s, uint32_t v,
         ip_compute(&x);
         ipr_store_value(d, v, before, x.value);
         act = x.active;
-        (*n_computed)++;
         if (act) (*n_active)++;
         if (x.ipr_bcast) {
             ((IP_MESSAGE_TYPE*)d.bcm[s & 1])[v] = x.ipr_bcast_msg;
@@ -680,27 +710,33 @@ __device__ inline void ipr_pull_vertex(const ipr_dev& d, uint64_t s, uint32_t v,
     if (act != act0) d.act[v] = act;
 }
 
+/* Counters of the fused pull kernel: reactivations finish ActiveVertexCount
+ * of superstep s-1, the active vertices after compute start that of s. */
+__device__ inline void ipr_pull_counts(const ipr_dev& d, uint64_t s, uint64_t n_react,
+                                       uint64_t n_active, uint64_t n_bcast)
+{
+    ipr_block_count3(d, s > 0 ? IPR_C_ACT0 + (int)((s + 1) & 1) : -1, n_react,
+                     IPR_C_ACT0 + (int)(s & 1), n_active, IPR_C_BCAST0 + (int)(s & 1), n_bcast);
+}
+
 /* ordered: one thread per vertex over all V */
 __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_pull_step_ordered(ipr_dev d)
 {
     ipr_ctl* ctl = d.ctl;
     if (ctl->done) return;
     const uint64_t s = ctl->superstep;
-    const uint8_t* bf = (const uint8_t*)d.bcf[(s + 1) & 1];
+    const uint8_t* bf = d.bcf[(s + 1) & 1];
     const IP_MESSAGE_TYPE* bm = (const IP_MESSAGE_TYPE*)d.bcm[(s + 1) & 1];
-    uint64_t n_react = 0, n_active = 0, n_computed = 0, n_bcast = 0;
+    uint64_t n_react = 0, n_active = 0, n_bcast = 0;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < d.V; i += stride) {
         uint32_t v = (uint32_t)i;
         IP_MESSAGE_TYPE msg = IP_MESSAGE_TYPE();
         bool hm = false;
         if (s > 0) hm = ipr_fetch_ordered(d, v, bf, bm, &msg);
-        ipr_pull_vertex(d, s, v, hm, msg, &n_react, &n_active, &n_computed, &n_bcast);
+        ipr_pull_vertex(d, s, v, hm, msg, &n_react, &n_active, &n_bcast);
     }
-    if (s > 0) ipr_wave_add((unsigned long long*)&ctl->cnt[(s + 1) & 1], n_react);
-    ipr_wave_add((unsigned long long*)&ctl->cnt[s & 1], n_active);
-    ipr_wave_add((unsigned long long*)&ctl->work_vertices, n_computed);
-    ipr_wave_add((unsigned long long*)&ctl->bcast[s & 1], n_bcast);
+    ipr_pull_counts(d, s, n_react, n_active, n_bcast);
 }
 
 /* reassociating fold for declared combiners: vertices binned by in-degree
@@ -708,12 +744,14 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_pull_step_ordered(ipr_dev d)
  * Blocks [0, nb0) serve bin 0, [nb0, nb0+nb1) bin 1, the rest bin 2. */
 __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_pull_step_binned(ipr_dev d, uint32_t nb0, uint32_t nb1)
 {
+    __shared__ IP_MESSAGE_TYPE part[IPR_WAVES];
+    __shared__ int parth[IPR_WAVES];
     ipr_ctl* ctl = d.ctl;
     if (ctl->done) return;
     const uint64_t s = ctl->superstep;
-    const uint8_t* bf = (const uint8_t*)d.bcf[(s + 1) & 1];
+    const uint8_t* bf = d.bcf[(s + 1) & 1];
     const IP_MESSAGE_TYPE* bm = (const IP_MESSAGE_TYPE*)d.bcm[(s + 1) & 1];
-    uint64_t n_react = 0, n_active = 0, n_computed = 0, n_bcast = 0;
+    uint64_t n_react = 0, n_active = 0, n_bcast = 0;
     const uint32_t lane = ipr_lane();
     const uint32_t b = blockIdx.x;
     if (b < nb0) {
@@ -723,12 +761,12 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_pull_step_binned(ipr_dev d, u
             IP_MESSAGE_TYPE msg = IP_MESSAGE_TYPE();
             bool hm = false;
             if (s > 0) hm = ipr_fetch_ordered(d, v, bf, bm, &msg);
-            ipr_pull_vertex(d, s, v, hm, msg, &n_react, &n_active, &n_computed, &n_bcast);
+            ipr_pull_vertex(d, s, v, hm, msg, &n_react, &n_active, &n_bcast);
         }
     } else if (b < nb0 + nb1) {
         const uint64_t n = d.pbin_size[1];
-        const uint64_t wave = (uint64_t)(b - nb0) * (IPR_BLOCK / IPR_WAVE) + threadIdx.x / IPR_WAVE;
-        const uint64_t nwaves = (uint64_t)nb1 * (IPR_BLOCK / IPR_WAVE);
+        const uint64_t wave = (uint64_t)(b - nb0) * IPR_WAVES + threadIdx.x / IPR_WAVE;
+        const uint64_t nwaves = (uint64_t)nb1 * IPR_WAVES;
         for (uint64_t i = wave; i < n; i += nwaves) {
             uint32_t v = d.pbins[1][i];
             IP_MESSAGE_TYPE acc = ipr_policy::ident();
@@ -745,12 +783,9 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_pull_step_binned(ipr_dev d, u
                 }
                 has = __ballot(has) != 0;
             }
-            if (lane == 0)
-                ipr_pull_vertex(d, s, v, has, acc, &n_react, &n_active, &n_computed, &n_bcast);
+            if (lane == 0) ipr_pull_vertex(d, s, v, has, acc, &n_react, &n_active, &n_bcast);
         }
     } else {
-        __shared__ IP_MESSAGE_TYPE part[IPR_BLOCK / IPR_WAVE];
-        __shared__ int parth[IPR_BLOCK / IPR_WAVE];
         const uint64_t n = d.pbin_size[2];
         const uint32_t nb2 = gridDim.x - nb0 - nb1;
         for (uint64_t i = b - nb0 - nb1; i < n; i += nb2) {
@@ -773,21 +808,17 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_pull_step_binned(ipr_dev d, u
                 if (threadIdx.x == 0) {
                     acc = part[0];
                     has = parth[0];
-                    for (int w = 1; w < IPR_BLOCK / IPR_WAVE; w++) {
+                    for (int w = 1; w < IPR_WAVES; w++) {
                         ipr_policy::fold(&acc, part[w]);
                         has |= parth[w] != 0;
                     }
                 }
                 __syncthreads();
             }
-            if (threadIdx.x == 0)
-                ipr_pull_vertex(d, s, v, has, acc, &n_react, &n_active, &n_computed, &n_bcast);
+            if (threadIdx.x == 0) ipr_pull_vertex(d, s, v, has, acc, &n_react, &n_active, &n_bcast);
         }
     }
-    if (s > 0) ipr_wave_add((unsigned long long*)&ctl->cnt[(s + 1) & 1], n_react);
-    ipr_wave_add((unsigned long long*)&ctl->cnt[s & 1], n_active);
-    ipr_wave_add((unsigned long long*)&ctl->work_vertices, n_computed);
-    ipr_wave_add((unsigned long long*)&ctl->bcast[s & 1], n_bcast);
+    ipr_pull_counts(d, s, n_react, n_active, n_bcast);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -798,7 +829,7 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_pull_step_binned(ipr_dev d, u
 static void ipr_bin_blocks(const ipr_dev& d, uint32_t nb[3])
 {
     uint64_t b0 = (d.pbin_size[0] + IPR_BLOCK - 1) / IPR_BLOCK;
-    uint64_t b1 = (d.pbin_size[1] + (IPR_BLOCK / IPR_WAVE) - 1) / (IPR_BLOCK / IPR_WAVE);
+    uint64_t b1 = (d.pbin_size[1] + IPR_WAVES - 1) / IPR_WAVES;
     uint64_t b2 = d.pbin_size[2];
     nb[0] = (uint32_t)(b0 > 8192 ? 8192 : b0);
     nb[1] = (uint32_t)(b1 > 8192 ? 8192 : b1);
@@ -810,21 +841,22 @@ static int ipr_app_launch(int kernel, const ipr_dev* dp, void* stream)
 {
     hipStream_t st = (hipStream_t)stream;
     const ipr_dev& d = *dp;
-    uint32_t vblocks = (uint32_t)((d.V + IPR_BLOCK - 1) / IPR_BLOCK);
-    if (vblocks == 0) vblocks = 1;
-    if (vblocks > 65536) vblocks = 65536;
+    uint64_t vb = (d.V + IPR_BLOCK - 1) / IPR_BLOCK;
+    uint32_t vblocks = (uint32_t)(vb == 0 ? 1 : (vb > 65536 ? 65536 : vb));
+    uint32_t gblocks = d.grid_blocks ? d.grid_blocks : 2048;
     switch (kernel) {
     case IPR_K_INIT:
         hipLaunchKernelGGL(ipr_k_init, dim3(vblocks), dim3(IPR_BLOCK), 0, st, d);
         break;
     case IPR_K_COMPUTE:
-        hipLaunchKernelGGL(ipr_k_compute_push, dim3(vblocks), dim3(IPR_BLOCK), 0, st, d);
+        hipLaunchKernelGGL(ipr_k_compute_push, dim3(vblocks < gblocks ? vblocks : gblocks),
+                           dim3(IPR_BLOCK), 0, st, d);
         break;
     case IPR_K_EXPAND:
-        hipLaunchKernelGGL(ipr_k_expand, dim3(d.grid_blocks), dim3(IPR_BLOCK), 0, st, d);
+        hipLaunchKernelGGL(ipr_k_expand, dim3(gblocks), dim3(IPR_BLOCK), 0, st, d);
         break;
     case IPR_K_SWAP:
-        hipLaunchKernelGGL(ipr_k_swap, dim3(vblocks), dim3(IPR_BLOCK), 0, st, d);
+        hipLaunchKernelGGL(ipr_k_swap, dim3(vblocks < gblocks ? vblocks : gblocks), dim3(IPR_BLOCK), 0, st, d);
         break;
     case IPR_K_GATHER: {
         uint32_t nb[3];
@@ -834,7 +866,8 @@ static int ipr_app_launch(int kernel, const ipr_dev* dp, void* stream)
     }
     case IPR_K_PULL_STEP:
         if (d.pull_ordered || IPR_COMBINER == IPR_COMBINER_GENERIC) {
-            hipLaunchKernelGGL(ipr_k_pull_step_ordered, dim3(vblocks), dim3(IPR_BLOCK), 0, st, d);
+            hipLaunchKernelGGL(ipr_k_pull_step_ordered, dim3(vblocks < gblocks ? vblocks : gblocks),
+                               dim3(IPR_BLOCK), 0, st, d);
         } else {
             uint32_t nb[3];
             ipr_bin_blocks(d, nb);

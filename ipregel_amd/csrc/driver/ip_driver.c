@@ -147,13 +147,12 @@ static void dev_free_all(void)
         ipr_hip_free(d->bcf[i]);
         ipr_hip_free(d->bcm[i]);
         ipr_hip_free(d->mbox[i]);
-        ipr_hip_free(d->claim[i]);
+        ipr_hip_free(d->recv[i]);
         ipr_hip_free(d->front[i]);
     }
-    for (int b = 0; b < IPR_NBINS; b++) {
-        ipr_hip_free(d->bins[b]);
-        ipr_hip_free(d->pbins[b]);
-    }
+    for (int b = 0; b < IPR_NBINS; b++) ipr_hip_free(d->pbins[b]);
+    ipr_hip_free(d->lbins);
+    ipr_hip_free(d->cnt);
     ipr_hip_free(d->trace_active);
     ipr_hip_free(d->trace_ts);
     ipr_hip_free(d->trace_edges);
@@ -191,7 +190,6 @@ static void dev_setup(void)
     memset(d, 0, sizeof(*d));
     d->V = V;
     d->E = E;
-    d->words = (V + 31) / 32;
     uint64_t vv = V ? V : 1;
 
     d->out_idx = dalloc((V + 1) * 8, "out_idx");
@@ -227,17 +225,20 @@ static void dev_setup(void)
             d->bcf[i] = dalloc(vv, "broadcast flags");
         } else {
             d->mbox[i] = dalloc(vv * a->mbox_size, "mailboxes");
-            d->claim[i] = dalloc(d->words * 4 + 4, "claim bits");
+            d->recv[i] = dalloc(((vv + 15) / 16) * 16 + 16, "received flags");
+            hip_check(ipr_hip_fill(d->recv[i], 0, ((vv + 15) / 16) * 16 + 16, st), "flags init");
         }
         if (a->variant == IPR_VARIANT_SPREAD || a->variant == IPR_VARIANT_SPREAD_PULL)
             d->front[i] = dalloc(vv * 4, "frontier");
     }
     if (!pull) {
-        for (int b = 0; b < IPR_NBINS; b++) d->bins[b] = dalloc(vv * 4, "broadcaster bins");
+        d->lbins = dalloc(vv * 4, "large broadcasters");
         d->bslot = dalloc(vv * a->bslot_size, "broadcast records");
         d->lbase = dalloc(vv * 8, "large-bin chunk bases");
     }
     d->ctl = dalloc(sizeof(ipr_ctl), "control block");
+    d->cnt = dalloc(IPR_NCNT * IPR_CSLOTS * 8, "counters");
+    hip_check(ipr_hip_fill(d->cnt, 0, IPR_NCNT * IPR_CSLOTS * 8, st), "counters init");
     d->trace_active = dalloc(IPR_TRACE_CAP * 8, "trace");
     d->trace_ts = dalloc((IPR_TRACE_CAP + 1) * 8, "trace");
     d->trace_edges = dalloc(IPR_TRACE_CAP * 8, "trace");
@@ -248,16 +249,14 @@ static void dev_setup(void)
 
     /* tuning */
     const ipr_harness_cfg* c = D.have_cfg ? &D.cfg : NULL;
-    d->bin_small_max = c && c->bin_small_max ? c->bin_small_max : env_u32("IPR_BIN_SMALL", 64);
-    d->bin_medium_max = c && c->bin_medium_max ? c->bin_medium_max : env_u32("IPR_BIN_MEDIUM", 4096);
+    d->large_min = c && c->bin_medium_max ? c->bin_medium_max : env_u32("IPR_LARGE_MIN", 2048);
     d->grid_blocks = c && c->grid_blocks ? c->grid_blocks : env_u32("IPR_GRID_BLOCKS", 2048);
-    d->block_threads = 256;
     uint32_t po = c && c->pull_ordered ? (uint32_t)c->pull_ordered : env_u32("IPR_PULL_ORDERED", 1);
     d->pull_ordered = po == 1;
     uint32_t dopt = c && c->dir_opt ? (uint32_t)c->dir_opt : env_u32("IPR_DIR_OPT", 1);
     d->dir_opt = dopt == 1 && a->can_gather;
     d->dir_alpha = c && c->dir_alpha ? c->dir_alpha : env_u32("IPR_DIR_ALPHA", 6);
-    d->large_chunk = env_u32("IPR_LARGE_CHUNK", 2048);
+    d->large_chunk = c && c->bin_small_max ? c->bin_small_max : env_u32("IPR_LARGE_CHUNK", 2048);
     if (d->large_chunk < 64) d->large_chunk = 64;
     if (!pull && d->dir_opt) {
         /* dense supersteps gather over the exact transpose; a symmetric graph

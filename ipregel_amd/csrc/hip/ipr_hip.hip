@@ -347,28 +347,29 @@ extern "C" int ipr_hip_build_pull_bins(ipr_dev* dev, uint32_t small_max, uint32_
 /* frontier compaction                                                       */
 /* ------------------------------------------------------------------------ */
 
-/* The claim bitmap of superstep s+1 becomes the frontier list: each thread
- * owns one 32-bit word (and clears it for reuse two supersteps on), a
- * wavefront prefix sum of the words' populations places the ids, one atomic
- * per wavefront reserves its run. Ids ascend within each 2048-vertex run. */
+/* The received flags of superstep s+1 become the frontier list: each thread
+ * owns 16 flag bytes (and clears them for reuse two supersteps on), a
+ * wavefront prefix sum of their counts places the ids, one atomic per
+ * wavefront reserves its run, so ids ascend within each 1024-vertex run. */
 __global__ void k_compact(ipr_dev d)
 {
     ipr_ctl* c = d.ctl;
     if (c->done) return;
     const uint64_t s = c->superstep;
-    uint32_t* bits = d.claim[(s + 1) & 1];
+    uint8_t* flags = d.recv[(s + 1) & 1];
     uint32_t* out = d.front[(s + 1) & 1];
     unsigned long long* counter = (unsigned long long*)&c->fsize[(s + 1) & 1];
+    const uint64_t groups = (d.V + 15) / 16;          /* flag arrays are padded to 16 */
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    const uint64_t Wr = (d.words + 63) & ~63ull;
+    const uint64_t Gr = (groups + 63) & ~63ull;
     const uint32_t lane = threadIdx.x & 63;
-    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < Wr; w += stride) {
-        uint32_t word = 0;
-        if (w < d.words) {
-            word = bits[w];
-            if (word) bits[w] = 0;
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < Gr; g += stride) {
+        uint4 q = make_uint4(0, 0, 0, 0);
+        if (g < groups) {
+            q = ((const uint4*)flags)[g];
+            if (q.x | q.y | q.z | q.w) ((uint4*)flags)[g] = make_uint4(0, 0, 0, 0);
         }
-        uint32_t n = __popc(word);
+        uint32_t n = __popc(q.x) + __popc(q.y) + __popc(q.z) + __popc(q.w);
         uint32_t incl = n;
         for (int o = 1; o < 64; o <<= 1) {
             uint32_t y = __shfl_up(incl, o);
@@ -380,17 +381,21 @@ __global__ void k_compact(ipr_dev d)
         if (lane == 0) base = atomicAdd(counter, (unsigned long long)total);
         base = __shfl(base, 0);
         uint64_t pos = base + incl - n;
-        while (word) {
-            uint32_t bit = __ffs(word) - 1;
-            out[pos++] = (uint32_t)(w * 32 + bit);
-            word &= word - 1;
+        uint32_t words[4] = {q.x, q.y, q.z, q.w};
+        for (int k = 0; k < 4; k++) {
+            uint32_t w = words[k];
+            while (w) {
+                uint32_t bit = __ffs(w) - 1;           /* flags are 0 or 1: bit 0, 8, 16, 24 */
+                out[pos++] = (uint32_t)(g * 16 + k * 4 + bit / 8);
+                w &= w - 1;
+            }
         }
     }
 }
 
 extern "C" int ipr_hip_compact_frontier(const ipr_dev* dev, void* stream)
 {
-    uint64_t blocks = (dev->words + 255) / 256;
+    uint64_t blocks = ((dev->V + 15) / 16 + 255) / 256;
     if (blocks < 1) blocks = 1;
     if (blocks > 16384) blocks = 16384;
     hipLaunchKernelGGL(k_compact, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, *dev);
@@ -434,12 +439,22 @@ extern "C" int ipr_hip_equal(const void* a, const void* b, uint64_t bytes, int* 
 
 __global__ void k_begin(ipr_dev d)
 {
+    for (uint32_t i = threadIdx.x; i < IPR_NCNT * IPR_CSLOTS; i += blockDim.x) d.cnt[i] = 0;
     if (threadIdx.x != 0) return;
     ipr_ctl* c = d.ctl;
     uint64_t cap = c->trace_cap;
     memset(c, 0, sizeof(ipr_ctl));
     c->trace_cap = cap;
     if (cap) d.trace_ts[0] = wall_clock64();
+}
+
+/* sum (and optionally clear) one spread counter; 64 threads */
+__device__ uint64_t ipr_take(const ipr_dev& d, int id, bool clear)
+{
+    uint64_t x = d.cnt[id * IPR_CSLOTS + threadIdx.x];
+    if (clear) d.cnt[id * IPR_CSLOTS + threadIdx.x] = 0;
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    return x;
 }
 
 /* End of one superstep: active count, halting test, superstep increment.
@@ -449,39 +464,43 @@ __global__ void k_begin(ipr_dev d)
  * active = |next frontier| (:410-411); PUSH: src/combiner_postamble.h:229. */
 __global__ void k_finalize(ipr_dev d, int variant)
 {
-    if (threadIdx.x != 0) return;
+    /* one wavefront: lane k owns counter slot k */
     ipr_ctl* c = d.ctl;
     if (c->done) return;
     const uint64_t s = c->superstep;
-    uint64_t fin, active, edges;
+    uint64_t fin, active, edges, computed;
     if (variant == IPR_VARIANT_PULL) {
-        if (s == 0) { c->superstep = 1; return; }
+        if (s == 0) {
+            if (threadIdx.x == 0) c->superstep = 1;
+            return;
+        }
         fin = s - 1;
-        active = c->cnt[fin & 1];
-        c->cnt[fin & 1] = 0;
-        edges = c->bcast[fin & 1] ? d.in_idx[d.V] : 0;
-        c->bcast[fin & 1] = 0;
+        active = ipr_take(d, IPR_C_ACT0 + (int)(fin & 1), true);
+        uint64_t bc = ipr_take(d, IPR_C_BCAST0 + (int)(fin & 1), true);
+        computed = ipr_take(d, IPR_C_WORKV, true);
+        edges = bc ? d.in_idx[d.V] : 0;
+    } else {
+        fin = s;
+        active = variant == IPR_VARIANT_PUSH ? ipr_take(d, IPR_C_ACT0 + (int)(s & 1), true)
+                                             : c->fsize[(s + 1) & 1];
+        edges = ipr_take(d, IPR_C_EDGES, true);
+        computed = ipr_take(d, IPR_C_WORKV, true);
+    }
+    if (threadIdx.x != 0) return;
+    if (variant == IPR_VARIANT_PULL) {
         c->done = active == 0;
         c->executed = fin + 1;
         c->superstep = c->done ? s : s + 1;
     } else {
-        fin = s;
-        if (variant == IPR_VARIANT_PUSH) {
-            active = c->cnt[s & 1];
-            c->cnt[s & 1] = 0;
-        } else {
-            active = c->fsize[(s + 1) & 1];
-            c->fsize[s & 1] = 0;
-        }
-        edges = c->step_edges;
+        if (variant != IPR_VARIANT_PUSH) c->fsize[s & 1] = 0;
         c->done = active == 0;
         c->executed = s + 1;
         c->superstep = s + 1;
     }
-    for (int b = 0; b < IPR_NBINS; b++) c->bsize[b] = 0;
     c->lpack = 0;
     c->step_edges = 0;
     c->work_edges += edges;
+    c->work_vertices += computed;
     if (c->pull_mode) c->gathered++;
     c->pull_mode = 0;
     c->last_active = active;
@@ -496,10 +515,12 @@ __global__ void k_finalize(ipr_dev d, int variant)
  * superstep by gather when the broadcasters' out-arcs are a large share of E */
 __global__ void k_direction(ipr_dev d)
 {
-    if (threadIdx.x != 0) return;
     ipr_ctl* c = d.ctl;
     if (c->done) return;
-    c->pull_mode = (d.dir_opt && c->step_edges * (uint64_t)d.dir_alpha > d.E) ? 1 : 0;
+    uint64_t w = ipr_take(d, IPR_C_EDGES, false);
+    if (threadIdx.x != 0) return;
+    c->step_edges = w;
+    c->pull_mode = (d.dir_opt && w * (uint64_t)d.dir_alpha > d.E) ? 1 : 0;
 }
 
 extern "C" int ipr_hip_begin(const ipr_dev* dev, void* stream)
