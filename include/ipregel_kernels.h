@@ -564,6 +564,69 @@ __device__ inline bool ipr_gather_range(const ipr_dev& d, uint64_t s, uint64_t f
     return has;
 }
 
+/* Lane-cooperative fold for vertices of small in-degree: lane L holds vertex
+ * v (valid), the in-lists [idx[v], idx[v+1]) of the wavefront's 64 vertices
+ * are laid end to end and taken 64 arcs at a time; load(k, &m) gives the
+ * message at arc position k (false: none); a segmented scan across the lanes
+ * folds each vertex's arcs and the segment's last lane hands the result to
+ * the vertex's lane. Coalesced index reads, 64 gathers in flight per step.
+ * Must be reached by all lanes of the wavefront. */
+template <typename LoadF>
+__device__ inline bool ipr_wave_segmented_fold(const uint64_t* idx, uint32_t v, bool valid, LoadF load,
+                                               IP_MESSAGE_TYPE* out)
+{
+    const uint32_t lane = ipr_lane();
+    uint64_t beg = 0;
+    uint32_t deg = 0;
+    if (valid) {
+        beg = idx[v];
+        deg = (uint32_t)(idx[v + 1] - beg);
+    }
+    const uint32_t incl = ipr_wave_incl_scan(deg);
+    const uint32_t excl = incl - deg;
+    const uint32_t total = __shfl(incl, IPR_WAVE - 1);
+    IP_MESSAGE_TYPE acc = ipr_policy::ident();
+    bool has = false;
+    for (uint32_t t0 = 0; t0 < total; t0 += IPR_WAVE) {
+        const uint32_t t = t0 + lane;
+        uint32_t lo = 0;
+        for (uint32_t step = 32; step > 0; step >>= 1) {
+            uint32_t probe = __shfl(excl, (lo + step) & (IPR_WAVE - 1));
+            if (lo + step < IPR_WAVE && probe <= t) lo += step;
+        }
+        const uint64_t o_beg = __shfl(beg, lo);
+        const uint32_t o_ex = __shfl(excl, lo);
+        IP_MESSAGE_TYPE m = ipr_policy::ident();
+        int h = 0;
+        uint32_t owner = IPR_WAVE + lane;          /* matches no other lane */
+        if (t < total) {
+            owner = lo;
+            h = load(o_beg + (t - o_ex), &m) ? 1 : 0;
+            if (!h) m = ipr_policy::ident();
+        }
+        for (int o = 1; o < IPR_WAVE; o <<= 1) {
+            IP_MESSAGE_TYPE y = __shfl_up(m, o);
+            int yh = __shfl_up(h, o);
+            uint32_t yo = __shfl_up(owner, o);
+            if (lane >= (uint32_t)o && yo == owner) {
+                ipr_policy::fold(&m, y);
+                h |= yh;
+            }
+        }
+        const uint32_t ss = excl > t0 ? excl : t0;
+        const uint32_t se = excl + deg < t0 + IPR_WAVE ? excl + deg : t0 + IPR_WAVE;
+        const uint32_t src = ss < se ? se - 1 - t0 : lane;
+        IP_MESSAGE_TYPE y = __shfl(m, src);
+        int yh = __shfl(h, src);
+        if (ss < se) {
+            ipr_policy::fold(&acc, y);
+            has |= yh != 0;
+        }
+    }
+    *out = acc;
+    return has;
+}
+
 /* the gathering vertex is the only writer of its mailbox and flag (messages
  * from ip_send_message were delivered before, in the compute kernel) */
 __device__ inline void ipr_gather_store(const ipr_dev& d, uint64_t s, uint32_t v, IP_MESSAGE_TYPE acc)
@@ -586,10 +649,24 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_gather(ipr_dev d, uint32_t nb
     const uint32_t b = blockIdx.x;
     if (b < nb0) {
         const uint64_t n = d.pbin_size[0];
-        for (uint64_t i = (uint64_t)b * blockDim.x + threadIdx.x; i < n; i += (uint64_t)nb0 * blockDim.x) {
-            uint32_t v = d.pbins[0][i];
-            IP_MESSAGE_TYPE acc = ipr_policy::ident();
-            if (ipr_gather_range(d, s, d.t_idx[v], d.t_idx[v + 1], 1, &acc)) ipr_gather_store(d, s, v, acc);
+        const ipr_bslot_t* bs = (const ipr_bslot_t*)d.bslot;
+        const uint32_t want = (uint32_t)(s + 1);
+        auto load = [&](uint64_t k, IP_MESSAGE_TYPE* m) -> bool {
+            ipr_bslot_t r = bs[d.t_adj[k]];
+            if ((r.step & ~IPR_BSTEP_WEIGHTED) != want) return false;
+            *m = r.msg;
+#ifdef IP_NEEDS_OUT_NEIGHBOUR_WEIGHTS
+            if (r.step & IPR_BSTEP_WEIGHTED) *m = (IP_MESSAGE_TYPE)(*m + (IP_MESSAGE_TYPE)d.t_wgt[k]);
+#endif
+            return true;
+        };
+        const uint64_t stride = (uint64_t)nb0 * blockDim.x;
+        for (uint64_t base = (uint64_t)b * blockDim.x + (threadIdx.x & ~(IPR_WAVE - 1)); base < n; base += stride) {
+            const uint64_t i = base + lane;
+            const uint32_t v = i < n ? d.pbins[0][i] : 0;
+            IP_MESSAGE_TYPE acc;
+            bool has = ipr_wave_segmented_fold(d.t_idx, v, i < n, load, &acc);
+            if (i < n && has) ipr_gather_store(d, s, v, acc);
         }
     } else if (b < nb0 + nb1) {
         const uint64_t n = d.pbin_size[1];
@@ -719,30 +796,48 @@ __device__ inline void ipr_pull_counts(const ipr_dev& d, uint64_t s, uint64_t n_
                      IPR_C_ACT0 + (int)(s & 1), n_active, IPR_C_BCAST0 + (int)(s & 1), n_bcast);
 }
 
-/* ordered: one thread per vertex over all V */
-__global__ void __launch_bounds__(IPR_BLOCK) ipr_k_pull_step_ordered(ipr_dev d)
+/* In-list order fold by one wavefront (vertices of larger in-degree): 64
+ * in-neighbours are fetched at once, then folded lane by lane in list order
+ * with readlane, so the result is bit-identical to the sequential fold. */
+__device__ inline bool ipr_fetch_ordered_wave(const ipr_dev& d, uint32_t v, const uint8_t* bf,
+                                              const IP_MESSAGE_TYPE* bm, IP_MESSAGE_TYPE* out)
 {
-    ipr_ctl* ctl = d.ctl;
-    if (ctl->done) return;
-    const uint64_t s = ctl->superstep;
-    const uint8_t* bf = d.bcf[(s + 1) & 1];
-    const IP_MESSAGE_TYPE* bm = (const IP_MESSAGE_TYPE*)d.bcm[(s + 1) & 1];
-    uint64_t n_react = 0, n_active = 0, n_bcast = 0;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < d.V; i += stride) {
-        uint32_t v = (uint32_t)i;
-        IP_MESSAGE_TYPE msg = IP_MESSAGE_TYPE();
-        bool hm = false;
-        if (s > 0) hm = ipr_fetch_ordered(d, v, bf, bm, &msg);
-        ipr_pull_vertex(d, s, v, hm, msg, &n_react, &n_active, &n_bcast);
+    const uint32_t lane = ipr_lane();
+    const uint64_t beg = d.in_idx[v], end = d.in_idx[v + 1];
+    bool has = false;
+    IP_MESSAGE_TYPE acc = IP_MESSAGE_TYPE();
+    for (uint64_t base = beg; base < end; base += IPR_WAVE) {
+        const uint64_t k = base + lane;
+        bool f = false;
+        IP_MESSAGE_TYPE m = IP_MESSAGE_TYPE();
+        if (k < end) {
+            uint32_t u = d.in_adj[k];
+            f = bf[u] != 0;
+            if (f) m = bm[u];
+        }
+        uint64_t mask = __ballot(f);
+        while (mask) {
+            const uint32_t j = __ffsll((unsigned long long)mask) - 1;
+            IP_MESSAGE_TYPE mj = __shfl(m, j);
+            if (has) ip_combine(&acc, mj);
+            else { acc = mj; has = true; }
+            mask &= mask - 1;
+        }
     }
-    ipr_pull_counts(d, s, n_react, n_active, n_bcast);
+    *out = acc;
+    return has;
 }
 
-/* reassociating fold for declared combiners: vertices binned by in-degree
- * (static bins built at load); thread, wavefront or workgroup per vertex.
- * Blocks [0, nb0) serve bin 0, [nb0, nb0+nb1) bin 1, the rest bin 2. */
-__global__ void __launch_bounds__(IPR_BLOCK) ipr_k_pull_step_binned(ipr_dev d, uint32_t nb0, uint32_t nb1)
+/* Fused fetch(s-1) + compute(s) over vertices binned by in-degree (static
+ * bins built at load). Blocks [0, nb0) serve bin 0, [nb0, nb0+nb1) bin 1,
+ * the rest bin 2.
+ * ORDERED (bit-exact with the reference's in-list order): bin 0 one thread
+ * per vertex, bins 1-2 one wavefront per vertex with the ordered fold.
+ * Otherwise (declared combiner, reassociating): bin 0 the lane-cooperative
+ * segmented fold, bin 1 a wavefront and bin 2 a workgroup per vertex with
+ * tree reductions. */
+template <bool ORDERED>
+__global__ void __launch_bounds__(IPR_BLOCK) ipr_k_pull_step(ipr_dev d, uint32_t nb0, uint32_t nb1)
 {
     __shared__ IP_MESSAGE_TYPE part[IPR_WAVES];
     __shared__ int parth[IPR_WAVES];
@@ -753,19 +848,54 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_pull_step_binned(ipr_dev d, u
     const IP_MESSAGE_TYPE* bm = (const IP_MESSAGE_TYPE*)d.bcm[(s + 1) & 1];
     uint64_t n_react = 0, n_active = 0, n_bcast = 0;
     const uint32_t lane = ipr_lane();
+    const uint32_t wave_in_block = threadIdx.x / IPR_WAVE;
     const uint32_t b = blockIdx.x;
+    const uint32_t nb2 = gridDim.x - nb0 - nb1;
     if (b < nb0) {
         const uint64_t n = d.pbin_size[0];
-        for (uint64_t i = (uint64_t)b * blockDim.x + threadIdx.x; i < n; i += (uint64_t)nb0 * blockDim.x) {
-            uint32_t v = d.pbins[0][i];
+        if (ORDERED) {
+            for (uint64_t i = (uint64_t)b * blockDim.x + threadIdx.x; i < n; i += (uint64_t)nb0 * blockDim.x) {
+                uint32_t v = d.pbins[0][i];
+                IP_MESSAGE_TYPE msg = IP_MESSAGE_TYPE();
+                bool hm = false;
+                if (s > 0) hm = ipr_fetch_ordered(d, v, bf, bm, &msg);
+                ipr_pull_vertex(d, s, v, hm, msg, &n_react, &n_active, &n_bcast);
+            }
+        } else {
+            auto load = [&](uint64_t k, IP_MESSAGE_TYPE* m) -> bool {
+                uint32_t u = d.in_adj[k];
+                if (!bf[u]) return false;
+                *m = bm[u];
+                return true;
+            };
+            const uint64_t stride = (uint64_t)nb0 * blockDim.x;
+            for (uint64_t base = (uint64_t)b * blockDim.x + (threadIdx.x & ~(IPR_WAVE - 1)); base < n;
+                 base += stride) {
+                const uint64_t i = base + lane;
+                const uint32_t v = i < n ? d.pbins[0][i] : 0;
+                IP_MESSAGE_TYPE acc = ipr_policy::ident();
+                bool hm = false;
+                if (s > 0) hm = ipr_wave_segmented_fold(d.in_idx, v, i < n, load, &acc);
+                if (i < n) ipr_pull_vertex(d, s, v, hm, acc, &n_react, &n_active, &n_bcast);
+            }
+        }
+    } else if (ORDERED) {
+        /* bins 1 and 2: one wavefront per vertex, ordered */
+        const bool one = b < nb0 + nb1;
+        const uint64_t n = d.pbin_size[one ? 1 : 2];
+        const uint32_t* list = d.pbins[one ? 1 : 2];
+        const uint64_t first = (uint64_t)(b - (one ? nb0 : nb0 + nb1)) * IPR_WAVES + wave_in_block;
+        const uint64_t nwaves = (uint64_t)(one ? nb1 : nb2) * IPR_WAVES;
+        for (uint64_t i = first; i < n; i += nwaves) {
+            uint32_t v = list[i];
             IP_MESSAGE_TYPE msg = IP_MESSAGE_TYPE();
             bool hm = false;
-            if (s > 0) hm = ipr_fetch_ordered(d, v, bf, bm, &msg);
-            ipr_pull_vertex(d, s, v, hm, msg, &n_react, &n_active, &n_bcast);
+            if (s > 0) hm = ipr_fetch_ordered_wave(d, v, bf, bm, &msg);
+            if (lane == 0) ipr_pull_vertex(d, s, v, hm, msg, &n_react, &n_active, &n_bcast);
         }
     } else if (b < nb0 + nb1) {
         const uint64_t n = d.pbin_size[1];
-        const uint64_t wave = (uint64_t)(b - nb0) * IPR_WAVES + threadIdx.x / IPR_WAVE;
+        const uint64_t wave = (uint64_t)(b - nb0) * IPR_WAVES + wave_in_block;
         const uint64_t nwaves = (uint64_t)nb1 * IPR_WAVES;
         for (uint64_t i = wave; i < n; i += nwaves) {
             uint32_t v = d.pbins[1][i];
@@ -787,7 +917,6 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_pull_step_binned(ipr_dev d, u
         }
     } else {
         const uint64_t n = d.pbin_size[2];
-        const uint32_t nb2 = gridDim.x - nb0 - nb1;
         for (uint64_t i = b - nb0 - nb1; i < n; i += nb2) {
             uint32_t v = d.pbins[2][i];
             IP_MESSAGE_TYPE acc = ipr_policy::ident();
@@ -803,7 +932,7 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_pull_step_binned(ipr_dev d, u
                     ipr_policy::fold(&acc, y);
                 }
                 bool wh = __ballot(has) != 0;
-                if (lane == 0) { part[threadIdx.x / IPR_WAVE] = acc; parth[threadIdx.x / IPR_WAVE] = wh; }
+                if (lane == 0) { part[wave_in_block] = acc; parth[wave_in_block] = wh; }
                 __syncthreads();
                 if (threadIdx.x == 0) {
                     acc = part[0];
@@ -864,17 +993,17 @@ static int ipr_app_launch(int kernel, const ipr_dev* dp, void* stream)
         hipLaunchKernelGGL(ipr_k_gather, dim3(nb[0] + nb[1] + nb[2]), dim3(IPR_BLOCK), 0, st, d, nb[0], nb[1]);
         break;
     }
-    case IPR_K_PULL_STEP:
-        if (d.pull_ordered || IPR_COMBINER == IPR_COMBINER_GENERIC) {
-            hipLaunchKernelGGL(ipr_k_pull_step_ordered, dim3(vblocks < gblocks ? vblocks : gblocks),
-                               dim3(IPR_BLOCK), 0, st, d);
-        } else {
-            uint32_t nb[3];
-            ipr_bin_blocks(d, nb);
-            hipLaunchKernelGGL(ipr_k_pull_step_binned, dim3(nb[0] + nb[1] + nb[2]), dim3(IPR_BLOCK), 0, st,
+    case IPR_K_PULL_STEP: {
+        uint32_t nb[3];
+        ipr_bin_blocks(d, nb);
+        if (d.pull_ordered || IPR_COMBINER == IPR_COMBINER_GENERIC)
+            hipLaunchKernelGGL(ipr_k_pull_step<true>, dim3(nb[0] + nb[1] + nb[2]), dim3(IPR_BLOCK), 0, st,
                                d, nb[0], nb[1]);
-        }
+        else
+            hipLaunchKernelGGL(ipr_k_pull_step<false>, dim3(nb[0] + nb[1] + nb[2]), dim3(IPR_BLOCK), 0, st,
+                               d, nb[0], nb[1]);
         break;
+    }
     default:
         return -1;
     }

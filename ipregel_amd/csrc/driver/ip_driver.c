@@ -285,7 +285,10 @@ static void dev_setup(void)
         }
     }
     if (pull || d->dir_opt) {
-        uint32_t ps = c && c->pull_small_max ? c->pull_small_max : env_u32("IPR_PULL_SMALL", 32);
+        /* ordered folds are sequential per vertex: keep the thread-per-vertex
+         * bin short; reassociating folds use the lane-cooperative one */
+        uint32_t ps = c && c->pull_small_max ? c->pull_small_max
+                                             : env_u32("IPR_PULL_SMALL", d->pull_ordered ? 4 : 32);
         uint32_t pm = c && c->pull_medium_max ? c->pull_medium_max : env_u32("IPR_PULL_MEDIUM", 2048);
         for (int b = 0; b < IPR_NBINS; b++) d->pbins[b] = dalloc(vv * 4, "pull bins");
         hip_check(ipr_hip_build_pull_bins(d, ps, pm, st), "pull bins");
