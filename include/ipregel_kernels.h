@@ -286,6 +286,19 @@ __device__ inline void ipr_deliver(const ipr_dev& d, uint64_t s, uint32_t u, IP_
     if (ipr_policy::deliver(mb + u, m)) ipr_mark_recv(d.recv[(s + 1) & 1], u);
 }
 
+/* what the expansion does for one arc: push variants deliver the message,
+ * SPREAD_PULL only marks the out-neighbour as a broadcast target
+ * (src/combiner_spread_single_broadcast_postamble.h ip_broadcast) */
+__device__ inline void ipr_emit(const ipr_dev& d, uint64_t s, uint32_t u, IP_MESSAGE_TYPE m)
+{
+#if IPR_VARIANT == 3
+    (void)m;
+    ipr_mark_recv(d.recv[(s + 1) & 1], u);
+#else
+    ipr_deliver(d, s, u, m);
+#endif
+}
+
 /* ip_send_message in push variants: immediate delivery */
 __device__ inline void ipr_send_immediate(IP_VERTEX_ID_TYPE id, IP_MESSAGE_TYPE m)
 {
@@ -347,9 +360,13 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_init(ipr_dev d)
         ((IP_VALUE_TYPE*)d.val)[v] = IP_VALUE_TYPE();
         d.act[v] = 1;  /* ip_init_vertex_range, e.g. src/combiner_postamble.h:189 */
         d.hm[v] = 0;
-#if IPR_IS_PULL
+#if IPR_VARIANT == 1
         d.bcf[0][v] = 0;
         d.bcf[1][v] = 0;
+#elif IPR_VARIANT == 3
+        ((ipr_bslot_t*)d.bslot)[v].step = 0;
+        d.recv[0][v] = 0;
+        d.recv[1][v] = 0;
 #else
         ((ipr_slot_t*)d.mbox[0])[v] = ipr_policy::empty();
         ((ipr_slot_t*)d.mbox[1])[v] = ipr_policy::empty();
@@ -378,7 +395,7 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_compute_push(ipr_dev d)
     const uint64_t s = ctl->superstep;
     uint64_t n = d.V;
     const uint32_t* list = nullptr;
-    if (IPR_VARIANT == IPR_VARIANT_SPREAD && s > 0) {
+    if ((IPR_VARIANT == IPR_VARIANT_SPREAD || IPR_VARIANT == IPR_VARIANT_SPREAD_PULL) && s > 0) {
         n = ctl->fsize[s & 1];
         list = d.front[s & 1];
     }
@@ -400,6 +417,11 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_compute_push(ipr_dev d)
                     x.message = ipr_policy::value(mcur[v]);
                     mcur[v] = ipr_policy::empty();
                 }
+            } else if (IPR_VARIANT == IPR_VARIANT_SPREAD_PULL) {
+                /* targets: message fetched at the end of the previous superstep */
+                x.active = true;
+                x.has_message = d.hm[v] != 0;
+                if (x.has_message) x.message = ((const IP_MESSAGE_TYPE*)d.msg)[v];
             } else {
                 x.active = d.act[v] != 0;
                 x.has_message = d.hm[v] != 0;
@@ -421,18 +443,18 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_compute_push(ipr_dev d)
                     n_active += x.active;
                 }
                 if (x.ipr_bcast) {
+                    /* recorded even without out-arcs: with directed=false a
+                     * SPREAD_PULL in-list may still name this vertex */
                     uint64_t deg = d.out_idx[v + 1] - d.out_idx[v];
                     n_edges += deg;
-                    if (deg > 0) {
-                        ipr_bslot_t rec;
-                        rec.step = (uint32_t)(s + 1) | (x.ipr_bcast_weighted ? IPR_BSTEP_WEIGHTED : 0u);
-                        rec.msg = x.ipr_bcast_msg;
-                        bs[v] = rec;
-                        if (deg > d.large_min) {
-                            large = 1;
-                            chunks = (uint32_t)((deg + d.large_chunk - 1) / d.large_chunk);
-                            entry = v;
-                        }
+                    ipr_bslot_t rec;
+                    rec.step = (uint32_t)(s + 1) | (x.ipr_bcast_weighted ? IPR_BSTEP_WEIGHTED : 0u);
+                    rec.msg = x.ipr_bcast_msg;
+                    bs[v] = rec;
+                    if (deg > d.large_min) {
+                        large = 1;
+                        chunks = (uint32_t)((deg + d.large_chunk - 1) / d.large_chunk);
+                        entry = v;
                     }
                 }
             }
@@ -491,14 +513,14 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_expand(ipr_dev d)
         uint64_t beg = (c - d.lbase[lo]) * d.large_chunk;
         uint64_t end = beg + d.large_chunk < deg ? beg + d.large_chunk : deg;
         for (uint64_t k = beg + threadIdx.x; k < end; k += IPR_BLOCK)
-            ipr_deliver(d, s, d.out_adj[off + k], ipr_edge_msg(d.out_wgt, b.msg, wf, off + k));
+            ipr_emit(d, s, d.out_adj[off + k], ipr_edge_msg(d.out_wgt, b.msg, wf, off + k));
     }
 
     /* the rest of the work set: 256 vertices per workgroup, their arcs laid
      * end to end by a prefix sum and taken 256 at a time */
     uint64_t n = d.V;
     const uint32_t* list = nullptr;
-    if (IPR_VARIANT == IPR_VARIANT_SPREAD && s > 0) {
+    if ((IPR_VARIANT == IPR_VARIANT_SPREAD || IPR_VARIANT == IPR_VARIANT_SPREAD_PULL) && s > 0) {
         n = ctl->fsize[s & 1];
         list = d.front[s & 1];
     }
@@ -531,7 +553,7 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_expand(ipr_dev d)
                 for (uint32_t step = IPR_BLOCK / 2; step > 0; step >>= 1)
                     if (s_ex[lo + step] <= t) lo += step;
                 uint64_t e = s_off[lo] + (t - s_ex[lo]);
-                ipr_deliver(d, s, d.out_adj[e], ipr_edge_msg(d.out_wgt, s_msg[lo], s_wf[lo] != 0, e));
+                ipr_emit(d, s, d.out_adj[e], ipr_edge_msg(d.out_wgt, s_msg[lo], s_wf[lo] != 0, e));
             }
         }
         __syncthreads();
@@ -551,7 +573,7 @@ __device__ inline bool ipr_gather_range(const ipr_dev& d, uint64_t s, uint64_t f
     const uint32_t want = (uint32_t)(s + 1);
     bool has = false;
     for (uint64_t k = first; k < end; k += step) {
-        ipr_bslot_t b = bs[d.t_adj[k]];
+        ipr_bslot_t b = bs[__builtin_nontemporal_load(d.t_adj + k)];
         if ((b.step & ~IPR_BSTEP_WEIGHTED) == want) {
             IP_MESSAGE_TYPE m = b.msg;
 #ifdef IP_NEEDS_OUT_NEIGHBOUR_WEIGHTS
@@ -652,7 +674,9 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_gather(ipr_dev d, uint32_t nb
         const ipr_bslot_t* bs = (const ipr_bslot_t*)d.bslot;
         const uint32_t want = (uint32_t)(s + 1);
         auto load = [&](uint64_t k, IP_MESSAGE_TYPE* m) -> bool {
-            ipr_bslot_t r = bs[d.t_adj[k]];
+            /* the index stream is read once: keep it out of L2, which holds
+             * the randomly gathered broadcast records */
+            ipr_bslot_t r = bs[__builtin_nontemporal_load(d.t_adj + k)];
             if ((r.step & ~IPR_BSTEP_WEIGHTED) != want) return false;
             *m = r.msg;
 #ifdef IP_NEEDS_OUT_NEIGHBOUR_WEIGHTS
@@ -951,6 +975,94 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_pull_step(ipr_dev d, uint32_t
 }
 
 /* ------------------------------------------------------------------------ */
+/* spread + single broadcast: fetch for the targets                          */
+/* ------------------------------------------------------------------------ */
+
+/* src/combiner_spread_single_broadcast_postamble.h ip_fetch_broadcast_messages
+ * for every target of superstep s+1: the broadcasting in-neighbours (records
+ * stamped with superstep s) folded in in-list order, no activation logic.
+ * One thread per target; targets with more than 64 in-neighbours are taken by
+ * their whole wavefront in turn (ordered wavefront fold). */
+__device__ inline bool ipr_fetch_ordered_epoch_wave(const ipr_dev& d, uint32_t v, uint32_t want,
+                                                    IP_MESSAGE_TYPE* out)
+{
+    const ipr_bslot_t* bs = (const ipr_bslot_t*)d.bslot;
+    const uint32_t lane = ipr_lane();
+    const uint64_t beg = d.in_idx[v], end = d.in_idx[v + 1];
+    bool has = false;
+    IP_MESSAGE_TYPE acc = IP_MESSAGE_TYPE();
+    for (uint64_t base = beg; base < end; base += IPR_WAVE) {
+        const uint64_t k = base + lane;
+        bool f = false;
+        IP_MESSAGE_TYPE m = IP_MESSAGE_TYPE();
+        if (k < end) {
+            ipr_bslot_t b = bs[d.in_adj[k]];
+            f = (b.step & ~IPR_BSTEP_WEIGHTED) == want;
+            m = b.msg;
+        }
+        uint64_t mask = __ballot(f);
+        while (mask) {
+            const uint32_t j = __ffsll((unsigned long long)mask) - 1;
+            IP_MESSAGE_TYPE mj = __shfl(m, j);
+            if (has) ip_combine(&acc, mj);
+            else { acc = mj; has = true; }
+            mask &= mask - 1;
+        }
+    }
+    *out = acc;
+    return has;
+}
+
+__global__ void __launch_bounds__(IPR_BLOCK) ipr_k_fetch_targets(ipr_dev d)
+{
+    ipr_ctl* ctl = d.ctl;
+    if (ctl->done) return;
+    const uint64_t s = ctl->superstep;
+    const uint32_t want = (uint32_t)(s + 1);
+    const uint64_t n = ctl->fsize[(s + 1) & 1];
+    const uint32_t* list = d.front[(s + 1) & 1];
+    const ipr_bslot_t* bs = (const ipr_bslot_t*)d.bslot;
+    const uint32_t lane = ipr_lane();
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t nr = (n + IPR_WAVE - 1) & ~(uint64_t)(IPR_WAVE - 1);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nr; i += stride) {
+        uint32_t v = 0;
+        bool big = false;
+        if (i < n) {
+            v = list[i];
+            uint64_t beg = d.in_idx[v], end = d.in_idx[v + 1];
+            if (end - beg > IPR_WAVE) {
+                big = true;
+            } else {
+                bool has = false;
+                IP_MESSAGE_TYPE acc = IP_MESSAGE_TYPE();
+                for (uint64_t k = beg; k < end; k++) {
+                    ipr_bslot_t b = bs[d.in_adj[k]];
+                    if ((b.step & ~IPR_BSTEP_WEIGHTED) == want) {
+                        if (has) ip_combine(&acc, b.msg);
+                        else { acc = b.msg; has = true; }
+                    }
+                }
+                d.hm[v] = has;
+                if (has) ((IP_MESSAGE_TYPE*)d.msg)[v] = acc;
+            }
+        }
+        uint64_t bigmask = __ballot(big);
+        while (bigmask) {
+            const uint32_t j = __ffsll((unsigned long long)bigmask) - 1;
+            const uint32_t vj = __shfl(v, j);
+            IP_MESSAGE_TYPE acc;
+            bool has = ipr_fetch_ordered_epoch_wave(d, vj, want, &acc);
+            if (lane == 0) {
+                d.hm[vj] = has;
+                if (has) ((IP_MESSAGE_TYPE*)d.msg)[vj] = acc;
+            }
+            bigmask &= bigmask - 1;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
 /* launch table                                                              */
 /* ------------------------------------------------------------------------ */
 
@@ -983,6 +1095,10 @@ static int ipr_app_launch(int kernel, const ipr_dev* dp, void* stream)
         break;
     case IPR_K_EXPAND:
         hipLaunchKernelGGL(ipr_k_expand, dim3(gblocks), dim3(IPR_BLOCK), 0, st, d);
+        break;
+    case IPR_K_FETCH:
+        hipLaunchKernelGGL(ipr_k_fetch_targets, dim3(vblocks < gblocks ? vblocks : gblocks), dim3(IPR_BLOCK),
+                           0, st, d);
         break;
     case IPR_K_SWAP:
         hipLaunchKernelGGL(ipr_k_swap, dim3(vblocks < gblocks ? vblocks : gblocks), dim3(IPR_BLOCK), 0, st, d);

@@ -46,10 +46,14 @@ APPS = [
     ("cc_spread", "cc.c", ["-DIP_USE_SPREAD", "-DIP_COMBINER_MIN"]),
     ("cc", "cc.c", ["-DIP_COMBINER_MIN"]),
     ("cc_single_broadcast", "cc.c", ["-DIP_USE_SINGLE_BROADCAST", "-DIP_COMBINER_MIN"]),
+    ("cc_single_broadcast_spread", "cc.c",
+     ["-DIP_USE_SINGLE_BROADCAST", "-DIP_USE_SPREAD", "-DIP_COMBINER_MIN"]),
     ("cc_generic_spread", "cc.c", ["-DIP_USE_SPREAD"]),
     ("sssp_spread", "sssp.c", ["-DIP_USE_SPREAD", "-DIP_COMBINER_MIN"]),
     ("sssp", "sssp.c", ["-DIP_COMBINER_MIN"]),
     ("sssp_single_broadcast", "sssp.c", ["-DIP_USE_SINGLE_BROADCAST", "-DIP_COMBINER_MIN"]),
+    ("sssp_single_broadcast_spread", "sssp.c",
+     ["-DIP_USE_SINGLE_BROADCAST", "-DIP_USE_SPREAD", "-DIP_COMBINER_MIN"]),
     ("sssp_weighted_spread", "sssp_weighted.c", ["-DIP_USE_SPREAD", "-DIP_COMBINER_MIN"]),
     ("sssp_weighted", "sssp_weighted.c", ["-DIP_COMBINER_MIN"]),
 ]

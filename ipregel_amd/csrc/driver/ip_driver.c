@@ -218,20 +218,25 @@ static void dev_setup(void)
     d->msg = dalloc(vv * a->msg_size, "messages");
     d->act = dalloc(vv, "active");
     d->hm = dalloc(vv, "has_message");
-    int pull = a->variant == IPR_VARIANT_PULL || a->variant == IPR_VARIANT_SPREAD_PULL;
+    /* PULL: broadcast slots by parity. PUSH / SPREAD: mailboxes, received
+     * flags, broadcast records. SPREAD_PULL: target flags and broadcast
+     * records (the fetch recognises this superstep's records by epoch). */
+    const int var = a->variant;
+    const int pull = var == IPR_VARIANT_PULL;
+    const int records = !pull;
     for (int i = 0; i < 2; i++) {
-        d->bcm[i] = dalloc(vv * a->msg_size, "broadcast messages");
         if (pull) {
+            d->bcm[i] = dalloc(vv * a->msg_size, "broadcast messages");
             d->bcf[i] = dalloc(vv, "broadcast flags");
         } else {
-            d->mbox[i] = dalloc(vv * a->mbox_size, "mailboxes");
+            if (var != IPR_VARIANT_SPREAD_PULL) d->mbox[i] = dalloc(vv * a->mbox_size, "mailboxes");
             d->recv[i] = dalloc(((vv + 15) / 16) * 16 + 16, "received flags");
             hip_check(ipr_hip_fill(d->recv[i], 0, ((vv + 15) / 16) * 16 + 16, st), "flags init");
         }
-        if (a->variant == IPR_VARIANT_SPREAD || a->variant == IPR_VARIANT_SPREAD_PULL)
+        if (var == IPR_VARIANT_SPREAD || var == IPR_VARIANT_SPREAD_PULL)
             d->front[i] = dalloc(vv * 4, "frontier");
     }
-    if (!pull) {
+    if (records) {
         d->lbins = dalloc(vv * 4, "large broadcasters");
         d->bslot = dalloc(vv * a->bslot_size, "broadcast records");
         d->lbase = dalloc(vv * 8, "large-bin chunk bases");
@@ -254,7 +259,7 @@ static void dev_setup(void)
     uint32_t po = c && c->pull_ordered ? (uint32_t)c->pull_ordered : env_u32("IPR_PULL_ORDERED", 1);
     d->pull_ordered = po == 1;
     uint32_t dopt = c && c->dir_opt ? (uint32_t)c->dir_opt : env_u32("IPR_DIR_OPT", 1);
-    d->dir_opt = dopt == 1 && a->can_gather;
+    d->dir_opt = dopt == 1 && a->can_gather && (var == IPR_VARIANT_PUSH || var == IPR_VARIANT_SPREAD);
     d->dir_alpha = c && c->dir_alpha ? c->dir_alpha : env_u32("IPR_DIR_ALPHA", 6);
     d->large_chunk = c && c->bin_small_max ? c->bin_small_max : env_u32("IPR_LARGE_CHUNK", 2048);
     if (d->large_chunk < 64) d->large_chunk = 64;
@@ -508,6 +513,18 @@ static void enqueue_superstep(void)
         kev_mark(0);
         launch_app(IPR_K_PULL_STEP);
         kev_mark(1);
+        break;
+    case IPR_VARIANT_SPREAD_PULL:
+        /* src/combiner_spread_single_broadcast_postamble.h ip_run: compute on
+         * the targets (broadcasts mark out-neighbours), target filtering,
+         * fetch for the targets; broadcast flags are epoch-stamped, so the
+         * reset phase has nothing to do */
+        launch_app(IPR_K_COMPUTE);
+        kev_mark(0);
+        launch_app(IPR_K_EXPAND);
+        kev_mark(1);
+        hip_check(ipr_hip_compact_frontier(&D.dev, D.stream), "compact");
+        launch_app(IPR_K_FETCH);
         break;
     case IPR_VARIANT_SPREAD:
     case IPR_VARIANT_PUSH:

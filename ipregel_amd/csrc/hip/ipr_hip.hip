@@ -348,50 +348,63 @@ extern "C" int ipr_hip_build_pull_bins(ipr_dev* dev, uint32_t small_max, uint32_
 /* ------------------------------------------------------------------------ */
 
 /* The received flags of superstep s+1 become the frontier list: each thread
- * owns 4 flag bytes (and clears them for reuse two supersteps on), a
- * wavefront prefix sum of their counts places the ids, one atomic per
- * wavefront reserves its run, so ids ascend within each 256-vertex run. */
-__global__ void k_compact(ipr_dev d)
+ * owns 16 flag bytes (and clears them for reuse two supersteps on), a
+ * workgroup prefix sum of their counts places the ids and one atomic per
+ * workgroup reserves its run, so ids ascend within each 4096-vertex run. */
+__global__ void __launch_bounds__(256) k_compact(ipr_dev d)
 {
+    __shared__ uint32_t wsum[4];
+    __shared__ unsigned long long blk_base;
     ipr_ctl* c = d.ctl;
     if (c->done) return;
     const uint64_t s = c->superstep;
-    uint32_t* flags = (uint32_t*)d.recv[(s + 1) & 1];
+    uint4* flags = (uint4*)d.recv[(s + 1) & 1];
     uint32_t* out = d.front[(s + 1) & 1];
     unsigned long long* counter = (unsigned long long*)&c->fsize[(s + 1) & 1];
-    const uint64_t groups = (d.V + 3) / 4;            /* flag arrays are padded to 16 */
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    const uint64_t Gr = (groups + 63) & ~63ull;
-    const uint32_t lane = threadIdx.x & 63;
-    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < Gr; g += stride) {
-        uint32_t w = 0;
+    const uint64_t groups = (d.V + 15) / 16;           /* flag arrays are padded to 16 */
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (uint64_t g0 = (uint64_t)blockIdx.x * 256; g0 < groups; g0 += (uint64_t)gridDim.x * 256) {
+        const uint64_t g = g0 + threadIdx.x;
+        uint4 q = make_uint4(0, 0, 0, 0);
         if (g < groups) {
-            w = flags[g];
-            if (w) flags[g] = 0;
+            q = flags[g];
+            if (q.x | q.y | q.z | q.w) flags[g] = make_uint4(0, 0, 0, 0);
         }
-        uint32_t n = __popc(w);                        /* flags are 0 or 1: bits 0, 8, 16, 24 */
+        /* flags are 0 or 1, so each set flag is one bit (0, 8, 16 or 24) */
+        const uint32_t n = __popc(q.x) + __popc(q.y) + __popc(q.z) + __popc(q.w);
         uint32_t incl = n;
         for (int o = 1; o < 64; o <<= 1) {
             uint32_t y = __shfl_up(incl, o);
             if (lane >= (uint32_t)o) incl += y;
         }
-        uint32_t total = __shfl(incl, 63);
-        if (total == 0) continue;
-        unsigned long long base = 0;
-        if (lane == 0) base = atomicAdd(counter, (unsigned long long)total);
-        base = __shfl(base, 0);
-        uint64_t pos = base + incl - n;
-        while (w) {
-            uint32_t bit = __ffs(w) - 1;
-            out[pos++] = (uint32_t)(g * 4 + bit / 8);
-            w &= w - 1;
+        if (lane == 63) wsum[wave] = incl;
+        __syncthreads();
+        uint32_t before = 0, total = 0;
+        for (uint32_t w = 0; w < 4; w++) {
+            if (w < wave) before += wsum[w];
+            total += wsum[w];
         }
+        if (threadIdx.x == 0 && total) blk_base = atomicAdd(counter, (unsigned long long)total);
+        __syncthreads();
+        if (total) {
+            uint64_t pos = blk_base + before + incl - n;
+            const uint32_t words[4] = {q.x, q.y, q.z, q.w};
+            for (int k = 0; k < 4; k++) {
+                uint32_t w = words[k];
+                while (w) {
+                    uint32_t bit = __ffs(w) - 1;
+                    out[pos++] = (uint32_t)(g * 16 + k * 4 + bit / 8);
+                    w &= w - 1;
+                }
+            }
+        }
+        __syncthreads();
     }
 }
 
 extern "C" int ipr_hip_compact_frontier(const ipr_dev* dev, void* stream)
 {
-    uint64_t blocks = ((dev->V + 3) / 4 + 255) / 256;
+    uint64_t blocks = ((dev->V + 15) / 16 + 255) / 256;
     if (blocks < 1) blocks = 1;
     if (blocks > 16384) blocks = 16384;
     hipLaunchKernelGGL(k_compact, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, *dev);

@@ -23,8 +23,6 @@ def ours(binname):
 
 def engine_name_for(binname):
     name = ours(binname)
-    if name.endswith("single_broadcast_spread"):
-        pytest.skip("spread + single_broadcast variant not built yet")
     assert name in BUILT, f"{name} missing from {sorted(BUILT)}"
     return name
 
@@ -65,7 +63,8 @@ def test_pagerank_scale16_config():
     assert r2.work_edges == ref.work_edges == 10 * g.E
 
 
-@pytest.mark.parametrize("name", ["cc_spread", "cc", "cc_single_broadcast", "cc_generic_spread"])
+@pytest.mark.parametrize("name", ["cc_spread", "cc", "cc_single_broadcast", "cc_generic_spread",
+                                  "cc_single_broadcast_spread"])
 def test_cc_medium(name):
     g = G.rmat(15, 16, seed=2, symmetrise=True)
     ref = O.run(g, O.CC, O.VARIANTS[name[2:].replace("_generic", "")])
@@ -73,11 +72,12 @@ def test_cc_medium(name):
     assert np.array_equal(r.values, ref.values)
     assert np.array_equal(r.values, O.min_ancestor(g))
     assert list(r.active) == list(ref.active[: ref.supersteps])
-    if name != "cc_single_broadcast":
+    if name in ("cc_spread", "cc", "cc_generic_spread"):
         assert r.work_edges == ref.work_edges
 
 
-@pytest.mark.parametrize("name", ["sssp_spread", "sssp", "sssp_single_broadcast"])
+@pytest.mark.parametrize("name", ["sssp_spread", "sssp", "sssp_single_broadcast",
+                                  "sssp_single_broadcast_spread"])
 def test_sssp_medium(name):
     g = G.rmat(15, 16, seed=3)
     src = g.max_degree_vertex()
