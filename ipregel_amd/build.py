@@ -114,6 +114,10 @@ def build_all(jobs: int | None = None, verbose: bool = True) -> None:
             f.result()
             if verbose:
                 print(f"built {futs[f]}_32", flush=True)
+    # the reference's own applications against this engine (needs the
+    # reference tree; the GPU box uses what was built here)
+    from . import dropin
+    dropin.build()
     # the oracle is test infrastructure with its own recipe
     sys.path.insert(0, str(ROOT))
     from oracle import build as oracle_build  # noqa: E402
