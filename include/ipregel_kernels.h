@@ -572,7 +572,28 @@ __device__ inline bool ipr_gather_range(const ipr_dev& d, uint64_t s, uint64_t f
     const ipr_bslot_t* bs = (const ipr_bslot_t*)d.bslot;
     const uint32_t want = (uint32_t)(s + 1);
     bool has = false;
-    for (uint64_t k = first; k < end; k += step) {
+    uint64_t k = first;
+    /* four index -> record chains in flight per lane */
+    for (; k + 3 * step < end; k += 4 * step) {
+        uint32_t u[4];
+        ipr_bslot_t b[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) u[j] = __builtin_nontemporal_load(d.t_adj + k + j * step);
+#pragma unroll
+        for (int j = 0; j < 4; j++) b[j] = bs[u[j]];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            if ((b[j].step & ~IPR_BSTEP_WEIGHTED) == want) {
+                IP_MESSAGE_TYPE m = b[j].msg;
+#ifdef IP_NEEDS_OUT_NEIGHBOUR_WEIGHTS
+                if (b[j].step & IPR_BSTEP_WEIGHTED) m = (IP_MESSAGE_TYPE)(m + (IP_MESSAGE_TYPE)d.t_wgt[k + j * step]);
+#endif
+                ipr_policy::fold(acc, m);
+                has = true;
+            }
+        }
+    }
+    for (; k < end; k += step) {
         ipr_bslot_t b = bs[__builtin_nontemporal_load(d.t_adj + k)];
         if ((b.step & ~IPR_BSTEP_WEIGHTED) == want) {
             IP_MESSAGE_TYPE m = b.msg;
@@ -609,40 +630,52 @@ __device__ inline bool ipr_wave_segmented_fold(const uint64_t* idx, uint32_t v, 
     const uint32_t total = __shfl(incl, IPR_WAVE - 1);
     IP_MESSAGE_TYPE acc = ipr_policy::ident();
     bool has = false;
-    for (uint32_t t0 = 0; t0 < total; t0 += IPR_WAVE) {
-        const uint32_t t = t0 + lane;
-        uint32_t lo = 0;
-        for (uint32_t step = 32; step > 0; step >>= 1) {
-            uint32_t probe = __shfl(excl, (lo + step) & (IPR_WAVE - 1));
-            if (lo + step < IPR_WAVE && probe <= t) lo += step;
-        }
-        const uint64_t o_beg = __shfl(beg, lo);
-        const uint32_t o_ex = __shfl(excl, lo);
-        IP_MESSAGE_TYPE m = ipr_policy::ident();
-        int h = 0;
-        uint32_t owner = IPR_WAVE + lane;          /* matches no other lane */
-        if (t < total) {
-            owner = lo;
-            h = load(o_beg + (t - o_ex), &m) ? 1 : 0;
-            if (!h) m = ipr_policy::ident();
-        }
-        for (int o = 1; o < IPR_WAVE; o <<= 1) {
-            IP_MESSAGE_TYPE y = __shfl_up(m, o);
-            int yh = __shfl_up(h, o);
-            uint32_t yo = __shfl_up(owner, o);
-            if (lane >= (uint32_t)o && yo == owner) {
-                ipr_policy::fold(&m, y);
-                h |= yh;
+    constexpr int U = 4;                       /* chunks of 64 arcs in flight */
+    for (uint32_t t00 = 0; t00 < total; t00 += U * IPR_WAVE) {
+        IP_MESSAGE_TYPE m[U];
+        int h[U];
+        uint32_t owner[U];
+#pragma unroll
+        for (int c = 0; c < U; c++) {
+            const uint32_t t = t00 + c * IPR_WAVE + lane;
+            uint32_t lo = 0;
+            for (uint32_t step = 32; step > 0; step >>= 1) {
+                uint32_t probe = __shfl(excl, (lo + step) & (IPR_WAVE - 1));
+                if (lo + step < IPR_WAVE && probe <= t) lo += step;
+            }
+            const uint64_t o_beg = __shfl(beg, lo);
+            const uint32_t o_ex = __shfl(excl, lo);
+            m[c] = ipr_policy::ident();
+            h[c] = 0;
+            owner[c] = IPR_WAVE + lane;        /* matches no other lane */
+            if (t < total) {
+                owner[c] = lo;
+                h[c] = load(o_beg + (t - o_ex), &m[c]) ? 1 : 0;
+                if (!h[c]) m[c] = ipr_policy::ident();
             }
         }
-        const uint32_t ss = excl > t0 ? excl : t0;
-        const uint32_t se = excl + deg < t0 + IPR_WAVE ? excl + deg : t0 + IPR_WAVE;
-        const uint32_t src = ss < se ? se - 1 - t0 : lane;
-        IP_MESSAGE_TYPE y = __shfl(m, src);
-        int yh = __shfl(h, src);
-        if (ss < se) {
-            ipr_policy::fold(&acc, y);
-            has |= yh != 0;
+#pragma unroll
+        for (int c = 0; c < U; c++) {
+            const uint32_t t0 = t00 + c * IPR_WAVE;
+            if (t0 >= total) break;
+            for (int o = 1; o < IPR_WAVE; o <<= 1) {
+                IP_MESSAGE_TYPE y = __shfl_up(m[c], o);
+                int yh = __shfl_up(h[c], o);
+                uint32_t yo = __shfl_up(owner[c], o);
+                if (lane >= (uint32_t)o && yo == owner[c]) {
+                    ipr_policy::fold(&m[c], y);
+                    h[c] |= yh;
+                }
+            }
+            const uint32_t ss = excl > t0 ? excl : t0;
+            const uint32_t se = excl + deg < t0 + IPR_WAVE ? excl + deg : t0 + IPR_WAVE;
+            const uint32_t src = ss < se ? se - 1 - t0 : lane;
+            IP_MESSAGE_TYPE y = __shfl(m[c], src);
+            int yh = __shfl(h[c], src);
+            if (ss < se) {
+                ipr_policy::fold(&acc, y);
+                has |= yh != 0;
+            }
         }
     }
     *out = acc;
