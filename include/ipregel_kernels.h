@@ -41,6 +41,9 @@
 #define IPR_BLOCK 256
 #define IPR_WAVE 64
 #define IPR_WAVES (IPR_BLOCK / IPR_WAVE)
+#ifndef IPR_GATHER_UNROLL
+#define IPR_GATHER_UNROLL 4   /* gathers in flight per lane (memory-level parallelism) */
+#endif
 
 /* device copy of the engine state, read by the device-side API functions */
 __device__ ipr_dev ipr_d_dev;
@@ -573,16 +576,17 @@ __device__ inline bool ipr_gather_range(const ipr_dev& d, uint64_t s, uint64_t f
     const uint32_t want = (uint32_t)(s + 1);
     bool has = false;
     uint64_t k = first;
-    /* four index -> record chains in flight per lane */
-    for (; k + 3 * step < end; k += 4 * step) {
-        uint32_t u[4];
-        ipr_bslot_t b[4];
+    /* IPR_GATHER_UNROLL index -> record chains in flight per lane */
+    constexpr int U = IPR_GATHER_UNROLL;
+    for (; k + (U - 1) * step < end; k += U * step) {
+        uint32_t u[U];
+        ipr_bslot_t b[U];
 #pragma unroll
-        for (int j = 0; j < 4; j++) u[j] = __builtin_nontemporal_load(d.t_adj + k + j * step);
+        for (int j = 0; j < U; j++) u[j] = __builtin_nontemporal_load(d.t_adj + k + j * step);
 #pragma unroll
-        for (int j = 0; j < 4; j++) b[j] = bs[u[j]];
+        for (int j = 0; j < U; j++) b[j] = bs[u[j]];
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
+        for (int j = 0; j < U; j++) {
             if ((b[j].step & ~IPR_BSTEP_WEIGHTED) == want) {
                 IP_MESSAGE_TYPE m = b[j].msg;
 #ifdef IP_NEEDS_OUT_NEIGHBOUR_WEIGHTS
@@ -630,7 +634,7 @@ __device__ inline bool ipr_wave_segmented_fold(const uint64_t* idx, uint32_t v, 
     const uint32_t total = __shfl(incl, IPR_WAVE - 1);
     IP_MESSAGE_TYPE acc = ipr_policy::ident();
     bool has = false;
-    constexpr int U = 4;                       /* chunks of 64 arcs in flight */
+    constexpr int U = IPR_GATHER_UNROLL;       /* chunks of 64 arcs in flight */
     for (uint32_t t00 = 0; t00 < total; t00 += U * IPR_WAVE) {
         IP_MESSAGE_TYPE m[U];
         int h[U];
