@@ -443,6 +443,43 @@ int ipr_oracle_dump(int app, uint64_t V, const void* values, const char* path)
     return 0;
 }
 
+/* ---------------------------------------------------- personalised PR ---- */
+
+int ipr_oracle_ppr(const ipr_csr* g, uint32_t seed, uint32_t iters, double* x)
+{
+    uint64_t V = g->V;
+    if (seed >= V) return -EINVAL;
+    ipr_csr t;
+    int rc = ipr_graph_transpose(g, &t);
+    if (rc) return rc;
+    double* y = malloc(sizeof(double) * V);
+    double* inv = malloc(sizeof(double) * V);
+    if (!y || !inv) { free(y); free(inv); ipr_graph_free(&t); return -ENOMEM; }
+    for (uint64_t v = 0; v < V; v++) {
+        uint64_t d = g->idx[v + 1] - g->idx[v];
+        inv[v] = d ? 1.0 / (double)d : 0.0;
+        x[v] = v == seed ? 1.0 : 0.0;
+    }
+    for (uint32_t it = 0; it < iters; it++) {
+        double dangling = 0.0;
+        for (uint64_t v = 0; v < V; v++)
+            if (g->idx[v + 1] == g->idx[v]) dangling += x[v];
+        double total = 0.0;
+        for (uint64_t v = 0; v < V; v++) {
+            double s = 0.0;
+            for (uint64_t k = t.idx[v]; k < t.idx[v + 1]; k++) s += x[t.adj[k]] * inv[t.adj[k]];
+            double r = 0.85 * s;
+            if (v == seed) r += 0.85 * dangling + 0.15;
+            y[v] = r;
+            total += r;
+        }
+        for (uint64_t v = 0; v < V; v++) x[v] = y[v] / total;
+    }
+    free(y); free(inv);
+    ipr_graph_free(&t);
+    return 0;
+}
+
 /* ------------------------------------------------------- closed forms ---- */
 
 int ipr_oracle_min_ancestor(const ipr_csr* g, uint32_t* label)

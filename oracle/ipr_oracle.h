@@ -60,6 +60,14 @@ int ipr_oracle_run(const ipr_csr* g, int directed, int variant, int app,
 /* Write values in ip_dump's text format (benchmarks/*.c ip_serialise_vertex). */
 int ipr_oracle_dump(int app, uint64_t V, const void* values, const char* path);
 
+/* Personalised PageRank (project definition, no reference counterpart), FP64:
+ *   x_0 = e_seed;  y = 0.85 * (P^T x_t + D_t e_seed) + 0.15 e_seed,
+ *   (P^T x)(v) = sum over in-neighbours u of x(u) / outdeg(u),
+ *   D_t = sum of x_t over vertices without out-arcs,  x_{t+1} = y / sum(y).
+ * g is the out-edge CSR; in-sums run over the exact transpose in ascending
+ * source order. `iters` applications; x gets V doubles. */
+int ipr_oracle_ppr(const ipr_csr* g, uint32_t seed, uint32_t iters, double* x);
+
 /* Closed forms. */
 /* label[v] = min{u : u reaches v}; on a symmetric graph the component minimum. */
 int ipr_oracle_min_ancestor(const ipr_csr* g, uint32_t* label);

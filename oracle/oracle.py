@@ -48,6 +48,7 @@ def lib() -> C.CDLL:
         L.ipr_oracle_dump.argtypes = [C.c_int, C.c_uint64, C.c_void_p, C.c_char_p]
         for f in ("ipr_oracle_min_ancestor",):
             getattr(L, f).argtypes = [C.POINTER(_Csr), C.c_void_p]
+        L.ipr_oracle_ppr.argtypes = [C.POINTER(_Csr), C.c_uint32, C.c_uint32, C.c_void_p]
         for f in ("ipr_oracle_bfs", "ipr_oracle_dijkstra"):
             getattr(L, f).argtypes = [C.POINTER(_Csr), C.c_uint32, C.c_void_p]
         _lib = L
@@ -117,4 +118,13 @@ def dijkstra(g, source: int) -> np.ndarray:
     rc = lib().ipr_oracle_dijkstra(C.byref(c), source, out.ctypes.data)
     if rc:
         raise RuntimeError(f"dijkstra failed ({rc})")
+    return out
+
+
+def ppr(g, seed: int, iters: int = 20) -> np.ndarray:
+    c, keep = _csr(g)
+    out = np.zeros(g.V, np.float64)
+    rc = lib().ipr_oracle_ppr(C.byref(c), seed, iters, out.ctypes.data)
+    if rc:
+        raise RuntimeError(f"ppr failed ({rc})")
     return out
