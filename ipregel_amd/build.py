@@ -3,6 +3,7 @@
 Produces, in-tree (git-ignored, shipped with the tree):
   ipregel_amd/lib/libipregel_graph.so   graph generators and file triple (C, OpenMP)
   ipregel_amd/lib/libipregel_hip.so     thin HIP C-ABI + type-independent kernels
+  ipregel_amd/lib/libipregel_batch.so   resident graphs, batched queries, serving loop
   ipregel_amd/lib/ip_driver.o           C host driver (linked into every app)
   ipregel_amd/lib/libipr_<app>_32.so    app TU + driver, loadable over ctypes
   ipregel_amd/bin/<app>_32              the same app as a command-line program
@@ -77,6 +78,19 @@ def build_hip_lib() -> Path:
     return out
 
 
+def build_batch_lib() -> Path:
+    """libipregel_batch.so: resident graphs, scalar and batched query kernels,
+    the serving loop. -ffp-contract=off keeps the compensated FP32 sums and the
+    FP64 reductions free of fused multiply-adds, so the scalar and batched
+    kernels round identically."""
+    out = LIB / "libipregel_batch.so"
+    src = PKG / "csrc" / "batch"
+    _run([HIPCC, *HIP_FLAGS, "-ffp-contract=off", "-shared", f"-I{INC}", f"-I{src}",
+          str(src / "ipr_batch.hip"), str(src / "ipr_serve.cpp"),
+          f"-L{LIB}", "-lipregel_hip", "-Wl,-rpath,$ORIGIN", "-o", str(out)])
+    return out
+
+
 def build_driver() -> Path:
     out = OBJ / "ip_driver.o"
     _run(["gcc", *C_FLAGS, f"-I{INC}", "-c", str(PKG / "csrc/driver/ip_driver.c"), "-o", str(out)])
@@ -109,11 +123,13 @@ def build_all(jobs: int | None = None, verbose: bool = True) -> None:
         drv = f_drv.result()
         f_hip.result()
         f_graph.result()
+        f_batch = ex.submit(build_batch_lib)
         futs = {ex.submit(build_app, n, s, d, drv): n for (n, s, d) in APPS}
         for f in cf.as_completed(futs):
             f.result()
             if verbose:
                 print(f"built {futs[f]}_32", flush=True)
+        f_batch.result()
     # the reference's own applications against this engine (needs the
     # reference tree; the GPU box uses what was built here)
     from . import dropin

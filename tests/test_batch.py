@@ -1,0 +1,227 @@
+"""GPU parity of the batched query kernels and the serving loop
+(libipregel_batch.so through its C-ABI).
+
+SSSP distances and CC labels are bit-exact against the oracle's Dijkstra / BFS
+/ min-ancestor closed forms and against the one-query kernels. Personalised
+PageRank in FP32 storage (compensated pull sum, FP64 sums) is compared with
+the FP64 oracle: L1 error <= 2e-6, max relative error <= 1e-4 on entries above
+1e-12, and at least 95 of the top 100 vertices shared. Scalar and batched PPR
+results, and a query alone versus in a full batch, are bit-identical. The
+serving loop's results and digests are identical in sequential and batched
+modes."""
+import numpy as np
+import pytest
+
+from ipregel_amd import batch as BA
+from ipregel_amd import graph as G
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+PPR_L1 = 2e-6
+PPR_MAXREL = 1e-4
+PPR_TOP = 95
+
+
+def hubby(scale=13, seed=2):
+    """R-MAT plus a star through vertex 7 (in- and out-degree V-1): the star's
+    in-list is folded in hub chunks, its out-list relaxed in large chunks."""
+    g = G.rmat(scale, 8, seed=seed)
+    V = g.V
+    src = np.repeat(np.arange(V, dtype=np.uint32), np.diff(g.idx).astype(np.int64))
+    others = np.setdiff1d(np.arange(V, dtype=np.uint32), [7])
+    s = np.concatenate([src, others, np.full(others.size, 7, np.uint32)])
+    d = np.concatenate([g.adj, np.full(others.size, 7, np.uint32), others])
+    return G.from_arcs(V, s, d).with_weights(9)
+
+
+@pytest.fixture(scope="module")
+def graphs():
+    gd = G.rmat(12, 16, seed=2).with_weights(4)        # directed, weights in [1, 255]
+    gs = G.rmat(11, 16, seed=2, symmetrise=True)       # symmetric, for CC
+    gg = G.grid(40, 50).with_weights(3)
+    gh = hubby()
+    out = {"dir": (gd, BA.ResidentGraph(gd)), "sym": (gs, BA.ResidentGraph(gs)),
+           "grid": (gg, BA.ResidentGraph(gg)), "hub": (gh, BA.ResidentGraph(gh))}
+    yield out
+    for _, r in out.values():
+        r.close()
+
+
+def sources(V, n, seed):
+    return np.random.default_rng(seed).integers(0, V, size=n).astype(np.uint32)
+
+
+@pytest.mark.parametrize("name", ["dir", "grid", "hub"])
+def test_batched_sssp_vs_dijkstra(graphs, name):
+    g, r = graphs[name]
+    src = sources(g.V, 64, 4)
+    src[1] = src[0]                      # a repeated source in one batch
+    print(f"batched sssp {name} V={g.V} E={g.E}", flush=True)
+    dist, st = r.batch_sssp(src)
+    assert st["supersteps"] > 1 and st["relaxations"] > 0
+    for q in range(64):
+        ref = O.dijkstra(g, int(src[q]))
+        assert np.array_equal(dist[q], ref), (name, q)
+
+
+def test_batched_sssp_unit_weights_vs_bfs(graphs):
+    g, _ = graphs["sym"]
+    r = BA.ResidentGraph(G.Graph(g.idx, g.adj, None))
+    src = sources(g.V, 17, 8)
+    dist, _ = r.batch_sssp(src)
+    for q in range(src.size):
+        assert np.array_equal(dist[q], O.bfs(g, int(src[q])))
+    r.close()
+
+
+@pytest.mark.parametrize("name", ["dir", "grid", "hub"])
+def test_single_sssp_matches_batched(graphs, name):
+    g, r = graphs[name]
+    src = sources(g.V, 6, 11)
+    dist, _ = r.batch_sssp(src)
+    for q, s in enumerate(src):
+        one, st = r.sssp(int(s))
+        assert np.array_equal(one, dist[q])
+        assert st["supersteps"] >= 1
+
+
+def test_sssp_alone_equals_in_mix(graphs):
+    g, r = graphs["dir"]
+    mix = sources(g.V, 64, 21)
+    full, _ = r.batch_sssp(mix)
+    alone, _ = r.batch_sssp(mix[37:38])
+    assert np.array_equal(alone[0], full[37])
+
+
+def test_sssp_from_sink(graphs):
+    g, r = graphs["dir"]
+    deg = np.diff(g.idx)
+    sink = int(np.flatnonzero(deg == 0)[0])
+    d, _ = r.batch_sssp(np.array([sink], np.uint32))
+    assert d[0][sink] == 0 and np.count_nonzero(d[0] != BA.INF) == 1
+
+
+@pytest.mark.parametrize("name", ["sym", "grid", "hub"])
+def test_cc_vs_min_ancestor(graphs, name):
+    g, r = graphs[name]
+    lab, st = r.cc()
+    assert np.array_equal(lab, O.min_ancestor(g))
+
+
+@pytest.mark.parametrize("name", ["dir", "hub"])
+def test_batched_ppr_vs_fp64_oracle(graphs, name):
+    g, r = graphs[name]
+    seeds = sources(g.V, 64, 4)
+    deg = np.diff(g.idx)
+    seeds[2] = int(np.flatnonzero(deg == 0)[0]) if (deg == 0).any() else seeds[2]   # a dangling seed
+    print(f"batched ppr {name} V={g.V}", flush=True)
+    x, res, st = r.batch_ppr(seeds, 20)
+    worst = {"l1": 0.0, "max_rel": 0.0, "top_overlap": 100}
+    for q in range(64):
+        ref = O.ppr(g, int(seeds[q]), 20)
+        m = BA.ppr_agreement(x[q], ref)
+        worst = {"l1": max(worst["l1"], m["l1"]), "max_rel": max(worst["max_rel"], m["max_rel"]),
+                 "top_overlap": min(worst["top_overlap"], m["top_overlap"])}
+        assert abs(float(x[q].astype(np.float64).sum()) - 1.0) < 1e-5
+        assert res[q] >= 0.0
+    print(f"ppr {name} worst over 64 queries: {worst}", flush=True)
+    assert worst["l1"] <= PPR_L1
+    assert worst["max_rel"] <= PPR_MAXREL
+    assert worst["top_overlap"] >= PPR_TOP
+
+
+@pytest.mark.parametrize("name", ["dir", "hub", "grid"])
+def test_single_ppr_bit_identical_to_batched(graphs, name):
+    g, r = graphs[name]
+    seeds = sources(g.V, 64, 31)
+    x, res, _ = r.batch_ppr(seeds, 20)
+    for q in (0, 5, 63):
+        one, r1, _ = r.ppr(int(seeds[q]), 20)
+        assert np.array_equal(one.view(np.uint32), x[q].view(np.uint32)), (name, q)
+        assert r1 == res[q]
+
+
+def test_ppr_alone_equals_in_mix(graphs):
+    g, r = graphs["hub"]
+    seeds = sources(g.V, 64, 41)
+    full, res, _ = r.batch_ppr(seeds, 20)
+    alone, res1, _ = r.batch_ppr(seeds[9:10], 20)
+    assert np.array_equal(alone[0].view(np.uint32), full[9].view(np.uint32))
+    assert res1[0] == res[9]
+
+
+def test_ppr_one_iteration_closed_form(graphs):
+    g, r = graphs["dir"]
+    deg = np.diff(g.idx)
+    s = int(np.argmax(deg))
+    x, res, _ = r.batch_ppr(np.array([s], np.uint32), 1)
+    want = np.zeros(g.V)
+    want[s] = 0.15
+    want[g.adj[g.idx[s]:g.idx[s + 1]]] += 0.85 / deg[s]
+    assert np.allclose(x[0], want, rtol=1e-6, atol=1e-9)
+    assert abs(res[0] - np.abs(want - np.eye(1, g.V, s)[0]).sum()) < 1e-6
+
+
+def test_ppr_residual_shrinks(graphs):
+    g, r = graphs["dir"]
+    s = sources(g.V, 4, 3)
+    deg = np.diff(g.idx)
+    s[0] = int(np.flatnonzero(deg == 0)[0])          # a seed without out-arcs is a fixed point
+    _, r5, _ = r.batch_ppr(s, 5)
+    _, r30, _ = r.batch_ppr(s, 30)
+    assert r5[0] == 0.0 and r30[0] == 0.0
+    moving = r5 > 0
+    assert moving.sum() >= 2 and np.all(r30[moving] < r5[moving])
+
+
+def test_serve_identical_in_both_modes(graphs):
+    names = ["dir", "sym", "grid", "hub"]
+    gl = [graphs[n][0] for n in names]
+    rl = [graphs[n][1] for n in names]
+    V = [g.V for g in gl]
+    trace = BA.poisson_trace(90, 400.0, {BA.SSSP: 0.4, BA.PPR: 0.4, BA.CC: 0.2},
+                             {BA.SSSP: [0, 2, 3], BA.PPR: [0, 3], BA.CC: [1, 2]}, V, seed=5)
+    print("serve batched", flush=True)
+    rb, vb = BA.serve(rl, trace, results=True)
+    print("serve sequential", flush=True)
+    rs, vs = BA.serve(rl, trace, sequential=True, results=True)
+    assert np.array_equal(rb.digest, rs.digest)
+    cc_ref = {gi: O.min_ancestor(gl[gi]) for gi in (1, 2)}
+    for i in range(trace.size):
+        kind, gi, p = int(trace["kind"][i]), int(trace["graph"][i]), int(trace["param"][i])
+        assert np.array_equal(vb[i].view(np.uint32), vs[i].view(np.uint32)), i
+        assert BA.digest(vb[i]) == int(rb.digest[i]), i
+        if kind == BA.SSSP:
+            assert np.array_equal(vb[i], O.dijkstra(gl[gi], p)), i
+        elif kind == BA.CC:
+            assert np.array_equal(vb[i], cc_ref[gi]), i
+    for r in (rb, rs):
+        assert np.all(r.t_done >= r.t_admit) and np.all(r.t_admit >= r.t_arrival - 1e-6)
+    s = rb.summary()
+    print(f"batched {s}\nsequential {rs.summary()}", flush=True)
+    assert s["peak_slots"] >= 1 and s["ticks"] > 0
+
+
+def test_serve_under_hbm_budget(graphs):
+    """A budget that holds one 64-slot state at a time: batchers wait and idle
+    states are evicted; results do not change."""
+    names = ["dir", "hub"]
+    rl = [graphs[n][1] for n in names]
+    V = [graphs[n][0].V for n in names]
+    budget = max(max(r.slot_bytes(BA.SSSP), r.slot_bytes(BA.PPR)) for r in rl) + 1
+    trace = BA.poisson_trace(40, 2000.0, {BA.SSSP: 0.5, BA.PPR: 0.5},
+                             {BA.SSSP: [0, 1], BA.PPR: [0, 1]}, V, seed=6)
+    rb, _ = BA.serve(rl, trace, hbm_budget=budget)
+    rs, _ = BA.serve(rl, trace, sequential=True)
+    assert np.array_equal(rb.digest, rs.digest)
+    assert rb.hbm_peak <= budget
+    assert rb.admitted_waits > 0
+
+
+def test_serve_rejects_bad_query(graphs):
+    _, r = graphs["dir"]
+    trace = BA.poisson_trace(3, 100.0, {BA.SSSP: 1.0}, {BA.SSSP: [0]}, [r.V], seed=1)
+    trace["param"][1] = r.V
+    with pytest.raises(RuntimeError, match="malformed"):
+        BA.serve([r], trace)

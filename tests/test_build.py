@@ -48,3 +48,13 @@ def test_all_apps_built():
     for name, _, _ in APPS:
         assert (LIB / f"libipr_{name}_32.so").exists(), name
         assert (ROOT / "ipregel_amd" / "bin" / f"{name}_32").exists(), name
+
+
+def test_batch_lib_exports():
+    L = C.CDLL(str(LIB / "libipregel_batch.so"))
+    names = declared("ipregel_batch.h")
+    assert "ipr_serve" in names and "ipr_batch_ppr" in names
+    for n in names:
+        assert hasattr(L, n), n
+    from ipregel_amd import batch as BA
+    assert set(BA.EXPORTS) == set(names)
