@@ -659,11 +659,13 @@ __global__ void k_digest_u32(const uint32_t* x, uint64_t n, unsigned long long* 
 /* ------------------------------------------------------------------------ */
 
 struct PprArgs {
-    uint64_t V, nseg;
+    uint64_t V, nseg, nseg64;
+    const uint32_t* seg_v0;
     uint32_t nhub, nchunk, coop;
     const uint64_t* oidx;
     const uint64_t* iidx;
     const uint32_t* iadj;
+    const float* inw;           /* per transpose arc: 1/out-degree of its source (FP32) */
     const float* invdeg;
     const uint32_t* hub_v;
     const uint32_t* hub_c0;
@@ -708,58 +710,66 @@ __device__ __forceinline__ float yval64(const float* yin, uint32_t u, int lane, 
     return yin[(uint64_t)u * 64 + lane];
 }
 
-/* batched: Kahan-add y_t(u)/deg(u) for the arcs [e0, e1) of an in-list, in order */
-__device__ __forceinline__ void fold64(const PprArgs& a, const float* yin, uint64_t e0, uint64_t e1, int lane,
-                                       uint32_t seed, bool fresh, bool live, float& s, float& c)
-{
-    for (uint64_t e = e0; e < e1; e += 64) {
-        const uint32_t cnt = (uint32_t)min<uint64_t>(64, e1 - e);
-        uint32_t ul = 0;
-        float wl = 0.f;
-        if ((uint32_t)lane < cnt) {
-            ul = a.iadj[e + lane];
-            wl = a.invdeg[ul];
-        }
-        uint32_t j = 0;
-        for (; j + 8 <= cnt; j += 8) {
-            float vv[8], ww[8];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                ww[k] = rlf(wl, j + k);
-                vv[k] = yval64(yin, rl(ul, j + k), lane, seed, fresh, live);
-            }
-#pragma unroll
-            for (int k = 0; k < 8; ++k) kahan(s, c, vv[k] * ww[k]);
-        }
-        for (; j < cnt; ++j) {
-            const float w = rlf(wl, j);
-            kahan(s, c, yval64(yin, rl(ul, j), lane, seed, fresh, live) * w);
-        }
+/* Both pulls walk a 64-vertex segment as runs of consecutive non-hub vertices
+ * (their in-lists are contiguous in the transpose) in 64-arc chunks: arc ids
+ * and weights are loaded coalesced, then folded in list order. Before arc a
+ * is folded, every vertex whose list ends at or before a is closed (PPR_CLOSE),
+ * so a vertex's Kahan sum covers exactly its own list, in order. */
+
+/* batched: lane q folds slot q; 16 rows of 256 bytes in flight per wavefront */
+#define PPR_STREAM64(RA, RB, EJ, CLOSE)                                                             \
+    for (uint64_t e_ = (RA); e_ < (RB); e_ += 64) {                                               \
+        const uint32_t cnt_ = (uint32_t)min<uint64_t>(64, (RB) - e_);                              \
+        uint32_t ul_ = 0;                                                                           \
+        float wl_ = 0.f;                                                                            \
+        if ((uint32_t)lane < cnt_) {                                                                \
+            ul_ = a.iadj[e_ + lane];                                                                \
+            wl_ = a.inw[e_ + lane];                                                                 \
+        }                                                                                           \
+        for (uint32_t k_ = 0; k_ < cnt_; k_ += 16) {                                                \
+            uint32_t uu_[16];                                                                       \
+            float vv_[16];                                                                          \
+            _Pragma("unroll") for (int i_ = 0; i_ < 16; ++i_) uu_[i_] = rl(ul_, (k_ + i_) & 63);    \
+            _Pragma("unroll") for (int i_ = 0; i_ < 16; ++i_) vv_[i_] =                             \
+                (k_ + i_ < cnt_) ? yval64(yin, uu_[i_], lane, seed, fresh, live) : 0.f;              \
+            /* products first: the fold then reads registers only, so a row    */                   \
+            /* store when a vertex closes does not stall the next fold step     */                   \
+            _Pragma("unroll") for (int i_ = 0; i_ < 16; ++i_) vv_[i_] *= rlf(wl_, (k_ + i_) & 63);  \
+            if (k_ + 16 <= cnt_ && e_ + k_ + 16 <= (EJ)) {                                          \
+                _Pragma("unroll") for (int i_ = 0; i_ < 16; ++i_) kahan(s, c, vv_[i_]);             \
+            } else {                                                                                \
+                _Pragma("unroll") for (int i_ = 0; i_ < 16; ++i_) {                                 \
+                    if (k_ + i_ < cnt_) {                                                           \
+                        while (e_ + k_ + i_ >= (EJ)) { CLOSE; }                                     \
+                        kahan(s, c, vv_[i_]);                                                       \
+                    }                                                                               \
+                }                                                                                   \
+            }                                                                                       \
+        }                                                                                           \
     }
-}
 
-/* scalar: the same fold for one vertex, loads spread over the wavefront and the
- * Kahan chain run in list order on every lane */
-__device__ __forceinline__ void fold1_coop(const PprArgs& a, const float* yin, uint64_t e0, uint64_t e1, int lane,
-                                           uint32_t seed, bool fresh, float& s, float& c)
-{
-    for (uint64_t e = e0; e < e1; e += 64) {
-        const uint32_t cnt = (uint32_t)min<uint64_t>(64, e1 - e);
-        float p = 0.f;
-        if ((uint32_t)lane < cnt) {
-            const uint32_t u = a.iadj[e + lane];
-            const float w = a.invdeg[u];
-            const float yv = fresh ? (u == seed ? 1.f : 0.f) : yin[u];
-            p = yv * w;
-        }
-        for (uint32_t j = 0; j < cnt; ++j) kahan(s, c, rlf(p, j));
+/* scalar: lanes gather and multiply 64 arcs, every lane runs the Kahan chain */
+#define PPR_STREAM1(RA, RB, EJ, CLOSE)                                                              \
+    for (uint64_t e_ = (RA); e_ < (RB); e_ += 64) {                                               \
+        const uint32_t cnt_ = (uint32_t)min<uint64_t>(64, (RB) - e_);                              \
+        float p_ = 0.f;                                                                             \
+        if ((uint32_t)lane < cnt_) {                                                                \
+            const uint32_t u_ = a.iadj[e_ + lane];                                                  \
+            const float w_ = a.inw[e_ + lane];                                                      \
+            p_ = (fresh ? (u_ == seed ? 1.f : 0.f) : yin[u_]) * w_;                                 \
+        }                                                                                           \
+        if (e_ + cnt_ <= (EJ)) {                                                                    \
+            for (uint32_t i_ = 0; i_ < cnt_; ++i_) kahan(s, c, rlf(p_, i_));                        \
+        } else {                                                                                    \
+            for (uint32_t i_ = 0; i_ < cnt_; ++i_) {                                                \
+                while (e_ + i_ >= (EJ)) { CLOSE; }                                                  \
+                kahan(s, c, rlf(p_, i_));                                                           \
+            }                                                                                       \
+        }                                                                                           \
     }
-}
 
-template <int B>
-__device__ __forceinline__ float* ybuf(const PprArgs& a, int p) { return p ? a.y1 : a.y0; }
-
-/* batched pull: wavefront per 64-vertex segment, vertices in id order, lane q = slot q */
+/* batched pull. Work items: the hub chunks first, then the 64-vertex segments;
+ * y and the segment sums follow id order. */
 __global__ __launch_bounds__(256) void kp_pull_b(PprArgs a, int cur)
 {
     const int lane = threadIdx.x & 63;
@@ -771,23 +781,51 @@ __global__ __launch_bounds__(256) void kp_pull_b(PprArgs a, int cur)
     const double invS = sl->invS[lane], Dn = sl->Dn[lane];
     const float* yin = cur ? a.y1 : a.y0;
     float* yout = cur ? a.y0 : a.y1;
-    for (uint64_t seg = wid; seg < a.nseg; seg += nw) {
-        const uint64_t base = seg * 64, vl = base + lane;
-        const bool in = vl < a.V;
-        const uint64_t E0 = in ? a.iidx[vl] : 0, E1 = in ? a.iidx[vl + 1] : 0;
-        const float dg = in ? a.invdeg[vl] : 1.f;
-        const uint32_t nv = (uint32_t)min<uint64_t>(64, a.V - base);
-        double sS = 0.0, sD = 0.0;
-        for (uint32_t j = 0; j < nv; ++j) {
-            const uint64_t e0 = rl64(E0, j), e1 = rl64(E1, j);
-            if (e1 - e0 > HUB) continue;
+    const uint64_t nitems = (uint64_t)a.nchunk + a.nseg;
+    for (uint64_t it = wid; it < nitems; it += nw) {
+        if (it < a.nchunk) {
+            const uint32_t h = a.chunk_h[it];
+            const uint32_t v = a.hub_v[h];
+            const uint64_t e0 = a.iidx[v] + (uint64_t)(it - a.hub_c0[h]) * HUB;
+            const uint64_t e1 = min<uint64_t>(e0 + HUB, a.iidx[v + 1]);
             float s = 0.f, c = 0.f;
-            fold64(a, yin, e0, e1, lane, seed, fresh, live, s, c);
-            const uint64_t v = base + j;
-            const float y = ppr_y(s - c, invS, Dn, v == seed);
-            if (live) yout[v * 64 + lane] = y;
-            sS += (double)y;
-            if (rlf(dg, j) == 0.f) sD += (double)y;
+            PPR_STREAM64(e0, e1, ~0ull, (void)0)
+            if (live) a.hubpart[it * 64 + lane] = s - c;
+            continue;
+        }
+        const uint64_t seg = it - a.nchunk;
+        const uint64_t base = a.seg_v0[seg];
+        const uint32_t nv = a.seg_v0[seg + 1] - (uint32_t)base;
+        const bool in = (uint32_t)lane < nv;
+        const uint64_t E0 = in ? a.iidx[base + lane] : 0, E1 = in ? a.iidx[base + lane + 1] : 0;
+        const float dg = in ? a.invdeg[base + lane] : 1.f;
+        const unsigned long long hubm = __ballot(in && E1 - E0 > HUB), dangm = __ballot(in && dg == 0.f);
+        double sS = 0.0, sD = 0.0;
+        uint32_t j = 0;
+        while (j < nv) {
+            if ((hubm >> j) & 1ull) {
+                j++;
+                continue;
+            }
+            const unsigned long long rest = hubm >> j;
+            const uint32_t h = rest ? j + (uint32_t)__builtin_ctzll(rest) : nv;
+            const uint64_t ra = rl64(E0, j), rb = rl64(E1, h - 1);
+            float s = 0.f, c = 0.f;
+            uint64_t ej = rl64(E1, j);
+#define CLOSE_B                                                    \
+    {                                                              \
+        const float y_ = ppr_y(s - c, invS, Dn, base + j == seed); \
+        if (live) yout[(base + j) * 64 + lane] = y_;               \
+        sS += (double)y_;                                          \
+        if ((dangm >> j) & 1ull) sD += (double)y_;                 \
+        s = 0.f;                                                   \
+        c = 0.f;                                                   \
+        j++;                                                       \
+        ej = j < h ? rl64(E1, j) : ~0ull;                          \
+    }
+            PPR_STREAM64(ra, rb, ej, CLOSE_B)
+            while (j < h) CLOSE_B
+#undef CLOSE_B
         }
         if (live) {
             a.partS[seg * 64 + lane] = sS;
@@ -796,8 +834,7 @@ __global__ __launch_bounds__(256) void kp_pull_b(PprArgs a, int cur)
     }
 }
 
-/* scalar pull: wavefront per segment, lane = vertex; short lists folded per
- * lane, longer ones by the whole wavefront; segment sums in id order */
+/* scalar pull: same work items and order, lanes over arcs */
 __global__ __launch_bounds__(256) void kp_pull_s(PprArgs a, int cur)
 {
     const int lane = threadIdx.x & 63;
@@ -810,76 +847,58 @@ __global__ __launch_bounds__(256) void kp_pull_s(PprArgs a, int cur)
     const double invS = sl->invS[0], Dn = sl->Dn[0];
     const float* yin = cur ? a.y1 : a.y0;
     float* yout = cur ? a.y0 : a.y1;
-    for (uint64_t seg = wid; seg < a.nseg; seg += nw) {
-        const uint64_t base = seg * 64, v = base + lane;
-        const bool in = v < a.V;
-        const uint64_t e0 = in ? a.iidx[v] : 0, e1 = in ? a.iidx[v + 1] : 0;
-        const uint64_t deg = e1 - e0;
-        const float dg = in ? a.invdeg[v] : 1.f;
-        const bool hub = deg > HUB;
-        float s = 0.f, c = 0.f;
-        if (in && deg <= a.coop) {
-            for (uint64_t e = e0; e < e1; ++e) {
-                const uint32_t u = a.iadj[e];
-                const float w = a.invdeg[u];
-                const float yv = fresh ? (u == seed ? 1.f : 0.f) : yin[u];
-                kahan(s, c, yv * w);
-            }
+    const uint64_t nitems = (uint64_t)a.nchunk + a.nseg;
+    for (uint64_t it = wid; it < nitems; it += nw) {
+        if (it < a.nchunk) {
+            const uint32_t h = a.chunk_h[it];
+            const uint32_t v = a.hub_v[h];
+            const uint64_t e0 = a.iidx[v] + (uint64_t)(it - a.hub_c0[h]) * HUB;
+            const uint64_t e1 = min<uint64_t>(e0 + HUB, a.iidx[v + 1]);
+            float s = 0.f, c = 0.f;
+            PPR_STREAM1(e0, e1, ~0ull, (void)0)
+            if (lane == 0) a.hubpart[it] = s - c;
+            continue;
         }
-        unsigned long long mm = __ballot(in && !hub && deg > a.coop);
-        while (mm) {
-            const int j = __ffsll((unsigned long long)mm) - 1;
-            mm &= mm - 1;
-            float S = 0.f, C = 0.f;
-            fold1_coop(a, yin, rl64(e0, j), rl64(e1, j), lane, seed, fresh, S, C);
-            if (lane == j) {
-                s = S;
-                c = C;
-            }
-        }
-        const float y = ppr_y(s - c, invS, Dn, v == seed);
-        if (in && !hub) yout[v] = y;
-        const unsigned long long skip = __ballot(!in || hub), dang = __ballot(dg == 0.f);
+        const uint64_t seg = it - a.nchunk;
+        const uint64_t base = a.seg_v0[seg];
+        const uint32_t nv = a.seg_v0[seg + 1] - (uint32_t)base;
+        const bool in = (uint32_t)lane < nv;
+        const uint64_t E0 = in ? a.iidx[base + lane] : 0, E1 = in ? a.iidx[base + lane + 1] : 0;
+        const float dg = in ? a.invdeg[base + lane] : 1.f;
+        const bool hub = in && E1 - E0 > HUB;
+        const unsigned long long hubm = __ballot(hub), dangm = __ballot(in && dg == 0.f);
         double sS = 0.0, sD = 0.0;
-        const uint32_t nv = (uint32_t)min<uint64_t>(64, a.V - base);
-        for (uint32_t j = 0; j < nv; ++j) {
-            if ((skip >> j) & 1ull) continue;
-            const float yj = rlf(y, j);
-            sS += (double)yj;
-            if ((dang >> j) & 1ull) sD += (double)yj;
+        float mine = 0.f;
+        uint32_t j = 0;
+        while (j < nv) {
+            if ((hubm >> j) & 1ull) {
+                j++;
+                continue;
+            }
+            const unsigned long long rest = hubm >> j;
+            const uint32_t h = rest ? j + (uint32_t)__builtin_ctzll(rest) : nv;
+            const uint64_t ra = rl64(E0, j), rb = rl64(E1, h - 1);
+            float s = 0.f, c = 0.f;
+            uint64_t ej = rl64(E1, j);
+#define CLOSE_S                                                    \
+    {                                                              \
+        const float y_ = ppr_y(s - c, invS, Dn, base + j == seed); \
+        if ((uint32_t)lane == j) mine = y_;                        \
+        sS += (double)y_;                                          \
+        if ((dangm >> j) & 1ull) sD += (double)y_;                 \
+        s = 0.f;                                                   \
+        c = 0.f;                                                   \
+        j++;                                                       \
+        ej = j < h ? rl64(E1, j) : ~0ull;                          \
+    }
+            PPR_STREAM1(ra, rb, ej, CLOSE_S)
+            while (j < h) CLOSE_S
+#undef CLOSE_S
         }
+        if (in && !hub) yout[base + lane] = mine;
         if (lane == 0) {
             a.partS[seg] = sS;
             a.partD[seg] = sD;
-        }
-    }
-}
-
-/* hub chunks: wavefront per chunk */
-template <int B>
-__global__ __launch_bounds__(256) void kp_hubchunk(PprArgs a, int cur)
-{
-    const int lane = threadIdx.x & 63;
-    const uint64_t wid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-    const PprSlots* sl = a.slots;
-    const int q = B == 64 ? lane : 0;
-    const bool live = (sl->live >> q) & 1ull, fresh = (sl->fresh >> q) & 1ull;
-    if (B == 1 && !live) return;
-    const uint32_t seed = sl->seed[q];
-    const float* yin = cur ? a.y1 : a.y0;
-    for (uint64_t c = wid; c < a.nchunk; c += nw) {
-        const uint32_t h = a.chunk_h[c];
-        const uint32_t v = a.hub_v[h];
-        const uint64_t e0 = a.iidx[v] + (uint64_t)(c - a.hub_c0[h]) * HUB;
-        const uint64_t e1 = min<uint64_t>(e0 + HUB, a.iidx[v + 1]);
-        float s = 0.f, cc = 0.f;
-        if (B == 64) {
-            fold64(a, yin, e0, e1, lane, seed, fresh, live, s, cc);
-            if (live) a.hubpart[c * 64 + lane] = s - cc;
-        } else {
-            fold1_coop(a, yin, e0, e1, lane, seed, fresh, s, cc);
-            if (lane == 0) a.hubpart[c] = s - cc;
         }
     }
 }
@@ -1034,7 +1053,7 @@ __global__ __launch_bounds__(256) void kx_ext_b(PprArgs a, int latest, unsigned 
     const float* yP = latest ? a.y0 : a.y1;
     float* o = on && outp ? outp[lane] : nullptr;
     unsigned long long dig = 0;
-    for (uint64_t seg = wid; seg < a.nseg; seg += nw) {
+    for (uint64_t seg = wid; seg < a.nseg64; seg += nw) {
         const uint64_t base = seg * 64;
         const uint32_t nv = (uint32_t)min<uint64_t>(64, a.V - base);
         double sr = 0.0;
@@ -1066,7 +1085,7 @@ __global__ __launch_bounds__(256) void kx_ext_s(PprArgs a, int latest, unsigned 
     const float* yT = latest ? a.y1 : a.y0;
     const float* yP = latest ? a.y0 : a.y1;
     unsigned long long dig = 0;
-    for (uint64_t seg = wid; seg < a.nseg; seg += nw) {
+    for (uint64_t seg = wid; seg < a.nseg64; seg += nw) {
         const uint64_t base = seg * 64, v = base + lane;
         const uint32_t nv = (uint32_t)min<uint64_t>(64, a.V - base);
         double d = 0.0;
@@ -1102,6 +1121,12 @@ __global__ void kx_finish(PprArgs a, int B, unsigned long long mask)
 /* ------------------------------------------------------------------------ */
 /* graph                                                                     */
 /* ------------------------------------------------------------------------ */
+
+__global__ void k_inw(uint64_t E, const uint32_t* iadj, const float* inv, float* inw)
+{
+    const uint64_t st = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E; e += st) inw[e] = inv[iadj[e]];
+}
 
 __global__ void k_invdeg(uint64_t V, const uint64_t* oidx, float* inv)
 {
@@ -1162,11 +1187,11 @@ extern "C" int ipr_batch_graph_create(uint64_t V, uint64_t E, const uint64_t* id
     g->V = V;
     g->E = E;
     g->tune = default_tuning();
-    g->nseg = (V + 63) / 64;
+    g->nseg64 = (V + 63) / 64;
     int rc = 0;
     uint64_t* kidx = nullptr;
     std::vector<uint64_t> hin;
-    std::vector<uint32_t> hv, hc0, ch;
+    std::vector<uint32_t> hv, hc0, ch, sv;
     unsigned long long* dsum = nullptr;
 #define GO(x) do { if ((rc = (x)) != 0) goto fail; } while (0)
 #define GH(x, w) do { hipError_t e_ = (x); if (e_ != hipSuccess) { rc = hip_fail(e_, w); goto fail; } } while (0)
@@ -1177,6 +1202,7 @@ extern "C" int ipr_batch_graph_create(uint64_t V, uint64_t E, const uint64_t* id
     GO(dmalloc(&g->iidx, V + 1, &g->bytes, "transpose offsets"));
     GO(dmalloc(&g->iadj, E, &g->bytes, "transpose arcs"));
     GO(dmalloc(&g->invdeg, V, &g->bytes, "inverse degrees"));
+    GO(dmalloc(&g->inw, E, &g->bytes, "transpose arc weights"));
     GH(hipMemcpyAsync(g->oidx, idx, (V + 1) * 8, hipMemcpyHostToDevice, g->stream), "upload offsets");
     if (E) GH(hipMemcpyAsync(g->oadj, adj, E * 4, hipMemcpyHostToDevice, g->stream), "upload arcs");
     if (wgt && E) GH(hipMemcpyAsync(g->owgt, wgt, E * 4, hipMemcpyHostToDevice, g->stream), "upload weights");
@@ -1187,6 +1213,7 @@ extern "C" int ipr_batch_graph_create(uint64_t V, uint64_t E, const uint64_t* id
         goto fail;
     }
     hipLaunchKernelGGL(k_invdeg, dim3(grid_for(V, 256, 4096)), dim3(256), 0, g->stream, V, g->oidx, g->invdeg);
+    hipLaunchKernelGGL(k_inw, dim3(grid_for(E, 256, 8192)), dim3(256), 0, g->stream, E, g->iadj, g->invdeg, g->inw);
     GH(hipGetLastError(), "inverse degrees");
     /* hubs and their chunks, from the transpose offsets */
     hin.resize(V + 1);
@@ -1202,11 +1229,32 @@ extern "C" int ipr_batch_graph_create(uint64_t V, uint64_t E, const uint64_t* id
         }
     }
     hc0.push_back((uint32_t)ch.size());
+    /* pull segments: <= 64 vertices and <= SEG_ARCS non-hub in-arcs */
+    {
+        uint64_t v = 0;
+        sv.push_back(0);
+        while (v < V) {
+            uint64_t arcs = 0;
+            uint32_t n = 0;
+            while (v < V && n < 64) {
+                const uint64_t d = hin[v + 1] - hin[v];
+                const uint64_t c = d > HUB ? 0 : d;
+                if (n > 0 && arcs + c > SEG_ARCS) break;
+                arcs += c;
+                v++;
+                n++;
+            }
+            sv.push_back((uint32_t)v);
+        }
+        g->nseg = sv.size() - 1;
+    }
     g->nhub = (uint32_t)hv.size();
     g->nchunk = (uint32_t)ch.size();
     GO(dmalloc(&g->hub_v, hv.size(), &g->bytes, "hub list"));
     GO(dmalloc(&g->hub_c0, hc0.size(), &g->bytes, "hub chunks"));
     GO(dmalloc(&g->chunk_h, ch.size(), &g->bytes, "chunk hubs"));
+    GO(dmalloc(&g->seg_v0, sv.size(), &g->bytes, "segments"));
+    GH(hipMemcpy(g->seg_v0, sv.data(), sv.size() * 4, hipMemcpyHostToDevice), "segments");
     if (!hv.empty()) GH(hipMemcpy(g->hub_v, hv.data(), hv.size() * 4, hipMemcpyHostToDevice), "hub list");
     GH(hipMemcpy(g->hub_c0, hc0.data(), hc0.size() * 4, hipMemcpyHostToDevice), "hub chunks");
     if (!ch.empty()) GH(hipMemcpy(g->chunk_h, ch.data(), ch.size() * 4, hipMemcpyHostToDevice), "chunk hubs");
@@ -1233,7 +1281,8 @@ extern "C" int ipr_batch_graph_destroy(ipr_bgraph* g)
 {
     if (!g) return 0;
     (void)hipSetDevice(g->device);
-    void* ps[] = {g->oidx, g->oadj, g->owgt, g->iidx, g->iadj, g->invdeg, g->hub_v, g->hub_c0, g->chunk_h};
+    void* ps[] = {g->oidx, g->oadj, g->owgt, g->iidx, g->iadj, g->inw, g->invdeg, g->hub_v, g->hub_c0, g->chunk_h,
+                  g->seg_v0};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     if (g->stream) (void)hipStreamDestroy(g->stream);
@@ -1572,12 +1621,15 @@ static PprArgs ppr_args(const PprState* s)
     const ipr_bgraph* g = s->g;
     a.V = g->V;
     a.nseg = g->nseg;
+    a.nseg64 = g->nseg64;
+    a.seg_v0 = g->seg_v0;
     a.nhub = g->nhub;
     a.nchunk = g->nchunk;
     a.coop = g->tune.coop;
     a.oidx = g->oidx;
     a.iidx = g->iidx;
     a.iadj = g->iadj;
+    a.inw = g->inw;
     a.invdeg = g->invdeg;
     a.hub_v = g->hub_v;
     a.hub_c0 = g->hub_c0;
@@ -1600,7 +1652,7 @@ static PprArgs ppr_args(const PprState* s)
 uint64_t PprState::footprint(const ipr_bgraph* g, int B)
 {
     const uint64_t b = (uint64_t)B;
-    return 2 * g->V * b * 4 + 2 * g->nseg * b * 8 + (uint64_t)g->nchunk * b * 4 + 2ull * g->nhub * b * 8 +
+    return 2 * g->V * b * 4 + 2 * std::max<uint64_t>(g->nseg, g->nseg64) * b * 8 + (uint64_t)g->nchunk * b * 4 + 2ull * g->nhub * b * 8 +
            (CHAINS + 1ull) * b * 16 + b * 16 + SPREAD * b * 8 + sizeof(PprSlots) + sizeof(PprOut) + b * 8;
 }
 
@@ -1623,8 +1675,9 @@ int PprState::create(ipr_bgraph* g, int B, int use_graph, PprState** out)
     GH(hipEventCreateWithFlags(&s->ev, hipEventDisableTiming), "state event");
     GO(dmalloc(&s->y[0], g->V * b, &s->bytes, "ppr values"));
     GO(dmalloc(&s->y[1], g->V * b, &s->bytes, "ppr values"));
-    GO(dmalloc(&s->partS, g->nseg * b, &s->bytes, "segment sums"));
-    GO(dmalloc(&s->partD, g->nseg * b, &s->bytes, "segment sums"));
+    const uint64_t np = std::max<uint64_t>(g->nseg, g->nseg64);
+    GO(dmalloc(&s->partS, np * b, &s->bytes, "segment sums"));
+    GO(dmalloc(&s->partD, np * b, &s->bytes, "segment sums"));
     GO(dmalloc(&s->hubpart, (uint64_t)g->nchunk * b, &s->bytes, "hub chunks"));
     GO(dmalloc(&s->hubS, (uint64_t)g->nhub * b, &s->bytes, "hub sums"));
     GO(dmalloc(&s->hubD, (uint64_t)g->nhub * b, &s->bytes, "hub sums"));
@@ -1690,13 +1743,9 @@ static int enqueue_iteration(const PprState* s, int cur)
     const unsigned G = g->tune.grid;
     hipStream_t st = s->st;
     const uint64_t b = (uint64_t)s->B;
-    if (g->nchunk) {
-        const unsigned gb = grid_for(g->nchunk, 4, G);
-        if (s->B == 64) hipLaunchKernelGGL(kp_hubchunk<64>, dim3(gb), dim3(256), 0, st, a, cur);
-        else hipLaunchKernelGGL(kp_hubchunk<1>, dim3(gb), dim3(256), 0, st, a, cur);
-    }
-    if (s->B == 64) hipLaunchKernelGGL(kp_pull_b, dim3(grid_for(g->nseg, 4, G)), dim3(256), 0, st, a, cur);
-    else hipLaunchKernelGGL(kp_pull_s, dim3(grid_for(g->nseg, 4, G)), dim3(256), 0, st, a, cur);
+    const unsigned gp = grid_for(g->nseg + g->nchunk, 4, G);
+    if (s->B == 64) hipLaunchKernelGGL(kp_pull_b, dim3(gp), dim3(256), 0, st, a, cur);
+    else hipLaunchKernelGGL(kp_pull_s, dim3(gp), dim3(256), 0, st, a, cur);
     if (g->nhub) {
         if (s->B == 64) hipLaunchKernelGGL(kp_hubfin<64>, dim3(grid_for(g->nhub, 4, G)), dim3(256), 0, st, a, cur);
         else hipLaunchKernelGGL(kp_hubfin<1>, dim3(grid_for(g->nhub, 256, G)), dim3(256), 0, st, a, cur);
@@ -1745,7 +1794,7 @@ int PprState::extract(unsigned long long mask, unsigned long long first, float* 
     if (!mask) return 0;
     const PprArgs a = ppr_args(this);
     const int latest = (int)(tick & 1);
-    const unsigned gs = grid_for(g->nseg, 4, g->tune.grid);
+    const unsigned gs = grid_for(g->nseg64, 4, g->tune.grid);
     bool any_out = false;
     if (dev_out)
         for (int q = 0; q < B; q++) any_out |= dev_out[q] != nullptr;
@@ -1759,7 +1808,7 @@ int PprState::extract(unsigned long long mask, unsigned long long first, float* 
     }
     const unsigned gc = (unsigned)(((CHAINS + 1) * (uint64_t)B + 255) / 256);
     hipLaunchKernelGGL(k_chain, dim3(gc), dim3(256), 0, st, (const double*)partS, (const double*)nullptr,
-                       (const double*)nullptr, (const double*)nullptr, g->nseg, 0u, B,
+                       (const double*)nullptr, (const double*)nullptr, g->nseg64, 0u, B,
                        (const unsigned long long*)&slots->ext, chain);
     hipLaunchKernelGGL(k_tree, dim3(B), dim3(256), 0, st, (const double*)chain, B,
                        (const unsigned long long*)&slots->ext, res);

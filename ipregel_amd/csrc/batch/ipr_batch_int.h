@@ -30,7 +30,11 @@ constexpr uint32_t LDS_LARGE = 4096; /* chunk-prefix entries of the large list c
  * compensated, then the chunk values folded in order. Part of the definition
  * of the FP32 result (scalar and batched kernels follow it alike), so it is a
  * constant and not a tuning knob. */
-constexpr uint32_t HUB = 4096;
+constexpr uint32_t HUB = 2048;
+/* PPR: pull segments hold at most 64 consecutive vertices and at most SEG_ARCS
+ * non-hub in-arcs (a single heavier vertex forms its own segment). The FP64
+ * sums follow the segments, so this is part of the definition as well. */
+constexpr uint32_t SEG_ARCS = 2048;
 
 /* launch and binning parameters (chosen per graph by the autotuner or defaults) */
 struct Tuning {
@@ -53,11 +57,14 @@ struct ipr_bgraph {
     uint64_t* iidx;       /* transpose offsets, V+1 */
     uint32_t* iadj;       /* transpose sources, ascending within each list */
     float* invdeg;        /* 1/out-degree in FP32, 0 for vertices without out-arcs */
+    float* inw;           /* per transpose arc: invdeg of its source */
     uint32_t nhub, nchunk;
     uint32_t* hub_v;      /* vertices with in-degree > HUB, ascending */
     uint32_t* hub_c0;     /* first chunk of each hub, nhub+1 */
     uint32_t* chunk_h;    /* hub of each chunk */
-    uint64_t nseg;        /* 64-vertex segments */
+    uint64_t nseg;        /* pull segments */
+    uint32_t* seg_v0;     /* first vertex of each pull segment, nseg+1 */
+    uint64_t nseg64;      /* fixed 64-vertex segments (result extraction) */
     unsigned long long inf_digest;   /* digest of the all-INF uint32 vector */
     uint64_t bytes;
     iprb::Tuning tune;
