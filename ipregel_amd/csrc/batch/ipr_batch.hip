@@ -295,8 +295,11 @@ __global__ __launch_bounds__(256) void k_relax_small_s(PushArgs a, int cur)
                 const uint32_t o = owner_of(s_pre, 256, k);
                 const uint64_t e = s_e0[o] + (k - s_pre[o]);
                 const uint32_t v = a.oadj[e];
-                atomicMin(&a.mbox[v], sadd(s_val[o], arc_w(a, e)));
-                a.touched[v] = 1;
+                const uint32_t c = sadd(s_val[o], arc_w(a, e));
+                if (c < a.mbox[v]) {
+                    atomicMin(&a.mbox[v], c);
+                    a.touched[v] = 1;
+                }
             }
         }
         __syncthreads();
@@ -342,19 +345,36 @@ __global__ __launch_bounds__(256) void k_relax_small_b(PushArgs a, int cur)
         for (uint32_t base = 0; base < A; base += 256) {
             const uint32_t k = base + t;
             uint32_t v = 0, w = 0, o = 0;
+            unsigned long long vmk = 0;
             if (k < A) {
                 o = owner_of(s_pre, 64, k);
                 const uint64_t e = s_e0[o] + (k - s_pre[o]);
                 v = a.oadj[e];
                 w = arc_w(a, e);
-                a.touched[v] = 1;
+                vmk = a.vmask[v];
             }
             const uint32_t wb = base + (uint32_t)wave * 64;
             const uint32_t cnt = A > wb ? min(64u, A - wb) : 0u;
-            for (uint32_t j = 0; j < cnt; ++j) {
-                const uint32_t vj = rl(v, j), wj = rl(w, j), oj = rl(o, j);
-                if ((s_m[oj] >> lane) & 1ull)
-                    atomicMin(&a.mbox[(uint64_t)vj * 64 + lane], sadd(s_d[oj * 64 + lane], wj));
+            for (uint32_t j0 = 0; j0 < cnt; j0 += 8) {
+                /* the targets' current distances, 8 rows in flight; a message
+                 * that does not beat a set distance is dropped */
+                uint32_t dv[8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const uint32_t j = (j0 + i) & 63;
+                    const unsigned long long vm = rl64(vmk, j);
+                    dv[i] = (j0 + i < cnt && ((vm >> lane) & 1ull)) ? a.val[(uint64_t)rl(v, j) * 64 + lane] : INF;
+                }
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const uint32_t j = (j0 + i) & 63;
+                    if (j0 + i >= cnt) break;
+                    const uint32_t vj = rl(v, j), oj = rl(o, j);
+                    const uint32_t cd = sadd(s_d[oj * 64 + lane], rl(w, j));
+                    const bool go = ((s_m[oj] >> lane) & 1ull) && cd < dv[i];
+                    if (go) atomicMin(&a.mbox[(uint64_t)vj * 64 + lane], cd);
+                    if (__ballot(go) && lane == 0) a.touched[vj] = 1;
+                }
             }
         }
         __syncthreads();
@@ -385,8 +405,11 @@ __global__ __launch_bounds__(256) void k_relax_large(PushArgs a, int cur)
             const uint32_t du = a.val[u];
             for (uint64_t e = e0 + t; e < e1; e += 256) {
                 const uint32_t v = a.oadj[e];
-                atomicMin(&a.mbox[v], sadd(du, arc_w(a, e)));
-                a.touched[v] = 1;
+                const uint32_t c = sadd(du, arc_w(a, e));
+                if (c < a.mbox[v]) {
+                    atomicMin(&a.mbox[v], c);
+                    a.touched[v] = 1;
+                }
             }
         } else {
             const unsigned long long m = a.act[u];
@@ -395,16 +418,34 @@ __global__ __launch_bounds__(256) void k_relax_large(PushArgs a, int cur)
             for (uint64_t base = e0; base < e1; base += 256) {
                 const uint64_t e = base + t;
                 uint32_t v = 0, w = 0;
+                unsigned long long vmk = 0;
                 if (e < e1) {
                     v = a.oadj[e];
                     w = arc_w(a, e);
-                    a.touched[v] = 1;
+                    vmk = a.vmask[v];
                 }
                 const uint64_t wb = base + (uint64_t)wave * 64;
                 const uint32_t cnt = e1 > wb ? (uint32_t)min<uint64_t>(64, e1 - wb) : 0u;
-                if (on)
-                    for (uint32_t j = 0; j < cnt; ++j)
-                        atomicMin(&a.mbox[(uint64_t)rl(v, j) * 64 + lane], sadd(du, rl(w, j)));
+                for (uint32_t j0 = 0; j0 < cnt; j0 += 8) {
+                    uint32_t dv[8];
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) {
+                        const uint32_t j = (j0 + i) & 63;
+                        const unsigned long long vm = rl64(vmk, j);
+                        dv[i] = (on && j0 + i < cnt && ((vm >> lane) & 1ull)) ? a.val[(uint64_t)rl(v, j) * 64 + lane]
+                                                                               : INF;
+                    }
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) {
+                        const uint32_t j = (j0 + i) & 63;
+                        if (j0 + i >= cnt) break;
+                        const uint32_t vj = rl(v, j);
+                        const uint32_t cd = sadd(du, rl(w, j));
+                        const bool go = on && cd < dv[i];
+                        if (go) atomicMin(&a.mbox[(uint64_t)vj * 64 + lane], cd);
+                        if (__ballot(go) && lane == 0) a.touched[vj] = 1;
+                    }
+                }
             }
         }
     }
