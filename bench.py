@@ -17,7 +17,26 @@ scale 22, edge factor 16, a/b/c/d = .57/.19/.19/.05, seed 2.
              the reference sources) on the same graph file, its
              "Total time of supersteps", with its thread count
 
+Two more sections go into the same JSON line (libipregel_batch.so):
+
+  batched    64 SSSP queries (sources uniform, seed 4, weights [1,255]) and 64
+             personalised-PageRank queries (restart 0.15, 20 iterations) on the
+             directed R-MAT scale 22 ef 16 seed 2 graph, each as one 64-lane
+             batch, against the plain loop (the same queries one at a time on
+             the one-query kernels); results checked identical; device time by
+             HIP events. Its cpu_baseline is the FP64 PPR restatement on a
+             sample of seeds, which also gives the FP32 result's L1 / max-rel /
+             top-100 agreement.
+  served     Poisson arrivals (seed 5) at several rates of a 40/40/20 SSSP /
+             PPR / CC mix over six resident graphs (R-MAT scales 18, 20, 22,
+             23, 24, ef 16, seed 2, weights seed 4; CC on their symmetrised
+             versions; and the 4096x4096 grid, weights seed 3), served with
+             continuous batching under an HBM budget for the batch states and,
+             on the same trace, by the sequential plain loop; p50/p99 latency,
+             queries/s, and whether every result digest matched.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--apps cc22,pr16,...]
+                       [--no-batched] [--no-served] [--serve-rates 10,100,500,2000]
 """
 from __future__ import annotations
 
@@ -36,6 +55,8 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
+from ipregel_amd import autotune as AT  # noqa: E402
+from ipregel_amd import batch as BA  # noqa: E402
 from ipregel_amd import engine as EN  # noqa: E402
 from ipregel_amd import graph as G  # noqa: E402
 
@@ -203,6 +224,141 @@ def run_workload(key: str, steps: int, warmup: int, with_cpu: bool) -> dict:
     return out
 
 
+def ppr_cpu_sample(g: G.Graph, seeds, x32: np.ndarray, iters: int = 20) -> dict:
+    """cpu_baseline of the batched PPR line: the FP64 restatement (oracle,
+    one thread) on a few seeds, timed, and the FP32 results compared with it."""
+    from oracle import oracle as O
+    t0 = time.perf_counter()
+    worst = {"l1": 0.0, "max_rel": 0.0, "top_overlap": 100}
+    for i, s in enumerate(seeds):
+        ref = O.ppr(g, int(s), iters)
+        m = BA.ppr_agreement(x32[i], ref)
+        worst = {"l1": max(worst["l1"], m["l1"]), "max_rel": max(worst["max_rel"], m["max_rel"]),
+                 "top_overlap": min(worst["top_overlap"], m["top_overlap"])}
+    t = time.perf_counter() - t0
+    return {"value": len(seeds) / t, "unit": "queries/s", "cores": 1, "kind": "port",
+            "sample": f"FP64 PPR restatement, {len(seeds)} seeds x {iters} iterations, "
+                      f"{t:.2f} s", "fp32_vs_fp64_worst": worst}
+
+
+def run_batched(with_cpu: bool) -> dict:
+    t0 = time.time()
+    g = G.rmat(22, 16, seed=2).with_weights(4)
+    rg = BA.ResidentGraph(g)
+    rng = np.random.default_rng(4)
+    src = rng.integers(0, g.V, 64).astype(np.uint32)
+    seeds = rng.integers(0, g.V, 64).astype(np.uint32)
+    log(f"batched: graph V={g.V} E={g.E} resident ({rg.bytes / 2**30:.2f} GiB) in {time.time() - t0:.1f}s")
+    tune = AT.tune_resident(rg, kinds=("sssp",), mean_weight=float(g.wgt.mean()))
+    log(f"batched: tuned {tune['final']} in {tune['seconds']:.1f}s")
+    rg.batch_sssp(src[:2])
+    rg.batch_ppr(seeds[:2], 2)
+    dist, sb = rg.batch_sssp(src)
+    plain_ms, same = 0.0, True
+    for q in range(64):
+        one, st = rg.sssp(int(src[q]))
+        plain_ms += st["ms"]
+        same &= bool(np.array_equal(one, dist[q]))
+    del dist
+    x, res, pb = rg.batch_ppr(seeds, 20)
+    pplain_ms, psame = 0.0, True
+    for q in range(64):
+        one, r1, st = rg.ppr(int(seeds[q]), 20)
+        pplain_ms += st["ms"]
+        psame &= bool(np.array_equal(one.view(np.uint32), x[q].view(np.uint32))) and bool(r1 == res[q])
+    E = g.E
+    # pull iteration, 64 live slots: 4 B id + 4 B weight + 256 B row per in-arc, 256 B row written per vertex
+    ppr_bytes_iter = E * (8 + 64 * 4) + g.V * 64 * 4
+    out = {
+        "workload": "directed R-MAT scale 22 ef 16 seed 2, weights [1,255] seed 4; 64 queries per kind "
+                    "(seed 4)", "V": g.V, "E": E,
+        "sssp": {"batch_ms": sb["ms"], "plain_loop_ms": plain_ms, "speedup": plain_ms / sb["ms"],
+                 "queries_per_s": 64 / (sb["ms"] * 1e-3), "plain_queries_per_s": 64 / (plain_ms * 1e-3),
+                 "supersteps": sb["supersteps"], "relaxations": sb["relaxations"],
+                 "G_relaxations_per_s": sb["relaxations"] / (sb["ms"] * 1e-3) / 1e9,
+                 "identical_to_plain": same},
+        "ppr": {"batch_ms": pb["ms"], "plain_loop_ms": pplain_ms, "speedup": pplain_ms / pb["ms"],
+                "queries_per_s": 64 / (pb["ms"] * 1e-3), "plain_queries_per_s": 64 / (pplain_ms * 1e-3),
+                "iterations": 20, "ms_per_iteration": pb["ms"] / 20,
+                "shared_bytes_per_iteration": ppr_bytes_iter,
+                "shared_GBps": ppr_bytes_iter / (pb["ms"] / 20 * 1e-3) / 1e9,
+                "hbm_frac_shared": ppr_bytes_iter / (pb["ms"] / 20 * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                "identical_to_plain": psame,
+                "residual_max": float(res.max())},
+        "mix_speedup": (plain_ms + pplain_ms) / (sb["ms"] + pb["ms"]),
+        "tuning": tune["final"],
+    }
+    log(f"batched: sssp {sb['ms']:.1f} ms vs plain {plain_ms:.1f} ms; ppr {pb['ms']:.1f} ms vs "
+        f"plain {pplain_ms:.1f} ms; identical {same} {psame}")
+    if with_cpu:
+        out["cpu_baseline"] = ppr_cpu_sample(g, seeds[:3], x[:3])
+        log(f"batched: cpu baseline {out['cpu_baseline']}")
+    rg.close()
+    return out
+
+
+def build_served_graphs() -> tuple[list, list, dict]:
+    """Six logical graphs; SSSP/PPR on the directed weighted ones (and the
+    grid), CC on the symmetrised ones (and the grid)."""
+    t0 = time.time()
+    res, names, tuned = [], [], {}
+
+    def add(name: str, g: G.Graph, kinds) -> None:
+        rg = BA.ResidentGraph(g)
+        mw = float(g.wgt.mean()) if g.wgt is not None and g.E else None
+        rep = AT.tune_resident(rg, kinds=kinds, mean_weight=mw)
+        tuned[name] = {k: rep[k]["choice"] for k in ("delta", "delta1", "large", "grid", "cc_mode") if k in rep}
+        tuned[name]["seconds"] = round(rep["seconds"], 2)
+        log(f"served: {name} V={g.V} E={g.E} tuned {tuned[name]}")
+        res.append(rg)
+        names.append(name)
+
+    for sc in (18, 20, 22, 23, 24):
+        add(f"rmat{sc}", G.rmat(sc, 16, seed=2).with_weights(4), ("sssp",))
+    add("grid4096", G.grid(4096, 4096).with_weights(3), ("sssp", "cc"))
+    for sc in (18, 20, 22, 23, 24):
+        add(f"rmat{sc}sym", G.rmat(sc, 16, seed=2, symmetrise=True), ("cc",))
+    by_kind = {BA.SSSP: [0, 1, 2, 3, 4, 5], BA.PPR: [0, 1, 2, 3, 4, 5], BA.CC: [6, 7, 8, 9, 10, 5]}
+    log(f"served: 11 resident graphs ({sum(r.bytes for r in res) / 2**30:.1f} GiB) built and tuned in "
+        f"{time.time() - t0:.1f}s")
+    return res, names, by_kind, tuned
+
+
+def run_served(rates: list[float], counts: list[int], budget: int) -> dict:
+    res, names, by_kind, tuned = build_served_graphs()
+    V = [r.V for r in res]
+    mix = {BA.SSSP: 0.4, BA.PPR: 0.4, BA.CC: 0.2}
+    warm = BA.poisson_trace(48, 2000.0, mix, by_kind, V, seed=99)
+    BA.serve(res, warm, hbm_budget=budget)
+    BA.serve(res, warm[:12], sequential=True)
+    sweep = []
+    for rate, n in zip(rates, counts):
+        tr = BA.poisson_trace(n, rate, mix, by_kind, V, seed=5)
+        rb, _ = BA.serve(res, tr, hbm_budget=budget)
+        rs, _ = BA.serve(res, tr, sequential=True)
+        b, s_ = rb.summary(tr, names), rs.summary(tr, names)
+        row = {"rate": rate, "queries": n, "batched": b, "plain": s_,
+               "throughput_ratio": b["qps"] / s_["qps"] if s_["qps"] else None,
+               "identical": bool(np.array_equal(rb.digest, rs.digest))}
+        sweep.append(row)
+        log(f"served @ {rate} q/s: batched {b['qps']:.1f} q/s p50 {b['p50_ms']:.1f} p99 {b['p99_ms']:.1f} ms; "
+            f"plain {s_['qps']:.1f} q/s p50 {s_['p50_ms']:.1f} p99 {s_['p99_ms']:.1f} ms; identical {row['identical']}")
+    slot = {n: {"sssp": r.slot_bytes(BA.SSSP), "ppr": r.slot_bytes(BA.PPR), "cc": r.slot_bytes(BA.CC)}
+            for n, r in zip(names, res)}
+    for r in res:
+        r.close()
+    return {"graphs": names, "tuning": tuned,
+            "mix": "40/40/20 SSSP/PPR/CC, graph uniform over six, parameter uniform",
+            "hbm_budget_bytes": budget, "ppr_iterations": 20, "sweep": sweep, "state_bytes": slot}
+
+
+def _plain(o):
+    """numpy scalars in the JSON line as plain numbers"""
+    if isinstance(o, np.generic):
+        return o.item()
+    raise TypeError(f"not serialisable: {type(o)}")
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -210,6 +366,11 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--apps", default="cc22,pr16")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--serve-rates", default="10,100,500,2000")
+    ap.add_argument("--serve-counts", default="40,200,300,400")
+    ap.add_argument("--serve-budget-gib", type=float, default=128.0)
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is a design "
@@ -219,6 +380,12 @@ def main() -> int:
     results = {}
     for k in keys:
         results[k] = run_workload(k, a.steps, a.warmup, not a.no_cpu)
+    batched = None if a.no_batched else run_batched(not a.no_cpu)
+    served = None
+    if not a.no_served:
+        rates = [float(x) for x in a.serve_rates.split(",") if x]
+        counts = [int(x) for x in a.serve_counts.split(",") if x]
+        served = run_served(rates, counts, int(a.serve_budget_gib * 2**30))
     head = results[keys[0]]
     k = head["kernel"]
     line = {
@@ -244,8 +411,10 @@ def main() -> int:
         "hbm_frac_model": head["hbm_frac_model"],
         "apps": {kk: {x: v for x, v in r.items() if x not in ("active_trace",)}
                  for kk, r in results.items()},
+        "batched": batched,
+        "served": served,
     }
-    print(json.dumps(line), flush=True)
+    print(json.dumps(line, default=_plain), flush=True)
     return 0
 
 

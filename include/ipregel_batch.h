@@ -11,9 +11,9 @@
  *          D = mass on vertices without out-arcs; `iters` applications.
  *          Values stored in FP32 with a Kahan-compensated pull sum; dangling
  *          mass, normalisation and the L1 residual in FP64.
- *   CC     connected-component labels (minimum id reaching each vertex) of a
- *          symmetric graph, computed once per graph and shared by all CC
- *          queries of that graph.
+ *   CC     connected-component labels of a symmetric graph (the smallest
+ *          vertex id of each component), by union-find; in serving, CC queries
+ *          of one graph that wait together share one computation.
  *
  * Batches hold up to 64 queries per graph and kind; a query's lane computes
  * only from its own column and every reduction runs in a fixed order, so a
@@ -47,6 +47,23 @@ int ipr_batch_graph_destroy(ipr_bgraph* g);
 /* HBM held by the graph and by each kind's 64-slot batch state */
 uint64_t ipr_batch_graph_bytes(const ipr_bgraph* g);
 uint64_t ipr_batch_slot_bytes(const ipr_bgraph* g, int kind);
+
+/* Launch shapes and scheduling of the push kernels of one graph (used by
+ * states created afterwards). None of them changes a result. */
+typedef struct ipr_batch_tuning {
+    uint32_t large;   /* out-degree above which a frontier vertex's arcs are split into chunks (1024) */
+    uint32_t chunk;   /* arcs per chunk of such a vertex (1024) */
+    uint32_t grid;    /* workgroups of the grid-stride kernels (2048) */
+    uint32_t delta;   /* SSSP near/far step: a vertex relaxes while its distance is below the
+                         query's threshold, which then moves to the smallest pending distance
+                         + delta; 0xFFFFFFFF = every superstep relaxes every improved vertex
+                         (default: 4 x mean arc weight, or 0xFFFFFFFF without weights); used by
+                         64-slot states */
+    uint32_t delta1;  /* the same for one-query states (the plain loop) */
+    uint32_t cc_mode; /* CC algorithm: 0 label propagation supersteps, 1 union-find (default) */
+} ipr_batch_tuning;
+int ipr_batch_graph_get_tuning(const ipr_bgraph* g, ipr_batch_tuning* t);
+int ipr_batch_graph_set_tuning(ipr_bgraph* g, const ipr_batch_tuning* t);
 
 typedef struct ipr_batch_stats {
     uint64_t supersteps;        /* batch supersteps (SSSP) or iterations (PPR) run */

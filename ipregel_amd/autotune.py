@@ -1,0 +1,148 @@
+"""Autotuner: picks launch shapes, vertex-bin thresholds and scheduling knobs
+by timing candidates on the device, per graph.
+
+Two targets:
+  tune_resident(rg)   libipregel_batch.so, per resident graph: the SSSP near/far
+                      step of 64-slot states (delta) and of one-query states
+                      (delta1), the CC algorithm (label propagation or
+                      union-find), the large-vertex split and the grid size.
+  tune_engine(app, g) the drop-in engine (libipr_<app>_32.so): the push bin
+                      thresholds (bin_small_max = arcs per large chunk,
+                      bin_medium_max = out-degree of a large vertex), the pull
+                      bins and the persistent grid.
+
+No knob changes a result (min-combined distances and labels are unique; the
+PPR arithmetic does not depend on them), so the tuner only compares times.
+Times are device times (HIP events) of complete runs; a candidate that
+finishes within half a second runs twice and keeps the faster. A candidate 1.5x slower than the best stops a
+one-dimensional scan (the scans are ordered so that times are unimodal).
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import batch as BA
+from . import engine as EN
+from .graph import Graph
+
+INF = BA.INF
+
+
+def _best_of(f, again_below_ms: float = 500.0) -> float:
+    """One run; a second one (keeping the faster) when the first was short."""
+    t = f()
+    return min(t, f()) if t < again_below_ms else t
+
+
+def _scan(values, measure, stop_ratio: float = 1.5) -> tuple[object, dict]:
+    best, best_t, seen = None, float("inf"), {}
+    for v in values:
+        t = measure(v)
+        seen[str(v)] = t
+        if t < best_t:
+            best, best_t = v, t
+        elif t > stop_ratio * best_t:
+            break
+    return best, seen
+
+
+def tune_resident(rg: BA.ResidentGraph, sample: int = 8, seed: int = 0, kinds=("sssp", "cc"),
+                  mean_weight: float | None = None) -> dict:
+    """Tune one resident graph in place; returns the choice and the timings."""
+    t0 = time.time()
+    rng = np.random.default_rng(seed)
+    src = rng.integers(0, rg.V, sample).astype(np.uint32)
+    cur = rg.tuning
+    report: dict = {"start": dict(cur)}
+    if "sssp" in kinds:
+        if mean_weight is None:
+            mean_weight = cur["delta"] / 4 if cur["delta"] != INF else 1.0
+        base = max(1, int(round(mean_weight)))
+        deltas = [base, 4 * base, 16 * base, 64 * base, 256 * base, INF]
+
+        def batch_ms(d):
+            rg.tune(delta=d)
+            return _best_of(lambda: rg.batch_sssp(src)[1]["ms"])
+
+        best, seen = _scan(deltas, batch_ms)
+        rg.tune(delta=best)
+        report["delta"] = {"choice": best, "ms": seen}
+
+        def one_ms(d):
+            rg.tune(delta1=d)
+            return _best_of(lambda: sum(rg.sssp(int(s))[1]["ms"] for s in src[:2]))
+
+        best1, seen1 = _scan(deltas, one_ms)
+        rg.tune(delta1=best1)
+        report["delta1"] = {"choice": best1, "ms": seen1}
+
+        def large_ms(L):
+            rg.tune(large=L, chunk=min(max(L, 256), 4096))
+            return _best_of(lambda: rg.batch_sssp(src)[1]["ms"])
+
+        bestL, seenL = _scan([4096, 1024, 256], large_ms)
+        rg.tune(large=bestL, chunk=min(max(bestL, 256), 4096))
+        report["large"] = {"choice": bestL, "ms": seenL}
+
+        def grid_ms(G):
+            rg.tune(grid=G)
+            return _best_of(lambda: rg.batch_sssp(src)[1]["ms"])
+
+        bestG, seenG = _scan([2048, 4096, 1024], grid_ms, stop_ratio=10.0)
+        rg.tune(grid=bestG)
+        report["grid"] = {"choice": bestG, "ms": seenG}
+    if "cc" in kinds:
+        seenC = {}
+        for m in (1, 0):
+            rg.tune(cc_mode=m)
+            seenC[m] = _best_of(lambda: rg.cc()[1]["ms"])
+        bestC = min(seenC, key=seenC.get)
+        rg.tune(cc_mode=bestC)
+        report["cc_mode"] = {"choice": bestC, "ms": {"union_find": seenC[1], "label_propagation": seenC[0]}}
+    report["final"] = rg.tuning
+    report["seconds"] = time.time() - t0
+    return report
+
+
+ENGINE_GRID = {
+    # push: (bin_small_max = arcs per large chunk, bin_medium_max = large-vertex out-degree)
+    "push": [(2048, 2048), (1024, 1024), (4096, 4096), (1024, 4096), (2048, 8192), (512, 2048)],
+    "grid_blocks": [0, 1024, 4096],
+    # pull (single broadcast): (pull_small_max, pull_medium_max)
+    "pull": [(0, 0), (2, 16), (4, 64), (8, 256), (16, 1024)],
+}
+
+
+def tune_engine(app: str, g: Graph, source: int = 0, repeat: int = 3, **fixed) -> dict:
+    """Tune the drop-in engine's bins for one application and graph: each
+    candidate is a resident session run `repeat` times (best device time)."""
+    t0 = time.time()
+    pull = "single_broadcast" in app or app.startswith("pagerank")
+    results = []
+
+    def measure(**kw) -> float:
+        s = EN.Session(app, g, source=source, **fixed, **kw)
+        try:
+            s.run(1)
+            before = s.stats().total_ms
+            best = float("inf")
+            for _ in range(repeat):
+                b0 = s.stats().total_ms
+                s.run(1)
+                best = min(best, s.stats().total_ms - b0)
+            del before
+        finally:
+            s.close()
+        results.append({"cfg": kw, "ms": best})
+        return best
+
+    if pull:
+        bins = [dict(pull_small_max=a, pull_medium_max=b) for a, b in ENGINE_GRID["pull"]]
+    else:
+        bins = [dict(bin_small_max=a, bin_medium_max=b) for a, b in ENGINE_GRID["push"]]
+    best_bins = min(bins, key=lambda kw: measure(**kw))
+    best_grid = min(ENGINE_GRID["grid_blocks"], key=lambda gb: measure(**best_bins, grid_blocks=gb))
+    choice = dict(best_bins, grid_blocks=best_grid)
+    return {"app": app, "choice": choice, "tried": results, "seconds": time.time() - t0}

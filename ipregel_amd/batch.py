@@ -34,6 +34,11 @@ class ServeOpts(C.Structure):
                 ("use_graph", C.c_int32), ("pad", C.c_int32)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("large", C.c_uint32), ("chunk", C.c_uint32), ("grid", C.c_uint32), ("delta", C.c_uint32),
+                ("delta1", C.c_uint32), ("cc_mode", C.c_uint32)]
+
+
 class ServeStats(C.Structure):
     _fields_ = [("wall_s", C.c_double), ("ticks", C.c_uint64), ("admitted_waits", C.c_uint64),
                 ("peak_slots", C.c_uint64), ("hbm_peak", C.c_uint64)]
@@ -43,7 +48,8 @@ _lib = None
 
 EXPORTS = ("ipr_batch_last_error", "ipr_batch_graph_create", "ipr_batch_graph_destroy",
            "ipr_batch_graph_bytes", "ipr_batch_slot_bytes", "ipr_batch_sssp", "ipr_batch_ppr",
-           "ipr_single_sssp", "ipr_single_ppr", "ipr_batch_cc", "ipr_serve")
+           "ipr_single_sssp", "ipr_single_ppr", "ipr_batch_cc", "ipr_serve", "ipr_batch_graph_get_tuning",
+           "ipr_batch_graph_set_tuning")
 
 
 def lib() -> C.CDLL:
@@ -60,6 +66,8 @@ def lib() -> C.CDLL:
         L.ipr_batch_graph_bytes.restype = C.c_uint64
         L.ipr_batch_slot_bytes.argtypes = [C.c_void_p, C.c_int]
         L.ipr_batch_slot_bytes.restype = C.c_uint64
+        L.ipr_batch_graph_get_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
+        L.ipr_batch_graph_set_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
         SP = C.POINTER(BatchStats)
         L.ipr_batch_sssp.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, SP]
         L.ipr_batch_ppr.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, SP]
@@ -113,6 +121,18 @@ class ResidentGraph:
 
     def slot_bytes(self, kind: int) -> int:
         return int(lib().ipr_batch_slot_bytes(self.h, kind))
+
+    @property
+    def tuning(self) -> dict:
+        t = Tuning()
+        _check(lib().ipr_batch_graph_get_tuning(self.h, C.byref(t)), "get tuning")
+        return {f: int(getattr(t, f)) for f, _ in Tuning._fields_}
+
+    def tune(self, **kw) -> None:
+        cur = self.tuning
+        cur.update(kw)
+        t = Tuning(*[cur[f] for f, _ in Tuning._fields_])
+        _check(lib().ipr_batch_graph_set_tuning(self.h, C.byref(t)), "set tuning")
 
     def batch_sssp(self, sources) -> tuple[np.ndarray, dict]:
         s = np.ascontiguousarray(sources, dtype=np.uint32)
@@ -207,14 +227,30 @@ class ServeResult:
     def latency(self) -> np.ndarray:
         return self.t_done - self.t_arrival
 
-    def summary(self) -> dict:
+    def summary(self, trace: np.ndarray | None = None, names: list[str] | None = None) -> dict:
         lat = self.latency
         n = lat.size
-        return {"queries": n, "qps": n / self.wall_s if self.wall_s > 0 else 0.0,
-                "p50_ms": float(np.percentile(lat, 50) * 1e3) if n else 0.0,
-                "p99_ms": float(np.percentile(lat, 99) * 1e3) if n else 0.0,
-                "wall_s": self.wall_s, "ticks": self.ticks, "admitted_waits": self.admitted_waits,
-                "peak_slots": self.peak_slots, "hbm_peak_bytes": self.hbm_peak}
+        out = {"queries": n, "qps": n / self.wall_s if self.wall_s > 0 else 0.0,
+               "p50_ms": float(np.percentile(lat, 50) * 1e3) if n else 0.0,
+               "p99_ms": float(np.percentile(lat, 99) * 1e3) if n else 0.0,
+               "wall_s": self.wall_s, "ticks": self.ticks, "admitted_waits": self.admitted_waits,
+               "peak_slots": self.peak_slots, "hbm_peak_bytes": self.hbm_peak}
+        if trace is not None:
+            per = {}
+            for k, kn in KIND_NAMES.items():
+                m = trace["kind"] == k
+                if m.any():
+                    per[kn] = {"n": int(m.sum()), "p50_ms": float(np.percentile(lat[m], 50) * 1e3),
+                               "p99_ms": float(np.percentile(lat[m], 99) * 1e3)}
+            out["by_kind"] = per
+            if names is not None:
+                pg = {}
+                for gi in np.unique(trace["graph"]):
+                    m = trace["graph"] == gi
+                    pg[names[int(gi)]] = {"n": int(m.sum()), "p50_ms": float(np.percentile(lat[m], 50) * 1e3),
+                                          "max_ms": float(lat[m].max() * 1e3)}
+                out["by_graph"] = pg
+        return out
 
 
 def poisson_trace(n: int, rate: float, mix: dict[int, float], graphs_by_kind: dict[int, list[int]],

@@ -67,7 +67,16 @@ double now_s()
     return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
 }
 
-Tuning default_tuning() { return Tuning{1024u, 1024u, 16u, 2048u}; }
+Tuning default_tuning() { return Tuning{1024u, 1024u, 2048u, INF, INF, 1u}; }
+
+/* IPR_BATCH_GRAPHS=0 makes every state launch its kernels directly instead of
+ * replaying captured HIP graphs (same kernels, same order) */
+static int graph_mode(int requested)
+{
+    const char* e = getenv("IPR_BATCH_GRAPHS");
+    if (e && e[0] == '0') return 2;
+    return requested;
+}
 
 unsigned long long digest_u32_host(const uint32_t* v, uint64_t n)
 {
@@ -207,15 +216,28 @@ struct PushArgs {
     unsigned long long* act;
     unsigned long long* vmask;
     uint8_t* touched;
+    uint8_t* tgran;     /* per 256 vertices: any touched byte set */
     uint32_t* small0;
     uint32_t* small1;
     uint32_t* large0;
     uint32_t* large1;
     uint32_t* lpre;
     PushCtl* ctl;
+    void* pend;         /* parked frontier entries (byte per vertex, or slot mask when batched) */
+    uint32_t* fst;      /* frontier stamps */
     uint32_t large, chunk;
     uint32_t wzero;     /* 1: every arc weighs 0 (CC) */
+    uint32_t delta;     /* near/far step (INF: plain BSP) */
 };
+
+/* a vertex receives a message: its byte in the touched map, and the byte of
+ * its 256-vertex granule, which lets the apply pass skip untouched granules
+ * without reading their bytes */
+__device__ __forceinline__ void mark(const PushArgs& a, uint32_t v)
+{
+    a.touched[v] = 1;
+    a.tgran[v >> 8] = 1;
+}
 
 __device__ __forceinline__ uint32_t* small_of(const PushArgs& a, int p) { return p ? a.small1 : a.small0; }
 __device__ __forceinline__ uint32_t* large_of(const PushArgs& a, int p) { return p ? a.large1 : a.large0; }
@@ -234,6 +256,23 @@ __global__ void k_push_begin(PushArgs a, int cur)
     if (t < 64) {
         a.ctl->orm[t] = 0;
         a.ctl->relax[t] = 0;
+        for (int k = 0; k < SPREAD; ++k) {
+            a.ctl->fmin[k][t] = INF;
+            a.ctl->pdmin[k][t] = INF;
+        }
+    }
+    if (t < SPREAD) a.ctl->pdef[t] = 0;
+}
+
+/* a new state: no parked entries, no thresholds */
+__global__ void k_ctl_init(PushArgs a)
+{
+    const int t = threadIdx.x;   /* 64 threads */
+    a.ctl->thr[t] = INF;
+    a.ctl->pmin[t] = INF;
+    for (int k = 0; k < SPREAD; ++k) {
+        a.ctl->pdmin[k][t] = INF;
+        a.ctl->pscan[k][t] = INF;
     }
 }
 
@@ -272,18 +311,30 @@ __global__ __launch_bounds__(256) void k_relax_small_s(PushArgs a, int cur)
     __shared__ uint32_t s_val[256];
     __shared__ uint32_t s_w[5];
     const uint32_t n = a.ctl->nsmall[cur];
+    const uint32_t thr = a.ctl->thr[0];
+    uint8_t* pend = (uint8_t*)a.pend;
     const uint32_t* F = small_of(a, cur);
     const uint32_t ntiles = (n + 255) / 256;
     const int t = threadIdx.x;
+    bool parked = false;
+    uint32_t pmn = INF;
     for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const uint32_t i = tile * 256 + t;
         uint32_t deg = 0;
         if (i < n) {
             const uint32_t u = F[i];
-            const uint64_t e0 = a.oidx[u];
-            deg = (uint32_t)(a.oidx[u + 1] - e0);
-            s_e0[t] = e0;
-            s_val[t] = a.val[u];
+            const uint32_t du = a.val[u];
+            if (du < thr) {
+                const uint64_t e0 = a.oidx[u];
+                deg = (uint32_t)(a.oidx[u + 1] - e0);
+                s_e0[t] = e0;
+                s_val[t] = du;
+            } else {
+                /* far: parked until the threshold passes it */
+                pend[u] = 1;
+                parked = true;
+                pmn = min(pmn, du);
+            }
         }
         uint32_t A;
         const uint32_t ex = block_excl_scan_u32<256>(deg, s_w, A);
@@ -298,20 +349,52 @@ __global__ __launch_bounds__(256) void k_relax_small_s(PushArgs a, int cur)
                 const uint32_t c = sadd(s_val[o], arc_w(a, e));
                 if (c < a.mbox[v]) {
                     atomicMin(&a.mbox[v], c);
-                    a.touched[v] = 1;
+                    mark(a, v);
                 }
             }
         }
         __syncthreads();
     }
+    if (__syncthreads_or(parked)) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) pmn = min(pmn, (uint32_t)__shfl_xor(pmn, o, 64));
+        if ((t & 63) == 0 && pmn != INF) {
+            atomicOr(&a.ctl->pdef[blockIdx.x % SPREAD], 1ull);
+            atomicMin(&a.ctl->pdmin[blockIdx.x % SPREAD][0], pmn);
+        }
+    }
 }
 
-/* batched: tiles of 64 frontier vertices; each thread loads one arc per round
- * and its wavefront then relaxes the 64 arcs it loaded, lane q for query q */
+/* position of the r-th (from 0) set bit of m; r < popcount(m) */
+__host__ __device__ __forceinline__ uint32_t nth_bit(unsigned long long m, uint32_t r)
+{
+    uint32_t pos = 0, c;
+    c = (uint32_t)__builtin_popcountll(m & 0xFFFFFFFFull);
+    if (r >= c) { r -= c; m >>= 32; pos += 32; }
+    c = (uint32_t)__builtin_popcountll(m & 0xFFFFull);
+    if (r >= c) { r -= c; m >>= 16; pos += 16; }
+    c = (uint32_t)__builtin_popcountll(m & 0xFFull);
+    if (r >= c) { r -= c; m >>= 8; pos += 8; }
+    c = (uint32_t)__builtin_popcountll(m & 0xFull);
+    if (r >= c) { r -= c; m >>= 4; pos += 4; }
+    c = (uint32_t)__builtin_popcountll(m & 0x3ull);
+    if (r >= c) { r -= c; m >>= 2; pos += 2; }
+    if (r >= (uint32_t)(m & 1ull)) pos += 1;
+    return pos;
+}
+
+/* a frontier vertex with at most PAIR_SLOTS active slots and at most PAIR_DEG
+ * out-arcs is relaxed as (vertex, slot) pairs, one pair per lane; the others
+ * as rows, one arc per wavefront step for all slots at once */
+constexpr uint32_t PAIR_SLOTS = 8, PAIR_DEG = 32;
+
+/* batched: tiles of 64 frontier vertices */
 __global__ __launch_bounds__(256) void k_relax_small_b(PushArgs a, int cur)
 {
-    __shared__ uint32_t s_pre[65];
+    __shared__ uint32_t s_pre[65];     /* arcs of row-mode vertices, exclusive prefix */
+    __shared__ uint32_t s_pp[65];      /* pairs of pair-mode vertices, exclusive prefix */
     __shared__ uint64_t s_e0[64];
+    __shared__ uint32_t s_deg[64];
     __shared__ uint32_t s_u[64];
     __shared__ unsigned long long s_m[64];
     __shared__ uint32_t s_d[64 * 64];
@@ -319,6 +402,10 @@ __global__ __launch_bounds__(256) void k_relax_small_b(PushArgs a, int cur)
     const uint32_t* F = small_of(a, cur);
     const uint32_t ntiles = (n + 63) / 64;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const uint32_t thrl = a.ctl->thr[lane];
+    unsigned long long* pend = (unsigned long long*)a.pend;
+    unsigned long long parked = 0;
+    uint32_t pmn = INF;
     for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         if (t < 64) {
             const uint32_t i = tile * 64 + t;
@@ -333,14 +420,44 @@ __global__ __launch_bounds__(256) void k_relax_small_b(PushArgs a, int cur)
             }
             s_u[t] = u;
             s_m[t] = m;
-            const uint32_t inc = wave_incl_scan_u32(deg);
-            s_pre[t] = inc - deg;
-            if (t == 63) s_pre[64] = inc;
+            s_deg[t] = deg;
         }
         __syncthreads();
         const uint32_t nt = min(64u, n - tile * 64);
-        for (uint32_t r = wave; r < nt; r += 4) s_d[r * 64 + lane] = a.val[(uint64_t)s_u[r] * 64 + lane];
+        for (uint32_t r = wave; r < nt; r += 4) {
+            /* lanes whose distance is below their threshold relax now; the
+             * others stay pending at the vertex */
+            const uint32_t d = a.val[(uint64_t)s_u[r] * 64 + lane];
+            s_d[r * 64 + lane] = d;
+            const bool on = (s_m[r] >> lane) & 1ull;
+            const bool near = on && d < thrl;
+            if (on && !near) pmn = min(pmn, d);
+            const unsigned long long nm = __ballot(near), dm = __ballot(on && !near);
+            if (lane == 0) {
+                s_m[r] = nm;
+                if (dm) {
+                    /* far slots: parked until their threshold passes them */
+                    pend[s_u[r]] |= dm;
+                    parked |= dm;
+                }
+            }
+        }
         __syncthreads();
+        if (t < 64) {
+            const unsigned long long nm = s_m[t];
+            const uint32_t deg = s_deg[t], k = (uint32_t)__popcll(nm);
+            const bool pair = k && k <= PAIR_SLOTS && deg <= PAIR_DEG;
+            const uint32_t rowdeg = (k && !pair) ? deg : 0u, pairs = pair ? k : 0u;
+            const uint32_t ia = wave_incl_scan_u32(rowdeg), ib = wave_incl_scan_u32(pairs);
+            s_pre[t] = ia - rowdeg;
+            s_pp[t] = ib - pairs;
+            if (t == 63) {
+                s_pre[64] = ia;
+                s_pp[64] = ib;
+            }
+        }
+        __syncthreads();
+        /* rows: 256 arcs per round, each wavefront relaxes the 64 it loaded */
         const uint32_t A = s_pre[64];
         for (uint32_t base = 0; base < A; base += 256) {
             const uint32_t k = base + t;
@@ -363,7 +480,8 @@ __global__ __launch_bounds__(256) void k_relax_small_b(PushArgs a, int cur)
                 for (int i = 0; i < 8; ++i) {
                     const uint32_t j = (j0 + i) & 63;
                     const unsigned long long vm = rl64(vmk, j);
-                    dv[i] = (j0 + i < cnt && ((vm >> lane) & 1ull)) ? a.val[(uint64_t)rl(v, j) * 64 + lane] : INF;
+                    const bool need = j0 + i < cnt && ((vm >> lane) & 1ull) && ((s_m[rl(o, j)] >> lane) & 1ull);
+                    dv[i] = need ? a.val[(uint64_t)rl(v, j) * 64 + lane] : INF;
                 }
 #pragma unroll
                 for (int i = 0; i < 8; ++i) {
@@ -373,12 +491,50 @@ __global__ __launch_bounds__(256) void k_relax_small_b(PushArgs a, int cur)
                     const uint32_t cd = sadd(s_d[oj * 64 + lane], rl(w, j));
                     const bool go = ((s_m[oj] >> lane) & 1ull) && cd < dv[i];
                     if (go) atomicMin(&a.mbox[(uint64_t)vj * 64 + lane], cd);
-                    if (__ballot(go) && lane == 0) a.touched[vj] = 1;
+                    if (__ballot(go) && lane == 0) mark(a, vj);
+                }
+            }
+        }
+        /* pairs: lane = (vertex, slot), walking the vertex's short out-list */
+        const uint32_t P = s_pp[64];
+        for (uint32_t pb = (uint32_t)wave * 64; pb < P; pb += 256) {
+            const uint32_t p = pb + lane;
+            if (p < P) {
+                const uint32_t o = owner_of(s_pp, 64, p);
+                const uint32_t q = nth_bit(s_m[o], p - s_pp[o]);
+                const uint32_t du = s_d[o * 64 + q];
+                const uint64_t e0 = s_e0[o];
+                const uint32_t deg = s_deg[o];
+                for (uint32_t k0 = 0; k0 < deg; k0 += 4) {
+                    /* four arcs: targets, weights, then their distances, then the messages */
+                    uint32_t vv[4], cd[4], dv[4];
+                    unsigned long long vm[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const bool ok = k0 + i < deg;
+                        vv[i] = ok ? a.oadj[e0 + k0 + i] : 0u;
+                        cd[i] = ok ? sadd(du, arc_w(a, e0 + k0 + i)) : INF;
+                    }
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        vm[i] = k0 + i < deg ? a.vmask[vv[i]] : 0ull;
+                        dv[i] = k0 + i < deg ? a.val[(uint64_t)vv[i] * 64 + q] : INF;
+                    }
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const uint32_t cur = ((vm[i] >> q) & 1ull) ? dv[i] : INF;
+                        if (k0 + i < deg && cd[i] < cur) {
+                            atomicMin(&a.mbox[(uint64_t)vv[i] * 64 + q], cd[i]);
+                            mark(a, vv[i]);
+                        }
+                    }
                 }
             }
         }
         __syncthreads();
     }
+    if (lane == 0 && parked) atomicOr(&a.ctl->pdef[blockIdx.x % SPREAD], parked);
+    if (pmn != INF) atomicMin(&a.ctl->pdmin[blockIdx.x % SPREAD][lane], pmn);
 }
 
 /* large frontier vertices: chunk c of the large list goes to workgroup c mod grid */
@@ -396,6 +552,7 @@ __global__ __launch_bounds__(256) void k_relax_large(PushArgs a, int cur)
         for (uint32_t i = t; i <= n; i += 256) s_lp[i] = a.lpre[i];
     __syncthreads();
     const uint32_t* lp = inl ? s_lp : a.lpre;
+    const uint32_t thrl = a.ctl->thr[BATCHED ? lane : 0];
     for (uint32_t c = blockIdx.x; c < T; c += gridDim.x) {
         const uint32_t i = owner_of(lp, n, c);
         const uint32_t u = L[i];
@@ -403,18 +560,34 @@ __global__ __launch_bounds__(256) void k_relax_large(PushArgs a, int cur)
         const uint64_t e1 = min(e0 + a.chunk, a.oidx[u + 1]);
         if (!BATCHED) {
             const uint32_t du = a.val[u];
+            if (du >= thrl) {
+                if (c == lp[i] && t == 0) {
+                    ((uint8_t*)a.pend)[u] = 1;
+                    atomicOr(&a.ctl->pdef[blockIdx.x % SPREAD], 1ull);
+                    atomicMin(&a.ctl->pdmin[blockIdx.x % SPREAD][0], du);
+                }
+                continue;
+            }
             for (uint64_t e = e0 + t; e < e1; e += 256) {
                 const uint32_t v = a.oadj[e];
                 const uint32_t c = sadd(du, arc_w(a, e));
                 if (c < a.mbox[v]) {
                     atomicMin(&a.mbox[v], c);
-                    a.touched[v] = 1;
+                    mark(a, v);
                 }
             }
         } else {
             const unsigned long long m = a.act[u];
-            const bool on = (m >> lane) & 1ull;
             const uint32_t du = a.val[(uint64_t)u * 64 + lane];
+            const bool act_l = (m >> lane) & 1ull;
+            const bool on = act_l && du < thrl;
+            const unsigned long long dm = __ballot(act_l && !on);
+            if (dm && c == lp[i] && t == 0) {
+                ((unsigned long long*)a.pend)[u] |= dm;
+                atomicOr(&a.ctl->pdef[blockIdx.x % SPREAD], dm);
+            }
+            if (act_l && !on && c == lp[i] && wave == 0) atomicMin(&a.ctl->pdmin[blockIdx.x % SPREAD][lane], du);
+            if (!__ballot(on)) continue;
             for (uint64_t base = e0; base < e1; base += 256) {
                 const uint64_t e = base + t;
                 uint32_t v = 0, w = 0;
@@ -443,7 +616,7 @@ __global__ __launch_bounds__(256) void k_relax_large(PushArgs a, int cur)
                         const uint32_t cd = sadd(du, rl(w, j));
                         const bool go = on && cd < dv[i];
                         if (go) atomicMin(&a.mbox[(uint64_t)vj * 64 + lane], cd);
-                        if (__ballot(go) && lane == 0) a.touched[vj] = 1;
+                        if (__ballot(go) && lane == 0) mark(a, vj);
                     }
                 }
             }
@@ -459,8 +632,15 @@ __global__ __launch_bounds__(256) void k_apply_s(PushArgs a, int cur)
     const int nxt = cur ^ 1, t = threadIdx.x;
     uint32_t* S = small_of(a, nxt);
     uint32_t* Lg = large_of(a, nxt);
+    uint8_t* pend = (uint8_t*)a.pend;
+    const uint32_t step = a.ctl->step;
     unsigned long long rel = 0;
+    uint32_t fmn = INF;
     for (uint64_t gb = (uint64_t)blockIdx.x * 1024; gb < a.V; gb += (uint64_t)gridDim.x * 1024) {
+        /* the four granule bytes of these 1024 vertices (padded map) */
+        if (*(const uint32_t*)(a.tgran + (gb >> 8)) == 0) continue;
+        __syncthreads();
+        if (t == 0) *(uint32_t*)(a.tgran + (gb >> 8)) = 0;
         const uint64_t v0 = gb + (uint64_t)t * 4;
         uint32_t f4 = 0;
         if (v0 < a.V) f4 = *(const uint32_t*)(a.touched + v0);   /* padded to 1024 */
@@ -472,9 +652,12 @@ __global__ __launch_bounds__(256) void k_apply_s(PushArgs a, int cur)
             for (int k = 0; k < 4; ++k) {
                 if (!((f4 >> (8 * k)) & 0xff)) continue;
                 const uint64_t v = v0 + k;
-                const uint32_t m = a.mbox[v];
-                if (m < a.val[v]) {
+                const uint32_t m = a.mbox[v], d = a.val[v];
+                if (m < d) {
                     a.val[v] = m;
+                    if (pend[v]) pend[v] = 0;     /* its parked entry is superseded */
+                    a.fst[v] = step;
+                    fmn = min(fmn, m);
                     const uint64_t deg = a.oidx[v + 1] - a.oidx[v];
                     rel += deg;
                     if (deg > a.large) ll[nl++] = (uint32_t)v; else ls[ns++] = (uint32_t)v;
@@ -495,9 +678,13 @@ __global__ __launch_bounds__(256) void k_apply_s(PushArgs a, int cur)
     }
     rel = wave_sum_u64(rel);
     if ((t & 63) == 0 && rel) atomicAdd(&a.ctl->relax[blockIdx.x & 63], rel);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) fmn = min(fmn, (uint32_t)__shfl_xor(fmn, o, 64));
+    if ((t & 63) == 0 && fmn != INF) atomicMin(&a.ctl->fmin[blockIdx.x % SPREAD][0], fmn);
 }
 
-/* batched apply: a wavefront scans 64 byte flags and folds each marked row */
+/* batched apply: a workgroup takes a 256-vertex granule; each wavefront scans
+ * 64 byte flags and folds the marked rows 8 at a time, all loads issued first */
 __global__ __launch_bounds__(256) void k_apply_b(PushArgs a, int cur)
 {
     __shared__ uint32_t s_buf[2][4][64];
@@ -506,8 +693,14 @@ __global__ __launch_bounds__(256) void k_apply_b(PushArgs a, int cur)
     const int nxt = cur ^ 1, t = threadIdx.x, lane = t & 63, wave = t >> 6;
     uint32_t* S = small_of(a, nxt);
     uint32_t* Lg = large_of(a, nxt);
+    unsigned long long* pend = (unsigned long long*)a.pend;
+    const uint32_t step = a.ctl->step;
     unsigned long long orm = 0, rel = 0, dig = 0;
+    uint32_t fmn = INF;
     for (uint64_t gb = (uint64_t)blockIdx.x * 256; gb < a.V; gb += (uint64_t)gridDim.x * 256) {
+        if (a.tgran[gb >> 8] == 0) continue;
+        __syncthreads();
+        if (t == 0) a.tgran[gb >> 8] = 0;
         const uint64_t vb = gb + (uint64_t)wave * 64, vl = vb + lane;
         const uint8_t fl = vl < a.V ? a.touched[vl] : 0;
         if (!__syncthreads_or(fl != 0)) continue;
@@ -515,33 +708,64 @@ __global__ __launch_bounds__(256) void k_apply_b(PushArgs a, int cur)
         if (fl) a.touched[vl] = 0;
         uint32_t ns = 0, nl = 0;
         while (tm) {
-            const int j = __ffsll((unsigned long long)tm) - 1;
-            tm &= tm - 1;
-            const uint64_t v = vb + j, ix = v * 64 + lane;
-            const uint32_t m = a.mbox[ix];
-            const unsigned long long vm = a.vmask[v];
-            const uint32_t d = ((vm >> lane) & 1ull) ? a.val[ix] : INF;
-            const bool imp = m < d;
-            if (imp) {
-                a.val[ix] = m;
-                dig += dterm(v, m) - dterm(v, d);
-            }
-            if (m != INF) a.mbox[ix] = INF;
-            const unsigned long long im = __ballot(imp);
-            if (im) {
-                if (lane == 0) {
-                    a.vmask[v] = vm | im;
-                    a.act[v] = im;
+            uint32_t js[8];
+            uint32_t nb = 0;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                js[i] = tm ? (uint32_t)(__ffsll((unsigned long long)tm) - 1) : 0u;
+                if (tm) {
+                    tm &= tm - 1;
+                    nb = i + 1;
                 }
-                const uint64_t deg = a.oidx[v + 1] - a.oidx[v];
-                orm |= im;
-                rel += deg * (unsigned long long)__popcll(im);
-                if (deg > a.large) {
-                    if (lane == 0) s_buf[1][wave][nl] = (uint32_t)v;
-                    nl++;
-                } else {
-                    if (lane == 0) s_buf[0][wave][ns] = (uint32_t)v;
-                    ns++;
+            }
+            uint32_t mm[8], dd[8];
+            unsigned long long vmv[8], pmv[8];
+            uint64_t dg[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                if ((uint32_t)i < nb) {
+                    const uint64_t v = vb + js[i], ix = v * 64 + lane;
+                    mm[i] = a.mbox[ix];
+                    dd[i] = a.val[ix];
+                    vmv[i] = a.vmask[v];
+                    pmv[i] = pend[v];
+                    dg[i] = a.oidx[v + 1] - a.oidx[v];
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                if ((uint32_t)i >= nb) break;
+                const uint64_t v = vb + js[i], ix = v * 64 + lane;
+                const uint32_t m = mm[i];
+                const unsigned long long vm = vmv[i];
+                const uint32_t d = ((vm >> lane) & 1ull) ? dd[i] : INF;
+                const bool imp = m < d;
+                if (imp) {
+                    a.val[ix] = m;
+                    dig += dterm(v, m) - dterm(v, d);
+                }
+                if (m != INF) a.mbox[ix] = INF;
+                /* next frontier at v: the improved slots (parked entries of
+                 * those slots at v are superseded) */
+                const unsigned long long im = __ballot(imp), pm = pmv[i];
+                if (im) {
+                    if (lane == 0) {
+                        a.vmask[v] = vm | im;
+                        a.act[v] = im;
+                        a.fst[v] = step;
+                        if (pm & im) pend[v] = pm & ~im;
+                    }
+                    if (imp) fmn = min(fmn, m);
+                    const uint64_t deg = dg[i];
+                    orm |= im;
+                    rel += deg * (unsigned long long)__popcll(im);
+                    if (deg > a.large) {
+                        if (lane == 0) s_buf[1][wave][nl] = (uint32_t)v;
+                        nl++;
+                    } else {
+                        if (lane == 0) s_buf[0][wave][ns] = (uint32_t)v;
+                        ns++;
+                    }
                 }
             }
         }
@@ -567,9 +791,135 @@ __global__ __launch_bounds__(256) void k_apply_b(PushArgs a, int cur)
     if (lane == 0 && orm) atomicOr(&a.ctl->orm[blockIdx.x & 63], orm);
     if (lane == 0 && rel) atomicAdd(&a.ctl->relax[blockIdx.x & 63], rel);
     if (dig) atomicAdd(&a.ctl->dig[blockIdx.x % SPREAD][lane], dig);
+    if (fmn != INF) atomicMin(&a.ctl->fmin[blockIdx.x % SPREAD][lane], fmn);
 }
 
-__global__ void k_push_finish(PushArgs a, int nxt)
+/* after the apply pass: per query, if no entry of the next frontier lies below
+ * the threshold, move the threshold (past the smallest next-frontier distance,
+ * or by delta when entries are parked) and ask for a wake-up scan */
+__global__ void k_push_thr(PushArgs a)
+{
+    const int t = threadIdx.x;   /* 64 threads, one per slot */
+    unsigned long long pd = 0;
+    for (int k = 0; k < SPREAD; ++k) pd |= a.ctl->pdef[k];
+    const unsigned long long pleft = a.ctl->pleft | pd;
+    uint32_t fm = INF;
+    for (int k = 0; k < SPREAD; ++k) fm = min(fm, a.ctl->fmin[k][t]);
+    const uint32_t th = a.ctl->thr[t];
+    uint32_t pm = a.ctl->pmin[t];
+    for (int k = 0; k < SPREAD; ++k) pm = min(pm, a.ctl->pdmin[k][t]);
+    a.ctl->pmin[t] = pm;
+    const bool parked = (pleft >> t) & 1ull;
+    bool wake = false;
+    if (fm >= th) {
+        /* past the smallest distance waiting, in the frontier or parked */
+        const uint32_t tg = parked ? min(fm, pm) : fm;
+        uint32_t nt = th;
+        if (tg != INF) nt = a.delta >= INF - tg ? INF : tg + a.delta;
+        else if (parked) nt = a.delta >= INF - th ? INF : th + a.delta;
+        a.ctl->thr[t] = nt;
+        wake = parked && (pm < nt || pm == INF);
+    }
+    const unsigned long long wm = __ballot(wake);
+    if (t == 0) {
+        a.ctl->pleft = pleft;
+        a.ctl->react = wm;
+    }
+}
+
+/* wake-up scan: parked entries of the queries in react that now lie below their
+ * threshold join the next frontier (once per vertex: frontier stamps); the set
+ * of queries with entries still parked is recomputed exactly */
+template <bool BATCHED>
+__global__ __launch_bounds__(256) void k_push_react(PushArgs a, int nxt)
+{
+    __shared__ uint32_t s_w[5];
+    __shared__ uint32_t s_base[2];
+    __shared__ uint32_t s_pmin[64];
+    const unsigned long long react = a.ctl->react;
+    if (!react) return;
+    if (threadIdx.x < 64) s_pmin[threadIdx.x] = INF;
+    __syncthreads();
+    const uint32_t step = a.ctl->step;
+    const int t = threadIdx.x;
+    uint32_t* S = small_of(a, nxt);
+    uint32_t* Lg = large_of(a, nxt);
+    unsigned long long rem = 0, orm = 0, rel = 0;
+    for (uint64_t gb = (uint64_t)blockIdx.x * 256; gb < a.V; gb += (uint64_t)gridDim.x * 256) {
+        const uint64_t v = gb + t;
+        unsigned long long nb = 0;
+        uint32_t isl = 0, iss = 0;
+        if (v < a.V) {
+            const unsigned long long pm =
+                BATCHED ? ((unsigned long long*)a.pend)[v] : (unsigned long long)((uint8_t*)a.pend)[v];
+            unsigned long long bits = pm & react;
+            while (bits) {
+                const int q = __ffsll(bits) - 1;
+                bits &= bits - 1;
+                const uint32_t d = BATCHED ? a.val[v * 64 + q] : a.val[v];
+                if (d < a.ctl->thr[q]) nb |= 1ull << q;
+                else atomicMin(&s_pmin[q], d);
+            }
+            rem |= pm & ~nb;
+            if (nb) {
+                if (BATCHED) {
+                    ((unsigned long long*)a.pend)[v] = pm & ~nb;
+                    if (a.fst[v] == step) {
+                        a.act[v] |= nb;
+                    } else {
+                        a.act[v] = nb;
+                        a.fst[v] = step;
+                        isl = 1;
+                    }
+                } else {
+                    ((uint8_t*)a.pend)[v] = 0;
+                    if (a.fst[v] != step) {
+                        a.fst[v] = step;
+                        isl = 1;
+                    }
+                }
+                const uint64_t deg = a.oidx[v + 1] - a.oidx[v];
+                orm |= nb;
+                rel += deg * (unsigned long long)__popcll(nb);
+                if (isl && deg <= a.large) {
+                    iss = 1;
+                    isl = 0;
+                }
+            }
+        }
+        if (!__syncthreads_or(iss | isl)) continue;
+        uint32_t ts, tl;
+        const uint32_t xs = block_excl_scan_u32<256>(iss, s_w, ts);
+        const uint32_t xl = block_excl_scan_u32<256>(isl, s_w, tl);
+        if (t == 0) {
+            s_base[0] = ts ? atomicAdd(&a.ctl->nsmall[nxt], ts) : 0;
+            s_base[1] = tl ? atomicAdd(&a.ctl->nlarge[nxt], tl) : 0;
+        }
+        __syncthreads();
+        if (iss) S[s_base[0] + xs] = (uint32_t)v;
+        if (isl) Lg[s_base[1] + xl] = (uint32_t)v;
+        __syncthreads();
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        uint32_t lo = __shfl_xor((uint32_t)rem, o, 64), hi = __shfl_xor((uint32_t)(rem >> 32), o, 64);
+        rem |= ((unsigned long long)hi << 32) | lo;
+        lo = __shfl_xor((uint32_t)orm, o, 64);
+        hi = __shfl_xor((uint32_t)(orm >> 32), o, 64);
+        orm |= ((unsigned long long)hi << 32) | lo;
+    }
+    rel = wave_sum_u64(rel);
+    __syncthreads();
+    if (t < 64 && s_pmin[t] != INF) atomicMin(&a.ctl->pscan[blockIdx.x % SPREAD][t], s_pmin[t]);
+    if ((t & 63) == 0) {
+        if (rem) atomicOr(&a.ctl->prem[blockIdx.x % SPREAD], rem);
+        if (orm) atomicOr(&a.ctl->orm[blockIdx.x & 63], orm);
+        if (rel) atomicAdd(&a.ctl->relax[blockIdx.x & 63], rel);
+    }
+}
+
+/* end of the superstep: what the host reads back */
+__global__ void k_push_out(PushArgs a, int nxt, int batched)
 {
     const int t = threadIdx.x;   /* 64 threads */
     unsigned long long o = a.ctl->orm[t], r = a.ctl->relax[t], d = 0;
@@ -581,12 +931,35 @@ __global__ void k_push_finish(PushArgs a, int nxt)
     }
     r = wave_sum_u64(r);
     a.ctl->out.dig[t] = d;
+    const unsigned long long scanned = a.ctl->react;
+    if ((scanned >> t) & 1ull) {
+        uint32_t pm = INF;
+        for (int k = 0; k < SPREAD; ++k) {
+            pm = min(pm, a.ctl->pscan[k][t]);
+            a.ctl->pscan[k][t] = INF;
+        }
+        a.ctl->pmin[t] = pm;
+    }
+    __syncthreads();
     if (t == 0) {
+        unsigned long long pleft = a.ctl->pleft;
+        if (a.ctl->react) {
+            pleft = 0;
+            for (int k = 0; k < SPREAD; ++k) {
+                pleft |= a.ctl->prem[k];
+                a.ctl->prem[k] = 0;
+            }
+            a.ctl->pleft = pleft;
+            a.ctl->react = 0;
+        }
+        const uint32_t ns = a.ctl->nsmall[nxt], nl = a.ctl->nlarge[nxt];
         a.ctl->relax_total += r;
-        a.ctl->out.orm = o;
+        /* running queries: in the next frontier, or with parked entries */
+        a.ctl->out.orm = (batched ? o : (ns + nl ? 1ull : 0ull)) | pleft;
         a.ctl->out.relax = a.ctl->relax_total;
-        a.ctl->out.nsmall = a.ctl->nsmall[nxt];
-        a.ctl->out.nlarge = a.ctl->nlarge[nxt];
+        a.ctl->out.nsmall = ns;
+        a.ctl->out.nlarge = nl;
+        a.ctl->step += 1;
     }
 }
 
@@ -599,10 +972,11 @@ __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t x)
 /* scalar SSSP: the source receives distance 0 in the first superstep */
 __global__ void k_start_sssp_s(PushArgs a, int cur, uint32_t src)
 {
+    a.ctl->thr[0] = a.delta;
     a.ctl->nsmall[cur] = 0;
     a.ctl->nlarge[cur] = 0;
     a.mbox[src] = 0;
-    a.touched[src] = 1;
+    mark(a, src);
 }
 
 /* scalar CC: labels = ids, every vertex in the first frontier */
@@ -639,6 +1013,10 @@ __global__ __launch_bounds__(256) void k_start_cc_s(PushArgs a, int cur)
 __global__ void k_ctl_zero(PushArgs a, int cur)
 {
     a.ctl->relax_total = 0;
+    a.ctl->thr[0] = INF;
+    a.ctl->pleft = 0;
+    a.ctl->react = 0;
+    a.ctl->pmin[0] = INF;
     a.ctl->nsmall[cur] = 0;
     a.ctl->nlarge[cur] = 0;
     a.ctl->nsmall[cur ^ 1] = 0;
@@ -659,7 +1037,9 @@ __global__ void k_admit_b(PushArgs a, AdmitArgs ad, unsigned long long base)
     const int q = ad.slot[i];
     const uint32_t s = ad.src[i];
     a.mbox[(uint64_t)s * 64 + q] = 0;
-    a.touched[s] = 1;
+    mark(a, s);
+    a.ctl->thr[q] = a.delta;
+    a.ctl->pmin[q] = INF;
     a.ctl->dig[0][q] = base;
     for (int k = 1; k < SPREAD; ++k) a.ctl->dig[k][q] = 0;
 }
@@ -696,13 +1076,74 @@ __global__ void k_digest_u32(const uint32_t* x, uint64_t n, unsigned long long* 
 }
 
 /* ------------------------------------------------------------------------ */
+/* CC by union-find                                                          */
+/* ------------------------------------------------------------------------ */
+/* parent[v] <= v always; a root is linked only under a smaller root, so each
+ * tree's root is the smallest id of its component: the labels equal the
+ * label-propagation fixed point (component minimum) on a symmetric graph,
+ * whatever the order of the links. Path halving writes only ancestors, so
+ * concurrent finds and links stay consistent. */
+
+__device__ __forceinline__ uint32_t uf_root(uint32_t* p, uint32_t x)
+{
+    volatile uint32_t* vp = p;
+    for (;;) {
+        const uint32_t px = vp[x];
+        if (px == x) return x;
+        const uint32_t ppx = vp[px];
+        if (ppx == px) return px;
+        vp[x] = ppx;
+        x = ppx;
+    }
+}
+
+__device__ __forceinline__ void uf_link(uint32_t* p, uint32_t u, uint32_t v)
+{
+    uint32_t ru = uf_root(p, u), rv = uf_root(p, v);
+    while (ru != rv) {
+        const uint32_t hi = max(ru, rv), lo = min(ru, rv);
+        const uint32_t old = atomicCAS(&p[hi], hi, lo);
+        if (old == hi) return;
+        /* hi stopped being a root meanwhile: retry from the current roots */
+        ru = uf_root(p, old);
+        rv = uf_root(p, lo);
+    }
+}
+
+__global__ void k_uf_init(uint32_t* p, uint64_t V)
+{
+    const uint64_t st = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += st) p[v] = (uint32_t)v;
+}
+
+/* wavefront per vertex, lanes over its arcs */
+__global__ __launch_bounds__(256) void k_uf_link(const uint64_t* idx, const uint32_t* adj, uint32_t* p, uint64_t V)
+{
+    const int lane = threadIdx.x & 63;
+    const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    for (uint64_t u = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; u < V; u += nw) {
+        const uint64_t b = idx[u], e = idx[u + 1];
+        for (uint64_t k = b + lane; k < e; k += 64) {
+            const uint32_t v = adj[k];
+            if (v != (uint32_t)u) uf_link(p, (uint32_t)u, v);
+        }
+    }
+}
+
+__global__ void k_uf_flatten(uint32_t* p, uint64_t V)
+{
+    const uint64_t st = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += st) p[v] = uf_root(p, (uint32_t)v);
+}
+
+/* ------------------------------------------------------------------------ */
 /* PPR kernels                                                               */
 /* ------------------------------------------------------------------------ */
 
 struct PprArgs {
     uint64_t V, nseg, nseg64;
     const uint32_t* seg_v0;
-    uint32_t nhub, nchunk, coop;
+    uint32_t nhub, nchunk;
     const uint64_t* oidx;
     const uint64_t* iidx;
     const uint32_t* iadj;
@@ -1228,6 +1669,14 @@ extern "C" int ipr_batch_graph_create(uint64_t V, uint64_t E, const uint64_t* id
     g->V = V;
     g->E = E;
     g->tune = default_tuning();
+    if (wgt && E) {
+        /* near/far step: four mean arc weights */
+        uint64_t sum = 0;
+        for (uint64_t e = 0; e < E; e++) sum += wgt[e];
+        const uint64_t d = 4 * ((sum + E - 1) / E);
+        g->tune.delta = d >= INF ? INF : (uint32_t)(d ? d : 1);
+        g->tune.delta1 = g->tune.delta;
+    }
     g->nseg64 = (V + 63) / 64;
     int rc = 0;
     uint64_t* kidx = nullptr;
@@ -1333,6 +1782,42 @@ extern "C" int ipr_batch_graph_destroy(ipr_bgraph* g)
 
 extern "C" uint64_t ipr_batch_graph_bytes(const ipr_bgraph* g) { return g ? g->bytes : 0; }
 
+extern "C" int ipr_batch_graph_get_tuning(const ipr_bgraph* g, ipr_batch_tuning* t)
+{
+    if (!g || !t) {
+        set_error("ipr_batch_graph_get_tuning: null argument");
+        return -2;
+    }
+    t->large = g->tune.large;
+    t->chunk = g->tune.chunk;
+    t->grid = g->tune.grid;
+    t->delta = g->tune.delta;
+    t->delta1 = g->tune.delta1;
+    t->cc_mode = g->tune.cc_mode;
+    return 0;
+}
+
+extern "C" int ipr_batch_graph_set_tuning(ipr_bgraph* g, const ipr_batch_tuning* t)
+{
+    if (!g || !t) {
+        set_error("ipr_batch_graph_set_tuning: null argument");
+        return -2;
+    }
+    if (t->large < 1 || t->chunk < 64 || t->chunk > (1u << 20) || t->grid < 1 || t->grid > 65535 ||
+        t->delta < 1 || t->delta1 < 1 || t->cc_mode > 1) {
+        set_error("ipr_batch_graph_set_tuning: large >= 1, chunk in [64, 2^20], grid in [1, 65535], "
+                  "delta and delta1 >= 1, cc_mode 0 or 1");
+        return -2;
+    }
+    g->tune.large = t->large;
+    g->tune.chunk = t->chunk;
+    g->tune.grid = t->grid;
+    g->tune.delta = t->delta;
+    g->tune.delta1 = t->delta1;
+    g->tune.cc_mode = t->cc_mode;
+    return 0;
+}
+
 extern "C" uint64_t ipr_batch_slot_bytes(const ipr_bgraph* g, int kind)
 {
     if (!g) return 0;
@@ -1361,15 +1846,19 @@ static PushArgs push_args(const PushState* s)
     a.act = s->act;
     a.vmask = s->vmask;
     a.touched = s->touched;
+    a.tgran = s->tgran;
     a.small0 = s->small[0];
     a.small1 = s->small[1];
     a.large0 = s->large[0];
     a.large1 = s->large[1];
     a.lpre = s->lpre;
     a.ctl = s->ctl;
+    a.pend = s->pend;
+    a.fst = s->fst;
     a.large = g->tune.large;
     a.chunk = g->tune.chunk;
     a.wzero = 0;
+    a.delta = s->batched ? g->tune.delta : g->tune.delta1;
     return a;
 }
 
@@ -1384,8 +1873,8 @@ uint64_t PushState::footprint(const ipr_bgraph* g, bool batched)
 {
     const uint64_t V = g->V, W = batched ? 64 : 1;
     const uint64_t lc = large_capacity(g);
-    uint64_t b = 2 * V * W * 4 + ((V + 1023) / 1024) * 1024 + 2 * V * 4 + 2 * lc * 4 + (lc + 1) * 4 + sizeof(PushCtl);
-    if (batched) b += 2 * V * 8;
+    uint64_t b = 2 * V * W * 4 + ((V + 1023) / 1024) * 1028 + 2 * V * 4 + 2 * lc * 4 + (lc + 1) * 4 + sizeof(PushCtl);
+    b += (batched ? 3 * V * 8 : V) + V * 4;
     return b;
 }
 
@@ -1395,7 +1884,7 @@ int PushState::create(ipr_bgraph* g, bool batched, int use_graph, PushState** ou
     PushState* s = new PushState();
     s->g = g;
     s->batched = batched;
-    s->use_graph = use_graph;
+    s->use_graph = graph_mode(use_graph);
     const uint64_t V = g->V, W = batched ? 64 : 1;
     s->large_cap = large_capacity(g);
     int rc = 0;
@@ -1412,6 +1901,10 @@ int PushState::create(ipr_bgraph* g, bool batched, int use_graph, PushState** ou
     }
     const uint64_t tb = ((V + 1023) / 1024) * 1024;
     GO(dmalloc(&s->touched, tb, &s->bytes, "touched map"));
+    GO(dmalloc(&s->tgran, tb / 256, &s->bytes, "touched granules"));
+    if (batched) GO(dmalloc((unsigned long long**)&s->pend, V, &s->bytes, "pending masks"));
+    else GO(dmalloc((uint8_t**)&s->pend, V, &s->bytes, "pending flags"));
+    GO(dmalloc(&s->fst, V, &s->bytes, "frontier stamps"));
     for (int p = 0; p < 2; p++) {
         GO(dmalloc(&s->small[p], V, &s->bytes, "frontier"));
         GO(dmalloc(&s->large[p], s->large_cap, &s->bytes, "large frontier"));
@@ -1421,7 +1914,12 @@ int PushState::create(ipr_bgraph* g, bool batched, int use_graph, PushState** ou
     GH(hipHostMalloc((void**)&s->hout, 2 * sizeof(TickOut), hipHostMallocDefault), "pinned tick output");
     memset(s->hout, 0, 2 * sizeof(TickOut));
     GH(hipMemsetAsync(s->touched, 0, tb, s->st), "clear touched");
+    GH(hipMemsetAsync(s->tgran, 0, tb / 256, s->st), "clear granules");
+    GH(hipMemsetAsync(s->pend, 0, batched ? V * 8 : V, s->st), "clear pending");
+    GH(hipMemsetAsync(s->fst, 0xFF, V * 4, s->st), "clear stamps");
     GH(hipMemsetAsync(s->ctl, 0, sizeof(PushCtl), s->st), "clear control");
+    hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(64), 0, s->st, push_args(s));
+    GH(hipGetLastError(), "init control");
     if (batched) {
         GH(hipMemsetAsync(s->mbox, 0xFF, V * 64 * 4, s->st), "clear mailboxes");
         GH(hipMemsetAsync(s->vmask, 0, V * 8, s->st), "clear valid masks");
@@ -1441,7 +1939,7 @@ PushState::~PushState()
     for (int w = 0; w < 2; w++)
         for (int p = 0; p < 2; p++)
             if (gx[w][p]) (void)hipGraphExecDestroy(gx[w][p]);
-    void* ps[] = {val, mbox, act, vmask, touched, small[0], small[1], large[0], large[1], lpre, ctl, dsum};
+    void* ps[] = {val, mbox, act, vmask, touched, tgran, pend, fst, small[0], small[1], large[0], large[1], lpre, ctl, dsum};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     if (hout) (void)hipHostFree(hout);
@@ -1468,7 +1966,10 @@ static int enqueue_superstep(const PushState* s, int cur, uint32_t wzero)
         hipLaunchKernelGGL(k_relax_large<false>, dim3(G), dim3(256), 0, st, a, cur);
         hipLaunchKernelGGL(k_apply_s, dim3(grid_for(g->V, 1024, G)), dim3(256), 0, st, a, cur);
     }
-    hipLaunchKernelGGL(k_push_finish, dim3(1), dim3(64), 0, st, a, cur ^ 1);
+    hipLaunchKernelGGL(k_push_thr, dim3(1), dim3(64), 0, st, a);
+    if (s->batched) hipLaunchKernelGGL(k_push_react<true>, dim3(grid_for(g->V, 256, G)), dim3(256), 0, st, a, cur ^ 1);
+    else hipLaunchKernelGGL(k_push_react<false>, dim3(grid_for(g->V, 256, G)), dim3(256), 0, st, a, cur ^ 1);
+    hipLaunchKernelGGL(k_push_out, dim3(1), dim3(64), 0, st, a, cur ^ 1, s->batched ? 1 : 0);
     HCK(hipGetLastError(), "superstep launch");
     return 0;
 }
@@ -1502,6 +2003,34 @@ int PushState::start_cc()
     HCK(hipGetLastError(), "start cc");
     wzero = 1;
     return 0;
+}
+
+int PushState::cc_union_find()
+{
+    if (batched) {
+        set_error("cc_union_find: batched state");
+        return -2;
+    }
+    const unsigned G = g->tune.grid;
+    hipLaunchKernelGGL(k_uf_init, dim3(grid_for(g->V, 256, G)), dim3(256), 0, st, val, g->V);
+    hipLaunchKernelGGL(k_uf_link, dim3(grid_for(g->V, 4, G)), dim3(256), 0, st, (const uint64_t*)g->oidx,
+                       (const uint32_t*)g->oadj, val, g->V);
+    hipLaunchKernelGGL(k_uf_flatten, dim3(grid_for(g->V, 256, G)), dim3(256), 0, st, val, g->V);
+    HCK(hipGetLastError(), "union-find");
+    HCK(hipEventRecord(ev[0], st), "union-find event");
+    supersteps++;
+    return 0;
+}
+
+int PushState::cc(uint64_t max_supersteps)
+{
+    int rc;
+    if (g->tune.cc_mode == 1) {
+        if ((rc = cc_union_find()) != 0) return rc;
+        return wait(0);
+    }
+    if ((rc = start_cc()) != 0) return rc;
+    return run_to_end(max_supersteps);
 }
 
 int PushState::admit(int n, const int* slots, const uint32_t* sources)
@@ -1586,7 +2115,7 @@ int PushState::run_to_end(uint64_t max_supersteps)
         done += k;
         const int rc = wait(p);
         if (rc) return rc;
-        if (hout[p].nsmall + hout[p].nlarge == 0) return 0;
+        if (hout[p].orm == 0) return 0;   /* empty frontier, nothing parked */
         if (done >= max_supersteps) {
             set_error("run did not converge within %llu supersteps", (unsigned long long)max_supersteps);
             return -3;
@@ -1666,7 +2195,6 @@ static PprArgs ppr_args(const PprState* s)
     a.seg_v0 = g->seg_v0;
     a.nhub = g->nhub;
     a.nchunk = g->nchunk;
-    a.coop = g->tune.coop;
     a.oidx = g->oidx;
     a.iidx = g->iidx;
     a.iadj = g->iadj;
@@ -1707,7 +2235,7 @@ int PprState::create(ipr_bgraph* g, int B, int use_graph, PprState** out)
     PprState* s = new PprState();
     s->g = g;
     s->B = B;
-    s->use_graph = use_graph;
+    s->use_graph = graph_mode(use_graph);
     const uint64_t b = (uint64_t)B;
     int rc = 0;
 #define GO(x) do { if ((rc = (x)) != 0) { delete s; return rc; } } while (0)
@@ -1935,7 +2463,7 @@ extern "C" int ipr_batch_sssp(ipr_bgraph* g, const uint32_t* sources, int nq, ui
             if ((rc = s->step(&p)) != 0 || (rc = s->wait(p)) != 0) break;
             relax = s->hout[p].relax;
             if (!(s->hout[p].orm & live)) break;
-            if (s->supersteps > g->V + 8) {
+            if (s->supersteps > 64 * g->V + 64) {
                 set_error("ipr_batch_sssp: no convergence");
                 rc = -3;
                 break;
@@ -1964,7 +2492,7 @@ extern "C" int ipr_single_sssp(ipr_bgraph* g, uint32_t source, uint32_t* out, ip
     PushState* s = nullptr;
     if ((rc = PushState::create(g, false, 1, &s)) != 0) return rc;
     Timer tm(s->st);
-    if ((rc = s->start_sssp(source)) == 0) rc = s->run_to_end(g->V + 8);
+    if ((rc = s->start_sssp(source)) == 0) rc = s->run_to_end(64 * g->V + 64);
     const double ms = tm.stop();
     if (!rc) rc = s->values(out);
     if (!rc && st) {
@@ -1987,12 +2515,12 @@ extern "C" int ipr_batch_cc(ipr_bgraph* g, uint32_t* out, ipr_batch_stats* st)
     PushState* s = nullptr;
     if ((rc = PushState::create(g, false, 1, &s)) != 0) return rc;
     Timer tm(s->st);
-    if ((rc = s->start_cc()) == 0) rc = s->run_to_end(g->V + 8);
+    rc = s->cc(g->V + 8);
     const double ms = tm.stop();
     if (!rc) rc = s->values(out);
     if (!rc && st) {
         st->supersteps = s->supersteps;
-        st->relaxations = s->hout[(s->cur ^ 1)].relax + g->E;
+        st->relaxations = g->tune.cc_mode == 1 ? g->E : s->hout[(s->cur ^ 1)].relax + g->E;
         st->ms = ms;
     }
     delete s;

@@ -40,8 +40,13 @@ constexpr uint32_t SEG_ARCS = 2048;
 struct Tuning {
     uint32_t large;        /* push: out-degree above which a frontier vertex is relaxed in chunks */
     uint32_t chunk;        /* push: arcs per chunk of a large vertex */
-    uint32_t coop;         /* scalar pull: in-degree above which a wavefront folds one vertex */
     uint32_t grid;         /* workgroups of the grid-stride kernels */
+    uint32_t delta;        /* SSSP near/far step of 64-slot states: a frontier vertex relaxes only
+                              while its distance is below the query's threshold, which moves to
+                              (smallest pending distance + delta) when nothing below it is left;
+                              INF = plain BSP */
+    uint32_t delta1;       /* the same for one-query states */
+    uint32_t cc_mode;      /* CC: 0 label propagation (BSP supersteps), 1 union-find */
 };
 Tuning default_tuning();
 
@@ -99,6 +104,16 @@ struct PushCtl {
     unsigned long long orm[64];
     unsigned long long relax[64];
     unsigned long long dig[SPREAD][64];
+    uint32_t thr[64];                 /* near/far threshold per query */
+    uint32_t fmin[SPREAD][64];        /* smallest distance in the next frontier, per query */
+    unsigned long long pdef[SPREAD];  /* queries that parked (far) entries this superstep */
+    unsigned long long prem[SPREAD];  /* queries with parked entries left after a wake-up scan */
+    unsigned long long pleft;         /* queries that may have parked entries */
+    unsigned long long react;         /* queries whose threshold moved: wake parked entries below it */
+    uint32_t pmin[64];                /* smallest parked distance per query (lower bound) */
+    uint32_t pdmin[SPREAD][64];       /* smallest distance parked this superstep */
+    uint32_t pscan[SPREAD][64];       /* smallest distance left parked after a wake-up scan */
+    uint32_t step, pad;               /* superstep counter (frontier stamps) */
     unsigned long long relax_total;   /* cumulative over the run */
     TickOut out;                      /* out.relax: relax_total after the superstep */
 };
@@ -111,6 +126,9 @@ struct PushState {
     unsigned long long* act = nullptr; /* batched: per-vertex mask of slots active this superstep */
     unsigned long long* vmask = nullptr; /* batched: per-vertex mask of slots whose dist is set */
     uint8_t* touched = nullptr;        /* V rounded up to 1024, zero outside supersteps */
+    uint8_t* tgran = nullptr;          /* one byte per 256 vertices of the touched map */
+    void* pend = nullptr;              /* parked (far) entries: byte per vertex, or slot mask */
+    uint32_t* fst = nullptr;           /* superstep in which a vertex was last put in the next frontier */
     uint32_t* small[2] = {nullptr, nullptr};
     uint32_t* large[2] = {nullptr, nullptr};
     uint32_t* lpre = nullptr;          /* chunk prefix of the current large list */
@@ -132,7 +150,9 @@ struct PushState {
     static uint64_t footprint(const ipr_bgraph* g, bool batched);
     /* scalar runs: reset values and seed the first frontier */
     int start_sssp(uint32_t source);
-    int start_cc();
+    int start_cc();                    /* label propagation (BSP supersteps) */
+    int cc_union_find();               /* CC labels by union-find into val; ev[0] marks the end */
+    int cc(uint64_t max_supersteps);   /* CC by the graph's cc_mode, to the end (blocking) */
     /* batched: inject distance 0 at sources for slots; clear released slots */
     int admit(int n, const int* slots, const uint32_t* sources);
     int release(unsigned long long mask);

@@ -16,8 +16,10 @@
  *               concurrently on their streams. CC queries of one graph that are
  *               waiting when a CC run starts share that run (the labels do not
  *               depend on the query). A batcher's state is allocated on first
- *               use and counted against the HBM budget; a batcher that cannot
- *               fit waits, after idle batchers have been evicted (oldest first).
+ *               use (states for the trace's graphs and kinds are made before
+ *               the clock starts while the budget allows) and counted against
+ *               the HBM budget; a batcher that cannot fit waits, after idle
+ *               batchers have been evicted (oldest first).
  *
  * A query's result is the same in both modes: the batched kernels compute a
  * slot only from its own column, in the same order as the scalar kernels.
@@ -72,10 +74,37 @@ struct Ctx {
 
 /* ---------------------------------------------------------------- sequential */
 
-int serve_sequential(ipr_bgraph** graphs, int ng, const std::vector<int>& order, Ctx& c, ipr_serve_stats* st)
+/* one-query states of every graph the trace uses, made before the clock starts */
+struct SeqStates {
+    std::vector<PushState*> push;
+    std::vector<PprState*> ppr;
+    ~SeqStates()
+    {
+        for (auto* s : push) delete s;
+        for (auto* s : ppr) delete s;
+    }
+    int prepare(ipr_bgraph** graphs, int ng, const ipr_query* q, int n, int use_graph)
+    {
+        push.assign(ng, nullptr);
+        ppr.assign(ng, nullptr);
+        for (int i = 0; i < n; i++) {
+            const int gi = q[i].graph;
+            int rc = 0;
+            if (q[i].kind == IPR_Q_PPR) {
+                if (!ppr[gi]) rc = PprState::create(graphs[gi], 1, use_graph, &ppr[gi]);
+            } else if (!push[gi]) {
+                rc = PushState::create(graphs[gi], false, use_graph, &push[gi]);
+            }
+            if (rc) return rc;
+        }
+        return 0;
+    }
+};
+
+int serve_sequential(ipr_bgraph** graphs, SeqStates& ss, const std::vector<int>& order, Ctx& c)
 {
-    std::vector<PushState*> push(ng, nullptr);
-    std::vector<PprState*> ppr(ng, nullptr);
+    std::vector<PushState*>& push = ss.push;
+    std::vector<PprState*>& ppr = ss.ppr;
     int rc = 0;
     float* dbuf = nullptr;
     uint64_t dbuf_n = 0;
@@ -85,17 +114,19 @@ int serve_sequential(ipr_bgraph** graphs, int ng, const std::vector<int>& order,
         while (c.t() < q.t_arrival) sleep_until(c.t0, q.t_arrival);
         c.t_admit[qi] = c.t();
         if (q.kind == IPR_Q_SSSP || q.kind == IPR_Q_CC) {
-            if (!push[q.graph] && (rc = PushState::create(g, false, c.use_graph, &push[q.graph])) != 0) break;
             PushState* s = push[q.graph];
-            rc = q.kind == IPR_Q_SSSP ? s->start_sssp(q.param) : s->start_cc();
-            if (!rc) rc = s->run_to_end(g->V + 8);
+            if (q.kind == IPR_Q_SSSP) {
+                rc = s->start_sssp(q.param);
+                if (!rc) rc = s->run_to_end(64 * g->V + 64);
+            } else {
+                rc = s->cc(g->V + 8);
+            }
             unsigned long long d = 0;
             if (!rc) rc = s->digest(&d);
             if (!rc && c.results && c.results[qi]) rc = s->values((uint32_t*)c.results[qi]);
             if (rc) break;
             c.finish(qi, d);
         } else {
-            if (!ppr[q.graph] && (rc = PprState::create(g, 1, c.use_graph, &ppr[q.graph])) != 0) break;
             PprState* s = ppr[q.graph];
             const int slot = 0;
             rc = s->update(1, &slot, &q.param, 1ull);
@@ -123,8 +154,6 @@ int serve_sequential(ipr_bgraph** graphs, int ng, const std::vector<int>& order,
             c.finish(qi, s->hout->dig[0]);
         }
     }
-    for (auto* s : push) delete s;
-    for (auto* s : ppr) delete s;
     if (dbuf) (void)hipFree(dbuf);
     return rc;
 }
@@ -144,7 +173,8 @@ struct Batcher {
     float* slot_buf[64];           /* PPR: device result buffers (only when results are asked for) */
     unsigned long long live = 0, release = 0;
     bool inflight = false;
-    int parity = 0;                /* SSSP/CC: parity of the tick in flight */
+    int parity = 0;                /* SSSP/CC: parity of the last superstep of the tick in flight */
+    uint64_t run = 1;              /* SSSP: supersteps per tick (doubles while nobody waits) */
     uint64_t ticks = 0;
     unsigned long long ext_mask = 0;   /* PPR: slots extracted with the tick in flight */
     /* CC */
@@ -281,19 +311,21 @@ struct Server {
             }
             b.ext_mask = 0;
         } else {
-            const TickOut& o = b.ps->hout[b.parity];
-            if (o.nsmall + o.nlarge == 0) {
-                unsigned long long d = 0;
-                if ((rc = b.ps->digest(&d)) != 0) return rc;
-                for (int qi : b.cc_attached) {
-                    if (c.results && c.results[qi] && (rc = b.ps->values((uint32_t*)c.results[qi])) != 0) return rc;
-                    c.finish(qi, d);
+            if (b.g->tune.cc_mode == 0) {
+                const TickOut& o = b.ps->hout[b.parity];
+                if (o.nsmall + o.nlarge != 0) {
+                    b.cc_chunk = std::min<uint64_t>(b.cc_chunk * 2, 32);
+                    return 0;
                 }
-                b.cc_attached.clear();
-                b.cc_running = false;
-            } else {
-                b.cc_chunk = std::min<uint64_t>(b.cc_chunk * 2, 16);
             }
+            unsigned long long d = 0;
+            if ((rc = b.ps->digest(&d)) != 0) return rc;
+            for (int qi : b.cc_attached) {
+                if (c.results && c.results[qi] && (rc = b.ps->values((uint32_t*)c.results[qi])) != 0) return rc;
+                c.finish(qi, d);
+            }
+            b.cc_attached.clear();
+            b.cc_running = false;
         }
         return 0;
     }
@@ -334,7 +366,11 @@ struct Server {
             mark_delayed(b);
             if (n && (rc = b.ps->admit(n, slots, srcs)) != 0) return rc;
             if (!b.live) return 0;
-            if ((rc = b.ps->step(&b.parity)) != 0) return rc;
+            /* one superstep per tick while queries wait for a slot, else a
+             * doubling run of supersteps between host checks */
+            b.run = (n || !b.waiting.empty()) ? 1 : std::min<uint64_t>(b.run * 2, 32);
+            for (uint64_t i = 0; i < b.run; i++)
+                if ((rc = b.ps->step(&b.parity)) != 0) return rc;
         } else if (b.kind == IPR_Q_PPR) {
             int slots[64];
             uint32_t seeds[64];
@@ -361,11 +397,18 @@ struct Server {
                 b.release = 0;
             }
             if (!b.live) return 0;
-            if ((rc = b.pp->step()) != 0) return rc;
+            /* iterations this tick: up to the next slot completion; one while
+             * queries wait for a free slot */
+            uint64_t k = 8;
+            for (int q = 0; q < 64; q++)
+                if ((b.live >> q) & 1ull) k = std::min<uint64_t>(k, (uint64_t)c.ppr_iters - b.slot_tick[q]);
+            if (!b.waiting.empty() && free_slots(b) > 0) k = 1;
+            for (uint64_t i = 0; i < k; i++)
+                if ((rc = b.pp->step()) != 0) return rc;
             unsigned long long fin = 0, first = 0;
             for (int q = 0; q < 64; q++) {
                 if (!((b.live >> q) & 1ull)) continue;
-                b.slot_tick[q]++;
+                b.slot_tick[q] += k;
                 if (b.slot_tick[q] == (uint64_t)c.ppr_iters) {
                     fin |= 1ull << q;
                     if (c.ppr_iters == 1) first |= 1ull << q;
@@ -385,16 +428,38 @@ struct Server {
                     c.t_admit[qi] = c.t();
                     b.cc_attached.push_back(qi);
                 }
-                if ((rc = b.ps->start_cc()) != 0) return rc;
+                if (b.g->tune.cc_mode == 1) {
+                    if ((rc = b.ps->cc_union_find()) != 0) return rc;
+                    b.parity = 0;
+                } else if ((rc = b.ps->start_cc()) != 0) {
+                    return rc;
+                }
                 b.cc_running = true;
                 b.cc_chunk = 1;
             }
-            for (uint64_t i = 0; i < b.cc_chunk; i++)
-                if ((rc = b.ps->step(&b.parity)) != 0) return rc;
+            /* label propagation: doubling runs of supersteps per tick */
+            if (b.g->tune.cc_mode == 0)
+                for (uint64_t i = 0; i < b.cc_chunk; i++)
+                    if ((rc = b.ps->step(&b.parity)) != 0) return rc;
         }
         b.inflight = true;
         b.ticks++;
         st->ticks++;
+        return 0;
+    }
+
+    /* 64-slot states of the (graph, kind) pairs of the trace, in order of first
+     * arrival, while the budget lasts; made before the clock starts */
+    int prewarm(const std::vector<int>& order)
+    {
+        for (int qi : order) {
+            Batcher& b = get(c.q[qi].kind, c.q[qi].graph);
+            if (b.allocated()) continue;
+            if (budget && used + b.footprint > budget) continue;
+            bool ok;
+            const int rc = ensure_state(b, &ok);
+            if (rc) return rc;
+        }
         return 0;
     }
 
@@ -485,13 +550,19 @@ extern "C" int ipr_serve(ipr_bgraph** graphs, int ng, const ipr_query* q, int n,
     std::stable_sort(order.begin(), order.end(), [q](int a, int b) { return q[a].t_arrival < q[b].t_arrival; });
     Ctx c{q, t_admit, t_done, digest, results, 0.0, o.ppr_iters, o.use_graph};
     c.delayed.assign((size_t)n, 0);
-    c.t0 = now_s();
     int rc;
     if (o.sequential) {
-        rc = serve_sequential(graphs, ng, order, c, &local);
+        SeqStates ss;
+        if ((rc = ss.prepare(graphs, ng, q, n, o.use_graph)) == 0) {
+            c.t0 = now_s();
+            rc = serve_sequential(graphs, ss, order, c);
+        }
     } else {
         Server s(graphs, ng, c, o.hbm_budget, &local);
-        rc = s.run(order, (uint64_t)n);
+        if ((rc = s.prewarm(order)) == 0) {
+            c.t0 = now_s();
+            rc = s.run(order, (uint64_t)n);
+        }
     }
     if (!rc) {
         double first = 1e300, last = 0.0;

@@ -86,6 +86,30 @@ def test_single_sssp_matches_batched(graphs, name):
         assert st["supersteps"] >= 1
 
 
+@pytest.mark.parametrize("delta", [1, 7, 300, BA.INF])
+@pytest.mark.parametrize("name", ["dir", "grid", "hub"])
+def test_sssp_near_far_any_delta(graphs, name, delta):
+    """The near/far step changes the order of relaxations, never a distance."""
+    g, r = graphs[name]
+    before = r.tuning
+    r.tune(delta=delta, large=64, chunk=64)
+    try:
+        src = sources(g.V, 9, 17)
+        dist, st = r.batch_sssp(src)
+        one, _ = r.sssp(int(src[3]))
+    finally:
+        r.tune(**before)
+    for q in range(src.size):
+        assert np.array_equal(dist[q], O.dijkstra(g, int(src[q]))), (name, delta, q)
+    assert np.array_equal(one, dist[3])
+
+
+def test_tuning_rejects_bad_values(graphs):
+    _, r = graphs["dir"]
+    with pytest.raises(RuntimeError, match="chunk"):
+        r.tune(chunk=8)
+
+
 def test_sssp_alone_equals_in_mix(graphs):
     g, r = graphs["dir"]
     mix = sources(g.V, 64, 21)

@@ -58,3 +58,21 @@ def test_batch_lib_exports():
         assert hasattr(L, n), n
     from ipregel_amd import batch as BA
     assert set(BA.EXPORTS) == set(names)
+
+
+def test_batch_kernels_call_and_scratch_free(tmp_path):
+    """Every kernel of the batch library is a single straight function with no
+    private (scratch) segment: an accidental self-call once recursed through
+    the scratch stack until the GPU faulted."""
+    import subprocess
+    src = ROOT / "ipregel_amd" / "csrc" / "batch" / "ipr_batch.hip"
+    out = tmp_path / "batch.s"
+    r = subprocess.run(["/opt/rocm/bin/hipcc", "--offload-arch=gfx950", "-O3", "-std=c++17",
+                        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-ffp-contract=off",
+                        "-Wno-unused-command-line-argument", f"-I{INC}", "--cuda-device-only", "-S",
+                        str(src), "-o", str(out)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    asm = out.read_text()
+    assert "s_swappc_b64" not in asm
+    sizes = re.findall(r"\.private_segment_fixed_size:\s+(\d+)", asm)
+    assert sizes and all(int(x) == 0 for x in sizes)
