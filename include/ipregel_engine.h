@@ -85,6 +85,9 @@ typedef struct ipr_ctl {
 } ipr_ctl;
 
 #define IPR_LPACK_SHIFT 40
+/* arc-balanced gather: arcs per lane per pass, and the end-of-list mark */
+#define IPR_GATHER_K 8
+#define IPR_GATHER_LAST 0x80000000u
 #define IPR_LPACK_MASK ((1ull << IPR_LPACK_SHIFT) - 1)
 
 typedef struct ipr_dev {
@@ -102,6 +105,16 @@ typedef struct ipr_dev {
     const uint64_t* t_idx;
     const uint32_t* t_adj;
     const uint32_t* t_wgt;
+    /* arc-balanced gather over the transpose: sources with IPR_GATHER_LAST on
+     * the last arc of each in-list, the receivers of the non-empty lists, and
+     * per IPR_GATHER_K-arc chunk the receiver (position in g_dst) of its first
+     * arc; each wavefront folds g_per_wave consecutive arcs */
+    const uint32_t* g_arc;
+    const uint32_t* g_own;
+    const uint32_t* g_dst;
+    uint64_t g_per_wave;
+    uint32_t gather_blocks;
+    uint32_t g_pad;
     void* val;
     void* msg;
     uint8_t* act;

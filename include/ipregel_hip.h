@@ -17,6 +17,8 @@
  *                                  increment (e.g. src/combiner_spread_postamble.h
  *                                  :341-354, :410-414, :476-532)
  *   ipr_hip_build_pull_bins        (new) in-degree bins of the pull gather
+ *   ipr_hip_build_gather           (new) arc-balanced index of the push
+ *                                  applications' dense-superstep gather
  *   ipr_hip_graph_*                (new) replay of a chunk of supersteps
  */
 #ifndef IPREGEL_HIP_H
@@ -69,6 +71,15 @@ int ipr_hip_build_csc(uint64_t V, uint64_t E, const uint64_t* out_idx,
 /* static in-degree bins for the pull gather; sizes returned in dev->pbin_size */
 int ipr_hip_build_pull_bins(ipr_dev* dev, uint32_t small_max, uint32_t medium_max,
                             void* stream);
+
+/* index of the arc-balanced gather over a device CSC (in-lists):
+ * g_arc[E] = t_adj with IPR_GATHER_LAST on the last arc of every non-empty
+ * list, g_dst[*n_dst] = the vertices with non-empty lists (ascending),
+ * g_own[ceil(E/K)] = position in g_dst of the list holding arc c*K.
+ * Needs V < 2^31 and K a multiple of 4. */
+int ipr_hip_build_gather(uint64_t V, uint64_t E, const uint64_t* t_idx, const uint32_t* t_adj,
+                         uint32_t K, uint32_t* g_arc, uint32_t* g_own, uint32_t* g_dst,
+                         uint64_t* n_dst, void* stream);
 
 /* byte-wise equality of two device buffers (multiple of 4 bytes) */
 int ipr_hip_equal(const void* a, const void* b, uint64_t bytes, int* equal, void* stream);

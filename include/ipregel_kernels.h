@@ -567,50 +567,6 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_expand(ipr_dev d)
 /* push variants: gather (dense supersteps)                                  */
 /* ------------------------------------------------------------------------ */
 
-/* fold of the messages broadcast in superstep s by the in-neighbours in
- * transpose positions first, first+step, ... < end */
-__device__ inline bool ipr_gather_range(const ipr_dev& d, uint64_t s, uint64_t first, uint64_t end,
-                                        uint64_t step, IP_MESSAGE_TYPE* acc)
-{
-    const ipr_bslot_t* bs = (const ipr_bslot_t*)d.bslot;
-    const uint32_t want = (uint32_t)(s + 1);
-    bool has = false;
-    uint64_t k = first;
-    /* IPR_GATHER_UNROLL index -> record chains in flight per lane */
-    constexpr int U = IPR_GATHER_UNROLL;
-    for (; k + (U - 1) * step < end; k += U * step) {
-        uint32_t u[U];
-        ipr_bslot_t b[U];
-#pragma unroll
-        for (int j = 0; j < U; j++) u[j] = __builtin_nontemporal_load(d.t_adj + k + j * step);
-#pragma unroll
-        for (int j = 0; j < U; j++) b[j] = bs[u[j]];
-#pragma unroll
-        for (int j = 0; j < U; j++) {
-            if ((b[j].step & ~IPR_BSTEP_WEIGHTED) == want) {
-                IP_MESSAGE_TYPE m = b[j].msg;
-#ifdef IP_NEEDS_OUT_NEIGHBOUR_WEIGHTS
-                if (b[j].step & IPR_BSTEP_WEIGHTED) m = (IP_MESSAGE_TYPE)(m + (IP_MESSAGE_TYPE)d.t_wgt[k + j * step]);
-#endif
-                ipr_policy::fold(acc, m);
-                has = true;
-            }
-        }
-    }
-    for (; k < end; k += step) {
-        ipr_bslot_t b = bs[__builtin_nontemporal_load(d.t_adj + k)];
-        if ((b.step & ~IPR_BSTEP_WEIGHTED) == want) {
-            IP_MESSAGE_TYPE m = b.msg;
-#ifdef IP_NEEDS_OUT_NEIGHBOUR_WEIGHTS
-            if (b.step & IPR_BSTEP_WEIGHTED) m = (IP_MESSAGE_TYPE)(m + (IP_MESSAGE_TYPE)d.t_wgt[k]);
-#endif
-            ipr_policy::fold(acc, m);
-            has = true;
-        }
-    }
-    return has;
-}
-
 /* Lane-cooperative fold for vertices of small in-degree: lane L holds vertex
  * v (valid), the in-lists [idx[v], idx[v+1]) of the wavefront's 64 vertices
  * are laid end to end and taken 64 arcs at a time; load(k, &m) gives the
@@ -686,89 +642,138 @@ __device__ inline bool ipr_wave_segmented_fold(const uint64_t* idx, uint32_t v, 
     return has;
 }
 
-/* the gathering vertex is the only writer of its mailbox and flag (messages
- * from ip_send_message were delivered before, in the compute kernel) */
-__device__ inline void ipr_gather_store(const ipr_dev& d, uint64_t s, uint32_t v, IP_MESSAGE_TYPE acc)
+typedef uint32_t ipr_u32x4 __attribute__((ext_vector_type(4)));
+
+/* a receiver's folded messages of superstep s: the list may be split between
+ * wavefronts, so the delivery is the combiner's atomic one */
+__device__ inline void ipr_gather_flush(const ipr_dev& d, uint64_t s, uint32_t v, IP_MESSAGE_TYPE acc)
 {
-    ipr_policy::merge((ipr_slot_t*)d.mbox[(s + 1) & 1] + v, acc);
+    ipr_policy::deliver((ipr_slot_t*)d.mbox[(s + 1) & 1] + v, acc);
     d.recv[(s + 1) & 1][v] = 1;
 }
 
-/* Vertices binned by in-degree; blocks [0, nb0) one thread per vertex,
- * [nb0, nb0+nb1) one wavefront per vertex, the rest one workgroup. */
-__global__ void __launch_bounds__(IPR_BLOCK) ipr_k_gather(ipr_dev d, uint32_t nb0, uint32_t nb1)
+/* Dense supersteps of push applications: every receiver folds the records of
+ * the in-neighbours that broadcast in superstep s, over the exact transpose.
+ * Arc-balanced, so hubs need no special case: wavefront w owns the arcs
+ * [w*g_per_wave, (w+1)*g_per_wave) and takes them 64*K at a time; lane L folds
+ * the K consecutive arcs base + L*K (sources by vector loads, then their K
+ * records together), delivers the in-lists that end inside its K arcs, and one
+ * segmented scan across the lanes joins the lists that span lanes. The list
+ * still open after a pass carries into the next; the one open at the end of
+ * the range is delivered and finished by the next wavefront. Gathering is
+ * only enabled for integral MIN/MAX/SUM combiners, whose result does not
+ * depend on the order or grouping of the folds. */
+__global__ void __launch_bounds__(IPR_BLOCK) ipr_k_gather(ipr_dev d)
 {
-    __shared__ IP_MESSAGE_TYPE part[IPR_WAVES];
-    __shared__ int parth[IPR_WAVES];
+    constexpr int K = IPR_GATHER_K;
+    static_assert(K % 4 == 0, "sources are read four at a time");
     ipr_ctl* ctl = d.ctl;
     if (ctl->done) return;
     if (!ctl->pull_mode) return;
     const uint64_t s = ctl->superstep;
+    const uint32_t want = (uint32_t)(s + 1);
     const uint32_t lane = ipr_lane();
-    const uint32_t b = blockIdx.x;
-    if (b < nb0) {
-        const uint64_t n = d.pbin_size[0];
-        const ipr_bslot_t* bs = (const ipr_bslot_t*)d.bslot;
-        const uint32_t want = (uint32_t)(s + 1);
-        auto load = [&](uint64_t k, IP_MESSAGE_TYPE* m) -> bool {
-            /* the index stream is read once: keep it out of L2, which holds
-             * the randomly gathered broadcast records */
-            ipr_bslot_t r = bs[__builtin_nontemporal_load(d.t_adj + k)];
-            if ((r.step & ~IPR_BSTEP_WEIGHTED) != want) return false;
-            *m = r.msg;
+    const uint64_t per = d.g_per_wave;
+    const uint64_t w0 = ((uint64_t)blockIdx.x * IPR_WAVES + threadIdx.x / IPR_WAVE) * per;
+    if (w0 >= d.E) return;
+    const uint64_t w1 = w0 + per < d.E ? w0 + per : d.E;
+    const ipr_bslot_t* bs = (const ipr_bslot_t*)d.bslot;
+    IP_MESSAGE_TYPE carry = ipr_policy::ident();
+    int carry_has = 0;
+    uint32_t carry_o = 0;
+    for (uint64_t base = w0; base < w1; base += (uint64_t)IPR_WAVE * K) {
+        const uint64_t t0 = base + (uint64_t)lane * K;
+        uint32_t a[K];
+        if (t0 + K <= w1) {
+            /* the source stream is read once: keep it out of L2, which holds
+             * the randomly read broadcast records */
+#pragma unroll
+            for (int j = 0; j < K; j += 4) {
+                const ipr_u32x4 q = __builtin_nontemporal_load((const ipr_u32x4*)(d.g_arc + t0 + j));
+                a[j] = q[0]; a[j + 1] = q[1]; a[j + 2] = q[2]; a[j + 3] = q[3];
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < K; j++) a[j] = t0 + j < w1 ? d.g_arc[t0 + j] : 0u;
+        }
+        const uint32_t o = t0 < w1 ? d.g_own[t0 / K] : 0u;   /* receiver of arc t0 */
+        ipr_bslot_t r[K];
+#pragma unroll
+        for (int j = 0; j < K; j++) r[j] = bs[a[j] & ~IPR_GATHER_LAST];
 #ifdef IP_NEEDS_OUT_NEIGHBOUR_WEIGHTS
-            if (r.step & IPR_BSTEP_WEIGHTED) *m = (IP_MESSAGE_TYPE)(*m + (IP_MESSAGE_TYPE)d.t_wgt[k]);
+        uint32_t wt[K];
+        if (t0 + K <= w1) {
+#pragma unroll
+            for (int j = 0; j < K; j += 4) {
+                const ipr_u32x4 q = __builtin_nontemporal_load((const ipr_u32x4*)(d.t_wgt + t0 + j));
+                wt[j] = q[0]; wt[j + 1] = q[1]; wt[j + 2] = q[2]; wt[j + 3] = q[3];
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < K; j++) wt[j] = t0 + j < w1 ? d.t_wgt[t0 + j] : 0u;
+        }
 #endif
-            return true;
-        };
-        const uint64_t stride = (uint64_t)nb0 * blockDim.x;
-        for (uint64_t base = (uint64_t)b * blockDim.x + (threadIdx.x & ~(IPR_WAVE - 1)); base < n; base += stride) {
-            const uint64_t i = base + lane;
-            const uint32_t v = i < n ? d.pbins[0][i] : 0;
-            IP_MESSAGE_TYPE acc;
-            bool has = ipr_wave_segmented_fold(d.t_idx, v, i < n, load, &acc);
-            if (i < n && has) ipr_gather_store(d, s, v, acc);
-        }
-    } else if (b < nb0 + nb1) {
-        const uint64_t n = d.pbin_size[1];
-        const uint64_t wave = (uint64_t)(b - nb0) * IPR_WAVES + threadIdx.x / IPR_WAVE;
-        const uint64_t nwaves = (uint64_t)nb1 * IPR_WAVES;
-        for (uint64_t i = wave; i < n; i += nwaves) {
-            uint32_t v = d.pbins[1][i];
-            IP_MESSAGE_TYPE acc = ipr_policy::ident();
-            bool has = ipr_gather_range(d, s, d.t_idx[v] + lane, d.t_idx[v + 1], IPR_WAVE, &acc);
-            for (int o = 32; o > 0; o >>= 1) {
-                IP_MESSAGE_TYPE y = __shfl_xor(acc, o);
-                ipr_policy::fold(&acc, y);
+        /* head: the part of the list open on entry (it began in an earlier
+         * lane); fv/fh: lists that begin and end in these K arcs */
+        IP_MESSAGE_TYPE head = ipr_policy::ident(), cur = ipr_policy::ident();
+        int head_has = 0, cur_has = 0;
+        uint32_t nb = 0;
+        IP_MESSAGE_TYPE fv[K];
+        int fh[K];
+#pragma unroll
+        for (int j = 0; j < K; j++) {
+            const bool live = t0 + j < w1;
+            if (live && (r[j].step & ~IPR_BSTEP_WEIGHTED) == want) {
+                IP_MESSAGE_TYPE m = r[j].msg;
+#ifdef IP_NEEDS_OUT_NEIGHBOUR_WEIGHTS
+                if (r[j].step & IPR_BSTEP_WEIGHTED) m = (IP_MESSAGE_TYPE)(m + (IP_MESSAGE_TYPE)wt[j]);
+#endif
+                ipr_policy::fold(&cur, m);
+                cur_has = 1;
             }
-            has = __ballot(has) != 0;
-            if (lane == 0 && has) ipr_gather_store(d, s, v, acc);
-        }
-    } else {
-        const uint64_t n = d.pbin_size[2];
-        const uint32_t nb2 = gridDim.x - nb0 - nb1;
-        for (uint64_t i = b - nb0 - nb1; i < n; i += nb2) {
-            uint32_t v = d.pbins[2][i];
-            IP_MESSAGE_TYPE acc = ipr_policy::ident();
-            bool has = ipr_gather_range(d, s, d.t_idx[v] + threadIdx.x, d.t_idx[v + 1], IPR_BLOCK, &acc);
-            for (int o = 32; o > 0; o >>= 1) {
-                IP_MESSAGE_TYPE y = __shfl_xor(acc, o);
-                ipr_policy::fold(&acc, y);
+            fv[j] = cur;
+            fh[j] = 0;
+            if (live && (a[j] & IPR_GATHER_LAST)) {
+                if (nb == 0) { head = cur; head_has = cur_has; }
+                else fh[j] = cur_has;
+                nb++;
+                cur = ipr_policy::ident();
+                cur_has = 0;
             }
-            bool wh = __ballot(has) != 0;
-            if (lane == 0) { part[threadIdx.x / IPR_WAVE] = acc; parth[threadIdx.x / IPR_WAVE] = wh; }
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                bool any = false;
-                for (int w = 0; w < IPR_WAVES; w++) {
-                    ipr_policy::fold(&acc, part[w]);
-                    any |= parth[w] != 0;
-                }
-                if (any) ipr_gather_store(d, s, v, acc);
-            }
-            __syncthreads();
         }
+        uint32_t k = 0;
+#pragma unroll
+        for (int j = 0; j < K; j++) {
+            if (t0 + j < w1 && (a[j] & IPR_GATHER_LAST)) {
+                if (fh[j]) ipr_gather_flush(d, s, d.g_dst[o + k], fv[j]);
+                k++;
+            }
+        }
+        /* lanes without a list end extend the list open before them */
+        const int f = nb > 0;
+        if (lane == 0 && !f) { ipr_policy::fold(&cur, carry); cur_has |= carry_has; }
+        IP_MESSAGE_TYPE S = cur;
+        int Sh = cur_has, F = f;
+        for (int dd = 1; dd < IPR_WAVE; dd <<= 1) {
+            const IP_MESSAGE_TYPE y = __shfl_up(S, dd);
+            const int yh = __shfl_up(Sh, dd), yf = __shfl_up(F, dd);
+            if (lane >= (uint32_t)dd && !F) { ipr_policy::fold(&S, y); Sh |= yh; F |= yf; }
+        }
+        IP_MESSAGE_TYPE in = __shfl_up(S, 1);
+        int inh = __shfl_up(Sh, 1);
+        if (lane == 0) { in = carry; inh = carry_has; }
+        if (f && (head_has | inh)) {
+            ipr_policy::fold(&head, in);
+            ipr_gather_flush(d, s, d.g_dst[o], head);
+        }
+        /* the last lane with arcs holds the list still open */
+        const uint64_t left = w1 - base;
+        const int last = left >= (uint64_t)IPR_WAVE * K ? IPR_WAVE - 1 : (int)((left - 1) / K);
+        carry = __shfl(S, last);
+        carry_has = __shfl(Sh, last);
+        carry_o = __shfl(o + nb, last);
     }
+    if (lane == 0 && carry_has) ipr_gather_flush(d, s, d.g_dst[carry_o], carry);
 }
 
 /* PUSH: src/combiner_postamble.h:263-278 - a delivered mailbox becomes the
@@ -1140,12 +1145,9 @@ static int ipr_app_launch(int kernel, const ipr_dev* dp, void* stream)
     case IPR_K_SWAP:
         hipLaunchKernelGGL(ipr_k_swap, dim3(vblocks < gblocks ? vblocks : gblocks), dim3(IPR_BLOCK), 0, st, d);
         break;
-    case IPR_K_GATHER: {
-        uint32_t nb[3];
-        ipr_bin_blocks(d, nb);
-        hipLaunchKernelGGL(ipr_k_gather, dim3(nb[0] + nb[1] + nb[2]), dim3(IPR_BLOCK), 0, st, d, nb[0], nb[1]);
+    case IPR_K_GATHER:
+        hipLaunchKernelGGL(ipr_k_gather, dim3(d.gather_blocks ? d.gather_blocks : 1), dim3(IPR_BLOCK), 0, st, d);
         break;
-    }
     case IPR_K_PULL_STEP: {
         uint32_t nb[3];
         ipr_bin_blocks(d, nb);

@@ -131,6 +131,9 @@ static void dev_free_all(void)
     if (d->t_idx && d->t_idx != d->out_idx) ipr_hip_free((void*)d->t_idx);
     if (d->t_adj && d->t_adj != d->out_adj) ipr_hip_free((void*)d->t_adj);
     if (d->t_wgt && d->t_wgt != d->out_wgt) ipr_hip_free((void*)d->t_wgt);
+    ipr_hip_free((void*)d->g_arc);
+    ipr_hip_free((void*)d->g_own);
+    ipr_hip_free((void*)d->g_dst);
     ipr_hip_free(d->bslot);
     ipr_hip_free(d->lbase);
     if (d->in_idx && d->in_idx != d->out_idx) ipr_hip_free((void*)d->in_idx);
@@ -259,7 +262,9 @@ static void dev_setup(void)
     uint32_t po = c && c->pull_ordered ? (uint32_t)c->pull_ordered : env_u32("IPR_PULL_ORDERED", 1);
     d->pull_ordered = po == 1;
     uint32_t dopt = c && c->dir_opt ? (uint32_t)c->dir_opt : env_u32("IPR_DIR_OPT", 1);
-    d->dir_opt = dopt == 1 && a->can_gather && (var == IPR_VARIANT_PUSH || var == IPR_VARIANT_SPREAD);
+    /* the gather index marks list ends in bit 31 of the source ids */
+    d->dir_opt = dopt == 1 && a->can_gather && (var == IPR_VARIANT_PUSH || var == IPR_VARIANT_SPREAD) &&
+                 V < (1ull << 31);
     d->dir_alpha = c && c->dir_alpha ? c->dir_alpha : env_u32("IPR_DIR_ALPHA", 6);
     d->large_chunk = c && c->bin_small_max ? c->bin_small_max : env_u32("IPR_LARGE_CHUNK", 2048);
     if (d->large_chunk < 64) d->large_chunk = 64;
@@ -288,8 +293,25 @@ static void dev_setup(void)
             d->t_adj = t_adj;
             d->t_wgt = t_wgt;
         }
+        const uint64_t K = IPR_GATHER_K, pass = 64 * K;
+        uint64_t nch = (E + K - 1) / K, ndst = 0;
+        uint32_t* g_arc = dalloc(E * 4, "gather sources");
+        uint32_t* g_own = dalloc(nch * 4, "gather chunk receivers");
+        uint32_t* g_dst = dalloc(vv * 4, "gather receivers");
+        hip_check(ipr_hip_build_gather(V, E, d->t_idx, d->t_adj, IPR_GATHER_K, g_arc, g_own, g_dst, &ndst, st),
+                  "gather index");
+        d->g_arc = g_arc;
+        d->g_own = g_own;
+        d->g_dst = g_dst;
+        /* wavefronts (4 per workgroup) own equal arc ranges, whole passes */
+        d->gather_blocks = env_u32("IPR_GATHER_BLOCKS", 8192);
+        if (d->gather_blocks < 1) d->gather_blocks = 1;
+        const uint64_t nw = (uint64_t)d->gather_blocks * 4;
+        uint64_t per = (E + nw - 1) / nw;
+        per = (per + pass - 1) / pass * pass;
+        d->g_per_wave = per ? per : pass;
     }
-    if (pull || d->dir_opt) {
+    if (pull) {
         /* ordered folds are sequential per vertex: keep the thread-per-vertex
          * bin short; reassociating folds use the lane-cooperative one */
         uint32_t ps = c && c->pull_small_max ? c->pull_small_max

@@ -9,6 +9,7 @@
 
 #include <hip/hip_runtime.h>
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include "ipregel_hip.h"
 
@@ -281,6 +282,92 @@ extern "C" int ipr_hip_build_csc(uint64_t V, uint64_t E, const uint64_t* out_idx
     if ((e = hipStreamSynchronize(st)) != hipSuccess) { rc = fail(e, "csc sync"); goto out; }
 out:
     (void)hipFree(kin); (void)hipFree(kout); (void)hipFree(pin); (void)hipFree(pout); (void)hipFree(tmp);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* arc-balanced gather index                                                 */
+/* ------------------------------------------------------------------------ */
+
+__global__ void k_nonempty(uint64_t V, const uint64_t* idx, uint32_t* flag)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += stride)
+        flag[v] = idx[v + 1] > idx[v] ? 1u : 0u;
+}
+
+/* list ranks -> g_dst; IPR_GATHER_LAST on the last arc of every non-empty list */
+__global__ void k_gather_index(uint64_t V, const uint64_t* idx, const uint32_t* rank, uint32_t* g_dst,
+                               uint32_t* g_arc)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += stride) {
+        const uint64_t e = idx[v + 1];
+        if (e > idx[v]) {
+            g_dst[rank[v]] = (uint32_t)v;
+            g_arc[e - 1] |= IPR_GATHER_LAST;
+        }
+    }
+}
+
+/* rank of the list holding arc c*K: the last v with idx[v] <= c*K (idx[V] = E
+ * is larger, and that v's list is non-empty) */
+__global__ void k_gather_owner(uint64_t V, uint64_t E, const uint64_t* idx, const uint32_t* rank, uint32_t K,
+                               uint32_t* g_own)
+{
+    const uint64_t nch = (E + K - 1) / K;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nch; c += stride) {
+        const uint64_t p = c * K;
+        uint64_t lo = 0, hi = V;
+        while (hi - lo > 1) {
+            const uint64_t mid = lo + (hi - lo) / 2;
+            if (idx[mid] <= p) lo = mid;
+            else hi = mid;
+        }
+        g_own[c] = rank[lo];
+    }
+}
+
+extern "C" int ipr_hip_build_gather(uint64_t V, uint64_t E, const uint64_t* t_idx, const uint32_t* t_adj,
+                                    uint32_t K, uint32_t* g_arc, uint32_t* g_own, uint32_t* g_dst,
+                                    uint64_t* n_dst, void* stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    *n_dst = 0;
+    if (V >= (1ull << 31) || K == 0 || K % 4) {
+        snprintf(g_err, sizeof(g_err), "gather index: needs V < 2^31 and K a multiple of 4 (V=%llu K=%u)",
+                 (unsigned long long)V, K);
+        return -1;
+    }
+    if (E == 0 || V == 0) return 0;
+    uint32_t *flag = nullptr, *rank = nullptr;
+    void* tmp = nullptr;
+    size_t tmp_bytes = 0;
+    uint32_t last[2] = {0, 0};
+    int rc = 0;
+    hipError_t e;
+    uint64_t vb = (V + 255) / 256, cb = ((E + K - 1) / K + 255) / 256;
+    if (vb > 65536) vb = 65536;
+    if (cb > 65536) cb = 65536;
+    if ((e = hipMemcpyAsync(g_arc, t_adj, E * 4, hipMemcpyDeviceToDevice, st)) != hipSuccess) { rc = fail(e, "gather arcs"); goto out; }
+    if ((e = hipMalloc(&flag, V * 4)) != hipSuccess) { rc = fail(e, "gather flags"); goto out; }
+    if ((e = hipMalloc(&rank, V * 4)) != hipSuccess) { rc = fail(e, "gather ranks"); goto out; }
+    hipLaunchKernelGGL(k_nonempty, dim3((unsigned)vb), dim3(256), 0, st, V, t_idx, flag);
+    e = rocprim::exclusive_scan(nullptr, tmp_bytes, flag, rank, 0u, (size_t)V, rocprim::plus<uint32_t>(), st);
+    if (e != hipSuccess) { rc = fail(e, "gather scan size"); goto out; }
+    if ((e = hipMalloc(&tmp, tmp_bytes ? tmp_bytes : 16)) != hipSuccess) { rc = fail(e, "gather scan temp"); goto out; }
+    e = rocprim::exclusive_scan(tmp, tmp_bytes, flag, rank, 0u, (size_t)V, rocprim::plus<uint32_t>(), st);
+    if (e != hipSuccess) { rc = fail(e, "gather scan"); goto out; }
+    hipLaunchKernelGGL(k_gather_index, dim3((unsigned)vb), dim3(256), 0, st, V, t_idx, rank, g_dst, g_arc);
+    hipLaunchKernelGGL(k_gather_owner, dim3((unsigned)cb), dim3(256), 0, st, V, E, t_idx, rank, K, g_own);
+    if ((e = hipGetLastError()) != hipSuccess) { rc = fail(e, "gather index"); goto out; }
+    if ((e = hipMemcpyAsync(&last[0], rank + V - 1, 4, hipMemcpyDeviceToHost, st)) != hipSuccess ||
+        (e = hipMemcpyAsync(&last[1], flag + V - 1, 4, hipMemcpyDeviceToHost, st)) != hipSuccess ||
+        (e = hipStreamSynchronize(st)) != hipSuccess) { rc = fail(e, "gather sync"); goto out; }
+    *n_dst = (uint64_t)last[0] + last[1];
+out:
+    (void)hipFree(flag); (void)hipFree(rank); (void)hipFree(tmp);
     return rc;
 }
 

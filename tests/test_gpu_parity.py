@@ -108,6 +108,30 @@ def test_cc_tuning_does_not_change_results(tune):
     assert np.array_equal(r.values, ref)
 
 
+@pytest.mark.parametrize("blocks", ["1", "3", "8192"])
+def test_gather_every_superstep(blocks, monkeypatch):
+    """Dense-superstep gather forced on every superstep (dir_alpha huge), with
+    1, 3 and 8192 workgroups: in-lists split across lanes, passes and
+    wavefronts, hubs, isolated vertices; symmetric and directed transposes,
+    with and without weights."""
+    monkeypatch.setenv("IPR_GATHER_BLOCKS", blocks)
+    g = G.rmat(13, 16, seed=5, symmetrise=True)
+    r = EN.run_app("cc_spread", g, dir_alpha=1 << 30)
+    assert np.array_equal(r.values, O.min_ancestor(g))
+    assert r.gathered > 0
+    gd = G.rmat(12, 16, seed=6)
+    r = EN.run_app("sssp_spread", gd, source=0, dir_alpha=1 << 30)
+    assert np.array_equal(r.values, O.bfs(gd, 0))
+    gw = gd.with_weights(2)
+    r = EN.run_app("sssp_weighted_spread", gw, source=0, dir_alpha=1 << 30)
+    assert np.array_equal(r.values, O.dijkstra(gw, 0))
+    assert r.gathered > 0
+    star = G.from_arcs(3100, np.r_[np.arange(1, 3000), np.zeros(5, int)],
+                       np.r_[np.full(2999, 2999), np.arange(3000, 3005)], keep_self_loops=False)
+    r = EN.run_app("cc_spread", star, dir_alpha=1 << 30)
+    assert np.array_equal(r.values, O.min_ancestor(star))
+
+
 def test_repeat_runs_identical():
     g = G.rmat(13, 16, seed=4)
     r = EN.run_app("sssp_spread", g, source=0, repeat=3)
