@@ -9,10 +9,13 @@ scale 22, edge factor 16, a/b/c/d = .57/.19/.19/.05, seed 2.
   value      GTEPS = W / t, W = sum over supersteps of the out-degrees of the
              vertices that broadcast (deterministic under BSP), t = device time
              of the K timed steps (HIP events on the engine's stream)
-  roofline   the dominant kernel (the message expansion), algorithmic bytes
-             8 B per relaxed arc (4 B destination id + 4 B mailbox update)
-             + 4 B per next-frontier entry, over its HIP-event-timed launch
-             durations, against the 8 TB/s HBM peak of MI355X
+  roofline   the dominant kernel: the gather of the dense supersteps, with
+             8 B per arc relaxed in them (4 B source id + 4 B mailbox update;
+             12 B with weights), over its HIP-event-timed launch durations
+             (plain launches), against the 8 TB/s HBM peak of MI355X; with no
+             dense superstep, the expansion (+ 4 B per next-frontier entry);
+             for PageRank the fused pull kernel. traffic = rocprofv3-measured
+             HBM bytes per launch where a profile exists
   cpu_baseline  the reference's own OpenMP build (oracle/_ref, compiled from
              the reference sources) on the same graph file, its
              "Total time of supersteps", with its thread count
@@ -87,6 +90,12 @@ def wl_pr16():
 # engine options per workload (the FP64 PageRank line uses the reassociating
 # fold; the ordered fold is bit-exact and is what the parity tests pin)
 TUNING = {"pr16": {"pull_ordered": 2}}
+
+# HBM bytes per launch of the dominant kernel, from rocprofv3 --pmc FETCH_SIZE
+# and WRITE_SIZE passes over `bench.py --apps cc22` (FETCH_SIZE scaled by 2.0
+# and WRITE_SIZE by 1.0, the factors a plain 1 GiB device copy gives); the
+# commands and per-pass figures are in profiles/ipr_k_gather_cc_rmat22.md
+TRAFFIC = {("cc22", "ipr_k_gather"): (2_074_354 * 2 + 58_251) * 1024}
 
 
 def wl_ss24a():
@@ -187,15 +196,33 @@ def run_workload(key: str, steps: int, warmup: int, with_cpu: bool) -> dict:
     step_bytes = model_bytes(app, g, W, F, n_bcast_steps)
     log(f"{key}: {n_super} supersteps, W={W}, {ms:.3f} ms/step, {gteps:.2f} GTEPS")
     # the dominant kernel, timed by HIP events around each of its launches
+    # (plain launches; per superstep: the gather or fused pull kernel, then
+    # the expansion)
     sess.kernel_events(True)
     k0_ms, k0_n = st.kernel_ms, st.kernel_launches
     sess.run(steps)
     st2 = sess.stats()
+    t_first, t_second = sess.kernel_times()
     sess.kernel_events(False)
-    k_ms = (st2.kernel_ms - k0_ms) / steps
-    k_launch = (st2.kernel_launches - k0_n) / steps
-    k_bytes = kernel_bytes(app, g, W, int(active.sum()) if not app.startswith("pagerank") else 0,
-                           n_bcast_steps)
+    phase_ms = (st2.kernel_ms - k0_ms) / steps
+    phase_launch = (st2.kernel_launches - k0_n) / steps
+    per_arc = 12 if "weighted" in app else 8
+    alpha = TUNING.get(key, {}).get("dir_alpha", 6)
+    gathered = [i for i in range(len(sedges)) if int(sedges[i]) * alpha > g.E]
+    if app.startswith("pagerank"):
+        k_name, k_launch = "ipr_k_pull_step", phase_launch
+        k_ms = phase_ms
+        k_bytes = kernel_bytes(app, g, W, 0, n_bcast_steps)
+    elif gathered and int(st.gathered) == len(gathered):
+        # dense supersteps: the gather kernel alone, 8 (12 weighted) B per arc
+        # relaxed in those supersteps
+        k_name, k_launch = "ipr_k_gather", len(gathered)
+        k_ms = float(sum(t_first[i] for i in gathered))
+        k_bytes = per_arc * int(sum(int(sedges[i]) for i in gathered))
+    else:
+        k_name, k_launch = "ipr_k_expand", phase_launch
+        k_ms = float(t_second.sum())
+        k_bytes = kernel_bytes(app, g, W, int(active.sum()), n_bcast_steps)
     achieved = k_bytes / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
     values_digest = int(np.bitwise_xor.reduce(sess.values().view(np.uint32))) if g.V else 0
     sess.close()
@@ -205,10 +232,14 @@ def run_workload(key: str, steps: int, warmup: int, with_cpu: bool) -> dict:
         "model_bytes_per_step": step_bytes,
         "model_GBps": step_bytes / (ms * 1e-3) / 1e9,
         "hbm_frac_model": step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-        "kernel": {"name": "ipr_k_pull_step" if app.startswith("pagerank") else "ipr_k_expand",
+        "kernel": {"name": k_name,
                    "ms_per_step": k_ms, "launches_per_step": k_launch,
                    "avg_launch_ms": k_ms / k_launch if k_launch else None,
                    "algorithmic_bytes_per_step": k_bytes, "achieved_GBps": achieved,
+                   "traffic_per_launch": TRAFFIC.get((key, k_name)),
+                   "delivery_phase_ms_per_step": phase_ms,
+                   "per_superstep_ms": {"first": [round(float(x), 4) for x in t_first[:64]],
+                                        "expansion": [round(float(x), 4) for x in t_second[:64]]},
                    "ms_per_step_plain_launches": (st2.total_ms - st.total_ms) / steps},
         "active_trace": [int(x) for x in active[:64]],
         "values_xor": values_digest,
@@ -405,7 +436,7 @@ def main() -> int:
                    "supersteps": head["supersteps"], "W": head["W"]},
         "roofline": {"bound": "hbm", "achieved": k["achieved_GBps"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": k["achieved_GBps"] / HBM_PEAK_GBS,
-                     "traffic": None, "kernel": k["name"],
+                     "traffic": k.get("traffic_per_launch"), "kernel": k["name"],
                      "avg_launch_ms": k["avg_launch_ms"]},
         "cpu_baseline": head.get("cpu_baseline"),
         "hbm_frac_model": head["hbm_frac_model"],

@@ -86,6 +86,9 @@ int ipr_harness_run(int repeat);
 /* 1: HIP events around every launch of the dominant kernel (plain launches);
  * 0: off (chunks replayed as HIP graphs) */
 int ipr_harness_kernel_events(int on);
+/* with kernel events on: per superstep of the last run, the milliseconds of
+ * the gather (or the fused pull kernel) and of the expansion that follows */
+int ipr_harness_kernel_times(double* first, double* second, uint64_t cap);
 /* the hipStream_t the driver launches on */
 void* ipr_harness_stream(void);
 /* release all device state of the current graph */

@@ -9,7 +9,8 @@ Two targets:
   tune_engine(app, g) the drop-in engine (libipr_<app>_32.so): the push bin
                       thresholds (bin_small_max = arcs per large chunk,
                       bin_medium_max = out-degree of a large vertex), the pull
-                      bins and the persistent grid.
+                      bins, the persistent grid and, for push apps, the
+                      push/gather switch point (dir_alpha).
 
 No knob changes a result (min-combined distances and labels are unique; the
 PPR arithmetic does not depend on them), so the tuner only compares times.
@@ -110,6 +111,8 @@ ENGINE_GRID = {
     # push: (bin_small_max = arcs per large chunk, bin_medium_max = large-vertex out-degree)
     "push": [(2048, 2048), (1024, 1024), (4096, 4096), (1024, 4096), (2048, 8192), (512, 2048)],
     "grid_blocks": [0, 1024, 4096],
+    # push apps: gather a superstep when its broadcasters' out-arcs * alpha > E
+    "dir_alpha": [6, 2, 16],
     # pull (single broadcast): (pull_small_max, pull_medium_max)
     "pull": [(0, 0), (2, 16), (4, 64), (8, 256), (16, 1024)],
 }
@@ -145,4 +148,6 @@ def tune_engine(app: str, g: Graph, source: int = 0, repeat: int = 3, **fixed) -
     best_bins = min(bins, key=lambda kw: measure(**kw))
     best_grid = min(ENGINE_GRID["grid_blocks"], key=lambda gb: measure(**best_bins, grid_blocks=gb))
     choice = dict(best_bins, grid_blocks=best_grid)
+    if not pull:
+        choice["dir_alpha"] = min(ENGINE_GRID["dir_alpha"], key=lambda al: measure(**choice, dir_alpha=al))
     return {"app": app, "choice": choice, "tried": results, "seconds": time.time() - t0}

@@ -52,8 +52,10 @@ static struct {
     uint32_t max_chunk;
     int quiet;
     int kev_on;              /* events around the dominant kernel */
-    void* kev[2][64];        /* start/stop per superstep slot of a chunk */
+    void* kev[3][64];        /* start / split / stop per superstep slot of a chunk */
     uint32_t kev_slot;       /* slot of the superstep being enqueued */
+    double* kev_ms[2];       /* per superstep of the last timed run: before / after the split */
+    uint64_t kev_n;          /* supersteps in kev_ms */
 } D;
 
 /* ------------------------------------------------------------------------ */
@@ -164,10 +166,16 @@ static void dev_free_all(void)
     if (D.h_ctl) ipr_hip_host_free(D.h_ctl);
     D.h_ctl = NULL;
     for (int i = 0; i < 64; i++) {
-        if (D.kev[0][i]) ipr_hip_event_destroy(D.kev[0][i]);
-        if (D.kev[1][i]) ipr_hip_event_destroy(D.kev[1][i]);
-        D.kev[0][i] = D.kev[1][i] = NULL;
+        for (int k = 0; k < 3; k++) {
+            if (D.kev[k][i]) ipr_hip_event_destroy(D.kev[k][i]);
+            D.kev[k][i] = NULL;
+        }
     }
+    for (int k = 0; k < 2; k++) {
+        free(D.kev_ms[k]);
+        D.kev_ms[k] = NULL;
+    }
+    D.kev_n = 0;
     D.kev_on = 0;
     if (D.ev0) ipr_hip_event_destroy(D.ev0);
     if (D.ev1) ipr_hip_event_destroy(D.ev1);
@@ -518,13 +526,14 @@ static void launch_app(int k)
     if (rc) die("HIP failure launching kernel %d (%d)\n", k, rc);
 }
 
-/* HIP events bracket the delivery phase of each superstep (gather and/or
- * expansion, or the fused pull kernel) when kernel timing is on */
+/* HIP events bracket the delivery phase of each superstep when kernel timing
+ * is on: mark 0 before it, 1 between the gather (or the fused pull kernel)
+ * and the expansion, 2 after it */
 static void kev_mark(int which)
 {
     if (!D.kev_on || D.kev_slot >= 64) return;
     hip_check(ipr_hip_event_record(D.kev[which][D.kev_slot], D.stream), "kernel event");
-    if (which == 1) D.kev_slot++;
+    if (which == 2) D.kev_slot++;
 }
 
 /* one superstep of the variant's BSP loop, in phase order */
@@ -535,6 +544,7 @@ static void enqueue_superstep(void)
         kev_mark(0);
         launch_app(IPR_K_PULL_STEP);
         kev_mark(1);
+        kev_mark(2);
         break;
     case IPR_VARIANT_SPREAD_PULL:
         /* src/combiner_spread_single_broadcast_postamble.h ip_run: compute on
@@ -543,8 +553,9 @@ static void enqueue_superstep(void)
          * reset phase has nothing to do */
         launch_app(IPR_K_COMPUTE);
         kev_mark(0);
-        launch_app(IPR_K_EXPAND);
         kev_mark(1);
+        launch_app(IPR_K_EXPAND);
+        kev_mark(2);
         hip_check(ipr_hip_compact_frontier(&D.dev, D.stream), "compact");
         launch_app(IPR_K_FETCH);
         break;
@@ -554,8 +565,9 @@ static void enqueue_superstep(void)
         if (D.dev.dir_opt) hip_check(ipr_hip_direction(&D.dev, D.stream), "direction");
         kev_mark(0);
         if (D.dev.dir_opt) launch_app(IPR_K_GATHER);
-        launch_app(IPR_K_EXPAND);
         kev_mark(1);
+        launch_app(IPR_K_EXPAND);
+        kev_mark(2);
         if (D.app.variant == IPR_VARIANT_SPREAD)
             hip_check(ipr_hip_compact_frontier(&D.dev, D.stream), "compact");
         else
@@ -613,10 +625,17 @@ static void run_once(void)
             uint64_t live = D.h_ctl->executed - prev_exec;
             if (D.app.variant == IPR_VARIANT_PULL && prev_exec == 0 && live < chunk) live++;
             for (uint32_t i = 0; i < D.kev_slot && i < live; i++) {
-                float kms = 0;
-                hip_check(ipr_hip_event_elapsed_ms(D.kev[0][i], D.kev[1][i], &kms), "kernel event");
-                D.stats.kernel_ms += kms;
+                float k1 = 0, k2 = 0;
+                hip_check(ipr_hip_event_elapsed_ms(D.kev[0][i], D.kev[1][i], &k1), "kernel event");
+                hip_check(ipr_hip_event_elapsed_ms(D.kev[1][i], D.kev[2][i], &k2), "kernel event");
+                D.stats.kernel_ms += k1 + k2;
                 D.stats.kernel_launches++;
+                uint64_t sidx = prev_exec + i;
+                if (sidx < IPR_TRACE_CAP) {
+                    D.kev_ms[0][sidx] = k1;
+                    D.kev_ms[1][sidx] = k2;
+                    if (sidx + 1 > D.kev_n) D.kev_n = sidx + 1;
+                }
             }
         }
         prev_exec = D.h_ctl->executed;
@@ -663,12 +682,29 @@ void* ipr_harness_stream(void) { return D.stream; }
 int ipr_harness_kernel_events(int on)
 {
     if (on && !D.kev[0][0]) {
-        for (int i = 0; i < 64; i++) {
-            if (ipr_hip_event_create(&D.kev[0][i]) || ipr_hip_event_create(&D.kev[1][i])) return -EIO;
+        for (int i = 0; i < 64; i++)
+            for (int k = 0; k < 3; k++)
+                if (ipr_hip_event_create(&D.kev[k][i])) return -EIO;
+    }
+    if (on && !D.kev_ms[0]) {
+        for (int k = 0; k < 2; k++) {
+            D.kev_ms[k] = (double*)calloc(IPR_TRACE_CAP, sizeof(double));
+            if (!D.kev_ms[k]) return -ENOMEM;
         }
     }
     D.kev_on = on != 0;
     return 0;
+}
+
+int ipr_harness_kernel_times(double* first, double* second, uint64_t cap)
+{
+    if (!D.initialised || !D.kev_ms[0]) return -EINVAL;
+    uint64_t n = D.kev_n < cap ? D.kev_n : cap;
+    for (uint64_t s = 0; s < n; s++) {
+        if (first) first[s] = D.kev_ms[0][s];
+        if (second) second[s] = D.kev_ms[1][s];
+    }
+    return (int)n;
 }
 
 static int run_and_report(int reps)

@@ -105,6 +105,7 @@ def load_app(name: str) -> C.CDLL:
     L.ipr_harness_stream.argtypes = []
     L.ipr_harness_stream.restype = C.c_void_p
     L.ipr_harness_kernel_events.argtypes = [C.c_int]
+    L.ipr_harness_kernel_times.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64]
     _loaded[name] = L
     return L
 
@@ -238,6 +239,16 @@ class Session:
         if n:
             self.L.ipr_harness_trace(a.ctypes.data, e.ctypes.data, t.ctypes.data, n)
         return a, e, t
+
+    def kernel_times(self) -> tuple[np.ndarray, np.ndarray]:
+        """With kernel events on: per superstep of the last run, milliseconds of
+        the gather (or fused pull kernel) and of the expansion after it."""
+        n = int(self.stats().supersteps)
+        g, x = np.zeros(n, np.float64), np.zeros(n, np.float64)
+        got = self.L.ipr_harness_kernel_times(g.ctypes.data, x.ctypes.data, n) if n else 0
+        if got < 0:
+            raise RuntimeError("kernel events were not on")
+        return g[:got], x[:got]
 
     def kernel_events(self, on: bool) -> None:
         if self.L.ipr_harness_kernel_events(int(on)):

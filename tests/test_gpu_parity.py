@@ -132,6 +132,17 @@ def test_gather_every_superstep(blocks, monkeypatch):
     assert np.array_equal(r.values, O.min_ancestor(star))
 
 
+def test_tune_engine_keeps_results():
+    from ipregel_amd import autotune as AT
+    g = G.rmat(12, 16, seed=5, symmetrise=True)
+    rep = AT.tune_engine("cc_spread", g, repeat=1)
+    assert {"bin_small_max", "bin_medium_max", "grid_blocks", "dir_alpha"} <= set(rep["choice"])
+    assert len(rep["tried"]) == len(AT.ENGINE_GRID["push"]) + len(AT.ENGINE_GRID["grid_blocks"]) \
+        + len(AT.ENGINE_GRID["dir_alpha"])
+    r = EN.run_app("cc_spread", g, **rep["choice"])
+    assert np.array_equal(r.values, O.min_ancestor(g))
+
+
 def test_repeat_runs_identical():
     g = G.rmat(13, 16, seed=4)
     r = EN.run_app("sssp_spread", g, source=0, repeat=3)
