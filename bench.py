@@ -116,8 +116,14 @@ def wl_grid():
             "seed 3, source 4097", None, 4097)
 
 
+def wl_cc24():
+    g = G.rmat(24, 16, seed=2, symmetrise=True)
+    return ("cc_spread", g, "cc_spread hashmin CC, symmetrised R-MAT scale 24 ef 16 "
+            "a/b/c/d .57/.19/.19/.05 seed 2", "cc_spread_32", 0)
+
+
 WORKLOADS = {"cc22": wl_cc22, "pr16": wl_pr16, "ss24a": wl_ss24a, "ss24b": wl_ss24b,
-             "grid": wl_grid}
+             "grid": wl_grid, "cc24": wl_cc24}
 
 
 def model_bytes(app: str, g: G.Graph, W: int, F: int, supersteps_with_bcast: int) -> int:

@@ -112,6 +112,10 @@ typedef struct ipr_dev {
     const uint32_t* g_arc;
     const uint32_t* g_own;
     const uint32_t* g_dst;
+    /* SPREAD with gather: the broadcast record of vertex v lives at
+     * bslot[bpos[v]], records ordered by descending out-degree so that the
+     * most-gathered ones share cache lines (NULL: bslot[v]) */
+    const uint32_t* bpos;
     uint64_t g_per_wave;
     uint32_t gather_blocks;
     uint32_t g_pad;

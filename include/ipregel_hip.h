@@ -72,14 +72,18 @@ int ipr_hip_build_csc(uint64_t V, uint64_t E, const uint64_t* out_idx,
 int ipr_hip_build_pull_bins(ipr_dev* dev, uint32_t small_max, uint32_t medium_max,
                             void* stream);
 
+/* record order: bpos[v] = rank of v by descending out-degree (ties by id) */
+int ipr_hip_build_record_order(uint64_t V, const uint64_t* out_idx, uint32_t* bpos, void* stream);
+
 /* index of the arc-balanced gather over a device CSC (in-lists):
- * g_arc[E] = t_adj with IPR_GATHER_LAST on the last arc of every non-empty
- * list, g_dst[*n_dst] = the vertices with non-empty lists (ascending),
+ * g_arc[E] = the sources (their record positions bpos[u] when bpos is not
+ * NULL) with IPR_GATHER_LAST on the last arc of every non-empty list,
+ * g_dst[*n_dst] = the vertices with non-empty lists (ascending),
  * g_own[ceil(E/K)] = position in g_dst of the list holding arc c*K.
  * Needs V < 2^31 and K a multiple of 4. */
 int ipr_hip_build_gather(uint64_t V, uint64_t E, const uint64_t* t_idx, const uint32_t* t_adj,
-                         uint32_t K, uint32_t* g_arc, uint32_t* g_own, uint32_t* g_dst,
-                         uint64_t* n_dst, void* stream);
+                         const uint32_t* bpos, uint32_t K, uint32_t* g_arc, uint32_t* g_own,
+                         uint32_t* g_dst, uint64_t* n_dst, void* stream);
 
 /* byte-wise equality of two device buffers (multiple of 4 bytes) */
 int ipr_hip_equal(const void* a, const void* b, uint64_t bytes, int* equal, void* stream);

@@ -272,6 +272,12 @@ struct ipr_bslot_t {
     uint32_t step;
     IP_MESSAGE_TYPE msg;
 };
+
+/* position of vertex v's broadcast record */
+__device__ __forceinline__ uint32_t ipr_brec(const ipr_dev& d, uint32_t v)
+{
+    return d.bpos ? d.bpos[v] : v;
+}
 #define IPR_BSTEP_WEIGHTED 0x80000000u
 
 /* raise the received flag of u by atomic OR on its 32-bit word (plain byte
@@ -453,7 +459,7 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_compute_push(ipr_dev d)
                     ipr_bslot_t rec;
                     rec.step = (uint32_t)(s + 1) | (x.ipr_bcast_weighted ? IPR_BSTEP_WEIGHTED : 0u);
                     rec.msg = x.ipr_bcast_msg;
-                    bs[v] = rec;
+                    bs[ipr_brec(d, v)] = rec;
                     if (deg > d.large_min) {
                         large = 1;
                         chunks = (uint32_t)((deg + d.large_chunk - 1) / d.large_chunk);
@@ -510,7 +516,7 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_expand(ipr_dev d)
             if (d.lbase[mid] <= c) lo = mid; else hi = mid;
         }
         uint32_t v = d.lbins[lo];
-        ipr_bslot_t b = bs[v];
+        ipr_bslot_t b = bs[ipr_brec(d, v)];
         bool wf = (b.step & IPR_BSTEP_WEIGHTED) != 0;
         uint64_t off = d.out_idx[v], deg = d.out_idx[v + 1] - off;
         uint64_t beg = (c - d.lbase[lo]) * d.large_chunk;
@@ -533,7 +539,7 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_expand(ipr_dev d)
         uint32_t deg = 0;
         if (i < n) {
             uint32_t v = list ? list[i] : (uint32_t)i;
-            ipr_bslot_t b = bs[v];
+            ipr_bslot_t b = bs[ipr_brec(d, v)];
             if ((b.step & ~IPR_BSTEP_WEIGHTED) == want) {
                 uint64_t off = d.out_idx[v];
                 uint64_t dg = d.out_idx[v + 1] - off;
@@ -1038,7 +1044,7 @@ __device__ inline bool ipr_fetch_ordered_epoch_wave(const ipr_dev& d, uint32_t v
         bool f = false;
         IP_MESSAGE_TYPE m = IP_MESSAGE_TYPE();
         if (k < end) {
-            ipr_bslot_t b = bs[d.in_adj[k]];
+            ipr_bslot_t b = bs[ipr_brec(d, d.in_adj[k])];
             f = (b.step & ~IPR_BSTEP_WEIGHTED) == want;
             m = b.msg;
         }
@@ -1079,7 +1085,7 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_fetch_targets(ipr_dev d)
                 bool has = false;
                 IP_MESSAGE_TYPE acc = IP_MESSAGE_TYPE();
                 for (uint64_t k = beg; k < end; k++) {
-                    ipr_bslot_t b = bs[d.in_adj[k]];
+                    ipr_bslot_t b = bs[ipr_brec(d, d.in_adj[k])];
                     if ((b.step & ~IPR_BSTEP_WEIGHTED) == want) {
                         if (has) ip_combine(&acc, b.msg);
                         else { acc = b.msg; has = true; }

@@ -136,6 +136,7 @@ static void dev_free_all(void)
     ipr_hip_free((void*)d->g_arc);
     ipr_hip_free((void*)d->g_own);
     ipr_hip_free((void*)d->g_dst);
+    ipr_hip_free((void*)d->bpos);
     ipr_hip_free(d->bslot);
     ipr_hip_free(d->lbase);
     if (d->in_idx && d->in_idx != d->out_idx) ipr_hip_free((void*)d->in_idx);
@@ -306,7 +307,15 @@ static void dev_setup(void)
         uint32_t* g_arc = dalloc(E * 4, "gather sources");
         uint32_t* g_own = dalloc(nch * 4, "gather chunk receivers");
         uint32_t* g_dst = dalloc(vv * 4, "gather receivers");
-        hip_check(ipr_hip_build_gather(V, E, d->t_idx, d->t_adj, IPR_GATHER_K, g_arc, g_own, g_dst, &ndst, st),
+        /* SPREAD reads records only for its frontier outside the gather, so
+         * reordering them by out-degree costs little there (PUSH scans all V) */
+        if (var == IPR_VARIANT_SPREAD && env_u32("IPR_RECORD_ORDER", 1) == 1) {
+            uint32_t* bpos = dalloc(vv * 4, "record positions");
+            hip_check(ipr_hip_build_record_order(V, d->out_idx, bpos, st), "record order");
+            d->bpos = bpos;
+        }
+        hip_check(ipr_hip_build_gather(V, E, d->t_idx, d->t_adj, d->bpos, IPR_GATHER_K, g_arc, g_own, g_dst,
+                                       &ndst, st),
                   "gather index");
         d->g_arc = g_arc;
         d->g_own = g_own;

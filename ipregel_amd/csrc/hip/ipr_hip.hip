@@ -289,6 +289,65 @@ out:
 /* arc-balanced gather index                                                 */
 /* ------------------------------------------------------------------------ */
 
+/* sort key for descending out-degree (radix sort is stable: ties by id) */
+__global__ void k_degree_keys(uint64_t V, const uint64_t* idx, uint32_t* key, uint32_t* id)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += stride) {
+        const uint64_t dg = idx[v + 1] - idx[v];
+        key[v] = dg >= 0xffffffffull ? 0u : 0xffffffffu - (uint32_t)dg;
+        id[v] = (uint32_t)v;
+    }
+}
+
+__global__ void k_scatter_pos(uint64_t V, const uint32_t* order, uint32_t* pos)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < V; i += stride)
+        pos[order[i]] = (uint32_t)i;
+}
+
+extern "C" int ipr_hip_build_record_order(uint64_t V, const uint64_t* out_idx, uint32_t* bpos, void* stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    if (V == 0) return 0;
+    if (V >= (1ull << 32)) {
+        snprintf(g_err, sizeof(g_err), "record order: V=%llu needs 32-bit ids", (unsigned long long)V);
+        return -1;
+    }
+    uint32_t *kin = nullptr, *kout = nullptr, *vin = nullptr, *vout = nullptr;
+    void* tmp = nullptr;
+    size_t tmp_bytes = 0;
+    int rc = 0;
+    hipError_t e;
+    uint64_t vb = (V + 255) / 256;
+    if (vb > 65536) vb = 65536;
+    if ((e = hipMalloc(&kin, V * 4)) != hipSuccess) { rc = fail(e, "record order keys"); goto out; }
+    if ((e = hipMalloc(&kout, V * 4)) != hipSuccess) { rc = fail(e, "record order keys"); goto out; }
+    if ((e = hipMalloc(&vin, V * 4)) != hipSuccess) { rc = fail(e, "record order ids"); goto out; }
+    if ((e = hipMalloc(&vout, V * 4)) != hipSuccess) { rc = fail(e, "record order ids"); goto out; }
+    hipLaunchKernelGGL(k_degree_keys, dim3((unsigned)vb), dim3(256), 0, st, V, out_idx, kin, vin);
+    e = rocprim::radix_sort_pairs(nullptr, tmp_bytes, kin, kout, vin, vout, (size_t)V, 0, 32, st);
+    if (e != hipSuccess) { rc = fail(e, "record order sort size"); goto out; }
+    if ((e = hipMalloc(&tmp, tmp_bytes ? tmp_bytes : 16)) != hipSuccess) { rc = fail(e, "record order temp"); goto out; }
+    e = rocprim::radix_sort_pairs(tmp, tmp_bytes, kin, kout, vin, vout, (size_t)V, 0, 32, st);
+    if (e != hipSuccess) { rc = fail(e, "record order sort"); goto out; }
+    hipLaunchKernelGGL(k_scatter_pos, dim3((unsigned)vb), dim3(256), 0, st, V, vout, bpos);
+    if ((e = hipGetLastError()) != hipSuccess) { rc = fail(e, "record order"); goto out; }
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) { rc = fail(e, "record order sync"); goto out; }
+out:
+    (void)hipFree(kin); (void)hipFree(kout); (void)hipFree(vin); (void)hipFree(vout); (void)hipFree(tmp);
+    return rc;
+}
+
+/* the transpose's sources, as record positions when records are reordered */
+__global__ void k_gather_sources(uint64_t E, const uint32_t* t_adj, const uint32_t* bpos, uint32_t* g_arc)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < E; k += stride)
+        g_arc[k] = bpos ? bpos[t_adj[k]] : t_adj[k];
+}
+
 __global__ void k_nonempty(uint64_t V, const uint64_t* idx, uint32_t* flag)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -330,8 +389,8 @@ __global__ void k_gather_owner(uint64_t V, uint64_t E, const uint64_t* idx, cons
 }
 
 extern "C" int ipr_hip_build_gather(uint64_t V, uint64_t E, const uint64_t* t_idx, const uint32_t* t_adj,
-                                    uint32_t K, uint32_t* g_arc, uint32_t* g_own, uint32_t* g_dst,
-                                    uint64_t* n_dst, void* stream)
+                                    const uint32_t* bpos, uint32_t K, uint32_t* g_arc, uint32_t* g_own,
+                                    uint32_t* g_dst, uint64_t* n_dst, void* stream)
 {
     hipStream_t st = (hipStream_t)stream;
     *n_dst = 0;
@@ -350,7 +409,11 @@ extern "C" int ipr_hip_build_gather(uint64_t V, uint64_t E, const uint64_t* t_id
     uint64_t vb = (V + 255) / 256, cb = ((E + K - 1) / K + 255) / 256;
     if (vb > 65536) vb = 65536;
     if (cb > 65536) cb = 65536;
-    if ((e = hipMemcpyAsync(g_arc, t_adj, E * 4, hipMemcpyDeviceToDevice, st)) != hipSuccess) { rc = fail(e, "gather arcs"); goto out; }
+    {
+        uint64_t eb = (E + 255) / 256;
+        if (eb > 65536) eb = 65536;
+        hipLaunchKernelGGL(k_gather_sources, dim3((unsigned)eb), dim3(256), 0, st, E, t_adj, bpos, g_arc);
+    }
     if ((e = hipMalloc(&flag, V * 4)) != hipSuccess) { rc = fail(e, "gather flags"); goto out; }
     if ((e = hipMalloc(&rank, V * 4)) != hipSuccess) { rc = fail(e, "gather ranks"); goto out; }
     hipLaunchKernelGGL(k_nonempty, dim3((unsigned)vb), dim3(256), 0, st, V, t_idx, flag);

@@ -108,13 +108,14 @@ def test_cc_tuning_does_not_change_results(tune):
     assert np.array_equal(r.values, ref)
 
 
-@pytest.mark.parametrize("blocks", ["1", "3", "8192"])
-def test_gather_every_superstep(blocks, monkeypatch):
+@pytest.mark.parametrize("blocks,order", [("1", "1"), ("3", "0"), ("8192", "1"), ("8192", "0")])
+def test_gather_every_superstep(blocks, order, monkeypatch):
     """Dense-superstep gather forced on every superstep (dir_alpha huge), with
     1, 3 and 8192 workgroups: in-lists split across lanes, passes and
     wavefronts, hubs, isolated vertices; symmetric and directed transposes,
-    with and without weights."""
+    with and without weights; broadcast records by vertex id or by degree."""
     monkeypatch.setenv("IPR_GATHER_BLOCKS", blocks)
+    monkeypatch.setenv("IPR_RECORD_ORDER", order)
     g = G.rmat(13, 16, seed=5, symmetrise=True)
     r = EN.run_app("cc_spread", g, dir_alpha=1 << 30)
     assert np.array_equal(r.values, O.min_ancestor(g))
