@@ -497,10 +497,14 @@ extern "C" int ipr_hip_build_pull_bins(ipr_dev* dev, uint32_t small_max, uint32_
 /* frontier compaction                                                       */
 /* ------------------------------------------------------------------------ */
 
-/* The received flags of superstep s+1 become the frontier list: each thread
- * owns 16 flag bytes (and clears them for reuse two supersteps on), a
- * workgroup prefix sum of their counts places the ids and one atomic per
- * workgroup reserves its run, so ids ascend within each 4096-vertex run. */
+/* The received flags of superstep s+1 become the frontier list. Each thread
+ * owns IPR_COMPACT_T consecutive 16-flag groups (and clears them for reuse two
+ * supersteps on); a workgroup prefix sum of the threads' counts places the
+ * ids, and one atomic per workgroup reserves its run, so ids ascend within
+ * each 256*16*IPR_COMPACT_T-vertex run. Few workgroups means few atomics on
+ * the one counter: contention there, not the flag bytes, bounds this kernel
+ * when most runs hold a frontier vertex. */
+#define IPR_COMPACT_T 8
 __global__ void __launch_bounds__(256) k_compact(ipr_dev d)
 {
     __shared__ uint32_t wsum[4];
@@ -512,16 +516,24 @@ __global__ void __launch_bounds__(256) k_compact(ipr_dev d)
     uint32_t* out = d.front[(s + 1) & 1];
     unsigned long long* counter = (unsigned long long*)&c->fsize[(s + 1) & 1];
     const uint64_t groups = (d.V + 15) / 16;           /* flag arrays are padded to 16 */
+    const uint64_t span = 256ull * IPR_COMPACT_T;
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (uint64_t g0 = (uint64_t)blockIdx.x * 256; g0 < groups; g0 += (uint64_t)gridDim.x * 256) {
-        const uint64_t g = g0 + threadIdx.x;
-        uint4 q = make_uint4(0, 0, 0, 0);
-        if (g < groups) {
-            q = flags[g];
-            if (q.x | q.y | q.z | q.w) flags[g] = make_uint4(0, 0, 0, 0);
+    for (uint64_t b0 = (uint64_t)blockIdx.x * span; b0 < groups; b0 += (uint64_t)gridDim.x * span) {
+        const uint64_t g0 = b0 + (uint64_t)threadIdx.x * IPR_COMPACT_T;
+        uint4 q[IPR_COMPACT_T];
+        uint32_t n = 0;
+#pragma unroll
+        for (int t = 0; t < IPR_COMPACT_T; t++) {
+            q[t] = make_uint4(0, 0, 0, 0);
+            if (g0 + t < groups) q[t] = flags[g0 + t];
         }
-        /* flags are 0 or 1, so each set flag is one bit (0, 8, 16 or 24) */
-        const uint32_t n = __popc(q.x) + __popc(q.y) + __popc(q.z) + __popc(q.w);
+#pragma unroll
+        for (int t = 0; t < IPR_COMPACT_T; t++) {
+            /* flags are 0 or 1, so each set flag is one bit (0, 8, 16 or 24) */
+            const uint32_t m = __popc(q[t].x) + __popc(q[t].y) + __popc(q[t].z) + __popc(q[t].w);
+            if (m) flags[g0 + t] = make_uint4(0, 0, 0, 0);
+            n += m;
+        }
         uint32_t incl = n;
         for (int o = 1; o < 64; o <<= 1) {
             uint32_t y = __shfl_up(incl, o);
@@ -536,15 +548,18 @@ __global__ void __launch_bounds__(256) k_compact(ipr_dev d)
         }
         if (threadIdx.x == 0 && total) blk_base = atomicAdd(counter, (unsigned long long)total);
         __syncthreads();
-        if (total) {
+        if (n) {
             uint64_t pos = blk_base + before + incl - n;
-            const uint32_t words[4] = {q.x, q.y, q.z, q.w};
-            for (int k = 0; k < 4; k++) {
-                uint32_t w = words[k];
-                while (w) {
-                    uint32_t bit = __ffs(w) - 1;
-                    out[pos++] = (uint32_t)(g * 16 + k * 4 + bit / 8);
-                    w &= w - 1;
+#pragma unroll
+            for (int t = 0; t < IPR_COMPACT_T; t++) {
+                const uint32_t words[4] = {q[t].x, q[t].y, q[t].z, q[t].w};
+                for (int k = 0; k < 4; k++) {
+                    uint32_t w = words[k];
+                    while (w) {
+                        uint32_t bit = __ffs(w) - 1;
+                        out[pos++] = (uint32_t)((g0 + t) * 16 + k * 4 + bit / 8);
+                        w &= w - 1;
+                    }
                 }
             }
         }
@@ -554,7 +569,8 @@ __global__ void __launch_bounds__(256) k_compact(ipr_dev d)
 
 extern "C" int ipr_hip_compact_frontier(const ipr_dev* dev, void* stream)
 {
-    uint64_t blocks = ((dev->V + 15) / 16 + 255) / 256;
+    const uint64_t span = 256ull * IPR_COMPACT_T;
+    uint64_t blocks = ((dev->V + 15) / 16 + span - 1) / span;
     if (blocks < 1) blocks = 1;
     if (blocks > 16384) blocks = 16384;
     hipLaunchKernelGGL(k_compact, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, *dev);
