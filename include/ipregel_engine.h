@@ -73,8 +73,8 @@ typedef struct ipr_ctl {
     uint32_t error;              /* device-side API misuse flags */
     uint64_t fsize[2];           /* frontier list sizes by parity */
     uint64_t lpack;              /* large broadcasters: entries << 40 | arc chunks */
-    uint64_t pull_mode;          /* push apps: 1 when this superstep gathers */
-    uint64_t step_edges;         /* W of this superstep (set by the direction kernel) */
+    uint64_t pull_mode;          /* push apps: 1 when the last finalized superstep gathered */
+    uint64_t step_edges;         /* W of the last finalized superstep */
     uint64_t work_edges;         /* W accumulated */
     uint64_t work_vertices;      /* vertices computed, accumulated */
     uint64_t trace_cap;          /* entries in the trace arrays */

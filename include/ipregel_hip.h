@@ -95,8 +95,6 @@ int ipr_hip_compact_frontier(const ipr_dev* dev, void* stream);
 /* superstep control (kernels on `stream`, graph-capturable) */
 int ipr_hip_begin(const ipr_dev* dev, void* stream);
 int ipr_hip_finalize(const ipr_dev* dev, int variant, void* stream);
-/* push applications: choose push or gather for the coming delivery */
-int ipr_hip_direction(const ipr_dev* dev, void* stream);
 
 #ifdef __cplusplus
 }

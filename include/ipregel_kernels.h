@@ -273,6 +273,19 @@ struct ipr_bslot_t {
     IP_MESSAGE_TYPE msg;
 };
 
+/* Push apps choose per superstep between pushing and gathering: gather when
+ * the broadcasters' out-arcs (counted by the compute kernel) times dir_alpha
+ * exceed E. Every wavefront of the delivery kernels derives the same answer
+ * from the 64 counter slots; the finalize kernel counts gathered supersteps
+ * by the same rule. */
+__device__ inline bool ipr_gathers(const ipr_dev& d)
+{
+    if (!d.dir_opt) return false;
+    uint64_t x = d.cnt[IPR_C_EDGES * IPR_CSLOTS + ipr_lane()];
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    return x * (uint64_t)d.dir_alpha > d.E;
+}
+
 /* position of vertex v's broadcast record */
 __device__ __forceinline__ uint32_t ipr_brec(const ipr_dev& d, uint32_t v)
 {
@@ -501,7 +514,7 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_expand(ipr_dev d)
     __shared__ uint32_t scan[IPR_WAVES];
     ipr_ctl* ctl = d.ctl;
     if (ctl->done) return;
-    if (ctl->pull_mode) return;
+    if (ipr_gathers(d)) return;
     const uint64_t s = ctl->superstep;
     const ipr_bslot_t* bs = (const ipr_bslot_t*)d.bslot;
     const uint32_t want = (uint32_t)(s + 1);
@@ -675,7 +688,7 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_gather(ipr_dev d)
     static_assert(K % 4 == 0, "sources are read four at a time");
     ipr_ctl* ctl = d.ctl;
     if (ctl->done) return;
-    if (!ctl->pull_mode) return;
+    if (!ipr_gathers(d)) return;
     const uint64_t s = ctl->superstep;
     const uint32_t want = (uint32_t)(s + 1);
     const uint32_t lane = ipr_lane();

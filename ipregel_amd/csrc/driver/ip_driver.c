@@ -571,7 +571,6 @@ static void enqueue_superstep(void)
     case IPR_VARIANT_SPREAD:
     case IPR_VARIANT_PUSH:
         launch_app(IPR_K_COMPUTE);
-        if (D.dev.dir_opt) hip_check(ipr_hip_direction(&D.dev, D.stream), "direction");
         kev_mark(0);
         if (D.dev.dir_opt) launch_app(IPR_K_GATHER);
         kev_mark(1);

@@ -673,29 +673,18 @@ __global__ void k_finalize(ipr_dev d, int variant)
         c->superstep = s + 1;
     }
     c->lpack = 0;
-    c->step_edges = 0;
+    c->step_edges = edges;
     c->work_edges += edges;
     c->work_vertices += computed;
+    /* the delivery kernels gathered this superstep by the same rule */
+    c->pull_mode = variant != IPR_VARIANT_PULL && d.dir_opt && edges * (uint64_t)d.dir_alpha > d.E;
     if (c->pull_mode) c->gathered++;
-    c->pull_mode = 0;
     c->last_active = active;
     if (fin < c->trace_cap) {
         d.trace_active[fin] = active;
         d.trace_edges[fin] = edges;
         d.trace_ts[fin + 1] = wall_clock64();
     }
-}
-
-/* push applications with an order-independent combiner: deliver this
- * superstep by gather when the broadcasters' out-arcs are a large share of E */
-__global__ void k_direction(ipr_dev d)
-{
-    ipr_ctl* c = d.ctl;
-    if (c->done) return;
-    uint64_t w = ipr_take(d, IPR_C_EDGES, false);
-    if (threadIdx.x != 0) return;
-    c->step_edges = w;
-    c->pull_mode = (d.dir_opt && w * (uint64_t)d.dir_alpha > d.E) ? 1 : 0;
 }
 
 extern "C" int ipr_hip_begin(const ipr_dev* dev, void* stream)
@@ -712,9 +701,3 @@ extern "C" int ipr_hip_finalize(const ipr_dev* dev, int variant, void* stream)
     return 0;
 }
 
-extern "C" int ipr_hip_direction(const ipr_dev* dev, void* stream)
-{
-    hipLaunchKernelGGL(k_direction, dim3(1), dim3(64), 0, (hipStream_t)stream, *dev);
-    CK(hipGetLastError());
-    return 0;
-}
