@@ -344,7 +344,8 @@ def build_served_graphs() -> tuple[list, list, dict]:
         rg = BA.ResidentGraph(g)
         mw = float(g.wgt.mean()) if g.wgt is not None and g.E else None
         rep = AT.tune_resident(rg, kinds=kinds, mean_weight=mw)
-        tuned[name] = {k: rep[k]["choice"] for k in ("delta", "delta1", "large", "grid", "cc_mode") if k in rep}
+        tuned[name] = {k: rep[k]["choice"] for k in ("delta", "delta1", "large", "grid", "cc_mode", "sssp_states")
+                       if k in rep}
         tuned[name]["seconds"] = round(rep["seconds"], 2)
         log(f"served: {name} V={g.V} E={g.E} tuned {tuned[name]}")
         res.append(rg)
@@ -407,7 +408,7 @@ def main() -> int:
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--serve-rates", default="10,100,500,2000")
     ap.add_argument("--serve-counts", default="40,200,300,400")
-    ap.add_argument("--serve-budget-gib", type=float, default=128.0)
+    ap.add_argument("--serve-budget-gib", type=float, default=192.0)
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is a design "

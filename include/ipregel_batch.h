@@ -61,6 +61,9 @@ typedef struct ipr_batch_tuning {
                          64-slot states */
     uint32_t delta1;  /* the same for one-query states (the plain loop) */
     uint32_t cc_mode; /* CC algorithm: 0 label propagation supersteps, 1 union-find (default) */
+    uint32_t sssp_states; /* ipr_serve: 64-slot SSSP states this graph's queries are spread
+                             over (each arrival joins the least loaded one), run concurrently on
+                             their own streams; 1..8 (default 1) */
 } ipr_batch_tuning;
 int ipr_batch_graph_get_tuning(const ipr_bgraph* g, ipr_batch_tuning* t);
 int ipr_batch_graph_set_tuning(ipr_bgraph* g, const ipr_batch_tuning* t);

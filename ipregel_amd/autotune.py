@@ -5,7 +5,9 @@ Two targets:
   tune_resident(rg)   libipregel_batch.so, per resident graph: the SSSP near/far
                       step of 64-slot states (delta) and of one-query states
                       (delta1), the CC algorithm (label propagation or
-                      union-find), the large-vertex split and the grid size.
+                      union-find), the large-vertex split, the grid size, and
+                      how many concurrent SSSP states the server spreads a
+                      graph's queries over (timed on a served burst).
   tune_engine(app, g) the drop-in engine (libipr_<app>_32.so): the push bin
                       thresholds (bin_small_max = arcs per large chunk,
                       bin_medium_max = out-degree of a large vertex), the pull
@@ -50,7 +52,7 @@ def _scan(values, measure, stop_ratio: float = 1.5) -> tuple[object, dict]:
 
 
 def tune_resident(rg: BA.ResidentGraph, sample: int = 8, seed: int = 0, kinds=("sssp", "cc"),
-                  mean_weight: float | None = None) -> dict:
+                  mean_weight: float | None = None, burst_size: int = 32) -> dict:
     """Tune one resident graph in place; returns the choice and the timings."""
     t0 = time.time()
     rng = np.random.default_rng(seed)
@@ -94,6 +96,18 @@ def tune_resident(rg: BA.ResidentGraph, sample: int = 8, seed: int = 0, kinds=("
         bestG, seenG = _scan([2048, 4096, 1024], grid_ms, stop_ratio=10.0)
         rg.tune(grid=bestG)
         report["grid"] = {"choice": bestG, "ms": seenG}
+
+        # serving: how many concurrent 64-slot states a burst of queries is
+        # spread over (a served burst, wall time)
+        burst = BA.poisson_trace(burst_size, 1e9, {BA.SSSP: 1.0}, {BA.SSSP: [0]}, [rg.V], seed=seed + 1)
+
+        def states_ms(S):
+            rg.tune(sssp_states=S)
+            return BA.serve([rg], burst)[0].wall_s * 1e3
+
+        bestS, seenS = _scan([1, 2, 4, 8], states_ms, stop_ratio=1.2)
+        rg.tune(sssp_states=bestS)
+        report["sssp_states"] = {"choice": bestS, "ms": seenS}
     if "cc" in kinds:
         seenC = {}
         for m in (1, 0):

@@ -36,7 +36,7 @@ class ServeOpts(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("large", C.c_uint32), ("chunk", C.c_uint32), ("grid", C.c_uint32), ("delta", C.c_uint32),
-                ("delta1", C.c_uint32), ("cc_mode", C.c_uint32)]
+                ("delta1", C.c_uint32), ("cc_mode", C.c_uint32), ("sssp_states", C.c_uint32)]
 
 
 class ServeStats(C.Structure):

@@ -67,7 +67,7 @@ double now_s()
     return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
 }
 
-Tuning default_tuning() { return Tuning{1024u, 1024u, 2048u, INF, INF, 1u}; }
+Tuning default_tuning() { return Tuning{1024u, 1024u, 2048u, INF, INF, 1u, 1u}; }
 
 /* IPR_BATCH_GRAPHS=0 makes every state launch its kernels directly instead of
  * replaying captured HIP graphs (same kernels, same order) */
@@ -1794,6 +1794,7 @@ extern "C" int ipr_batch_graph_get_tuning(const ipr_bgraph* g, ipr_batch_tuning*
     t->delta = g->tune.delta;
     t->delta1 = g->tune.delta1;
     t->cc_mode = g->tune.cc_mode;
+    t->sssp_states = g->tune.sssp_states;
     return 0;
 }
 
@@ -1804,9 +1805,9 @@ extern "C" int ipr_batch_graph_set_tuning(ipr_bgraph* g, const ipr_batch_tuning*
         return -2;
     }
     if (t->large < 1 || t->chunk < 64 || t->chunk > (1u << 20) || t->grid < 1 || t->grid > 65535 ||
-        t->delta < 1 || t->delta1 < 1 || t->cc_mode > 1) {
+        t->delta < 1 || t->delta1 < 1 || t->cc_mode > 1 || t->sssp_states < 1 || t->sssp_states > 8) {
         set_error("ipr_batch_graph_set_tuning: large >= 1, chunk in [64, 2^20], grid in [1, 65535], "
-                  "delta and delta1 >= 1, cc_mode 0 or 1");
+                  "delta and delta1 >= 1, cc_mode 0 or 1, sssp_states in [1, 8]");
         return -2;
     }
     g->tune.large = t->large;
@@ -1815,6 +1816,7 @@ extern "C" int ipr_batch_graph_set_tuning(ipr_bgraph* g, const ipr_batch_tuning*
     g->tune.delta = t->delta;
     g->tune.delta1 = t->delta1;
     g->tune.cc_mode = t->cc_mode;
+    g->tune.sssp_states = t->sssp_states;
     return 0;
 }
 

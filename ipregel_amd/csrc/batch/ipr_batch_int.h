@@ -47,6 +47,7 @@ struct Tuning {
                               INF = plain BSP */
     uint32_t delta1;       /* the same for one-query states */
     uint32_t cc_mode;      /* CC: 0 label propagation (BSP supersteps), 1 union-find */
+    uint32_t sssp_states;  /* serving: concurrent 64-slot SSSP states per graph */
 };
 Tuning default_tuning();
 

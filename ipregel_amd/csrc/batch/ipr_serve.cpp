@@ -9,7 +9,10 @@
  *               alone on the scalar kernels (a one-lane state per graph) to
  *               completion before the next one starts.
  *   batched     continuous batching. Per (graph, kind) there is one batcher
- *               with a 64-slot state on its own stream. At every tick boundary
+ *               with a 64-slot state on its own stream (for SSSP, the graph's
+ *               tuning may ask for several, each arrival joining the least
+ *               loaded: on long-diameter graphs a superstep is latency-bound
+ *               and concurrent states overlap). At every tick boundary
  *               a batcher retires finished slots, admits waiting queries into
  *               free slots and launches the next superstep (SSSP) or iteration
  *               (PPR) as a captured HIP graph; batchers of different graphs run
@@ -163,6 +166,7 @@ int serve_sequential(ipr_bgraph** graphs, SeqStates& ss, const std::vector<int>&
 struct Batcher {
     int kind = 0;
     int gi = 0;
+    int rep = 0;                   /* SSSP: which of the graph's concurrent states */
     ipr_bgraph* g = nullptr;
     uint64_t footprint = 0;
     PushState* ps = nullptr;
@@ -214,19 +218,42 @@ struct Server {
     Server(ipr_bgraph** g, int n, Ctx& ctx, uint64_t b, ipr_serve_stats* s)
         : graphs(g), ng(n), c(ctx), budget(b), st(s) {}
 
-    Batcher& get(int kind, int gi)
+    Batcher& get(int kind, int gi, int rep = 0)
     {
-        const long key = (long)gi * 4 + kind;
+        const long key = ((long)gi * 4 + kind) * 16 + rep;
         auto it = batchers.find(key);
         if (it != batchers.end()) return it->second;
         Batcher& b = batchers[key];
         b.kind = kind;
         b.gi = gi;
+        b.rep = rep;
         b.g = graphs[gi];
         b.footprint = kind == IPR_Q_SSSP  ? PushState::footprint(b.g, true)
                       : kind == IPR_Q_PPR ? PprState::footprint(b.g, 64)
                                           : PushState::footprint(b.g, false);
         return b;
+    }
+
+    int states_of(int kind, int gi) const
+    {
+        return kind == IPR_Q_SSSP ? (int)std::max<uint32_t>(1u, graphs[gi]->tune.sssp_states) : 1;
+    }
+
+    /* where an arriving query waits: a graph's SSSP queries are spread over
+     * its concurrent states, each arrival joining the least loaded one */
+    Batcher& route(int kind, int gi)
+    {
+        Batcher* best = nullptr;
+        size_t best_load = 0;
+        for (int r = 0; r < states_of(kind, gi); r++) {
+            Batcher& b = get(kind, gi, r);
+            const size_t load = (size_t)__builtin_popcountll(b.live) + b.waiting.size();
+            if (!best || load < best_load) {
+                best = &b;
+                best_load = load;
+            }
+        }
+        return *best;
     }
 
     /* allocate b's state within the budget; false: it has to wait */
@@ -453,12 +480,14 @@ struct Server {
     int prewarm(const std::vector<int>& order)
     {
         for (int qi : order) {
-            Batcher& b = get(c.q[qi].kind, c.q[qi].graph);
-            if (b.allocated()) continue;
-            if (budget && used + b.footprint > budget) continue;
-            bool ok;
-            const int rc = ensure_state(b, &ok);
-            if (rc) return rc;
+            for (int r = 0; r < states_of(c.q[qi].kind, c.q[qi].graph); r++) {
+                Batcher& b = get(c.q[qi].kind, c.q[qi].graph, r);
+                if (b.allocated()) continue;
+                if (budget && used + b.footprint > budget) continue;
+                bool ok;
+                const int rc = ensure_state(b, &ok);
+                if (rc) return rc;
+            }
         }
         return 0;
     }
@@ -477,7 +506,7 @@ struct Server {
             const double now = c.t();
             while (next < order.size() && c.q[order[next]].t_arrival <= now) {
                 const int qi = order[next++];
-                get(c.q[qi].kind, c.q[qi].graph).waiting.push_back(qi);
+                route(c.q[qi].kind, c.q[qi].graph).waiting.push_back(qi);
             }
             bool any_inflight = false;
             uint64_t live_slots = 0;

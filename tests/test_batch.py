@@ -108,6 +108,10 @@ def test_tuning_rejects_bad_values(graphs):
     _, r = graphs["dir"]
     with pytest.raises(RuntimeError, match="chunk"):
         r.tune(chunk=8)
+    for bad in (0, 9):
+        with pytest.raises(RuntimeError, match="sssp_states"):
+            r.tune(sssp_states=bad)
+    assert r.tuning["sssp_states"] == 1
 
 
 def test_sssp_alone_equals_in_mix(graphs):
@@ -241,6 +245,34 @@ def test_serve_under_hbm_budget(graphs):
     assert np.array_equal(rb.digest, rs.digest)
     assert rb.hbm_peak <= budget
     assert rb.admitted_waits > 0
+
+
+def test_serve_sssp_spread_over_states(graphs):
+    """A graph's SSSP queries spread over three concurrent states, then under a
+    budget that holds two states: results do not change."""
+    names = ["grid", "dir"]
+    gl = [graphs[n][0] for n in names]
+    rl = [graphs[n][1] for n in names]
+    V = [g.V for g in gl]
+    trace = BA.poisson_trace(30, 5000.0, {BA.SSSP: 1.0}, {BA.SSSP: [0, 1]}, V, seed=8)
+    rs, _ = BA.serve(rl, trace, sequential=True)
+    for r in rl:
+        r.tune(sssp_states=3)
+    try:
+        print("serve over 3 states", flush=True)
+        rb, vb = BA.serve(rl, trace, results=True)
+        assert np.array_equal(rb.digest, rs.digest)
+        for i in range(trace.size):
+            gi, p = int(trace["graph"][i]), int(trace["param"][i])
+            assert np.array_equal(vb[i], O.dijkstra(gl[gi], p)), i
+        budget = 2 * max(r.slot_bytes(BA.SSSP) for r in rl) + 1
+        print("serve over 3 states under a budget", flush=True)
+        rb2, _ = BA.serve(rl, trace, hbm_budget=budget)
+        assert np.array_equal(rb2.digest, rs.digest)
+        assert rb2.hbm_peak <= budget
+    finally:
+        for r in rl:
+            r.tune(sssp_states=1)
 
 
 def test_serve_rejects_bad_query(graphs):
