@@ -19,10 +19,12 @@
 #include "ipregel_hip.h"
 
 #include <errno.h>
+#include <fcntl.h>
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <unistd.h>
 #include <time.h>
 
 #define IPR_TRACE_CAP (1u << 20)
@@ -98,10 +100,29 @@ static FILE* safe_fopen(const char* path, const char* mode)
     return f;
 }
 
-static void safe_fread(void* p, size_t size, size_t count, FILE* f)
+/* count elements of size bytes from the start of a file, read in 64 MiB
+ * pieces by all threads (positional reads; the reference likewise reads its
+ * offset file in per-thread chunks, src/iPregel_postamble.h:295-321) */
+static void read_elements(const char* path, void* dst, size_t size, uint64_t count)
 {
-    if (count && fread(p, size, count, f) != count)
-        die("Failed to read %zu elements of %zu bytes each.\n", count, size);
+    int fd = open(path, O_RDONLY);
+    if (fd < 0) die("Cannot open the file \"%s\" in mode \"%s\".\n", path, "rb");
+    const uint64_t bytes = (uint64_t)size * count, piece = 64ull << 20;
+    const int64_t pieces = (int64_t)((bytes + piece - 1) / piece);
+    int bad = 0;
+    #pragma omp parallel for schedule(dynamic, 1) reduction(|:bad)
+    for (int64_t i = 0; i < pieces; i++) {
+        const uint64_t off = (uint64_t)i * piece;
+        const uint64_t len = bytes - off < piece ? bytes - off : piece;
+        uint64_t done = 0;
+        while (done < len) {
+            const ssize_t r = pread(fd, (char*)dst + off + done, len - done, (off_t)(off + done));
+            if (r <= 0) { bad = 1; break; }
+            done += (uint64_t)r;
+        }
+    }
+    close(fd);
+    if (bad) die("Failed to read %zu elements of %zu bytes each.\n", (size_t)count, size);
 }
 
 static char* with_ext(const char* root, const char* ext)
@@ -355,7 +376,7 @@ static void dev_setup(void)
 /* loader                                                                    */
 /* ------------------------------------------------------------------------ */
 
-static void load_from_files(const char* root, int want_weights)
+static void load_from_files(const char* root, int want_weights, uint32_t id_size)
 {
     char* p = with_ext(root, ".config");
     printf("\t- Loading configuration file from: \"%s\".\n", p);
@@ -376,25 +397,34 @@ static void load_from_files(const char* root, int want_weights)
     D.h_adj = safe_malloc((E ? E : 1) * sizeof(uint32_t));
     p = with_ext(root, ".idx");
     printf("\t- Loading offset file from: \"%s\".\n", p);
-    f = safe_fopen(p, "rb");
+    read_elements(p, D.h_idx, sizeof(uint64_t), V);
     free(p);
-    safe_fread(D.h_idx, sizeof(uint64_t), V, f);
-    fclose(f);
     D.h_idx[V] = E;
+    /* .adj holds IP_VERTEX_ID_TYPE ids: 4 bytes in _32 builds, 8 in _64 ones;
+     * the device keeps 32-bit ids (validate_graph bounds them) */
     p = with_ext(root, ".adj");
     printf("\t- Loading adjacency file from: \"%s\".\n", p);
-    f = safe_fopen(p, "rb");
+    if (id_size == 8) {
+        uint64_t* wide = safe_malloc((E ? E : 1) * sizeof(uint64_t));
+        read_elements(p, wide, sizeof(uint64_t), E);
+        int big = 0;
+        #pragma omp parallel for schedule(static) reduction(|:big)
+        for (uint64_t e = 0; e < E; e++) {
+            big |= wide[e] > 0xffffffffull;
+            D.h_adj[e] = (uint32_t)wide[e];
+        }
+        free(wide);
+        if (big) die("Invalid graph: an arc points outside [0, V).\n");
+    } else {
+        read_elements(p, D.h_adj, sizeof(uint32_t), E);
+    }
     free(p);
-    safe_fread(D.h_adj, sizeof(uint32_t), E, f);
-    fclose(f);
     if (want_weights) {
         p = with_ext(root, ".wgt");
         printf("\t- Loading weight file from: \"%s\".\n", p);
-        f = safe_fopen(p, "rb");
-        free(p);
         D.h_wgt = safe_malloc((E ? E : 1) * sizeof(uint32_t));
-        safe_fread(D.h_wgt, sizeof(uint32_t), E, f);
-        fclose(f);
+        read_elements(p, D.h_wgt, sizeof(uint32_t), E);
+        free(p);
     }
     D.own_host = 1;
 }
@@ -486,7 +516,7 @@ void ipr_drv_init(const ipr_app_desc* app, const char* file_path, int number_of_
         if (want_w && !D.h_wgt) die("The application needs edge weights but none were given.\n");
         printf("\t\t- %llu vertices\n\t\t- %llu edges\n", (unsigned long long)D.V, (unsigned long long)D.E);
     } else {
-        load_from_files(file_path, want_w);
+        load_from_files(file_path, want_w, D.app.id_size);
     }
     validate_graph();
 

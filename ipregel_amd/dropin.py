@@ -8,11 +8,14 @@ made under /tmp (the copy is never written into this tree):
     assigns (pagerank.c: ratio, initial_value; sssp.c: start_vertex), and one
     `IP_PUBLISH(name);` per such global just before `ip_run();`.
 Everything else (types, hooks, main, printf formats) is compiled as written.
-Outputs: ipregel_amd/dropin/bin/<app><variant>_32 (git-ignored, shipped with
-the tree like the engine's own binaries).
+Outputs: ipregel_amd/dropin/bin/<app><variant>_32 and _64 (IP_VERTEX_ID_TYPE
+uint32_t / uint64_t, as the reference makefile builds them; git-ignored,
+shipped with the tree like the engine's own binaries).
 """
 from __future__ import annotations
 
+import concurrent.futures as cf
+import os
 import re
 import shutil
 import subprocess
@@ -37,6 +40,10 @@ APPS = {
                                 ("_spread", ["-DIP_USE_SPREAD", "-DIP_COMBINER_MIN"]),
                                 ("_single_broadcast", ["-DIP_USE_SINGLE_BROADCAST", "-DIP_COMBINER_MIN"])]),
 }
+
+
+# the reference makefile's id widths (makefile:16-17): <name>_32 and <name>_64
+WIDTHS = [(32, "uint32_t"), (64, "uint64_t")]
 
 
 def qualify(text: str, device_globals: list[str]) -> str:
@@ -65,24 +72,31 @@ def build(ref: Path = REF_DEFAULT) -> list[Path]:
     drv = B.OBJ / "ip_driver.o"
     if not drv.exists():
         B.build_driver()
-    built = []
     with tempfile.TemporaryDirectory(prefix="ipr_dropin_", dir="/tmp") as td:
+        jobs = []
         for app, (globs, variants) in APPS.items():
             qsrc = Path(td) / f"{app}.c"
             qsrc.write_text(qualify((src_dir / f"{app}.c").read_text(), globs))
             for suffix, defines in variants:
-                name = f"{app}{suffix}_32"
-                obj = Path(td) / f"{name}.o"
-                B._run([B.HIPCC, *B.HIP_FLAGS, "-x", "hip", "-ffp-contract=off", "-w",
-                        f"-I{B.INC}", "-DIP_VERTEX_ID_TYPE=uint32_t",
-                        f'-DIP_APPLICATION="{app}{suffix}"', '-DIP_MACHINE="MI355X"',
-                        '-DVERSION="1.0.0"', '-DCOMMITS="none"', *defines, "-c", str(qsrc),
-                        "-o", str(obj)])
-                exe = OUT / name
-                B._run([B.HIPCC, f"-Wl,-rpath,{B.LIB}", "-Wl,-rpath,$ORIGIN/../../lib", str(obj), str(drv),
-                        f"-L{B.LIB}", "-lipregel_hip", "-fopenmp", "-o", str(exe)])
-                built.append(exe)
-    return built
+                for width, id_type in WIDTHS:
+                    jobs.append((app, suffix, defines, width, id_type, qsrc))
+
+        def one(job) -> Path:
+            app, suffix, defines, width, id_type, qsrc = job
+            name = f"{app}{suffix}_{width}"
+            obj = Path(td) / f"{name}.o"
+            B._run([B.HIPCC, *B.HIP_FLAGS, "-x", "hip", "-ffp-contract=off", "-w",
+                    f"-I{B.INC}", f"-DIP_VERTEX_ID_TYPE={id_type}",
+                    f'-DIP_APPLICATION="{app}{suffix}"', '-DIP_MACHINE="MI355X"',
+                    '-DVERSION="1.0.0"', '-DCOMMITS="none"', *defines, "-c", str(qsrc),
+                    "-o", str(obj)])
+            exe = OUT / name
+            B._run([B.HIPCC, f"-Wl,-rpath,{B.LIB}", "-Wl,-rpath,$ORIGIN/../../lib", str(obj), str(drv),
+                    f"-L{B.LIB}", "-lipregel_hip", "-fopenmp", "-o", str(exe)])
+            return exe
+
+        with cf.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 4)) as ex:
+            return list(ex.map(one, jobs))
 
 
 if __name__ == "__main__":

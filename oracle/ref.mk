@@ -14,6 +14,8 @@ BINS := $(OUT)/bin/cc_32 $(OUT)/bin/cc_spread_32 $(OUT)/bin/cc_single_broadcast_
         $(OUT)/bin/pagerank_32 $(OUT)/bin/pagerank_single_broadcast_32 \
         $(OUT)/bin/sssp_32 $(OUT)/bin/sssp_spread_32 $(OUT)/bin/sssp_single_broadcast_32 \
         $(OUT)/bin/sssp_single_broadcast_spread_32
+# the _64 builds (IP_VERTEX_ID_TYPE=uint64_t, makefile:17)
+BINS += $(patsubst %_32,%_64,$(BINS))
 
 all: $(BINS)
 
@@ -27,4 +29,9 @@ endef
 $(OUT)/bin/%_32: | $(OUT)/bin
 	gcc $(CFLAGS) $(DEFS) -DIP_APPLICATION=\"$*\" $(call variant_flags,$*) \
 	    -DIP_VERTEX_ID_TYPE=uint32_t -I$(REF)/src \
+	    $(REF)/benchmarks/$(firstword $(subst _, ,$*)).c -o $@
+
+$(OUT)/bin/%_64: | $(OUT)/bin
+	gcc $(CFLAGS) $(DEFS) -DIP_APPLICATION=\"$*\" $(call variant_flags,$*) \
+	    -DIP_VERTEX_ID_TYPE=uint64_t -I$(REF)/src \
 	    $(REF)/benchmarks/$(firstword $(subst _, ,$*)).c -o $@

@@ -18,12 +18,15 @@ def golden_cases():
     return sorted(p.stem for p in GOLDEN.glob("*.config"))
 
 
-def golden_runs():
-    """(case, reference binary) pairs that have a golden output."""
+def golden_runs(wide: bool = False):
+    """(case, reference binary) pairs that have a golden output: the _32
+    builds on the cases, or (wide) the _64 builds on the <case>_w64 copies
+    written with 64-bit ids."""
     out = []
     for p in sorted(GOLDEN.glob("*__*.out")):
         case, binname = p.stem.split("__")
-        out.append((case, binname))
+        if case.endswith("_w64") == wide:
+            out.append((case, binname))
     return out
 
 

@@ -5,6 +5,8 @@ oracle/ref.mk) on small graphs.
 For every case: the graph triple (<case>.config/.idx/.adj), and for every
 reference binary that applies, its ip_dump output (<case>__<bin>.out) and its
 per-superstep ActiveVertexCount trace (<case>__<bin>.trace, one number a line).
+Four cases are also written with 64-bit ids (<case>_w64.*) and run by the
+reference's _64 builds (<case>_w64__<bin>_64.out / .trace).
 SSSP runs from vertex 0: the reference cannot run from another source
 (benchmarks/sssp.c:101 reads the schedule string).
 
@@ -62,6 +64,17 @@ def run_ref(binname: str, root: Path, out: Path) -> list[int]:
     return [int(line.split(":")[1]) for line in p.stdout.splitlines() if "ActiveVertexCount" in line]
 
 
+# cases also written with 64-bit ids (<case>_w64.*: the .adj holds uint64
+# entries, as the _64 builds read it) and run by the _64 builds
+WIDE_CASES = ["chain10", "star65", "twocomp10", "rmat10"]
+
+
+def write_wide(g: G.Graph, root: Path) -> None:
+    g.write(root)
+    adj = np.fromfile(f"{root}.adj", dtype=np.uint32)
+    adj.astype(np.uint64).tofile(f"{root}.adj")
+
+
 def main() -> None:
     if not REF_BIN.exists():
         raise SystemExit("oracle/_ref/bin is missing: build the reference first (oracle/build.py)")
@@ -73,6 +86,15 @@ def main() -> None:
                 out = HERE / f"{name}__{b}.out"
                 trace = run_ref(b, root, out)
                 (HERE / f"{name}__{b}.trace").write_text("".join(f"{t}\n" for t in trace))
+        if name in WIDE_CASES:
+            wroot = HERE / f"{name}_w64"
+            write_wide(g, wroot)
+            for app, bins in BINS.items():
+                for b in bins:
+                    b64 = b.replace("_32", "_64")
+                    out = HERE / f"{name}_w64__{b64}.out"
+                    trace = run_ref(b64, wroot, out)
+                    (HERE / f"{name}_w64__{b64}.trace").write_text("".join(f"{t}\n" for t in trace))
         print(name, g.V, g.E)
 
 

@@ -29,6 +29,21 @@ def test_oracle_matches_reference_binary(case, binname, tmp_path):
     assert list(r.active[: r.supersteps]) == trace
 
 
+@pytest.mark.parametrize("case,binname", golden_runs(wide=True))
+def test_reference_64bit_build_matches_32bit(case, binname):
+    """The reference's _64 builds on the graph written with 64-bit ids print
+    what its _32 builds print on the 32-bit copy (so the _64 drop-in runs are
+    pinned by the same oracle)."""
+    narrow = case[: -len("_w64")]
+    wide = np.fromfile(GOLDEN / f"{case}.adj", dtype=np.uint64)
+    assert np.array_equal(wide, np.fromfile(GOLDEN / f"{narrow}.adj", dtype=np.uint32).astype(np.uint64))
+    for ext in (".idx", ".config"):
+        assert (GOLDEN / f"{case}{ext}").read_bytes() == (GOLDEN / f"{narrow}{ext}").read_bytes()
+    b32 = binname[: -len("_64")] + "_32"
+    for ext in (".out", ".trace"):
+        assert (GOLDEN / f"{case}__{binname}{ext}").read_text() == (GOLDEN / f"{narrow}__{b32}{ext}").read_text()
+
+
 @pytest.mark.parametrize("seed", [3, 4])
 @pytest.mark.parametrize("variant", [O.PUSH, O.PULL, O.SPREAD, O.SPREAD_PULL])
 def test_cc_is_component_minimum(seed, variant):
