@@ -116,6 +116,12 @@ typedef struct ipr_dev {
      * bslot[bpos[v]], records ordered by descending out-degree so that the
      * most-gathered ones share cache lines (NULL: bslot[v]) */
     const uint32_t* bpos;
+    /* dense supersteps whose broadcasters' out-arcs are below live_pct % of
+     * E: bit p of g_live is set when the record at position p was broadcast
+     * this superstep, and the gather reads only those records */
+    uint64_t* g_live;
+    uint32_t live_pct;
+    uint32_t live_pad;
     uint64_t g_per_wave;
     uint32_t gather_blocks;
     uint32_t g_pad;
@@ -155,6 +161,7 @@ enum {
     IPR_K_PULL_STEP,     /* PULL: fetch(s-1) fused with compute(s) */
     IPR_K_FETCH,         /* SPREAD_PULL: fetch for targets */
     IPR_K_GATHER,        /* push apps, dense superstep: receivers gather */
+    IPR_K_GATHER_LIVE,   /* push apps, dense superstep with few broadcasters: their bitmap */
     IPR_K_COUNT
 };
 

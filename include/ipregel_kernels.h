@@ -278,12 +278,25 @@ struct ipr_bslot_t {
  * exceed E. Every wavefront of the delivery kernels derives the same answer
  * from the 64 counter slots; the finalize kernel counts gathered supersteps
  * by the same rule. */
-__device__ inline bool ipr_gathers(const ipr_dev& d)
+__device__ inline uint64_t ipr_bcast_arcs(const ipr_dev& d)
 {
-    if (!d.dir_opt) return false;
     uint64_t x = d.cnt[IPR_C_EDGES * IPR_CSLOTS + ipr_lane()];
     for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
-    return x * (uint64_t)d.dir_alpha > d.E;
+    return x;
+}
+
+__device__ inline bool ipr_gathers(const ipr_dev& d)
+{
+    return d.dir_opt && ipr_bcast_arcs(d) * (uint64_t)d.dir_alpha > d.E;
+}
+
+/* a gathered superstep with few live arcs reads the broadcasters' bitmap
+ * before any record (wavefront-uniform) */
+__device__ inline bool ipr_gathers_live(const ipr_dev& d)
+{
+    if (!d.dir_opt || !d.g_live) return false;
+    const uint64_t w = ipr_bcast_arcs(d);
+    return w * (uint64_t)d.dir_alpha > d.E && w * 100 < d.E * (uint64_t)d.live_pct;
 }
 
 /* position of vertex v's broadcast record */
@@ -671,6 +684,27 @@ __device__ inline void ipr_gather_flush(const ipr_dev& d, uint64_t s, uint32_t v
     d.recv[(s + 1) & 1][v] = 1;
 }
 
+/* The bitmap of this superstep's broadcasters over record positions: one
+ * wavefront per 64 positions, a ballot of their records' stamps, one word
+ * written; no atomics, nothing to clear. Runs only when the gather will read
+ * it. */
+__global__ void __launch_bounds__(IPR_BLOCK) ipr_k_gather_live(ipr_dev d)
+{
+    ipr_ctl* ctl = d.ctl;
+    if (ctl->done) return;
+    if (!ipr_gathers_live(d)) return;
+    const uint32_t want = (uint32_t)(ctl->superstep + 1);
+    const ipr_bslot_t* bs = (const ipr_bslot_t*)d.bslot;
+    const uint64_t words = (d.V + 63) / 64;
+    const uint64_t nw = (uint64_t)gridDim.x * IPR_WAVES;
+    for (uint64_t wd = (uint64_t)blockIdx.x * IPR_WAVES + threadIdx.x / IPR_WAVE; wd < words; wd += nw) {
+        const uint64_t p = wd * 64 + ipr_lane();
+        const bool on = p < d.V && (bs[p].step & ~IPR_BSTEP_WEIGHTED) == want;
+        const uint64_t m = __ballot(on);
+        if (ipr_lane() == 0) d.g_live[wd] = m;
+    }
+}
+
 /* Dense supersteps of push applications: every receiver folds the records of
  * the in-neighbours that broadcast in superstep s, over the exact transpose.
  * Arc-balanced, so hubs need no special case: wavefront w owns the arcs
@@ -689,6 +723,7 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_gather(ipr_dev d)
     ipr_ctl* ctl = d.ctl;
     if (ctl->done) return;
     if (!ipr_gathers(d)) return;
+    const bool live = ipr_gathers_live(d);
     const uint64_t s = ctl->superstep;
     const uint32_t want = (uint32_t)(s + 1);
     const uint32_t lane = ipr_lane();
@@ -717,8 +752,21 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_gather(ipr_dev d)
         }
         const uint32_t o = t0 < w1 ? d.g_own[t0 / K] : 0u;   /* receiver of arc t0 */
         ipr_bslot_t r[K];
+        if (live) {
+            /* the bitmap (V bits, cache-resident) first; records only of
+             * sources that broadcast */
+            uint64_t w[K];
 #pragma unroll
-        for (int j = 0; j < K; j++) r[j] = bs[a[j] & ~IPR_GATHER_LAST];
+            for (int j = 0; j < K; j++) w[j] = d.g_live[(a[j] & ~IPR_GATHER_LAST) >> 6];
+#pragma unroll
+            for (int j = 0; j < K; j++) {
+                r[j].step = 0;
+                if ((w[j] >> (a[j] & 63)) & 1ull) r[j] = bs[a[j] & ~IPR_GATHER_LAST];
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < K; j++) r[j] = bs[a[j] & ~IPR_GATHER_LAST];
+        }
 #ifdef IP_NEEDS_OUT_NEIGHBOUR_WEIGHTS
         uint32_t wt[K];
         if (t0 + K <= w1) {
@@ -1167,6 +1215,13 @@ static int ipr_app_launch(int kernel, const ipr_dev* dp, void* stream)
     case IPR_K_GATHER:
         hipLaunchKernelGGL(ipr_k_gather, dim3(d.gather_blocks ? d.gather_blocks : 1), dim3(IPR_BLOCK), 0, st, d);
         break;
+    case IPR_K_GATHER_LIVE: {
+        /* a small grid: launched every superstep, it usually exits at once */
+        uint64_t wb = ((d.V + 63) / 64 + IPR_WAVES - 1) / IPR_WAVES;
+        hipLaunchKernelGGL(ipr_k_gather_live, dim3((uint32_t)(wb < 1 ? 1 : (wb > 512 ? 512 : wb))),
+                           dim3(IPR_BLOCK), 0, st, d);
+        break;
+    }
     case IPR_K_PULL_STEP: {
         uint32_t nb[3];
         ipr_bin_blocks(d, nb);

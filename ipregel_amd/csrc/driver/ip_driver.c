@@ -158,6 +158,7 @@ static void dev_free_all(void)
     ipr_hip_free((void*)d->g_own);
     ipr_hip_free((void*)d->g_dst);
     ipr_hip_free((void*)d->bpos);
+    ipr_hip_free(d->g_live);
     ipr_hip_free(d->bslot);
     ipr_hip_free(d->lbase);
     if (d->in_idx && d->in_idx != d->out_idx) ipr_hip_free((void*)d->in_idx);
@@ -341,6 +342,9 @@ static void dev_setup(void)
         d->g_arc = g_arc;
         d->g_own = g_own;
         d->g_dst = g_dst;
+        /* broadcasters' bitmap for gathers with few live arcs (0 turns it off) */
+        d->live_pct = env_u32("IPR_GATHER_LIVE", 60);
+        if (d->live_pct > 0) d->g_live = dalloc(((vv + 63) / 64) * 8, "gather live bitmap");
         /* wavefronts (4 per workgroup) own equal arc ranges, whole passes */
         d->gather_blocks = env_u32("IPR_GATHER_BLOCKS", 8192);
         if (d->gather_blocks < 1) d->gather_blocks = 1;
@@ -601,6 +605,7 @@ static void enqueue_superstep(void)
     case IPR_VARIANT_SPREAD:
     case IPR_VARIANT_PUSH:
         launch_app(IPR_K_COMPUTE);
+        if (D.dev.dir_opt && D.dev.g_live) launch_app(IPR_K_GATHER_LIVE);
         kev_mark(0);
         if (D.dev.dir_opt) launch_app(IPR_K_GATHER);
         kev_mark(1);

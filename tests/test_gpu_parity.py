@@ -108,14 +108,17 @@ def test_cc_tuning_does_not_change_results(tune):
     assert np.array_equal(r.values, ref)
 
 
-@pytest.mark.parametrize("blocks,order", [("1", "1"), ("3", "0"), ("8192", "1"), ("8192", "0")])
-def test_gather_every_superstep(blocks, order, monkeypatch):
+@pytest.mark.parametrize("blocks,order,live", [("1", "1", "60"), ("3", "0", "0"), ("8192", "1", "100"),
+                                               ("8192", "0", "60")])
+def test_gather_every_superstep(blocks, order, live, monkeypatch):
     """Dense-superstep gather forced on every superstep (dir_alpha huge), with
     1, 3 and 8192 workgroups: in-lists split across lanes, passes and
     wavefronts, hubs, isolated vertices; symmetric and directed transposes,
-    with and without weights; broadcast records by vertex id or by degree."""
+    with and without weights; broadcast records by vertex id or by degree;
+    the broadcasters' bitmap off, below 60% live arcs, or always."""
     monkeypatch.setenv("IPR_GATHER_BLOCKS", blocks)
     monkeypatch.setenv("IPR_RECORD_ORDER", order)
+    monkeypatch.setenv("IPR_GATHER_LIVE", live)
     g = G.rmat(13, 16, seed=5, symmetrise=True)
     r = EN.run_app("cc_spread", g, dir_alpha=1 << 30)
     assert np.array_equal(r.values, O.min_ancestor(g))
