@@ -330,6 +330,9 @@ void ip_init(const char* file_path, int number_of_threads, const char* schedule,
     d.bind = ipr_app_bind;
     d.serialise = ipr_app_serialise;
     d.check_combiner = ipr_app_check_combiner;
+#ifdef IP_ENABLE_THREAD_PROFILING
+    d.thread_profiling = 1;
+#endif
     ipr_drv_init(&d, file_path, number_of_threads, schedule, chunk_size, directed, weighted);
     ip_vertices_count = (size_t)ipr_drv_vertices_count();
     ip_edges_count = (size_t)ipr_drv_edges_count();

@@ -186,6 +186,8 @@ typedef struct ipr_app_desc {
                       uint64_t first, uint64_t count);
     /* host-side self-check of the declared combiner (0 = consistent) */
     int (*check_combiner)(void);
+    /* built with IP_ENABLE_THREAD_PROFILING: a per-superstep phase table */
+    int thread_profiling;
 } ipr_app_desc;
 
 #ifdef __cplusplus

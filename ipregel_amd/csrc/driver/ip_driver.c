@@ -58,6 +58,11 @@ static struct {
     uint32_t kev_slot;       /* slot of the superstep being enqueued */
     double* kev_ms[2];       /* per superstep of the last timed run: before / after the split */
     uint64_t kev_n;          /* supersteps in kev_ms */
+    /* phase profile (IP_ENABLE_THREAD_PROFILING or IPR_PHASE_PROFILE=1):
+     * events before compute, after compute, after delivery, after merging */
+    int prof_on;
+    void* pev[4][64];
+    double* prof_ms[3];      /* per superstep: compute, delivery (mailbox), merging */
 } D;
 
 /* ------------------------------------------------------------------------ */
@@ -200,6 +205,17 @@ static void dev_free_all(void)
     }
     D.kev_n = 0;
     D.kev_on = 0;
+    for (int i = 0; i < 64; i++) {
+        for (int k = 0; k < 4; k++) {
+            if (D.pev[k][i]) ipr_hip_event_destroy(D.pev[k][i]);
+            D.pev[k][i] = NULL;
+        }
+    }
+    for (int k = 0; k < 3; k++) {
+        free(D.prof_ms[k]);
+        D.prof_ms[k] = NULL;
+    }
+    D.prof_on = 0;
     if (D.ev0) ipr_hip_event_destroy(D.ev0);
     if (D.ev1) ipr_hip_event_destroy(D.ev1);
     D.ev0 = D.ev1 = NULL;
@@ -372,6 +388,15 @@ static void dev_setup(void)
     if (D.max_chunk > 64) D.max_chunk = 64;
     D.kev_on = 0;
     if (c && c->kernel_events) ipr_harness_kernel_events(1);
+    D.prof_on = a->thread_profiling || env_u32("IPR_PHASE_PROFILE", 0) == 1;
+    if (D.prof_on) {
+        for (int i = 0; i < 64; i++)
+            for (int k = 0; k < 4; k++) hip_check(ipr_hip_event_create(&D.pev[k][i]), "profile event");
+        for (int k = 0; k < 3; k++) {
+            D.prof_ms[k] = (double*)safe_malloc(IPR_TRACE_CAP * sizeof(double));
+            memset(D.prof_ms[k], 0, IPR_TRACE_CAP * sizeof(double));
+        }
+    }
     int rc = a->bind(d);
     if (rc) die("HIP failure in binding device state (%d)\n", rc);
 }
@@ -576,18 +601,29 @@ static void kev_mark(int which)
 {
     if (!D.kev_on || D.kev_slot >= 64) return;
     hip_check(ipr_hip_event_record(D.kev[which][D.kev_slot], D.stream), "kernel event");
-    if (which == 2) D.kev_slot++;
+}
+
+/* phase profile marks: 0 before compute, 1 after compute, 2 after the
+ * delivery (the reference's "Mailbox" phase), 3 after merging the frontier */
+static void pmark(int which)
+{
+    if (!D.prof_on || D.kev_slot >= 64) return;
+    hip_check(ipr_hip_event_record(D.pev[which][D.kev_slot], D.stream), "profile event");
 }
 
 /* one superstep of the variant's BSP loop, in phase order */
 static void enqueue_superstep(void)
 {
+    pmark(0);
     switch (D.app.variant) {
     case IPR_VARIANT_PULL:
+        /* the fused kernel is fetch(s-1) + compute(s): counted as compute */
         kev_mark(0);
         launch_app(IPR_K_PULL_STEP);
         kev_mark(1);
         kev_mark(2);
+        pmark(1);
+        pmark(2);
         break;
     case IPR_VARIANT_SPREAD_PULL:
         /* src/combiner_spread_single_broadcast_postamble.h ip_run: compute on
@@ -595,22 +631,26 @@ static void enqueue_superstep(void)
          * fetch for the targets; broadcast flags are epoch-stamped, so the
          * reset phase has nothing to do */
         launch_app(IPR_K_COMPUTE);
+        pmark(1);
         kev_mark(0);
         kev_mark(1);
         launch_app(IPR_K_EXPAND);
         kev_mark(2);
         hip_check(ipr_hip_compact_frontier(&D.dev, D.stream), "compact");
         launch_app(IPR_K_FETCH);
+        pmark(2);
         break;
     case IPR_VARIANT_SPREAD:
     case IPR_VARIANT_PUSH:
         launch_app(IPR_K_COMPUTE);
+        pmark(1);
         if (D.dev.dir_opt && D.dev.g_live) launch_app(IPR_K_GATHER_LIVE);
         kev_mark(0);
         if (D.dev.dir_opt) launch_app(IPR_K_GATHER);
         kev_mark(1);
         launch_app(IPR_K_EXPAND);
         kev_mark(2);
+        pmark(2);
         if (D.app.variant == IPR_VARIANT_SPREAD)
             hip_check(ipr_hip_compact_frontier(&D.dev, D.stream), "compact");
         else
@@ -619,7 +659,10 @@ static void enqueue_superstep(void)
     default:
         die("Engine variant %d is not available in this build.\n", D.app.variant);
     }
+    pmark(3);
     hip_check(ipr_hip_finalize(&D.dev, D.app.variant, D.stream), "finalize");
+    /* the next superstep of the chunk records into the next event slot */
+    if ((D.kev_on || D.prof_on) && D.kev_slot < 64) D.kev_slot++;
 }
 
 static void enqueue_chunk(uint32_t n)
@@ -627,7 +670,7 @@ static void enqueue_chunk(uint32_t n)
     int slot = 0;
     while ((1u << slot) < n) slot++;
     D.kev_slot = 0;
-    if (D.use_graph && !D.kev_on && slot < IPR_MAX_GRAPHS) {
+    if (D.use_graph && !D.kev_on && !D.prof_on && slot < IPR_MAX_GRAPHS) {
         if (!D.graphs[slot]) {
             hip_check(ipr_hip_graph_begin(D.stream), "graph capture");
             for (uint32_t i = 0; i < n; i++) enqueue_superstep();
@@ -678,6 +721,19 @@ static void run_once(void)
                     D.kev_ms[0][sidx] = k1;
                     D.kev_ms[1][sidx] = k2;
                     if (sidx + 1 > D.kev_n) D.kev_n = sidx + 1;
+                }
+            }
+        }
+        if (D.prof_on) {
+            uint64_t live = D.h_ctl->executed - prev_exec;
+            if (D.app.variant == IPR_VARIANT_PULL && prev_exec == 0 && live < chunk) live++;
+            for (uint32_t i = 0; i < D.kev_slot && i < live; i++) {
+                const uint64_t sidx = prev_exec + i;
+                if (sidx >= IPR_TRACE_CAP) break;
+                for (int k = 0; k < 3; k++) {
+                    float ms = 0;
+                    hip_check(ipr_hip_event_elapsed_ms(D.pev[k][i], D.pev[k + 1][i], &ms), "profile event");
+                    D.prof_ms[k][sidx] = ms;
                 }
             }
         }
@@ -750,6 +806,23 @@ int ipr_harness_kernel_times(double* first, double* second, uint64_t cap)
     return (int)n;
 }
 
+/* the reference's IP_ENABLE_THREAD_PROFILING table
+ * (src/combiner_spread_postamble.h:352-399), with the device as its one
+ * column: phase times of the superstep in seconds, and its share of the
+ * superstep's relaxed arcs (the device's: 100%) with their count */
+static void print_phase_table(uint64_t s, uint64_t arcs)
+{
+    const double c = D.prof_ms[0][s] * 1e-3, mb = D.prof_ms[1][s] * 1e-3, mg = D.prof_ms[2][s] * 1e-3;
+    printf("            +-----------+\n            |    Device |\n+-----------+-----------+\n");
+    printf("|   Compute | %8.6fs |\n", c);
+    printf("|   Merging | %8.6fs |\n", mg);
+    printf("|   Mailbox | %8.6fs |\n", mb);
+    printf("|     Total | %8.6fs |\n", c + mg + mb);
+    printf("| EdgeCount | %8.3f%% |\n", arcs ? 100.0 : 0.0);
+    printf("|      Arcs | %9llu |\n", (unsigned long long)arcs);
+    printf("+-----------+-----------+\n");
+}
+
 static int run_and_report(int reps)
 {
     if (D.V == 0) {
@@ -766,12 +839,20 @@ static int run_and_report(int reps)
     hip_check(ipr_hip_memcpy_d2h(ts, D.dev.trace_ts, (m + 1) * 8, D.stream), "trace");
     hip_check(ipr_hip_stream_sync(D.stream), "trace");
     double tick = D.wall_hz ? 1.0 / (double)D.wall_hz : 1e-8;
+    uint64_t* sedges = NULL;
+    if (!D.quiet && D.prof_on) {
+        sedges = safe_malloc((m + 1) * 8);
+        hip_check(ipr_hip_memcpy_d2h(sedges, D.dev.trace_edges, m * 8, D.stream), "trace");
+        hip_check(ipr_hip_stream_sync(D.stream), "trace");
+    }
     if (!D.quiet) {
         for (uint64_t s = 0; s < m; s++) {
             printf("Superstep%zuDuration:%f\n", (size_t)s, (double)(ts[s + 1] - ts[s]) * tick);
             printf("Superstep%zuActiveVertexCount:%zu\n", (size_t)s, (size_t)act[s]);
+            if (sedges) print_phase_table(s, sedges[s]);
         }
     }
+    free(sedges);
     D.stats.device_ms = m ? (double)(ts[m] - ts[0]) * tick * 1e3 : 0.0;
     double t = D.stats.last_ms * 1e-3;
     printf("Total time of supersteps: %fs.\n", t);

@@ -3,6 +3,8 @@ the reference binaries' golden outputs and against the oracle.
 Bit-exact for CC/SSSP (labels, distances, ip_dump text, active-count trace);
 FP64 PageRank bit-exact in ordered mode and within 1e-12 relative per vertex
 in the reassociating mode."""
+from pathlib import Path
+
 import numpy as np
 import pytest
 
@@ -10,6 +12,8 @@ from conftest import GOLDEN, golden_runs
 from ipregel_amd import engine as EN
 from ipregel_amd import graph as G
 from oracle import oracle as O
+
+ROOT = Path(__file__).resolve().parents[1]
 
 pytestmark = pytest.mark.gpu
 
@@ -145,6 +149,34 @@ def test_tune_engine_keeps_results():
         + len(AT.ENGINE_GRID["dir_alpha"])
     r = EN.run_app("cc_spread", g, **rep["choice"])
     assert np.array_equal(r.values, O.min_ancestor(g))
+
+
+@pytest.mark.parametrize("binname", ["cc_spread_32", "sssp_32", "pagerank_single_broadcast_32"])
+def test_phase_profile_table(binname, tmp_path):
+    """IPR_PHASE_PROFILE=1 (as IP_ENABLE_THREAD_PROFILING at build time) adds
+    the per-superstep phase table to the log; the dump and the active counts
+    do not change, and the tables' arcs add up to WorkEdges."""
+    import os
+    import re
+    import subprocess
+    case = "rmat10"
+    exe = ROOT / "ipregel_amd" / "bin" / binname
+    out = tmp_path / "out.txt"
+    args = [str(exe), str(GOLDEN / case), str(out), "1", "static", "0"] + (["0"] if binname.startswith("sssp") else [])
+    p = subprocess.run(args, capture_output=True, text=True, timeout=120,
+                       env=dict(os.environ, IPR_PHASE_PROFILE="1"))
+    assert p.returncode == 0, p.stdout[-2000:]
+    assert out.read_text() == (GOLDEN / f"{case}__{binname}.out").read_text()
+    trace = [int(x.split(":")[1]) for x in p.stdout.splitlines() if "ActiveVertexCount" in x]
+    assert trace == [int(x) for x in (GOLDEN / f"{case}__{binname}.trace").read_text().split()]
+    rows = {k: [float(v) for v in re.findall(rf"\|\s+{k} \|\s+([0-9.]+)", p.stdout)]
+            for k in ("Compute", "Merging", "Mailbox", "Total", "Arcs")}
+    assert len(rows["Total"]) == len(trace) > 0
+    for c, mg, mb, t in zip(rows["Compute"], rows["Merging"], rows["Mailbox"], rows["Total"]):
+        assert min(c, mg, mb) >= 0 and abs(c + mg + mb - t) < 3e-6
+    assert all(t > 0 for t in rows["Total"]) and sum(rows["Compute"]) > 0
+    work = int(re.search(r"WorkEdges:(\d+)", p.stdout).group(1))
+    assert sum(rows["Arcs"]) == work
 
 
 def test_repeat_runs_identical():
