@@ -599,23 +599,23 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_expand(ipr_dev d)
 /* push variants: gather (dense supersteps)                                  */
 /* ------------------------------------------------------------------------ */
 
-/* Lane-cooperative fold for vertices of small in-degree: lane L holds vertex
- * v (valid), the in-lists [idx[v], idx[v+1]) of the wavefront's 64 vertices
- * are laid end to end and taken 64 arcs at a time; load(k, &m) gives the
- * message at arc position k (false: none); a segmented scan across the lanes
- * folds each vertex's arcs and the segment's last lane hands the result to
- * the vertex's lane. Coalesced index reads, 64 gathers in flight per step.
+/* Lane-cooperative fold for vertices of small in-degree (reassociating
+ * pull): lane L holds vertex v (valid); the in-lists of the wavefront's 64
+ * vertices are laid end to end and taken 64*U arcs at a time. For the U
+ * chunks in flight the source ids are loaded first, then their broadcast
+ * flags, then the messages of the sources that broadcast, so no load waits on
+ * another of its kind. A segmented scan across the lanes folds each vertex's
+ * arcs and the segment's last lane hands the result to the vertex's lane.
  * Must be reached by all lanes of the wavefront. */
-template <typename LoadF>
-__device__ inline bool ipr_wave_segmented_fold(const uint64_t* idx, uint32_t v, bool valid, LoadF load,
-                                               IP_MESSAGE_TYPE* out)
+__device__ inline bool ipr_wave_pull_fold(const ipr_dev& d, uint32_t v, bool valid, const uint8_t* bf,
+                                          const IP_MESSAGE_TYPE* bm, IP_MESSAGE_TYPE* out)
 {
     const uint32_t lane = ipr_lane();
     uint64_t beg = 0;
     uint32_t deg = 0;
     if (valid) {
-        beg = idx[v];
-        deg = (uint32_t)(idx[v + 1] - beg);
+        beg = d.in_idx[v];
+        deg = (uint32_t)(d.in_idx[v + 1] - beg);
     }
     const uint32_t incl = ipr_wave_incl_scan(deg);
     const uint32_t excl = incl - deg;
@@ -624,27 +624,35 @@ __device__ inline bool ipr_wave_segmented_fold(const uint64_t* idx, uint32_t v, 
     bool has = false;
     constexpr int U = IPR_GATHER_UNROLL;       /* chunks of 64 arcs in flight */
     for (uint32_t t00 = 0; t00 < total; t00 += U * IPR_WAVE) {
+        uint64_t pos[U];
+        uint32_t owner[U], u[U];
+        uint8_t f[U];
         IP_MESSAGE_TYPE m[U];
         int h[U];
-        uint32_t owner[U];
 #pragma unroll
         for (int c = 0; c < U; c++) {
             const uint32_t t = t00 + c * IPR_WAVE + lane;
             uint32_t lo = 0;
+#pragma unroll
             for (uint32_t step = 32; step > 0; step >>= 1) {
                 uint32_t probe = __shfl(excl, (lo + step) & (IPR_WAVE - 1));
                 if (lo + step < IPR_WAVE && probe <= t) lo += step;
             }
             const uint64_t o_beg = __shfl(beg, lo);
             const uint32_t o_ex = __shfl(excl, lo);
+            /* lanes past the end read arc 0 (total > 0, so it exists) */
+            owner[c] = t < total ? lo : IPR_WAVE + lane;   /* matches no other lane */
+            pos[c] = t < total ? o_beg + (t - o_ex) : 0;
+        }
+#pragma unroll
+        for (int c = 0; c < U; c++) u[c] = d.in_adj[pos[c]];
+#pragma unroll
+        for (int c = 0; c < U; c++) f[c] = bf[u[c]];
+#pragma unroll
+        for (int c = 0; c < U; c++) {
+            h[c] = owner[c] < IPR_WAVE && f[c];
             m[c] = ipr_policy::ident();
-            h[c] = 0;
-            owner[c] = IPR_WAVE + lane;        /* matches no other lane */
-            if (t < total) {
-                owner[c] = lo;
-                h[c] = load(o_beg + (t - o_ex), &m[c]) ? 1 : 0;
-                if (!h[c]) m[c] = ipr_policy::ident();
-            }
+            if (h[c]) m[c] = bm[u[c]];
         }
 #pragma unroll
         for (int c = 0; c < U; c++) {
@@ -995,12 +1003,6 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_pull_step(ipr_dev d, uint32_t
                 ipr_pull_vertex(d, s, v, hm, msg, &n_react, &n_active, &n_bcast);
             }
         } else {
-            auto load = [&](uint64_t k, IP_MESSAGE_TYPE* m) -> bool {
-                uint32_t u = d.in_adj[k];
-                if (!bf[u]) return false;
-                *m = bm[u];
-                return true;
-            };
             const uint64_t stride = (uint64_t)nb0 * blockDim.x;
             for (uint64_t base = (uint64_t)b * blockDim.x + (threadIdx.x & ~(IPR_WAVE - 1)); base < n;
                  base += stride) {
@@ -1008,7 +1010,7 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_pull_step(ipr_dev d, uint32_t
                 const uint32_t v = i < n ? d.pbins[0][i] : 0;
                 IP_MESSAGE_TYPE acc = ipr_policy::ident();
                 bool hm = false;
-                if (s > 0) hm = ipr_wave_segmented_fold(d.in_idx, v, i < n, load, &acc);
+                if (s > 0) hm = ipr_wave_pull_fold(d, v, i < n, bf, bm, &acc);
                 if (i < n) ipr_pull_vertex(d, s, v, hm, acc, &n_react, &n_active, &n_bcast);
             }
         }
