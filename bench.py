@@ -91,11 +91,15 @@ def wl_pr16():
 # fold; the ordered fold is bit-exact and is what the parity tests pin)
 TUNING = {"pr16": {"pull_ordered": 2}}
 
-# HBM bytes per launch of the dominant kernel, from rocprofv3 --pmc FETCH_SIZE
-# and WRITE_SIZE passes over `bench.py --apps cc22` (FETCH_SIZE scaled by 2.0
-# and WRITE_SIZE by 1.0, the factors a plain 1 GiB device copy gives); the
-# commands and per-pass figures are in profiles/ipr_k_gather_cc_rmat22.md
-TRAFFIC = {("cc22", "ipr_k_gather"): (2_074_354 * 2 + 58_251) * 1024}
+# HBM bytes per dense launch of the gather, from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE passes over `bench.py --apps <workload>` (FETCH_SIZE scaled by 2.0
+# and WRITE_SIZE by 1.0, the factors a plain 1 GiB device copy gives; KiB,
+# mean over the dense launches); commands and per-launch figures are in
+# profiles/ipr_k_gather_cc_rmat22.md, ipr_k_gather_cc_rmat24.md and
+# ipr_k_gather_sssp_rmat24.md
+TRAFFIC = {("cc22", "ipr_k_gather"): (1_412_909 * 2 + 57_539) * 1024,
+           ("cc24", "ipr_k_gather"): (11_713_738 * 2 + 222_378) * 1024,
+           ("ss24a", "ipr_k_gather"): (3_641_891 * 2 + 159_829) * 1024}
 
 
 def wl_ss24a():
