@@ -113,8 +113,10 @@ typedef struct ipr_dev {
     const uint32_t* g_own;
     const uint32_t* g_dst;
     /* SPREAD with gather: the broadcast record of vertex v lives at
-     * bslot[bpos[v]], records ordered by descending out-degree so that the
-     * most-gathered ones share cache lines (NULL: bslot[v]) */
+     * bslot[bpos[v]], records ordered by descending out-degree class so that
+     * the most-gathered ones share cache lines, ids ascending within a class
+     * so that consecutive frontier vertices write nearby records (NULL:
+     * bslot[v]) */
     const uint32_t* bpos;
     /* dense supersteps whose broadcasters' out-arcs are below live_pct % of
      * E: bit p of g_live is set when the record at position p was broadcast

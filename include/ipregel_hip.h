@@ -72,7 +72,8 @@ int ipr_hip_build_csc(uint64_t V, uint64_t E, const uint64_t* out_idx,
 int ipr_hip_build_pull_bins(ipr_dev* dev, uint32_t small_max, uint32_t medium_max,
                             void* stream);
 
-/* record order: bpos[v] = rank of v by descending out-degree (ties by id) */
+/* record order: bpos[v] = rank of v by descending out-degree class
+ * [2^k, 2^(k+1)), ids ascending within a class; no out-arcs last */
 int ipr_hip_build_record_order(uint64_t V, const uint64_t* out_idx, uint32_t* bpos, void* stream);
 
 /* index of the arc-balanced gather over a device CSC (in-lists):

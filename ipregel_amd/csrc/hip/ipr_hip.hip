@@ -289,13 +289,16 @@ out:
 /* arc-balanced gather index                                                 */
 /* ------------------------------------------------------------------------ */
 
-/* sort key for descending out-degree (radix sort is stable: ties by id) */
+/* sort key: out-degree classes [2^k, 2^(k+1)) in descending order, vertices
+ * without out-arcs last; the radix sort is stable, so ids ascend within a
+ * class. The hot records gather at the front, and a run of consecutive
+ * frontier vertices of one class still writes consecutive records. */
 __global__ void k_degree_keys(uint64_t V, const uint64_t* idx, uint32_t* key, uint32_t* id)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += stride) {
         const uint64_t dg = idx[v + 1] - idx[v];
-        key[v] = dg >= 0xffffffffull ? 0u : 0xffffffffu - (uint32_t)dg;
+        key[v] = dg == 0 ? 64u : 63u - (63u - (uint32_t)__clzll(dg));
         id[v] = (uint32_t)v;
     }
 }
@@ -327,10 +330,10 @@ extern "C" int ipr_hip_build_record_order(uint64_t V, const uint64_t* out_idx, u
     if ((e = hipMalloc(&vin, V * 4)) != hipSuccess) { rc = fail(e, "record order ids"); goto out; }
     if ((e = hipMalloc(&vout, V * 4)) != hipSuccess) { rc = fail(e, "record order ids"); goto out; }
     hipLaunchKernelGGL(k_degree_keys, dim3((unsigned)vb), dim3(256), 0, st, V, out_idx, kin, vin);
-    e = rocprim::radix_sort_pairs(nullptr, tmp_bytes, kin, kout, vin, vout, (size_t)V, 0, 32, st);
+    e = rocprim::radix_sort_pairs(nullptr, tmp_bytes, kin, kout, vin, vout, (size_t)V, 0, 7, st);
     if (e != hipSuccess) { rc = fail(e, "record order sort size"); goto out; }
     if ((e = hipMalloc(&tmp, tmp_bytes ? tmp_bytes : 16)) != hipSuccess) { rc = fail(e, "record order temp"); goto out; }
-    e = rocprim::radix_sort_pairs(tmp, tmp_bytes, kin, kout, vin, vout, (size_t)V, 0, 32, st);
+    e = rocprim::radix_sort_pairs(tmp, tmp_bytes, kin, kout, vin, vout, (size_t)V, 0, 7, st);
     if (e != hipSuccess) { rc = fail(e, "record order sort"); goto out; }
     hipLaunchKernelGGL(k_scatter_pos, dim3((unsigned)vb), dim3(256), 0, st, V, vout, bpos);
     if ((e = hipGetLastError()) != hipSuccess) { rc = fail(e, "record order"); goto out; }
