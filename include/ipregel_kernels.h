@@ -423,8 +423,6 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_init(ipr_dev d)
  * has_message). */
 __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_compute_push(ipr_dev d)
 {
-    __shared__ uint32_t scan_a[IPR_WAVES], scan_b[IPR_WAVES];
-    __shared__ unsigned long long lp_base;
     ipr_ctl* ctl = d.ctl;
     if (ctl->done) return;
     const uint64_t s = ctl->superstep;
@@ -494,20 +492,24 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_compute_push(ipr_dev d)
                 }
             }
         }
-        /* large broadcasters of this tile take consecutive list slots and
-         * consecutive chunk ranges with one atomic */
-        if (__syncthreads_or(large)) {
-            uint32_t tl = 0, tc = 0;
-            uint32_t el = ipr_block_excl_scan(large, scan_a, &tl);
-            uint32_t ec = ipr_block_excl_scan(chunks, scan_b, &tc);
-            if (threadIdx.x == 0)
-                lp_base = atomicAdd((unsigned long long*)&ctl->lpack,
-                                    ((unsigned long long)tl << IPR_LPACK_SHIFT) | tc);
-            __syncthreads();
+        /* large broadcasters of this wavefront take consecutive list slots
+         * and consecutive chunk ranges with one atomic (no workgroup barrier:
+         * a barrier would wait for this tile's stores before the next tile) */
+        const uint64_t lmask = __ballot(large);
+        if (lmask) {
+            const uint32_t lane = ipr_lane();
+            const uint32_t el = (uint32_t)__popcll(lmask & ((1ull << lane) - 1ull));
+            const uint32_t ci = ipr_wave_incl_scan(chunks);
+            const uint32_t ec = ci - chunks;
+            const uint32_t tl = (uint32_t)__popcll(lmask), tc = __shfl(ci, IPR_WAVE - 1);
+            unsigned long long lb = 0;
+            if (lane == 0)
+                lb = atomicAdd((unsigned long long*)&ctl->lpack, ((unsigned long long)tl << IPR_LPACK_SHIFT) | tc);
+            lb = __shfl(lb, 0);
             if (large) {
-                uint64_t slot = (lp_base >> IPR_LPACK_SHIFT) + el;
+                const uint64_t slot = (lb >> IPR_LPACK_SHIFT) + el;
                 d.lbins[slot] = entry;
-                d.lbase[slot] = (lp_base & IPR_LPACK_MASK) + ec;
+                d.lbase[slot] = (lb & IPR_LPACK_MASK) + ec;
             }
         }
     }
