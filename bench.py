@@ -17,7 +17,8 @@ scale 22, edge factor 16, a/b/c/d = .57/.19/.19/.05, seed 2.
              for PageRank the fused pull kernel. traffic = rocprofv3-measured
              HBM bytes per launch where a profile exists
   cpu_baseline  the reference's own OpenMP build (oracle/_ref, compiled from
-             the reference sources) on the same graph file, its
+             the reference sources) on the same graph file, run again until
+             about 10 s of supersteps (at most 12 runs); the median of its
              "Total time of supersteps", with its thread count
 
 Two more sections go into the same JSON line (libipregel_batch.so):
@@ -149,7 +150,12 @@ def kernel_bytes(app: str, g: G.Graph, W: int, F_next: int, supersteps_with_bcas
     return per_arc * W + 4 * F_next
 
 
-def cpu_baseline(binname: str | None, g: G.Graph, W: int, source: int) -> dict | None:
+def cpu_baseline(binname: str | None, g: G.Graph, W: int, source: int,
+                 budget_s: float = 10.0, max_runs: int = 12) -> dict | None:
+    """The reference's own OpenMP binary on the same graph file, run again until
+    its supersteps have taken about `budget_s` seconds (at most `max_runs`
+    runs, each a full process: load, supersteps, dump to /dev/null); the
+    median of its 'Total time of supersteps' is the baseline."""
     if binname is None or not (REF_BIN / binname).exists():
         return None
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
@@ -161,24 +167,28 @@ def cpu_baseline(binname: str | None, g: G.Graph, W: int, source: int) -> dict |
         args = [str(REF_BIN / binname), root, "/dev/null", str(threads), "static", "0"]
         if binname.startswith("sssp"):
             args.append(str(source))
-        t0 = time.time()
-        try:
-            p = subprocess.run(args, env=env, capture_output=True, text=True, timeout=150)
-        except subprocess.TimeoutExpired:
-            return {"value": None, "unit": "GTEPS", "cores": threads, "kind": "reference",
-                    "sample": f"{binname} did not finish within 150 s"}
+        times, trace, t0 = [], None, time.time()
+        while len(times) < max_runs and (not times or sum(times) < budget_s):
+            try:
+                p = subprocess.run(args, env=env, capture_output=True, text=True, timeout=150)
+            except subprocess.TimeoutExpired:
+                return {"value": None, "unit": "GTEPS", "cores": threads, "kind": "reference",
+                        "sample": f"{binname} did not finish within 150 s"}
+            m = re.search(r"Total time of supersteps: ([0-9.]+)s", p.stdout)
+            if p.returncode != 0 or not m:
+                return {"value": None, "unit": "GTEPS", "cores": threads, "kind": "reference",
+                        "sample": f"{binname} failed rc={p.returncode}"}
+            times.append(float(m.group(1)))
+            if trace is None:
+                trace = [int(x.split(":")[1]) for x in p.stdout.splitlines() if "ActiveVertexCount" in x]
         wall = time.time() - t0
-        m = re.search(r"Total time of supersteps: ([0-9.]+)s", p.stdout)
-        if p.returncode != 0 or not m:
-            return {"value": None, "unit": "GTEPS", "cores": threads, "kind": "reference",
-                    "sample": f"{binname} failed rc={p.returncode}"}
-        t = float(m.group(1))
-        trace = [int(x.split(":")[1]) for x in p.stdout.splitlines() if "ActiveVertexCount" in x]
+        t = float(np.median(times))
         return {"value": W / t / 1e9, "unit": "GTEPS", "cores": threads, "kind": "reference",
-                "sample": f"one full run of the reference {binname} (OpenMP, static schedule) on "
-                          f"the same graph file, its 'Total time of supersteps' {t:.3f} s "
-                          f"(process wall {wall:.1f} s)",
-                "seconds": t, "trace": trace}
+                "sample": f"{len(times)} full runs of the reference {binname} (OpenMP, {threads} threads, "
+                          f"static schedule) on the same graph file; median 'Total time of supersteps' "
+                          f"{t:.3f} s (min {min(times):.3f}, max {max(times):.3f}; {sum(times):.1f} s of "
+                          f"supersteps, {wall:.1f} s of process wall time)",
+                "seconds": t, "runs": len(times), "trace": trace}
 
 
 def run_workload(key: str, steps: int, warmup: int, with_cpu: bool) -> dict:
