@@ -697,96 +697,111 @@ __global__ __launch_bounds__(256) void k_apply_b(PushArgs a, int cur)
     const uint32_t step = a.ctl->step;
     unsigned long long orm = 0, rel = 0, dig = 0;
     uint32_t fmn = INF;
-    for (uint64_t gb = (uint64_t)blockIdx.x * 256; gb < a.V; gb += (uint64_t)gridDim.x * 256) {
-        if (a.tgran[gb >> 8] == 0) continue;
+    __shared__ unsigned long long s_gm;
+    /* granules g = blockIdx.x + k * gridDim.x; their touched bytes are read
+     * 64 at a time by one wavefront, so untouched granules cost no round trip */
+    const uint64_t ngran = (a.V + 255) >> 8;
+    for (uint64_t g0 = blockIdx.x; g0 < ngran; g0 += (uint64_t)gridDim.x * 64) {
         __syncthreads();
-        if (t == 0) a.tgran[gb >> 8] = 0;
-        const uint64_t vb = gb + (uint64_t)wave * 64, vl = vb + lane;
-        const uint8_t fl = vl < a.V ? a.touched[vl] : 0;
-        if (!__syncthreads_or(fl != 0)) continue;
-        unsigned long long tm = __ballot(fl != 0);
-        if (fl) a.touched[vl] = 0;
-        uint32_t ns = 0, nl = 0;
-        while (tm) {
-            uint32_t js[8];
-            uint32_t nb = 0;
+        if (wave == 0) {
+            const uint64_t g = g0 + (uint64_t)lane * gridDim.x;
+            const unsigned long long m = __ballot(g < ngran && a.tgran[g] != 0);
+            if (lane == 0) s_gm = m;
+        }
+        __syncthreads();
+        unsigned long long gm = s_gm;
+        while (gm) {
+            const uint64_t gb = (g0 + (uint64_t)(__ffsll(gm) - 1) * gridDim.x) * 256;
+            gm &= gm - 1;
+            __syncthreads();
+            if (t == 0) a.tgran[gb >> 8] = 0;
+            const uint64_t vb = gb + (uint64_t)wave * 64, vl = vb + lane;
+            const uint8_t fl = vl < a.V ? a.touched[vl] : 0;
+            if (!__syncthreads_or(fl != 0)) continue;
+            unsigned long long tm = __ballot(fl != 0);
+            if (fl) a.touched[vl] = 0;
+            uint32_t ns = 0, nl = 0;
+            while (tm) {
+                uint32_t js[8];
+                uint32_t nb = 0;
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                js[i] = tm ? (uint32_t)(__ffsll((unsigned long long)tm) - 1) : 0u;
-                if (tm) {
-                    tm &= tm - 1;
-                    nb = i + 1;
+                for (int i = 0; i < 8; ++i) {
+                    js[i] = tm ? (uint32_t)(__ffsll((unsigned long long)tm) - 1) : 0u;
+                    if (tm) {
+                        tm &= tm - 1;
+                        nb = i + 1;
+                    }
                 }
-            }
-            uint32_t mm[8], dd[8];
-            unsigned long long vmv[8], pmv[8];
-            uint64_t dg[8];
+                uint32_t mm[8], dd[8];
+                unsigned long long vmv[8], pmv[8];
+                uint64_t dg[8];
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                if ((uint32_t)i < nb) {
+                for (int i = 0; i < 8; ++i) {
+                    if ((uint32_t)i < nb) {
+                        const uint64_t v = vb + js[i], ix = v * 64 + lane;
+                        mm[i] = a.mbox[ix];
+                        dd[i] = a.val[ix];
+                        vmv[i] = a.vmask[v];
+                        pmv[i] = pend[v];
+                        dg[i] = a.oidx[v + 1] - a.oidx[v];
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    if ((uint32_t)i >= nb) break;
                     const uint64_t v = vb + js[i], ix = v * 64 + lane;
-                    mm[i] = a.mbox[ix];
-                    dd[i] = a.val[ix];
-                    vmv[i] = a.vmask[v];
-                    pmv[i] = pend[v];
-                    dg[i] = a.oidx[v + 1] - a.oidx[v];
-                }
-            }
-#pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                if ((uint32_t)i >= nb) break;
-                const uint64_t v = vb + js[i], ix = v * 64 + lane;
-                const uint32_t m = mm[i];
-                const unsigned long long vm = vmv[i];
-                const uint32_t d = ((vm >> lane) & 1ull) ? dd[i] : INF;
-                const bool imp = m < d;
-                if (imp) {
-                    a.val[ix] = m;
-                    dig += dterm(v, m) - dterm(v, d);
-                }
-                if (m != INF) a.mbox[ix] = INF;
-                /* next frontier at v: the improved slots (parked entries of
-                 * those slots at v are superseded) */
-                const unsigned long long im = __ballot(imp), pm = pmv[i];
-                if (im) {
-                    if (lane == 0) {
-                        a.vmask[v] = vm | im;
-                        a.act[v] = im;
-                        a.fst[v] = step;
-                        if (pm & im) pend[v] = pm & ~im;
+                    const uint32_t m = mm[i];
+                    const unsigned long long vm = vmv[i];
+                    const uint32_t d = ((vm >> lane) & 1ull) ? dd[i] : INF;
+                    const bool imp = m < d;
+                    if (imp) {
+                        a.val[ix] = m;
+                        dig += dterm(v, m) - dterm(v, d);
                     }
-                    if (imp) fmn = min(fmn, m);
-                    const uint64_t deg = dg[i];
-                    orm |= im;
-                    rel += deg * (unsigned long long)__popcll(im);
-                    if (deg > a.large) {
-                        if (lane == 0) s_buf[1][wave][nl] = (uint32_t)v;
-                        nl++;
-                    } else {
-                        if (lane == 0) s_buf[0][wave][ns] = (uint32_t)v;
-                        ns++;
+                    if (m != INF) a.mbox[ix] = INF;
+                    /* next frontier at v: the improved slots (parked entries of
+                     * those slots at v are superseded) */
+                    const unsigned long long im = __ballot(imp), pm = pmv[i];
+                    if (im) {
+                        if (lane == 0) {
+                            a.vmask[v] = vm | im;
+                            a.act[v] = im;
+                            a.fst[v] = step;
+                            if (pm & im) pend[v] = pm & ~im;
+                        }
+                        if (imp) fmn = min(fmn, m);
+                        const uint64_t deg = dg[i];
+                        orm |= im;
+                        rel += deg * (unsigned long long)__popcll(im);
+                        if (deg > a.large) {
+                            if (lane == 0) s_buf[1][wave][nl] = (uint32_t)v;
+                            nl++;
+                        } else {
+                            if (lane == 0) s_buf[0][wave][ns] = (uint32_t)v;
+                            ns++;
+                        }
                     }
                 }
             }
-        }
-        if (lane == 0) {
-            s_cnt[0][wave] = ns;
-            s_cnt[1][wave] = nl;
-        }
-        __syncthreads();
-        if (t < 2) {
-            uint32_t tot = 0, pre[4];
-            for (int w = 0; w < 4; ++w) {
-                pre[w] = tot;
-                tot += s_cnt[t][w];
+            if (lane == 0) {
+                s_cnt[0][wave] = ns;
+                s_cnt[1][wave] = nl;
             }
-            const uint32_t b = tot ? atomicAdd(t ? &a.ctl->nlarge[nxt] : &a.ctl->nsmall[nxt], tot) : 0;
-            for (int w = 0; w < 4; ++w) s_base[t][w] = b + pre[w];
+            __syncthreads();
+            if (t < 2) {
+                uint32_t tot = 0, pre[4];
+                for (int w = 0; w < 4; ++w) {
+                    pre[w] = tot;
+                    tot += s_cnt[t][w];
+                }
+                const uint32_t b = tot ? atomicAdd(t ? &a.ctl->nlarge[nxt] : &a.ctl->nsmall[nxt], tot) : 0;
+                for (int w = 0; w < 4; ++w) s_base[t][w] = b + pre[w];
+            }
+            __syncthreads();
+            if ((uint32_t)lane < ns) S[s_base[0][wave] + lane] = s_buf[0][wave][lane];
+            if ((uint32_t)lane < nl) Lg[s_base[1][wave] + lane] = s_buf[1][wave][lane];
+            __syncthreads();
         }
-        __syncthreads();
-        if ((uint32_t)lane < ns) S[s_base[0][wave] + lane] = s_buf[0][wave][lane];
-        if ((uint32_t)lane < nl) Lg[s_base[1][wave] + lane] = s_buf[1][wave][lane];
-        __syncthreads();
     }
     if (lane == 0 && orm) atomicOr(&a.ctl->orm[blockIdx.x & 63], orm);
     if (lane == 0 && rel) atomicAdd(&a.ctl->relax[blockIdx.x & 63], rel);
