@@ -175,7 +175,7 @@ extern "C" {
 __host__ __device__ inline size_t ip_get_superstep(void)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
-    return (size_t)ipr_d_dev.ctl->superstep;
+    return (size_t)ipr_sm_superstep;
 #else
     return ip_superstep;
 #endif

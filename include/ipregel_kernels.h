@@ -45,8 +45,15 @@
 #define IPR_GATHER_UNROLL 4   /* gathers in flight per lane (memory-level parallelism) */
 #endif
 
-/* device copy of the engine state, read by the device-side API functions */
-__device__ ipr_dev ipr_d_dev;
+/* device copy of the engine state, read by the device-side API functions
+ * (constant memory: scalar loads, cached, never aliased by the kernels' stores) */
+__constant__ ipr_dev ipr_d_dev;
+
+/* the superstep, for ip_get_superstep() in application code: every kernel
+ * that can run application code (ip_compute, ip_combine) stores it here first,
+ * each thread the same value, so the call costs an LDS read instead of two
+ * dependent global loads */
+__shared__ uint64_t ipr_sm_superstep;
 
 enum { IPR_ERR_SEND_IN_PULL = 1, IPR_ERR_MIXED_BROADCAST = 2 };
 
@@ -426,6 +433,7 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_compute_push(ipr_dev d)
     ipr_ctl* ctl = d.ctl;
     if (ctl->done) return;
     const uint64_t s = ctl->superstep;
+    ipr_sm_superstep = s;
     uint64_t n = d.V;
     const uint32_t* list = nullptr;
     if ((IPR_VARIANT == IPR_VARIANT_SPREAD || IPR_VARIANT == IPR_VARIANT_SPREAD_PULL) && s > 0) {
@@ -463,6 +471,9 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_compute_push(ipr_dev d)
                 x.active = true;
             }
             if (run) {
+                /* the broadcast bookkeeping's loads go out with the vertex's own */
+                const uint64_t o0 = d.out_idx[v], o1 = d.out_idx[v + 1];
+                const uint32_t rp = ipr_brec(d, v);
                 ipr_fill_vertex(d, &x, v);
                 const IP_VALUE_TYPE before = x.value;
                 const bool hm_before = x.has_message;
@@ -478,12 +489,12 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_compute_push(ipr_dev d)
                 if (x.ipr_bcast) {
                     /* recorded even without out-arcs: with directed=false a
                      * SPREAD_PULL in-list may still name this vertex */
-                    uint64_t deg = d.out_idx[v + 1] - d.out_idx[v];
+                    uint64_t deg = o1 - o0;
                     n_edges += deg;
                     ipr_bslot_t rec;
                     rec.step = (uint32_t)(s + 1) | (x.ipr_bcast_weighted ? IPR_BSTEP_WEIGHTED : 0u);
                     rec.msg = x.ipr_bcast_msg;
-                    bs[ipr_brec(d, v)] = rec;
+                    bs[rp] = rec;
                     if (deg > d.large_min) {
                         large = 1;
                         chunks = (uint32_t)((deg + d.large_chunk - 1) / d.large_chunk);
@@ -531,6 +542,7 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_expand(ipr_dev d)
     if (ctl->done) return;
     if (ipr_gathers(d)) return;
     const uint64_t s = ctl->superstep;
+    ipr_sm_superstep = s;
     const ipr_bslot_t* bs = (const ipr_bslot_t*)d.bslot;
     const uint32_t want = (uint32_t)(s + 1);
 
@@ -860,6 +872,7 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_swap(ipr_dev d)
     ipr_ctl* ctl = d.ctl;
     if (ctl->done) return;
     const uint64_t s = ctl->superstep;
+    ipr_sm_superstep = s;
     ipr_slot_t* mb = (ipr_slot_t*)d.mbox[(s + 1) & 1];
     uint8_t* rv = d.recv[(s + 1) & 1];
     uint64_t n_react = 0;
@@ -987,6 +1000,7 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_pull_step(ipr_dev d, uint32_t
     ipr_ctl* ctl = d.ctl;
     if (ctl->done) return;
     const uint64_t s = ctl->superstep;
+    ipr_sm_superstep = s;
     const uint8_t* bf = d.bcf[(s + 1) & 1];
     const IP_MESSAGE_TYPE* bm = (const IP_MESSAGE_TYPE*)d.bcm[(s + 1) & 1];
     uint64_t n_react = 0, n_active = 0, n_bcast = 0;
@@ -1131,6 +1145,7 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_fetch_targets(ipr_dev d)
     ipr_ctl* ctl = d.ctl;
     if (ctl->done) return;
     const uint64_t s = ctl->superstep;
+    ipr_sm_superstep = s;
     const uint32_t want = (uint32_t)(s + 1);
     const uint64_t n = ctl->fsize[(s + 1) & 1];
     const uint32_t* list = d.front[(s + 1) & 1];
