@@ -500,17 +500,17 @@ extern "C" int ipr_hip_build_pull_bins(ipr_dev* dev, uint32_t small_max, uint32_
 /* frontier compaction                                                       */
 /* ------------------------------------------------------------------------ */
 
-/* The received flags of superstep s+1 become the frontier list. Each thread
- * owns IPR_COMPACT_T consecutive 16-flag groups (and clears them for reuse two
- * supersteps on); a workgroup prefix sum of the threads' counts places the
- * ids, and one atomic per workgroup reserves its run, so ids ascend within
- * each 256*16*IPR_COMPACT_T-vertex run. Few workgroups means few atomics on
- * the one counter: contention there, not the flag bytes, bounds this kernel
- * when most runs hold a frontier vertex. */
+/* The received flags of superstep s+1 become the frontier list. A workgroup
+ * takes IPR_COMPACT_T rounds of 256 consecutive 16-flag groups (coalesced
+ * loads; the flags are cleared for reuse two supersteps on). Positions follow
+ * (round, thread) order, which is ascending vertex order: one workgroup scan
+ * per round, one atomic per workgroup on the list counter, and neighbouring
+ * threads write neighbouring runs. Few workgroups means few atomics on the
+ * one counter, whose contention bounds this kernel on large graphs. */
 #define IPR_COMPACT_T 8
 __global__ void __launch_bounds__(256) k_compact(ipr_dev d)
 {
-    __shared__ uint32_t wsum[4];
+    __shared__ uint32_t wsum[IPR_COMPACT_T][4];
     __shared__ unsigned long long blk_base;
     ipr_ctl* c = d.ctl;
     if (c->done) return;
@@ -522,45 +522,53 @@ __global__ void __launch_bounds__(256) k_compact(ipr_dev d)
     const uint64_t span = 256ull * IPR_COMPACT_T;
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (uint64_t b0 = (uint64_t)blockIdx.x * span; b0 < groups; b0 += (uint64_t)gridDim.x * span) {
-        const uint64_t g0 = b0 + (uint64_t)threadIdx.x * IPR_COMPACT_T;
         uint4 q[IPR_COMPACT_T];
-        uint32_t n = 0;
+        uint32_t n[IPR_COMPACT_T], incl[IPR_COMPACT_T];
 #pragma unroll
         for (int t = 0; t < IPR_COMPACT_T; t++) {
+            const uint64_t g = b0 + (uint64_t)t * 256 + threadIdx.x;
             q[t] = make_uint4(0, 0, 0, 0);
-            if (g0 + t < groups) q[t] = flags[g0 + t];
+            if (g < groups) q[t] = flags[g];
         }
 #pragma unroll
         for (int t = 0; t < IPR_COMPACT_T; t++) {
             /* flags are 0 or 1, so each set flag is one bit (0, 8, 16 or 24) */
-            const uint32_t m = __popc(q[t].x) + __popc(q[t].y) + __popc(q[t].z) + __popc(q[t].w);
-            if (m) flags[g0 + t] = make_uint4(0, 0, 0, 0);
-            n += m;
+            n[t] = __popc(q[t].x) + __popc(q[t].y) + __popc(q[t].z) + __popc(q[t].w);
+            if (n[t]) flags[b0 + (uint64_t)t * 256 + threadIdx.x] = make_uint4(0, 0, 0, 0);
+            uint32_t x = n[t];
+            for (int o = 1; o < 64; o <<= 1) {
+                uint32_t y = __shfl_up(x, o);
+                if (lane >= (uint32_t)o) x += y;
+            }
+            incl[t] = x;
+            if (lane == 63) wsum[t][wave] = x;
         }
-        uint32_t incl = n;
-        for (int o = 1; o < 64; o <<= 1) {
-            uint32_t y = __shfl_up(incl, o);
-            if (lane >= (uint32_t)o) incl += y;
-        }
-        if (lane == 63) wsum[wave] = incl;
         __syncthreads();
-        uint32_t before = 0, total = 0;
-        for (uint32_t w = 0; w < 4; w++) {
-            if (w < wave) before += wsum[w];
-            total += wsum[w];
+        uint32_t total = 0;
+        uint32_t before[IPR_COMPACT_T];
+#pragma unroll
+        for (int t = 0; t < IPR_COMPACT_T; t++) {
+            uint32_t b = total;
+            for (uint32_t w = 0; w < 4; w++) {
+                if (w < wave) b += wsum[t][w];
+                total += wsum[t][w];
+            }
+            before[t] = b;
         }
         if (threadIdx.x == 0 && total) blk_base = atomicAdd(counter, (unsigned long long)total);
         __syncthreads();
-        if (n) {
-            uint64_t pos = blk_base + before + incl - n;
+        if (total) {
 #pragma unroll
             for (int t = 0; t < IPR_COMPACT_T; t++) {
+                if (!n[t]) continue;
+                uint64_t pos = blk_base + before[t] + incl[t] - n[t];
+                const uint64_t g = b0 + (uint64_t)t * 256 + threadIdx.x;
                 const uint32_t words[4] = {q[t].x, q[t].y, q[t].z, q[t].w};
                 for (int k = 0; k < 4; k++) {
                     uint32_t w = words[k];
                     while (w) {
                         uint32_t bit = __ffs(w) - 1;
-                        out[pos++] = (uint32_t)((g0 + t) * 16 + k * 4 + bit / 8);
+                        out[pos++] = (uint32_t)(g * 16 + k * 4 + bit / 8);
                         w &= w - 1;
                     }
                 }
