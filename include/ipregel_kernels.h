@@ -579,10 +579,21 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_expand(ipr_dev d)
         uint32_t deg = 0;
         if (i < n) {
             uint32_t v = list ? list[i] : (uint32_t)i;
+            /* a frontier vertex's offsets are read with its record (one round
+             * trip less); the full sweep of PUSH reads them only for
+             * broadcasters */
+            uint64_t off = 0, end = 0;
+            if (list) {
+                off = d.out_idx[v];
+                end = d.out_idx[v + 1];
+            }
             ipr_bslot_t b = bs[ipr_brec(d, v)];
             if ((b.step & ~IPR_BSTEP_WEIGHTED) == want) {
-                uint64_t off = d.out_idx[v];
-                uint64_t dg = d.out_idx[v + 1] - off;
+                if (!list) {
+                    off = d.out_idx[v];
+                    end = d.out_idx[v + 1];
+                }
+                uint64_t dg = end - off;
                 if (dg <= d.large_min) {
                     deg = (uint32_t)dg;
                     s_off[threadIdx.x] = off;
