@@ -8,13 +8,16 @@
  * Push variants (PUSH, SPREAD), per superstep s:
  *   compute  every vertex of the work set (all V at superstep 0 and in PUSH,
  *            the frontier list in SPREAD) runs ip_compute; a broadcast is
- *            recorded in bslot[v] = {s+1, message}; broadcasters with more
- *            than large_min out-arcs are also listed, with their first arc
- *            chunk, in lbins/lbase (one atomic per workgroup).
+ *            recorded in bslot[bpos[v]] = {s+1, message} (bpos: records by
+ *            out-degree class for SPREAD, identity otherwise); broadcasters
+ *            with more than large_min out-arcs are also listed, with their
+ *            first arc chunk, in lbins/lbase (one atomic per wavefront).
  *   gather   (dense supersteps: broadcasters' out-arcs > E/dir_alpha) every
  *            vertex folds the records of its in-neighbours over the exact
- *            transpose: same combined message and same receivers as the push,
- *            one writer per mailbox, no atomics.
+ *            transpose, arc-balanced across wavefronts: same combined message
+ *            and same receivers as the push; lists split between wavefronts
+ *            are delivered with the combiner's atomic. With few live arcs a
+ *            bitmap of this superstep's broadcasters is read first.
  *   expand   (sparse supersteps) large broadcasters one workgroup per chunk
  *            of large_chunk arcs; the rest of the work set by a workgroup
  *            prefix sum over the out-degrees of 256 vertices, so the
