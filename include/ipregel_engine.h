@@ -11,8 +11,9 @@
  *   val[V]        application value                    (ip_vertex_t.value)
  *   msg[V]        delivered combined message            (ip_vertex_t.message)
  *   act[V], hm[V] active / has_message flags, one byte each
- *   bcf[2][V]     broadcast flags (bytes) by superstep parity (pull variants)
- *   bcm[2][V]     broadcast messages by superstep parity (pull variants)
+ *   bcm[2][V]     broadcast records {superstep+1, message} by superstep
+ *                 parity (pull variants; a record counts only when its stamp
+ *                 is the superstep that reads it, so nothing is cleared)
  *   bslot[V]      push variants: last broadcast {superstep+1 | weighted<<31, msg}
  *   mbox[2][V]    combiner mailboxes by parity (push variants)
  *   recv[2][V]    received-a-message flags (bytes) by parity (push variants);
@@ -131,7 +132,6 @@ typedef struct ipr_dev {
     void* msg;
     uint8_t* act;
     uint8_t* hm;
-    uint8_t* bcf[2];
     void* bcm[2];
     void* mbox[2];
     uint8_t* recv[2];

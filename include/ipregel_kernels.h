@@ -406,8 +406,8 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_init(ipr_dev d)
         d.act[v] = 1;  /* ip_init_vertex_range, e.g. src/combiner_postamble.h:189 */
         d.hm[v] = 0;
 #if IPR_VARIANT == 1
-        d.bcf[0][v] = 0;
-        d.bcf[1][v] = 0;
+        ((ipr_bslot_t*)d.bcm[0])[v].step = 0;
+        ((ipr_bslot_t*)d.bcm[1])[v].step = 0;
 #elif IPR_VARIANT == 3
         ((ipr_bslot_t*)d.bslot)[v].step = 0;
         d.recv[0][v] = 0;
@@ -631,12 +631,12 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_expand(ipr_dev d)
  * pull): lane L holds vertex v (valid); the in-lists of the wavefront's 64
  * vertices are laid end to end and taken 64*U arcs at a time. For the U
  * chunks in flight the source ids are loaded first, then their broadcast
- * flags, then the messages of the sources that broadcast, so no load waits on
- * another of its kind. A segmented scan across the lanes folds each vertex's
+ * records (stamp and message in one load), so no load waits on another of
+ * its kind. A segmented scan across the lanes folds each vertex's
  * arcs and the segment's last lane hands the result to the vertex's lane.
  * Must be reached by all lanes of the wavefront. */
-__device__ inline bool ipr_wave_pull_fold(const ipr_dev& d, uint32_t v, bool valid, const uint8_t* bf,
-                                          const IP_MESSAGE_TYPE* bm, IP_MESSAGE_TYPE* out)
+__device__ inline bool ipr_wave_pull_fold(const ipr_dev& d, uint32_t v, bool valid, const ipr_bslot_t* pr,
+                                          uint32_t want, IP_MESSAGE_TYPE* out)
 {
     const uint32_t lane = ipr_lane();
     uint64_t beg = 0;
@@ -654,7 +654,7 @@ __device__ inline bool ipr_wave_pull_fold(const ipr_dev& d, uint32_t v, bool val
     for (uint32_t t00 = 0; t00 < total; t00 += U * IPR_WAVE) {
         uint64_t pos[U];
         uint32_t owner[U], u[U];
-        uint8_t f[U];
+        ipr_bslot_t r[U];
         IP_MESSAGE_TYPE m[U];
         int h[U];
 #pragma unroll
@@ -675,12 +675,11 @@ __device__ inline bool ipr_wave_pull_fold(const ipr_dev& d, uint32_t v, bool val
 #pragma unroll
         for (int c = 0; c < U; c++) u[c] = d.in_adj[pos[c]];
 #pragma unroll
-        for (int c = 0; c < U; c++) f[c] = bf[u[c]];
+        for (int c = 0; c < U; c++) r[c] = pr[u[c]];
 #pragma unroll
         for (int c = 0; c < U; c++) {
-            h[c] = owner[c] < IPR_WAVE && f[c];
-            m[c] = ipr_policy::ident();
-            if (h[c]) m[c] = bm[u[c]];
+            h[c] = owner[c] < IPR_WAVE && r[c].step == want;
+            m[c] = h[c] ? r[c].msg : ipr_policy::ident();
         }
 #pragma unroll
         for (int c = 0; c < U; c++) {
@@ -908,19 +907,19 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_swap(ipr_dev d)
 /* ------------------------------------------------------------------------ */
 
 /* fold of the broadcasting in-neighbours of v in in-list order
- * (src/combiner_single_broadcast_postamble.h:161-193) */
-__device__ inline bool ipr_fetch_ordered(const ipr_dev& d, uint32_t v, const uint8_t* bf,
-                                         const IP_MESSAGE_TYPE* bm, IP_MESSAGE_TYPE* out)
+ * (src/combiner_single_broadcast_postamble.h:161-193); pr holds superstep
+ * s-1's broadcast records, stamped want = s */
+__device__ inline bool ipr_fetch_ordered(const ipr_dev& d, uint32_t v, const ipr_bslot_t* pr, uint32_t want,
+                                         IP_MESSAGE_TYPE* out)
 {
     uint64_t i = d.in_idx[v], end = d.in_idx[v + 1];
     bool has = false;
     IP_MESSAGE_TYPE acc = IP_MESSAGE_TYPE();
     for (; i < end; i++) {
-        uint32_t u = d.in_adj[i];
-        if (bf[u]) {
-            IP_MESSAGE_TYPE m = bm[u];
-            if (has) ip_combine(&acc, m);
-            else { acc = m; has = true; }
+        const ipr_bslot_t r = pr[d.in_adj[i]];
+        if (r.step == want) {
+            if (has) ip_combine(&acc, r.msg);
+            else { acc = r.msg; has = true; }
         }
     }
     *out = acc;
@@ -935,8 +934,6 @@ __device__ inline void ipr_pull_vertex(const ipr_dev& d, uint64_t s, uint32_t v,
     bool act = d.act[v] != 0;
     const bool act0 = act;
     if (hm && !act) { act = true; (*n_react)++; }
-    uint8_t* bf_cur = d.bcf[s & 1];
-    bf_cur[v] = 0;
     if (act) {
         struct ip_vertex_t x;
         ipr_fill_vertex(d, &x, v);
@@ -949,8 +946,11 @@ __device__ inline void ipr_pull_vertex(const ipr_dev& d, uint64_t s, uint32_t v,
         act = x.active;
         if (act) (*n_active)++;
         if (x.ipr_bcast) {
-            ((IP_MESSAGE_TYPE*)d.bcm[s & 1])[v] = x.ipr_bcast_msg;
-            bf_cur[v] = 1;
+            /* stamped s+1: read in superstep s+1, and no clearing needed */
+            ipr_bslot_t rec;
+            rec.step = (uint32_t)(s + 1);
+            rec.msg = x.ipr_bcast_msg;
+            ((ipr_bslot_t*)d.bcm[s & 1])[v] = rec;
             (*n_bcast)++;
         }
     }
@@ -969,8 +969,8 @@ __device__ inline void ipr_pull_counts(const ipr_dev& d, uint64_t s, uint64_t n_
 /* In-list order fold by one wavefront (vertices of larger in-degree): 64
  * in-neighbours are fetched at once, then folded lane by lane in list order
  * with readlane, so the result is bit-identical to the sequential fold. */
-__device__ inline bool ipr_fetch_ordered_wave(const ipr_dev& d, uint32_t v, const uint8_t* bf,
-                                              const IP_MESSAGE_TYPE* bm, IP_MESSAGE_TYPE* out)
+__device__ inline bool ipr_fetch_ordered_wave(const ipr_dev& d, uint32_t v, const ipr_bslot_t* pr,
+                                              uint32_t want, IP_MESSAGE_TYPE* out)
 {
     const uint32_t lane = ipr_lane();
     const uint64_t beg = d.in_idx[v], end = d.in_idx[v + 1];
@@ -981,9 +981,9 @@ __device__ inline bool ipr_fetch_ordered_wave(const ipr_dev& d, uint32_t v, cons
         bool f = false;
         IP_MESSAGE_TYPE m = IP_MESSAGE_TYPE();
         if (k < end) {
-            uint32_t u = d.in_adj[k];
-            f = bf[u] != 0;
-            if (f) m = bm[u];
+            const ipr_bslot_t r = pr[d.in_adj[k]];
+            f = r.step == want;
+            if (f) m = r.msg;
         }
         uint64_t mask = __ballot(f);
         while (mask) {
@@ -1015,8 +1015,9 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_pull_step(ipr_dev d, uint32_t
     if (ctl->done) return;
     const uint64_t s = ctl->superstep;
     ipr_sm_superstep = s;
-    const uint8_t* bf = d.bcf[(s + 1) & 1];
-    const IP_MESSAGE_TYPE* bm = (const IP_MESSAGE_TYPE*)d.bcm[(s + 1) & 1];
+    /* superstep s-1's broadcast records, stamped s */
+    const ipr_bslot_t* pr = (const ipr_bslot_t*)d.bcm[(s + 1) & 1];
+    const uint32_t want = (uint32_t)s;
     uint64_t n_react = 0, n_active = 0, n_bcast = 0;
     const uint32_t lane = ipr_lane();
     const uint32_t wave_in_block = threadIdx.x / IPR_WAVE;
@@ -1029,7 +1030,7 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_pull_step(ipr_dev d, uint32_t
                 uint32_t v = d.pbins[0][i];
                 IP_MESSAGE_TYPE msg = IP_MESSAGE_TYPE();
                 bool hm = false;
-                if (s > 0) hm = ipr_fetch_ordered(d, v, bf, bm, &msg);
+                if (s > 0) hm = ipr_fetch_ordered(d, v, pr, want, &msg);
                 ipr_pull_vertex(d, s, v, hm, msg, &n_react, &n_active, &n_bcast);
             }
         } else {
@@ -1040,7 +1041,7 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_pull_step(ipr_dev d, uint32_t
                 const uint32_t v = i < n ? d.pbins[0][i] : 0;
                 IP_MESSAGE_TYPE acc = ipr_policy::ident();
                 bool hm = false;
-                if (s > 0) hm = ipr_wave_pull_fold(d, v, i < n, bf, bm, &acc);
+                if (s > 0) hm = ipr_wave_pull_fold(d, v, i < n, pr, want, &acc);
                 if (i < n) ipr_pull_vertex(d, s, v, hm, acc, &n_react, &n_active, &n_bcast);
             }
         }
@@ -1055,7 +1056,7 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_pull_step(ipr_dev d, uint32_t
             uint32_t v = list[i];
             IP_MESSAGE_TYPE msg = IP_MESSAGE_TYPE();
             bool hm = false;
-            if (s > 0) hm = ipr_fetch_ordered_wave(d, v, bf, bm, &msg);
+            if (s > 0) hm = ipr_fetch_ordered_wave(d, v, pr, want, &msg);
             if (lane == 0) ipr_pull_vertex(d, s, v, hm, msg, &n_react, &n_active, &n_bcast);
         }
     } else if (b < nb0 + nb1) {
@@ -1069,8 +1070,8 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_pull_step(ipr_dev d, uint32_t
             if (s > 0) {
                 uint64_t beg = d.in_idx[v], end = d.in_idx[v + 1];
                 for (uint64_t k = beg + lane; k < end; k += IPR_WAVE) {
-                    uint32_t u = d.in_adj[k];
-                    if (bf[u]) { ipr_policy::fold(&acc, bm[u]); has = true; }
+                    const ipr_bslot_t r = pr[d.in_adj[k]];
+                    if (r.step == want) { ipr_policy::fold(&acc, r.msg); has = true; }
                 }
                 for (int o = 32; o > 0; o >>= 1) {
                     IP_MESSAGE_TYPE y = __shfl_xor(acc, o);
@@ -1089,8 +1090,8 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_pull_step(ipr_dev d, uint32_t
             if (s > 0) {
                 uint64_t beg = d.in_idx[v], end = d.in_idx[v + 1];
                 for (uint64_t k = beg + threadIdx.x; k < end; k += IPR_BLOCK) {
-                    uint32_t u = d.in_adj[k];
-                    if (bf[u]) { ipr_policy::fold(&acc, bm[u]); has = true; }
+                    const ipr_bslot_t r = pr[d.in_adj[k]];
+                    if (r.step == want) { ipr_policy::fold(&acc, r.msg); has = true; }
                 }
                 for (int o = 32; o > 0; o >>= 1) {
                     IP_MESSAGE_TYPE y = __shfl_xor(acc, o);

@@ -177,7 +177,6 @@ static void dev_free_all(void)
     ipr_hip_free(d->act);
     ipr_hip_free(d->hm);
     for (int i = 0; i < 2; i++) {
-        ipr_hip_free(d->bcf[i]);
         ipr_hip_free(d->bcm[i]);
         ipr_hip_free(d->mbox[i]);
         ipr_hip_free(d->recv[i]);
@@ -276,8 +275,7 @@ static void dev_setup(void)
     const int records = !pull;
     for (int i = 0; i < 2; i++) {
         if (pull) {
-            d->bcm[i] = dalloc(vv * a->msg_size, "broadcast messages");
-            d->bcf[i] = dalloc(vv, "broadcast flags");
+            d->bcm[i] = dalloc(vv * a->bslot_size, "broadcast records");
         } else {
             if (var != IPR_VARIANT_SPREAD_PULL) d->mbox[i] = dalloc(vv * a->mbox_size, "mailboxes");
             d->recv[i] = dalloc(((vv + 15) / 16) * 16 + 16, "received flags");
