@@ -44,6 +44,9 @@
 #define IPR_BLOCK 256
 #define IPR_WAVE 64
 #define IPR_WAVES (IPR_BLOCK / IPR_WAVE)
+#ifndef IPR_GATHER_HOT
+#define IPR_GATHER_HOT 1024   /* hottest broadcast records kept in LDS by the gather */
+#endif
 #ifndef IPR_GATHER_UNROLL
 #define IPR_GATHER_UNROLL 4   /* gathers in flight per lane (memory-level parallelism) */
 #endif
@@ -755,6 +758,9 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_gather(ipr_dev d)
 {
     constexpr int K = IPR_GATHER_K;
     static_assert(K % 4 == 0, "sources are read four at a time");
+    /* with records in out-degree order the first IPR_GATHER_HOT are the
+     * most-read ones: each workgroup keeps a copy in LDS */
+    __shared__ ipr_bslot_t hot[IPR_GATHER_HOT];
     ipr_ctl* ctl = d.ctl;
     if (ctl->done) return;
     if (!ipr_gathers(d)) return;
@@ -762,11 +768,14 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_gather(ipr_dev d)
     const uint64_t s = ctl->superstep;
     const uint32_t want = (uint32_t)(s + 1);
     const uint32_t lane = ipr_lane();
+    const ipr_bslot_t* bs = (const ipr_bslot_t*)d.bslot;
+    const uint32_t nhot = d.bpos ? (uint32_t)(d.V < IPR_GATHER_HOT ? d.V : IPR_GATHER_HOT) : 0u;
+    for (uint32_t h = threadIdx.x; h < nhot; h += IPR_BLOCK) hot[h] = bs[h];
+    __syncthreads();
     const uint64_t per = d.g_per_wave;
     const uint64_t w0 = ((uint64_t)blockIdx.x * IPR_WAVES + threadIdx.x / IPR_WAVE) * per;
     if (w0 >= d.E) return;
     const uint64_t w1 = w0 + per < d.E ? w0 + per : d.E;
-    const ipr_bslot_t* bs = (const ipr_bslot_t*)d.bslot;
     IP_MESSAGE_TYPE carry = ipr_policy::ident();
     int carry_has = 0;
     uint32_t carry_o = 0;
@@ -795,12 +804,16 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_gather(ipr_dev d)
             for (int j = 0; j < K; j++) w[j] = d.g_live[(a[j] & ~IPR_GATHER_LAST) >> 6];
 #pragma unroll
             for (int j = 0; j < K; j++) {
+                const uint32_t p = a[j] & ~IPR_GATHER_LAST;
                 r[j].step = 0;
-                if ((w[j] >> (a[j] & 63)) & 1ull) r[j] = bs[a[j] & ~IPR_GATHER_LAST];
+                if ((w[j] >> (a[j] & 63)) & 1ull) r[j] = p < nhot ? hot[p] : bs[p];
             }
         } else {
 #pragma unroll
-            for (int j = 0; j < K; j++) r[j] = bs[a[j] & ~IPR_GATHER_LAST];
+            for (int j = 0; j < K; j++) {
+                const uint32_t p = a[j] & ~IPR_GATHER_LAST;
+                r[j] = p < nhot ? hot[p] : bs[p];
+            }
         }
 #ifdef IP_NEEDS_OUT_NEIGHBOUR_WEIGHTS
         uint32_t wt[K];
