@@ -44,8 +44,8 @@
 #define IPR_BLOCK 256
 #define IPR_WAVE 64
 #define IPR_WAVES (IPR_BLOCK / IPR_WAVE)
-#ifndef IPR_GATHER_HOT
-#define IPR_GATHER_HOT 1024   /* hottest broadcast records kept in LDS by the gather */
+#ifndef IPR_GATHER_HOT_BYTES
+#define IPR_GATHER_HOT_BYTES 32768   /* LDS per gather workgroup for the hottest broadcast records */
 #endif
 #ifndef IPR_GATHER_UNROLL
 #define IPR_GATHER_UNROLL 4   /* gathers in flight per lane (memory-level parallelism) */
@@ -758,9 +758,11 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_gather(ipr_dev d)
 {
     constexpr int K = IPR_GATHER_K;
     static_assert(K % 4 == 0, "sources are read four at a time");
-    /* with records in out-degree order the first IPR_GATHER_HOT are the
-     * most-read ones: each workgroup keeps a copy in LDS */
-    __shared__ ipr_bslot_t hot[IPR_GATHER_HOT];
+    /* with records in out-degree order the first ones are the most read:
+     * each workgroup keeps a copy of IPR_GATHER_HOT_BYTES of them in LDS
+     * (4096 records of 8 bytes; 32 KB still lets 5 workgroups share a CU) */
+    constexpr uint32_t HOT = IPR_GATHER_HOT_BYTES / sizeof(ipr_bslot_t);
+    __shared__ ipr_bslot_t hot[HOT];
     ipr_ctl* ctl = d.ctl;
     if (ctl->done) return;
     if (!ipr_gathers(d)) return;
@@ -769,7 +771,7 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_gather(ipr_dev d)
     const uint32_t want = (uint32_t)(s + 1);
     const uint32_t lane = ipr_lane();
     const ipr_bslot_t* bs = (const ipr_bslot_t*)d.bslot;
-    const uint32_t nhot = d.bpos ? (uint32_t)(d.V < IPR_GATHER_HOT ? d.V : IPR_GATHER_HOT) : 0u;
+    const uint32_t nhot = d.bpos ? (uint32_t)(d.V < HOT ? d.V : HOT) : 0u;
     for (uint32_t h = threadIdx.x; h < nhot; h += IPR_BLOCK) hot[h] = bs[h];
     __syncthreads();
     const uint64_t per = d.g_per_wave;
