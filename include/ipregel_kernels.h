@@ -1298,6 +1298,7 @@ static int ipr_app_bind(const ipr_dev* d)
 /* the declared combiner must agree with ip_combine on sample pairs */
 __global__ void ipr_k_check_combiner(const IP_MESSAGE_TYPE* a, const IP_MESSAGE_TYPE* b, int n, int* bad)
 {
+    ipr_sm_superstep = 0;   /* ip_combine may ask (it runs before the first superstep) */
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     IP_MESSAGE_TYPE x = a[i], y = x;
