@@ -685,11 +685,18 @@ __global__ __launch_bounds__(256) void k_apply_s(PushArgs a, int cur)
 
 /* batched apply: a workgroup takes a 256-vertex granule; each wavefront scans
  * 64 byte flags and folds the marked rows 8 at a time, all loads issued first */
+/* next-frontier entries a workgroup of k_apply_b buffers in LDS before one
+ * reservation on the list counter (entries beyond the buffer are reserved one
+ * at a time) */
+#define APPLY_BUF_S 1024
+#define APPLY_BUF_L 256
+
 __global__ __launch_bounds__(256) void k_apply_b(PushArgs a, int cur)
 {
-    __shared__ uint32_t s_buf[2][4][64];
-    __shared__ uint32_t s_cnt[2][4];
-    __shared__ uint32_t s_base[2][4];
+    __shared__ uint32_t buf_s[APPLY_BUF_S], buf_l[APPLY_BUF_L];
+    __shared__ uint32_t cnt_s, cnt_l;
+    __shared__ uint32_t base_s, base_l;
+    __shared__ unsigned long long s_gm;
     const int nxt = cur ^ 1, t = threadIdx.x, lane = t & 63, wave = t >> 6;
     uint32_t* S = small_of(a, nxt);
     uint32_t* Lg = large_of(a, nxt);
@@ -697,9 +704,14 @@ __global__ __launch_bounds__(256) void k_apply_b(PushArgs a, int cur)
     const uint32_t step = a.ctl->step;
     unsigned long long orm = 0, rel = 0, dig = 0;
     uint32_t fmn = INF;
-    __shared__ unsigned long long s_gm;
+    if (t == 0) {
+        cnt_s = 0;
+        cnt_l = 0;
+    }
     /* granules g = blockIdx.x + k * gridDim.x; their touched bytes are read
-     * 64 at a time by one wavefront, so untouched granules cost no round trip */
+     * 64 at a time by one wavefront, so untouched granules cost no round trip.
+     * Inside a granule each wavefront owns 64 vertices and works alone: no
+     * workgroup barrier, no global atomic per granule. */
     const uint64_t ngran = (a.V + 255) >> 8;
     for (uint64_t g0 = blockIdx.x; g0 < ngran; g0 += (uint64_t)gridDim.x * 64) {
         __syncthreads();
@@ -713,14 +725,12 @@ __global__ __launch_bounds__(256) void k_apply_b(PushArgs a, int cur)
         while (gm) {
             const uint64_t gb = (g0 + (uint64_t)(__ffsll(gm) - 1) * gridDim.x) * 256;
             gm &= gm - 1;
-            __syncthreads();
             if (t == 0) a.tgran[gb >> 8] = 0;
             const uint64_t vb = gb + (uint64_t)wave * 64, vl = vb + lane;
             const uint8_t fl = vl < a.V ? a.touched[vl] : 0;
-            if (!__syncthreads_or(fl != 0)) continue;
             unsigned long long tm = __ballot(fl != 0);
+            if (!tm) continue;
             if (fl) a.touched[vl] = 0;
-            uint32_t ns = 0, nl = 0;
             while (tm) {
                 uint32_t js[8];
                 uint32_t nb = 0;
@@ -763,46 +773,40 @@ __global__ __launch_bounds__(256) void k_apply_b(PushArgs a, int cur)
                      * those slots at v are superseded) */
                     const unsigned long long im = __ballot(imp), pm = pmv[i];
                     if (im) {
+                        const uint64_t deg = dg[i];
                         if (lane == 0) {
                             a.vmask[v] = vm | im;
                             a.act[v] = im;
                             a.fst[v] = step;
                             if (pm & im) pend[v] = pm & ~im;
+                            if (deg > a.large) {
+                                const uint32_t q = atomicAdd(&cnt_l, 1u);
+                                if (q < APPLY_BUF_L) buf_l[q] = (uint32_t)v;
+                                else Lg[atomicAdd(&a.ctl->nlarge[nxt], 1u)] = (uint32_t)v;
+                            } else {
+                                const uint32_t q = atomicAdd(&cnt_s, 1u);
+                                if (q < APPLY_BUF_S) buf_s[q] = (uint32_t)v;
+                                else S[atomicAdd(&a.ctl->nsmall[nxt], 1u)] = (uint32_t)v;
+                            }
                         }
                         if (imp) fmn = min(fmn, m);
-                        const uint64_t deg = dg[i];
                         orm |= im;
                         rel += deg * (unsigned long long)__popcll(im);
-                        if (deg > a.large) {
-                            if (lane == 0) s_buf[1][wave][nl] = (uint32_t)v;
-                            nl++;
-                        } else {
-                            if (lane == 0) s_buf[0][wave][ns] = (uint32_t)v;
-                            ns++;
-                        }
                     }
                 }
             }
-            if (lane == 0) {
-                s_cnt[0][wave] = ns;
-                s_cnt[1][wave] = nl;
-            }
-            __syncthreads();
-            if (t < 2) {
-                uint32_t tot = 0, pre[4];
-                for (int w = 0; w < 4; ++w) {
-                    pre[w] = tot;
-                    tot += s_cnt[t][w];
-                }
-                const uint32_t b = tot ? atomicAdd(t ? &a.ctl->nlarge[nxt] : &a.ctl->nsmall[nxt], tot) : 0;
-                for (int w = 0; w < 4; ++w) s_base[t][w] = b + pre[w];
-            }
-            __syncthreads();
-            if ((uint32_t)lane < ns) S[s_base[0][wave] + lane] = s_buf[0][wave][lane];
-            if ((uint32_t)lane < nl) Lg[s_base[1][wave] + lane] = s_buf[1][wave][lane];
-            __syncthreads();
         }
     }
+    __syncthreads();
+    const uint32_t ns = cnt_s < APPLY_BUF_S ? cnt_s : APPLY_BUF_S;
+    const uint32_t nl = cnt_l < APPLY_BUF_L ? cnt_l : APPLY_BUF_L;
+    if (t == 0) {
+        base_s = ns ? atomicAdd(&a.ctl->nsmall[nxt], ns) : 0u;
+        base_l = nl ? atomicAdd(&a.ctl->nlarge[nxt], nl) : 0u;
+    }
+    __syncthreads();
+    for (uint32_t i = t; i < ns; i += 256) S[base_s + i] = buf_s[i];
+    for (uint32_t i = t; i < nl; i += 256) Lg[base_l + i] = buf_l[i];
     if (lane == 0 && orm) atomicOr(&a.ctl->orm[blockIdx.x & 63], orm);
     if (lane == 0 && rel) atomicAdd(&a.ctl->relax[blockIdx.x & 63], rel);
     if (dig) atomicAdd(&a.ctl->dig[blockIdx.x % SPREAD][lane], dig);
