@@ -624,11 +624,33 @@ __global__ __launch_bounds__(256) void k_relax_large(PushArgs a, int cur)
     }
 }
 
-/* scalar apply: 4 vertices per thread; fold mailbox, append improved vertices */
+/* next-frontier entries a workgroup of an apply kernel buffers in LDS before
+ * one reservation on the list counter (entries beyond the buffer are reserved
+ * one at a time) */
+#define APPLY_BUF_S 1024
+#define APPLY_BUF_L 256
+
+/* flush of an apply workgroup's buffered next-frontier entries */
+__device__ inline void apply_flush(PushArgs& a, int nxt, uint32_t* S, uint32_t* Lg, const uint32_t* buf_s,
+                                   const uint32_t* buf_l, uint32_t cnt_s, uint32_t cnt_l, uint32_t* base)
+{
+    const uint32_t ns = cnt_s < APPLY_BUF_S ? cnt_s : APPLY_BUF_S;
+    const uint32_t nl = cnt_l < APPLY_BUF_L ? cnt_l : APPLY_BUF_L;
+    if (threadIdx.x == 0) {
+        base[0] = ns ? atomicAdd(&a.ctl->nsmall[nxt], ns) : 0u;
+        base[1] = nl ? atomicAdd(&a.ctl->nlarge[nxt], nl) : 0u;
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < ns; i += blockDim.x) S[base[0] + i] = buf_s[i];
+    for (uint32_t i = threadIdx.x; i < nl; i += blockDim.x) Lg[base[1] + i] = buf_l[i];
+}
+
+/* one-query apply: a workgroup takes 1024-vertex spans (four granule bytes),
+ * a thread 4 vertices; improved vertices go to the LDS buffers */
 __global__ __launch_bounds__(256) void k_apply_s(PushArgs a, int cur)
 {
-    __shared__ uint32_t s_w[5];
-    __shared__ uint32_t s_base[2];
+    __shared__ uint32_t buf_s[APPLY_BUF_S], buf_l[APPLY_BUF_L];
+    __shared__ uint32_t cnt_s, cnt_l, base[2];
     const int nxt = cur ^ 1, t = threadIdx.x;
     uint32_t* S = small_of(a, nxt);
     uint32_t* Lg = large_of(a, nxt);
@@ -636,46 +658,46 @@ __global__ __launch_bounds__(256) void k_apply_s(PushArgs a, int cur)
     const uint32_t step = a.ctl->step;
     unsigned long long rel = 0;
     uint32_t fmn = INF;
+    if (t == 0) {
+        cnt_s = 0;
+        cnt_l = 0;
+    }
     for (uint64_t gb = (uint64_t)blockIdx.x * 1024; gb < a.V; gb += (uint64_t)gridDim.x * 1024) {
         /* the four granule bytes of these 1024 vertices (padded map) */
         if (*(const uint32_t*)(a.tgran + (gb >> 8)) == 0) continue;
-        __syncthreads();
+        __syncthreads();   /* every thread has read the bytes before they are cleared */
         if (t == 0) *(uint32_t*)(a.tgran + (gb >> 8)) = 0;
         const uint64_t v0 = gb + (uint64_t)t * 4;
         uint32_t f4 = 0;
         if (v0 < a.V) f4 = *(const uint32_t*)(a.touched + v0);   /* padded to 1024 */
-        if (!__syncthreads_or(f4 != 0)) continue;
-        uint32_t ls[4], ll[4], ns = 0, nl = 0;
-        if (f4) {
-            *(uint32_t*)(a.touched + v0) = 0;
+        if (!f4) continue;
+        *(uint32_t*)(a.touched + v0) = 0;
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                if (!((f4 >> (8 * k)) & 0xff)) continue;
-                const uint64_t v = v0 + k;
-                const uint32_t m = a.mbox[v], d = a.val[v];
-                if (m < d) {
-                    a.val[v] = m;
-                    if (pend[v]) pend[v] = 0;     /* its parked entry is superseded */
-                    a.fst[v] = step;
-                    fmn = min(fmn, m);
-                    const uint64_t deg = a.oidx[v + 1] - a.oidx[v];
-                    rel += deg;
-                    if (deg > a.large) ll[nl++] = (uint32_t)v; else ls[ns++] = (uint32_t)v;
+        for (int k = 0; k < 4; ++k) {
+            if (!((f4 >> (8 * k)) & 0xff)) continue;
+            const uint64_t v = v0 + k;
+            const uint32_t m = a.mbox[v], d = a.val[v];
+            if (m < d) {
+                a.val[v] = m;
+                if (pend[v]) pend[v] = 0;     /* its parked entry is superseded */
+                a.fst[v] = step;
+                fmn = min(fmn, m);
+                const uint64_t deg = a.oidx[v + 1] - a.oidx[v];
+                rel += deg;
+                if (deg > a.large) {
+                    const uint32_t q = atomicAdd(&cnt_l, 1u);
+                    if (q < APPLY_BUF_L) buf_l[q] = (uint32_t)v;
+                    else Lg[atomicAdd(&a.ctl->nlarge[nxt], 1u)] = (uint32_t)v;
+                } else {
+                    const uint32_t q = atomicAdd(&cnt_s, 1u);
+                    if (q < APPLY_BUF_S) buf_s[q] = (uint32_t)v;
+                    else S[atomicAdd(&a.ctl->nsmall[nxt], 1u)] = (uint32_t)v;
                 }
             }
         }
-        uint32_t ts, tl;
-        const uint32_t xs = block_excl_scan_u32<256>(ns, s_w, ts);
-        const uint32_t xl = block_excl_scan_u32<256>(nl, s_w, tl);
-        if (t == 0) {
-            s_base[0] = ts ? atomicAdd(&a.ctl->nsmall[nxt], ts) : 0;
-            s_base[1] = tl ? atomicAdd(&a.ctl->nlarge[nxt], tl) : 0;
-        }
-        __syncthreads();
-        for (uint32_t k = 0; k < ns; ++k) S[s_base[0] + xs + k] = ls[k];
-        for (uint32_t k = 0; k < nl; ++k) Lg[s_base[1] + xl + k] = ll[k];
-        __syncthreads();
     }
+    __syncthreads();
+    apply_flush(a, nxt, S, Lg, buf_s, buf_l, cnt_s, cnt_l, base);
     rel = wave_sum_u64(rel);
     if ((t & 63) == 0 && rel) atomicAdd(&a.ctl->relax[blockIdx.x & 63], rel);
 #pragma unroll
@@ -683,19 +705,13 @@ __global__ __launch_bounds__(256) void k_apply_s(PushArgs a, int cur)
     if ((t & 63) == 0 && fmn != INF) atomicMin(&a.ctl->fmin[blockIdx.x % SPREAD][0], fmn);
 }
 
-/* batched apply: a workgroup takes a 256-vertex granule; each wavefront scans
- * 64 byte flags and folds the marked rows 8 at a time, all loads issued first */
-/* next-frontier entries a workgroup of k_apply_b buffers in LDS before one
- * reservation on the list counter (entries beyond the buffer are reserved one
- * at a time) */
-#define APPLY_BUF_S 1024
-#define APPLY_BUF_L 256
-
+/* batched apply: a workgroup takes 256-vertex granules; each wavefront scans
+ * its 64 byte flags and folds the marked rows 8 at a time, all loads issued
+ * first */
 __global__ __launch_bounds__(256) void k_apply_b(PushArgs a, int cur)
 {
     __shared__ uint32_t buf_s[APPLY_BUF_S], buf_l[APPLY_BUF_L];
-    __shared__ uint32_t cnt_s, cnt_l;
-    __shared__ uint32_t base_s, base_l;
+    __shared__ uint32_t cnt_s, cnt_l, base[2];
     __shared__ unsigned long long s_gm;
     const int nxt = cur ^ 1, t = threadIdx.x, lane = t & 63, wave = t >> 6;
     uint32_t* S = small_of(a, nxt);
@@ -798,15 +814,7 @@ __global__ __launch_bounds__(256) void k_apply_b(PushArgs a, int cur)
         }
     }
     __syncthreads();
-    const uint32_t ns = cnt_s < APPLY_BUF_S ? cnt_s : APPLY_BUF_S;
-    const uint32_t nl = cnt_l < APPLY_BUF_L ? cnt_l : APPLY_BUF_L;
-    if (t == 0) {
-        base_s = ns ? atomicAdd(&a.ctl->nsmall[nxt], ns) : 0u;
-        base_l = nl ? atomicAdd(&a.ctl->nlarge[nxt], nl) : 0u;
-    }
-    __syncthreads();
-    for (uint32_t i = t; i < ns; i += 256) S[base_s + i] = buf_s[i];
-    for (uint32_t i = t; i < nl; i += 256) Lg[base_l + i] = buf_l[i];
+    apply_flush(a, nxt, S, Lg, buf_s, buf_l, cnt_s, cnt_l, base);
     if (lane == 0 && orm) atomicOr(&a.ctl->orm[blockIdx.x & 63], orm);
     if (lane == 0 && rel) atomicAdd(&a.ctl->relax[blockIdx.x & 63], rel);
     if (dig) atomicAdd(&a.ctl->dig[blockIdx.x % SPREAD][lane], dig);
