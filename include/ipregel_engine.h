@@ -125,6 +125,10 @@ typedef struct ipr_dev {
     uint64_t* g_live;
     uint32_t live_pct;
     uint32_t live_pad;
+    /* push apps with integral messages that gather: a 4-byte copy of each
+     * broadcast record at the same position as in bslot (IPR_REC4_* layout),
+     * read by the gather in place of the 8-byte record (NULL: off) */
+    uint32_t* brec4;
     uint64_t g_per_wave;
     uint32_t gather_blocks;
     uint32_t g_pad;
@@ -153,6 +157,17 @@ typedef struct ipr_dev {
     uint32_t dir_opt;            /* push apps: allow gathering dense supersteps */
     uint32_t dir_alpha;          /* gather when bcast out-arcs * alpha > E */
 } ipr_dev;
+
+/* 4-byte broadcast record: bits 31-26 the superstep's stamp (superstep mod
+ * IPR_REC4_WINDOW, plus one; 0 never matches), bit 25 ip_broadcast_weighted,
+ * bits 24-0 the message, or IPR_REC4_ESC when it does not fit (the reader
+ * then takes the message from the 8-byte record). The stamps of a window of
+ * IPR_REC4_WINDOW supersteps are distinct; every record is cleared to 0 at
+ * the end of a window's last superstep, so a stale record never matches. */
+#define IPR_REC4_SHIFT 26
+#define IPR_REC4_WINDOW 63u
+#define IPR_REC4_WEIGHTED 0x02000000u
+#define IPR_REC4_ESC 0x01FFFFFFu
 
 /* kernels the application TU provides (typed by the app's message/value) */
 enum {

@@ -319,6 +319,18 @@ __device__ __forceinline__ uint32_t ipr_brec(const ipr_dev& d, uint32_t v)
 }
 #define IPR_BSTEP_WEIGHTED 0x80000000u
 
+/* the 4-byte gather record of a broadcast in superstep s (integral
+ * messages; a value that does not fit below IPR_REC4_ESC escapes) */
+__device__ __forceinline__ uint32_t ipr_rec4(uint32_t stamp, IP_MESSAGE_TYPE m, bool weighted)
+{
+    uint32_t low = IPR_REC4_ESC;
+    if constexpr (ipr_policy::exact_gather) {
+        const uint64_t u = (uint64_t)m;
+        if (u < IPR_REC4_ESC) low = (uint32_t)u;
+    }
+    return (stamp << IPR_REC4_SHIFT) | (weighted ? IPR_REC4_WEIGHTED : 0u) | low;
+}
+
 /* raise the received flag of u by atomic OR on its 32-bit word (plain byte
  * read first: most deliveries find it already raised) */
 __device__ inline void ipr_mark_recv(uint8_t* recv, uint32_t u)
@@ -422,6 +434,7 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_init(ipr_dev d)
         d.recv[0][v] = 0;
         d.recv[1][v] = 0;
 #endif
+        if (d.brec4) d.brec4[v] = 0;
     }
 }
 
@@ -448,6 +461,7 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_compute_push(ipr_dev d)
     }
     ipr_slot_t* mcur = (ipr_slot_t*)d.mbox[s & 1];
     ipr_bslot_t* bs = (ipr_bslot_t*)d.bslot;
+    const uint32_t stamp4 = (uint32_t)(s % IPR_REC4_WINDOW) + 1u;
     uint64_t n_active = 0, n_computed = 0, n_edges = 0;
     const uint64_t tile_stride = (uint64_t)gridDim.x * IPR_BLOCK;
     for (uint64_t base = (uint64_t)blockIdx.x * IPR_BLOCK; base < n; base += tile_stride) {
@@ -501,6 +515,7 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_compute_push(ipr_dev d)
                     rec.step = (uint32_t)(s + 1) | (x.ipr_bcast_weighted ? IPR_BSTEP_WEIGHTED : 0u);
                     rec.msg = x.ipr_bcast_msg;
                     bs[rp] = rec;
+                    if (d.brec4) d.brec4[rp] = ipr_rec4(stamp4, rec.msg, x.ipr_bcast_weighted);
                     if (deg > d.large_min) {
                         large = 1;
                         chunks = (uint32_t)((deg + d.large_chunk - 1) / d.large_chunk);
@@ -760,19 +775,29 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_gather(ipr_dev d)
     static_assert(K % 4 == 0, "sources are read four at a time");
     /* with records in out-degree order the first ones are the most read:
      * each workgroup keeps a copy of IPR_GATHER_HOT_BYTES of them in LDS
-     * (4096 records of 8 bytes; 32 KB still lets 5 workgroups share a CU) */
+     * (8192 records of 4 bytes or 4096 of 8; 32 KB still lets 5 workgroups
+     * share a CU) */
     constexpr uint32_t HOT = IPR_GATHER_HOT_BYTES / sizeof(ipr_bslot_t);
     __shared__ ipr_bslot_t hot[HOT];
+    uint32_t* hot4 = (uint32_t*)hot;
     ipr_ctl* ctl = d.ctl;
     if (ctl->done) return;
     if (!ipr_gathers(d)) return;
     const bool live = ipr_gathers_live(d);
     const uint64_t s = ctl->superstep;
     const uint32_t want = (uint32_t)(s + 1);
+    const uint32_t want4 = (uint32_t)(s % IPR_REC4_WINDOW) + 1u;
     const uint32_t lane = ipr_lane();
     const ipr_bslot_t* bs = (const ipr_bslot_t*)d.bslot;
-    const uint32_t nhot = d.bpos ? (uint32_t)(d.V < HOT ? d.V : HOT) : 0u;
-    for (uint32_t h = threadIdx.x; h < nhot; h += IPR_BLOCK) hot[h] = bs[h];
+    const uint32_t* b4 = d.brec4;
+    /* 4-byte records: twice as many of them in the same LDS */
+    const uint32_t cap = b4 ? 2 * HOT : HOT;
+    const uint32_t nhot = d.bpos ? (uint32_t)(d.V < cap ? d.V : cap) : 0u;
+    if (b4) {
+        for (uint32_t h = threadIdx.x; h < nhot; h += IPR_BLOCK) hot4[h] = b4[h];
+    } else {
+        for (uint32_t h = threadIdx.x; h < nhot; h += IPR_BLOCK) hot[h] = bs[h];
+    }
     __syncthreads();
     const uint64_t per = d.g_per_wave;
     const uint64_t w0 = ((uint64_t)blockIdx.x * IPR_WAVES + threadIdx.x / IPR_WAVE) * per;
@@ -797,24 +822,45 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_gather(ipr_dev d)
             for (int j = 0; j < K; j++) a[j] = t0 + j < w1 ? d.g_arc[t0 + j] : 0u;
         }
         const uint32_t o = t0 < w1 ? d.g_own[t0 / K] : 0u;   /* receiver of arc t0 */
-        ipr_bslot_t r[K];
+        /* the K records: ok[j] bit 0 when source j broadcast in superstep s,
+         * bit 1 when weighted; mv[j] its message */
+        IP_MESSAGE_TYPE mv[K];
+        uint32_t ok[K];
+        uint64_t w[K];
         if (live) {
             /* the bitmap (V bits, cache-resident) first; records only of
              * sources that broadcast */
-            uint64_t w[K];
 #pragma unroll
             for (int j = 0; j < K; j++) w[j] = d.g_live[(a[j] & ~IPR_GATHER_LAST) >> 6];
+        }
+        if (b4) {
+            uint32_t q[K];
+#pragma unroll
+            for (int j = 0; j < K; j++) {
+                const uint32_t p = a[j] & ~IPR_GATHER_LAST;
+                q[j] = 0;
+                if (!live || ((w[j] >> (a[j] & 63)) & 1ull)) q[j] = p < nhot ? hot4[p] : b4[p];
+            }
+#pragma unroll
+            for (int j = 0; j < K; j++) {
+                const uint32_t low = q[j] & IPR_REC4_ESC;
+                ok[j] = (q[j] >> IPR_REC4_SHIFT) == want4 ? 1u | ((q[j] & IPR_REC4_WEIGHTED) ? 2u : 0u) : 0u;
+                mv[j] = (IP_MESSAGE_TYPE)low;
+                if (ok[j] && low == IPR_REC4_ESC) mv[j] = bs[a[j] & ~IPR_GATHER_LAST].msg;
+            }
+        } else {
+            ipr_bslot_t r[K];
 #pragma unroll
             for (int j = 0; j < K; j++) {
                 const uint32_t p = a[j] & ~IPR_GATHER_LAST;
                 r[j].step = 0;
-                if ((w[j] >> (a[j] & 63)) & 1ull) r[j] = p < nhot ? hot[p] : bs[p];
+                if (!live || ((w[j] >> (a[j] & 63)) & 1ull)) r[j] = p < nhot ? hot[p] : bs[p];
             }
-        } else {
 #pragma unroll
             for (int j = 0; j < K; j++) {
-                const uint32_t p = a[j] & ~IPR_GATHER_LAST;
-                r[j] = p < nhot ? hot[p] : bs[p];
+                ok[j] = (r[j].step & ~IPR_BSTEP_WEIGHTED) == want
+                            ? 1u | ((r[j].step & IPR_BSTEP_WEIGHTED) ? 2u : 0u) : 0u;
+                mv[j] = r[j].msg;
             }
         }
 #ifdef IP_NEEDS_OUT_NEIGHBOUR_WEIGHTS
@@ -840,10 +886,10 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_gather(ipr_dev d)
 #pragma unroll
         for (int j = 0; j < K; j++) {
             const bool live = t0 + j < w1;
-            if (live && (r[j].step & ~IPR_BSTEP_WEIGHTED) == want) {
-                IP_MESSAGE_TYPE m = r[j].msg;
+            if (live && ok[j]) {
+                IP_MESSAGE_TYPE m = mv[j];
 #ifdef IP_NEEDS_OUT_NEIGHBOUR_WEIGHTS
-                if (r[j].step & IPR_BSTEP_WEIGHTED) m = (IP_MESSAGE_TYPE)(m + (IP_MESSAGE_TYPE)wt[j]);
+                if (ok[j] & 2u) m = (IP_MESSAGE_TYPE)(m + (IP_MESSAGE_TYPE)wt[j]);
 #endif
                 ipr_policy::fold(&cur, m);
                 cur_has = 1;
@@ -904,8 +950,11 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_swap(ipr_dev d)
     ipr_slot_t* mb = (ipr_slot_t*)d.mbox[(s + 1) & 1];
     uint8_t* rv = d.recv[(s + 1) & 1];
     uint64_t n_react = 0;
+    /* the last superstep of a stamp window clears the 4-byte records */
+    uint32_t* clr = d.brec4 && (s + 1) % IPR_REC4_WINDOW == 0 ? d.brec4 : nullptr;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < d.V; v += stride) {
+        if (clr) clr[v] = 0;
         if (rv[v]) {
             rv[v] = 0;   /* clean for superstep s+2 */
             ((IP_MESSAGE_TYPE*)d.msg)[v] = ipr_policy::value(mb[v]);

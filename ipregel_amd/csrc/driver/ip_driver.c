@@ -164,6 +164,7 @@ static void dev_free_all(void)
     ipr_hip_free((void*)d->g_dst);
     ipr_hip_free((void*)d->bpos);
     ipr_hip_free(d->g_live);
+    ipr_hip_free(d->brec4);
     ipr_hip_free(d->bslot);
     ipr_hip_free(d->lbase);
     if (d->in_idx && d->in_idx != d->out_idx) ipr_hip_free((void*)d->in_idx);
@@ -359,6 +360,11 @@ static void dev_setup(void)
         /* broadcasters' bitmap for gathers with few live arcs (0 turns it off) */
         d->live_pct = env_u32("IPR_GATHER_LIVE", 60);
         if (d->live_pct > 0) d->g_live = dalloc(((vv + 63) / 64) * 8, "gather live bitmap");
+        /* 4-byte records for the gather (0 turns them off); padded to 16
+         * entries for the window clear of the frontier compaction, cleared by
+         * the init kernel */
+        if (env_u32("IPR_GATHER_RECORD4", 1) == 1)
+            d->brec4 = dalloc(((vv + 15) / 16) * 64, "gather records");
         /* wavefronts (4 per workgroup) own equal arc ranges, whole passes */
         d->gather_blocks = env_u32("IPR_GATHER_BLOCKS", 8192);
         if (d->gather_blocks < 1) d->gather_blocks = 1;

@@ -521,6 +521,9 @@ __global__ void __launch_bounds__(256) k_compact(ipr_dev d)
     const uint64_t groups = (d.V + 15) / 16;           /* flag arrays are padded to 16 */
     const uint64_t span = 256ull * IPR_COMPACT_T;
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    /* the last superstep of a stamp window clears the 4-byte gather records
+     * (padded to 16 like the flags) */
+    uint4* clr = d.brec4 && (s + 1) % IPR_REC4_WINDOW == 0 ? (uint4*)d.brec4 : nullptr;
     for (uint64_t b0 = (uint64_t)blockIdx.x * span; b0 < groups; b0 += (uint64_t)gridDim.x * span) {
         uint4 q[IPR_COMPACT_T];
         uint32_t n[IPR_COMPACT_T], incl[IPR_COMPACT_T];
@@ -529,6 +532,13 @@ __global__ void __launch_bounds__(256) k_compact(ipr_dev d)
             const uint64_t g = b0 + (uint64_t)t * 256 + threadIdx.x;
             q[t] = make_uint4(0, 0, 0, 0);
             if (g < groups) q[t] = flags[g];
+            if (clr && g < groups) {
+                const uint4 z = make_uint4(0, 0, 0, 0);
+                clr[4 * g] = z;
+                clr[4 * g + 1] = z;
+                clr[4 * g + 2] = z;
+                clr[4 * g + 3] = z;
+            }
         }
 #pragma unroll
         for (int t = 0; t < IPR_COMPACT_T; t++) {

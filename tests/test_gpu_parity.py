@@ -140,6 +140,35 @@ def test_gather_every_superstep(blocks, order, live, monkeypatch):
     assert np.array_equal(r.values, O.min_ancestor(star))
 
 
+@pytest.mark.parametrize("record4", ["1", "0"])
+def test_gather_stamp_window_and_escapes(record4, monkeypatch):
+    """The gather forced on every superstep for more supersteps than the
+    4-byte records' stamp window (63): CC and BFS on a 40x40 grid (78
+    supersteps) in the SPREAD and PUSH variants; weighted SSSP whose distances
+    pass 2^25, so its messages escape to the 8-byte records; with the 4-byte
+    records and without them."""
+    monkeypatch.setenv("IPR_GATHER_RECORD4", record4)
+    gg = G.grid(40, 40)
+    for name in ("cc_spread", "cc"):
+        print(f"app={name} grid40", flush=True)
+        r = EN.run_app(name, gg, dir_alpha=1 << 30)
+        assert np.array_equal(r.values, O.min_ancestor(gg))
+        assert r.supersteps > 63 and r.gathered > 63
+    for name in ("sssp_spread", "sssp"):
+        print(f"app={name} grid40", flush=True)
+        r = EN.run_app(name, gg, source=0, dir_alpha=1 << 30)
+        assert np.array_equal(r.values, O.bfs(gg, 0))
+        assert r.gathered > 63
+    gw = G.rmat(12, 16, seed=6).with_weights(2, 1 << 24, 1 << 26)
+    ref = O.dijkstra(gw, 0)
+    assert ref[ref != 2**32 - 1].max() >= 1 << 25
+    for name in ("sssp_weighted_spread", "sssp_weighted"):
+        print(f"app={name} rmat12 wide weights", flush=True)
+        r = EN.run_app(name, gw, source=0, dir_alpha=1 << 30)
+        assert np.array_equal(r.values, ref)
+        assert r.gathered > 0
+
+
 def test_tune_engine_keeps_results():
     from ipregel_amd import autotune as AT
     g = G.rmat(12, 16, seed=5, symmetrise=True)
