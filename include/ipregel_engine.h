@@ -164,6 +164,11 @@ typedef struct ipr_dev {
  * then takes the message from the 8-byte record). The stamps of a window of
  * IPR_REC4_WINDOW supersteps are distinct; every record is cleared to 0 at
  * the end of a window's last superstep, so a stale record never matches. */
+/* wavefronts per gather workgroup */
+#ifndef IPR_GATHER_WAVES
+#define IPR_GATHER_WAVES 4
+#endif
+
 #define IPR_REC4_SHIFT 26
 #define IPR_REC4_WINDOW 63u
 #define IPR_REC4_WEIGHTED 0x02000000u

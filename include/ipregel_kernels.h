@@ -331,6 +331,29 @@ __device__ __forceinline__ uint32_t ipr_rec4(uint32_t stamp, IP_MESSAGE_TYPE m, 
     return (stamp << IPR_REC4_SHIFT) | (weighted ? IPR_REC4_WEIGHTED : 0u) | low;
 }
 
+/* the record at position p: true when its vertex broadcast in superstep s
+ * (want = s + 1, want4 its 4-byte stamp), with the message and whether it
+ * is weighted. With 4-byte records on, the 8-byte record is written (and
+ * read) only for messages that escape. */
+__device__ __forceinline__ bool ipr_bcast_at(const ipr_dev& d, uint32_t p, uint32_t want, uint32_t want4,
+                                             IP_MESSAGE_TYPE* m, bool* wf)
+{
+    const ipr_bslot_t* bs = (const ipr_bslot_t*)d.bslot;
+    if (d.brec4) {
+        const uint32_t q = d.brec4[p];
+        const uint32_t low = q & IPR_REC4_ESC;
+        *wf = (q & IPR_REC4_WEIGHTED) != 0;
+        *m = (IP_MESSAGE_TYPE)low;
+        if ((q >> IPR_REC4_SHIFT) != want4) return false;
+        if (low == IPR_REC4_ESC) *m = bs[p].msg;
+        return true;
+    }
+    const ipr_bslot_t b = bs[p];
+    *wf = (b.step & IPR_BSTEP_WEIGHTED) != 0;
+    *m = b.msg;
+    return (b.step & ~IPR_BSTEP_WEIGHTED) == want;
+}
+
 /* raise the received flag of u by atomic OR on its 32-bit word (plain byte
  * read first: most deliveries find it already raised) */
 __device__ inline void ipr_mark_recv(uint8_t* recv, uint32_t u)
@@ -514,8 +537,13 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_compute_push(ipr_dev d)
                     ipr_bslot_t rec;
                     rec.step = (uint32_t)(s + 1) | (x.ipr_bcast_weighted ? IPR_BSTEP_WEIGHTED : 0u);
                     rec.msg = x.ipr_bcast_msg;
-                    bs[rp] = rec;
-                    if (d.brec4) d.brec4[rp] = ipr_rec4(stamp4, rec.msg, x.ipr_bcast_weighted);
+                    if (d.brec4) {
+                        const uint32_t r4 = ipr_rec4(stamp4, rec.msg, x.ipr_bcast_weighted);
+                        d.brec4[rp] = r4;
+                        if ((r4 & IPR_REC4_ESC) == IPR_REC4_ESC) bs[rp] = rec;
+                    } else {
+                        bs[rp] = rec;
+                    }
                     if (deg > d.large_min) {
                         large = 1;
                         chunks = (uint32_t)((deg + d.large_chunk - 1) / d.large_chunk);
@@ -564,8 +592,8 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_expand(ipr_dev d)
     if (ipr_gathers(d)) return;
     const uint64_t s = ctl->superstep;
     ipr_sm_superstep = s;
-    const ipr_bslot_t* bs = (const ipr_bslot_t*)d.bslot;
     const uint32_t want = (uint32_t)(s + 1);
+    const uint32_t want4 = (uint32_t)(s % IPR_REC4_WINDOW) + 1u;
 
     /* large broadcasters: one workgroup per chunk of large_chunk arcs */
     const uint64_t lp = ctl->lpack;
@@ -577,13 +605,14 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_expand(ipr_dev d)
             if (d.lbase[mid] <= c) lo = mid; else hi = mid;
         }
         uint32_t v = d.lbins[lo];
-        ipr_bslot_t b = bs[ipr_brec(d, v)];
-        bool wf = (b.step & IPR_BSTEP_WEIGHTED) != 0;
+        IP_MESSAGE_TYPE bm;
+        bool wf;
+        ipr_bcast_at(d, ipr_brec(d, v), want, want4, &bm, &wf);   /* a listed broadcaster */
         uint64_t off = d.out_idx[v], deg = d.out_idx[v + 1] - off;
         uint64_t beg = (c - d.lbase[lo]) * d.large_chunk;
         uint64_t end = beg + d.large_chunk < deg ? beg + d.large_chunk : deg;
         for (uint64_t k = beg + threadIdx.x; k < end; k += IPR_BLOCK)
-            ipr_emit(d, s, d.out_adj[off + k], ipr_edge_msg(d.out_wgt, b.msg, wf, off + k));
+            ipr_emit(d, s, d.out_adj[off + k], ipr_edge_msg(d.out_wgt, bm, wf, off + k));
     }
 
     /* the rest of the work set: 256 vertices per workgroup, their arcs laid
@@ -608,8 +637,9 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_expand(ipr_dev d)
                 off = d.out_idx[v];
                 end = d.out_idx[v + 1];
             }
-            ipr_bslot_t b = bs[ipr_brec(d, v)];
-            if ((b.step & ~IPR_BSTEP_WEIGHTED) == want) {
+            IP_MESSAGE_TYPE bm;
+            bool wf;
+            if (ipr_bcast_at(d, ipr_brec(d, v), want, want4, &bm, &wf)) {
                 if (!list) {
                     off = d.out_idx[v];
                     end = d.out_idx[v + 1];
@@ -618,8 +648,8 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_expand(ipr_dev d)
                 if (dg <= d.large_min) {
                     deg = (uint32_t)dg;
                     s_off[threadIdx.x] = off;
-                    s_msg[threadIdx.x] = b.msg;
-                    s_wf[threadIdx.x] = (b.step & IPR_BSTEP_WEIGHTED) != 0;
+                    s_msg[threadIdx.x] = bm;
+                    s_wf[threadIdx.x] = wf;
                 }
             }
         }
@@ -747,12 +777,14 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_gather_live(ipr_dev d)
     if (ctl->done) return;
     if (!ipr_gathers_live(d)) return;
     const uint32_t want = (uint32_t)(ctl->superstep + 1);
+    const uint32_t want4 = (uint32_t)(ctl->superstep % IPR_REC4_WINDOW) + 1u;
     const ipr_bslot_t* bs = (const ipr_bslot_t*)d.bslot;
     const uint64_t words = (d.V + 63) / 64;
     const uint64_t nw = (uint64_t)gridDim.x * IPR_WAVES;
     for (uint64_t wd = (uint64_t)blockIdx.x * IPR_WAVES + threadIdx.x / IPR_WAVE; wd < words; wd += nw) {
         const uint64_t p = wd * 64 + ipr_lane();
-        const bool on = p < d.V && (bs[p].step & ~IPR_BSTEP_WEIGHTED) == want;
+        const bool on = p < d.V && (d.brec4 ? (d.brec4[p] >> IPR_REC4_SHIFT) == want4
+                                            : (bs[p].step & ~IPR_BSTEP_WEIGHTED) == want);
         const uint64_t m = __ballot(on);
         if (ipr_lane() == 0) d.g_live[wd] = m;
     }
@@ -769,7 +801,7 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_gather_live(ipr_dev d)
  * the range is delivered and finished by the next wavefront. Gathering is
  * only enabled for integral MIN/MAX/SUM combiners, whose result does not
  * depend on the order or grouping of the folds. */
-__global__ void __launch_bounds__(IPR_BLOCK) ipr_k_gather(ipr_dev d)
+__global__ void __launch_bounds__(IPR_GATHER_WAVES * IPR_WAVE) ipr_k_gather(ipr_dev d)
 {
     constexpr int K = IPR_GATHER_K;
     static_assert(K % 4 == 0, "sources are read four at a time");
@@ -793,14 +825,15 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_gather(ipr_dev d)
     /* 4-byte records: twice as many of them in the same LDS */
     const uint32_t cap = b4 ? 2 * HOT : HOT;
     const uint32_t nhot = d.bpos ? (uint32_t)(d.V < cap ? d.V : cap) : 0u;
+    constexpr uint32_t NT = IPR_GATHER_WAVES * IPR_WAVE;
     if (b4) {
-        for (uint32_t h = threadIdx.x; h < nhot; h += IPR_BLOCK) hot4[h] = b4[h];
+        for (uint32_t h = threadIdx.x; h < nhot; h += NT) hot4[h] = b4[h];
     } else {
-        for (uint32_t h = threadIdx.x; h < nhot; h += IPR_BLOCK) hot[h] = bs[h];
+        for (uint32_t h = threadIdx.x; h < nhot; h += NT) hot[h] = bs[h];
     }
     __syncthreads();
     const uint64_t per = d.g_per_wave;
-    const uint64_t w0 = ((uint64_t)blockIdx.x * IPR_WAVES + threadIdx.x / IPR_WAVE) * per;
+    const uint64_t w0 = ((uint64_t)blockIdx.x * IPR_GATHER_WAVES + threadIdx.x / IPR_WAVE) * per;
     if (w0 >= d.E) return;
     const uint64_t w1 = w0 + per < d.E ? w0 + per : d.E;
     IP_MESSAGE_TYPE carry = ipr_policy::ident();
@@ -1311,7 +1344,8 @@ static int ipr_app_launch(int kernel, const ipr_dev* dp, void* stream)
         hipLaunchKernelGGL(ipr_k_swap, dim3(vblocks < gblocks ? vblocks : gblocks), dim3(IPR_BLOCK), 0, st, d);
         break;
     case IPR_K_GATHER:
-        hipLaunchKernelGGL(ipr_k_gather, dim3(d.gather_blocks ? d.gather_blocks : 1), dim3(IPR_BLOCK), 0, st, d);
+        hipLaunchKernelGGL(ipr_k_gather, dim3(d.gather_blocks ? d.gather_blocks : 1), dim3(IPR_GATHER_WAVES * IPR_WAVE), 0,
+                           st, d);
         break;
     case IPR_K_GATHER_LIVE: {
         /* a small grid: launched every superstep, it usually exits at once */

@@ -357,18 +357,21 @@ static void dev_setup(void)
         d->g_arc = g_arc;
         d->g_own = g_own;
         d->g_dst = g_dst;
-        /* broadcasters' bitmap for gathers with few live arcs (0 turns it off) */
-        d->live_pct = env_u32("IPR_GATHER_LIVE", 60);
+        /* broadcasters' bitmap for gathers with few live arcs: below this
+         * percentage of E (0, the default, turns it off: with 4-byte records
+         * the bitmap's loads cost more than the records they skip) */
+        d->live_pct = env_u32("IPR_GATHER_LIVE", 0);
         if (d->live_pct > 0) d->g_live = dalloc(((vv + 63) / 64) * 8, "gather live bitmap");
         /* 4-byte records for the gather (0 turns them off); padded to 16
          * entries for the window clear of the frontier compaction, cleared by
          * the init kernel */
         if (env_u32("IPR_GATHER_RECORD4", 1) == 1)
             d->brec4 = dalloc(((vv + 15) / 16) * 64, "gather records");
-        /* wavefronts (4 per workgroup) own equal arc ranges, whole passes */
-        d->gather_blocks = env_u32("IPR_GATHER_BLOCKS", 8192);
+        /* wavefronts (IPR_GATHER_WAVES per workgroup) own equal arc ranges,
+         * whole passes */
+        d->gather_blocks = env_u32("IPR_GATHER_BLOCKS", 32768 / IPR_GATHER_WAVES);
         if (d->gather_blocks < 1) d->gather_blocks = 1;
-        const uint64_t nw = (uint64_t)d->gather_blocks * 4;
+        const uint64_t nw = (uint64_t)d->gather_blocks * IPR_GATHER_WAVES;
         uint64_t per = (E + nw - 1) / nw;
         per = (per + pass - 1) / pass * pass;
         d->g_per_wave = per ? per : pass;

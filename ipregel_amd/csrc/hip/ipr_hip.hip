@@ -501,16 +501,18 @@ extern "C" int ipr_hip_build_pull_bins(ipr_dev* dev, uint32_t small_max, uint32_
 /* ------------------------------------------------------------------------ */
 
 /* The received flags of superstep s+1 become the frontier list. A workgroup
- * takes IPR_COMPACT_T rounds of 256 consecutive 16-flag groups (coalesced
- * loads; the flags are cleared for reuse two supersteps on). Positions follow
- * (round, thread) order, which is ascending vertex order: one workgroup scan
- * per round, one atomic per workgroup on the list counter, and neighbouring
- * threads write neighbouring runs. Few workgroups means few atomics on the
- * one counter, whose contention bounds this kernel on large graphs. */
-#define IPR_COMPACT_T 8
+ * takes T rounds of 256 consecutive 16-flag groups (coalesced loads; the
+ * flags are cleared for reuse two supersteps on). Positions follow (round,
+ * thread) order, which is ascending vertex order: one workgroup scan per
+ * round and one atomic per workgroup on the list counter. Each round's
+ * entries are staged in LDS and written out by the whole workgroup, 64
+ * consecutive entries per wavefront store. T is picked per graph so that the
+ * grid has about 1024 workgroups or more. */
+template <int T>
 __global__ void __launch_bounds__(256) k_compact(ipr_dev d)
 {
-    __shared__ uint32_t wsum[IPR_COMPACT_T][4];
+    __shared__ uint32_t wsum[T][4];
+    __shared__ uint32_t stage[256 * 16];
     __shared__ unsigned long long blk_base;
     ipr_ctl* c = d.ctl;
     if (c->done) return;
@@ -519,16 +521,16 @@ __global__ void __launch_bounds__(256) k_compact(ipr_dev d)
     uint32_t* out = d.front[(s + 1) & 1];
     unsigned long long* counter = (unsigned long long*)&c->fsize[(s + 1) & 1];
     const uint64_t groups = (d.V + 15) / 16;           /* flag arrays are padded to 16 */
-    const uint64_t span = 256ull * IPR_COMPACT_T;
+    const uint64_t span = 256ull * T;
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     /* the last superstep of a stamp window clears the 4-byte gather records
      * (padded to 16 like the flags) */
     uint4* clr = d.brec4 && (s + 1) % IPR_REC4_WINDOW == 0 ? (uint4*)d.brec4 : nullptr;
     for (uint64_t b0 = (uint64_t)blockIdx.x * span; b0 < groups; b0 += (uint64_t)gridDim.x * span) {
-        uint4 q[IPR_COMPACT_T];
-        uint32_t n[IPR_COMPACT_T], incl[IPR_COMPACT_T];
+        uint4 q[T];
+        uint32_t n[T], incl[T];
 #pragma unroll
-        for (int t = 0; t < IPR_COMPACT_T; t++) {
+        for (int t = 0; t < T; t++) {
             const uint64_t g = b0 + (uint64_t)t * 256 + threadIdx.x;
             q[t] = make_uint4(0, 0, 0, 0);
             if (g < groups) q[t] = flags[g];
@@ -541,7 +543,7 @@ __global__ void __launch_bounds__(256) k_compact(ipr_dev d)
             }
         }
 #pragma unroll
-        for (int t = 0; t < IPR_COMPACT_T; t++) {
+        for (int t = 0; t < T; t++) {
             /* flags are 0 or 1, so each set flag is one bit (0, 8, 16 or 24) */
             n[t] = __popc(q[t].x) + __popc(q[t].y) + __popc(q[t].z) + __popc(q[t].w);
             if (n[t]) flags[b0 + (uint64_t)t * 256 + threadIdx.x] = make_uint4(0, 0, 0, 0);
@@ -555,46 +557,63 @@ __global__ void __launch_bounds__(256) k_compact(ipr_dev d)
         }
         __syncthreads();
         uint32_t total = 0;
-        uint32_t before[IPR_COMPACT_T];
+        uint32_t first[T], cnt[T], before[T];
 #pragma unroll
-        for (int t = 0; t < IPR_COMPACT_T; t++) {
-            uint32_t b = total;
+        for (int t = 0; t < T; t++) {
+            first[t] = total;
+            uint32_t b = 0;
             for (uint32_t w = 0; w < 4; w++) {
                 if (w < wave) b += wsum[t][w];
                 total += wsum[t][w];
             }
             before[t] = b;
+            cnt[t] = total - first[t];
         }
         if (threadIdx.x == 0 && total) blk_base = atomicAdd(counter, (unsigned long long)total);
         __syncthreads();
         if (total) {
+            const uint64_t base = blk_base;
 #pragma unroll
-            for (int t = 0; t < IPR_COMPACT_T; t++) {
-                if (!n[t]) continue;
-                uint64_t pos = blk_base + before[t] + incl[t] - n[t];
+            for (int t = 0; t < T; t++) {
+                if (!cnt[t]) continue;   /* uniform across the workgroup */
+                uint32_t pos = before[t] + incl[t] - n[t];
                 const uint64_t g = b0 + (uint64_t)t * 256 + threadIdx.x;
                 const uint32_t words[4] = {q[t].x, q[t].y, q[t].z, q[t].w};
                 for (int k = 0; k < 4; k++) {
                     uint32_t w = words[k];
                     while (w) {
                         uint32_t bit = __ffs(w) - 1;
-                        out[pos++] = (uint32_t)(g * 16 + k * 4 + bit / 8);
+                        stage[pos++] = (uint32_t)(g * 16 + k * 4 + bit / 8);
                         w &= w - 1;
                     }
                 }
+                __syncthreads();
+                for (uint32_t i = threadIdx.x; i < cnt[t]; i += 256) out[base + first[t] + i] = stage[i];
+                __syncthreads();
             }
         }
-        __syncthreads();
     }
 }
 
 extern "C" int ipr_hip_compact_frontier(const ipr_dev* dev, void* stream)
 {
-    const uint64_t span = 256ull * IPR_COMPACT_T;
-    uint64_t blocks = ((dev->V + 15) / 16 + span - 1) / span;
+    const uint64_t groups = (dev->V + 15) / 16;
+    /* rounds per workgroup: as many as keep the grid at 1024 workgroups or
+     * more (fewer workgroups: fewer counter atomics, longer serial work) */
+    int T = 8;
+    while (T > 1 && groups < 1024ull * 256 * T) T >>= 1;
+    const uint64_t span = 256ull * T;
+    uint64_t blocks = (groups + span - 1) / span;
     if (blocks < 1) blocks = 1;
     if (blocks > 16384) blocks = 16384;
-    hipLaunchKernelGGL(k_compact, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, *dev);
+    const dim3 grid((unsigned)blocks), block(256);
+    hipStream_t st = (hipStream_t)stream;
+    switch (T) {
+    case 8: hipLaunchKernelGGL(k_compact<8>, grid, block, 0, st, *dev); break;
+    case 4: hipLaunchKernelGGL(k_compact<4>, grid, block, 0, st, *dev); break;
+    case 2: hipLaunchKernelGGL(k_compact<2>, grid, block, 0, st, *dev); break;
+    default: hipLaunchKernelGGL(k_compact<1>, grid, block, 0, st, *dev); break;
+    }
     CK(hipGetLastError());
     return 0;
 }
