@@ -62,7 +62,9 @@ enum {
     IPR_C_EDGES = 3,     /* out-arcs of this superstep's broadcasters (W) */
     IPR_C_BCAST0 = 4,    /* broadcasts, superstep parity 0 */
     IPR_C_BCAST1 = 5,    /* ... parity 1 */
-    IPR_NCNT = 6
+    IPR_C_GNEXT = 6,     /* gather range counters: IPR_GATHER_CTRS of them, 128 bytes apart
+                          * (spans this id and the next; zeroed at every superstep's end) */
+    IPR_NCNT = 8
 };
 
 /* device-resident control block: every kernel reads the superstep and the
@@ -129,8 +131,8 @@ typedef struct ipr_dev {
      * broadcast record at the same position as in bslot (IPR_REC4_* layout),
      * read by the gather in place of the 8-byte record (NULL: off) */
     uint32_t* brec4;
-    uint64_t g_per_wave;
-    uint32_t gather_blocks;
+    uint64_t g_per_wave;         /* arcs per range (whole passes) */
+    uint32_t gather_blocks;      /* resident gather workgroups; their wavefronts take ranges in turn */
     uint32_t g_pad;
     void* val;
     void* msg;
@@ -164,6 +166,8 @@ typedef struct ipr_dev {
  * then takes the message from the 8-byte record). The stamps of a window of
  * IPR_REC4_WINDOW supersteps are distinct; every record is cleared to 0 at
  * the end of a window's last superstep, so a stale record never matches. */
+/* gather range counters (workgroup b takes ranges from counter b mod 8) */
+#define IPR_GATHER_CTRS 8
 /* wavefronts per gather workgroup */
 #ifndef IPR_GATHER_WAVES
 #define IPR_GATHER_WAVES 4

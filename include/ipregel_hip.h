@@ -36,6 +36,8 @@ const char* ipr_hip_last_error(void);
 
 int ipr_hip_init(int device, char* name, size_t name_len, uint64_t* hbm_bytes);
 int ipr_hip_wallclock_hz(uint64_t* hz);
+/* compute units of the current device */
+int ipr_hip_cu_count(uint32_t* cus);
 
 int ipr_hip_malloc(void** p, size_t bytes);
 int ipr_hip_free(void* p);

@@ -792,13 +792,16 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_gather_live(ipr_dev d)
 
 /* Dense supersteps of push applications: every receiver folds the records of
  * the in-neighbours that broadcast in superstep s, over the exact transpose.
- * Arc-balanced, so hubs need no special case: wavefront w owns the arcs
- * [w*g_per_wave, (w+1)*g_per_wave) and takes them 64*K at a time; lane L folds
+ * Arc-balanced, so hubs need no special case: the arcs split into ranges of
+ * g_per_wave; one resident round of workgroups is launched, and each wavefront
+ * takes ranges from a counter until none is left. A range is taken 64*K arcs
+ * at a time; lane L folds
  * the K consecutive arcs base + L*K (sources by vector loads, then their K
  * records together), delivers the in-lists that end inside its K arcs, and one
  * segmented scan across the lanes joins the lists that span lanes. The list
  * still open after a pass carries into the next; the one open at the end of
- * the range is delivered and finished by the next wavefront. Gathering is
+ * the range is delivered and finished by whichever wavefront takes the next
+ * range. Gathering is
  * only enabled for integral MIN/MAX/SUM combiners, whose result does not
  * depend on the order or grouping of the folds. */
 __global__ void __launch_bounds__(IPR_GATHER_WAVES * IPR_WAVE) ipr_k_gather(ipr_dev d)
@@ -833,143 +836,156 @@ __global__ void __launch_bounds__(IPR_GATHER_WAVES * IPR_WAVE) ipr_k_gather(ipr_
     }
     __syncthreads();
     const uint64_t per = d.g_per_wave;
-    const uint64_t w0 = ((uint64_t)blockIdx.x * IPR_GATHER_WAVES + threadIdx.x / IPR_WAVE) * per;
-    if (w0 >= d.E) return;
-    const uint64_t w1 = w0 + per < d.E ? w0 + per : d.E;
-    IP_MESSAGE_TYPE carry = ipr_policy::ident();
-    int carry_has = 0;
-    uint32_t carry_o = 0;
-    for (uint64_t base = w0; base < w1; base += (uint64_t)IPR_WAVE * K) {
-        const uint64_t t0 = base + (uint64_t)lane * K;
-        uint32_t a[K];
-        if (t0 + K <= w1) {
-            /* the source stream is read once: keep it out of L2, which holds
-             * the randomly read broadcast records */
+    const uint64_t nranges = (d.E + per - 1) / per;
+    /* ranges gc, gc + G, gc + 2G ... come from range counter gc: G counters on
+     * their own cache lines share the atomics (zeroed at every superstep's
+     * end) */
+    const uint32_t G = gridDim.x < IPR_GATHER_CTRS ? gridDim.x : IPR_GATHER_CTRS;
+    const uint32_t gc = blockIdx.x % G;
+    unsigned long long* ctr = (unsigned long long*)&d.cnt[IPR_C_GNEXT * IPR_CSLOTS + gc * 16];
+    for (;;) {
+        unsigned long long k = 0;
+        if (lane == 0) k = atomicAdd(ctr, 1ull);
+        k = __shfl(k, 0);
+        const uint64_t rg = (uint64_t)k * G + gc;
+        if (rg >= nranges) break;
+        const uint64_t w0 = (uint64_t)rg * per;
+        const uint64_t w1 = w0 + per < d.E ? w0 + per : d.E;
+        IP_MESSAGE_TYPE carry = ipr_policy::ident();
+        int carry_has = 0;
+        uint32_t carry_o = 0;
+        for (uint64_t base = w0; base < w1; base += (uint64_t)IPR_WAVE * K) {
+            const uint64_t t0 = base + (uint64_t)lane * K;
+            uint32_t a[K];
+            if (t0 + K <= w1) {
+                /* the source stream is read once: keep it out of L2, which holds
+                 * the randomly read broadcast records */
 #pragma unroll
-            for (int j = 0; j < K; j += 4) {
-                const ipr_u32x4 q = __builtin_nontemporal_load((const ipr_u32x4*)(d.g_arc + t0 + j));
-                a[j] = q[0]; a[j + 1] = q[1]; a[j + 2] = q[2]; a[j + 3] = q[3];
+                for (int j = 0; j < K; j += 4) {
+                    const ipr_u32x4 q = __builtin_nontemporal_load((const ipr_u32x4*)(d.g_arc + t0 + j));
+                    a[j] = q[0]; a[j + 1] = q[1]; a[j + 2] = q[2]; a[j + 3] = q[3];
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < K; j++) a[j] = t0 + j < w1 ? d.g_arc[t0 + j] : 0u;
             }
-        } else {
+            const uint32_t o = t0 < w1 ? d.g_own[t0 / K] : 0u;   /* receiver of arc t0 */
+            /* the K records: ok[j] bit 0 when source j broadcast in superstep s,
+             * bit 1 when weighted; mv[j] its message */
+            IP_MESSAGE_TYPE mv[K];
+            uint32_t ok[K];
+            uint64_t w[K];
+            if (live) {
+                /* the bitmap (V bits, cache-resident) first; records only of
+                 * sources that broadcast */
 #pragma unroll
-            for (int j = 0; j < K; j++) a[j] = t0 + j < w1 ? d.g_arc[t0 + j] : 0u;
-        }
-        const uint32_t o = t0 < w1 ? d.g_own[t0 / K] : 0u;   /* receiver of arc t0 */
-        /* the K records: ok[j] bit 0 when source j broadcast in superstep s,
-         * bit 1 when weighted; mv[j] its message */
-        IP_MESSAGE_TYPE mv[K];
-        uint32_t ok[K];
-        uint64_t w[K];
-        if (live) {
-            /* the bitmap (V bits, cache-resident) first; records only of
-             * sources that broadcast */
-#pragma unroll
-            for (int j = 0; j < K; j++) w[j] = d.g_live[(a[j] & ~IPR_GATHER_LAST) >> 6];
-        }
-        if (b4) {
-            uint32_t q[K];
-#pragma unroll
-            for (int j = 0; j < K; j++) {
-                const uint32_t p = a[j] & ~IPR_GATHER_LAST;
-                q[j] = 0;
-                if (!live || ((w[j] >> (a[j] & 63)) & 1ull)) q[j] = p < nhot ? hot4[p] : b4[p];
+                for (int j = 0; j < K; j++) w[j] = d.g_live[(a[j] & ~IPR_GATHER_LAST) >> 6];
             }
+            if (b4) {
+                uint32_t q[K];
 #pragma unroll
-            for (int j = 0; j < K; j++) {
-                const uint32_t low = q[j] & IPR_REC4_ESC;
-                ok[j] = (q[j] >> IPR_REC4_SHIFT) == want4 ? 1u | ((q[j] & IPR_REC4_WEIGHTED) ? 2u : 0u) : 0u;
-                mv[j] = (IP_MESSAGE_TYPE)low;
-                if (ok[j] && low == IPR_REC4_ESC) mv[j] = bs[a[j] & ~IPR_GATHER_LAST].msg;
-            }
-        } else {
-            ipr_bslot_t r[K];
+                for (int j = 0; j < K; j++) {
+                    const uint32_t p = a[j] & ~IPR_GATHER_LAST;
+                    q[j] = 0;
+                    if (!live || ((w[j] >> (a[j] & 63)) & 1ull)) q[j] = p < nhot ? hot4[p] : b4[p];
+                }
 #pragma unroll
-            for (int j = 0; j < K; j++) {
-                const uint32_t p = a[j] & ~IPR_GATHER_LAST;
-                r[j].step = 0;
-                if (!live || ((w[j] >> (a[j] & 63)) & 1ull)) r[j] = p < nhot ? hot[p] : bs[p];
-            }
+                for (int j = 0; j < K; j++) {
+                    const uint32_t low = q[j] & IPR_REC4_ESC;
+                    ok[j] = (q[j] >> IPR_REC4_SHIFT) == want4 ? 1u | ((q[j] & IPR_REC4_WEIGHTED) ? 2u : 0u) : 0u;
+                    mv[j] = (IP_MESSAGE_TYPE)low;
+                    if (ok[j] && low == IPR_REC4_ESC) mv[j] = bs[a[j] & ~IPR_GATHER_LAST].msg;
+                }
+            } else {
+                ipr_bslot_t r[K];
 #pragma unroll
-            for (int j = 0; j < K; j++) {
-                ok[j] = (r[j].step & ~IPR_BSTEP_WEIGHTED) == want
-                            ? 1u | ((r[j].step & IPR_BSTEP_WEIGHTED) ? 2u : 0u) : 0u;
-                mv[j] = r[j].msg;
+                for (int j = 0; j < K; j++) {
+                    const uint32_t p = a[j] & ~IPR_GATHER_LAST;
+                    r[j].step = 0;
+                    if (!live || ((w[j] >> (a[j] & 63)) & 1ull)) r[j] = p < nhot ? hot[p] : bs[p];
+                }
+#pragma unroll
+                for (int j = 0; j < K; j++) {
+                    ok[j] = (r[j].step & ~IPR_BSTEP_WEIGHTED) == want
+                                ? 1u | ((r[j].step & IPR_BSTEP_WEIGHTED) ? 2u : 0u) : 0u;
+                    mv[j] = r[j].msg;
+                }
             }
-        }
 #ifdef IP_NEEDS_OUT_NEIGHBOUR_WEIGHTS
-        uint32_t wt[K];
-        if (t0 + K <= w1) {
+            uint32_t wt[K];
+            if (t0 + K <= w1) {
 #pragma unroll
-            for (int j = 0; j < K; j += 4) {
-                const ipr_u32x4 q = __builtin_nontemporal_load((const ipr_u32x4*)(d.t_wgt + t0 + j));
-                wt[j] = q[0]; wt[j + 1] = q[1]; wt[j + 2] = q[2]; wt[j + 3] = q[3];
+                for (int j = 0; j < K; j += 4) {
+                    const ipr_u32x4 q = __builtin_nontemporal_load((const ipr_u32x4*)(d.t_wgt + t0 + j));
+                    wt[j] = q[0]; wt[j + 1] = q[1]; wt[j + 2] = q[2]; wt[j + 3] = q[3];
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < K; j++) wt[j] = t0 + j < w1 ? d.t_wgt[t0 + j] : 0u;
             }
-        } else {
-#pragma unroll
-            for (int j = 0; j < K; j++) wt[j] = t0 + j < w1 ? d.t_wgt[t0 + j] : 0u;
-        }
 #endif
-        /* head: the part of the list open on entry (it began in an earlier
-         * lane); fv/fh: lists that begin and end in these K arcs */
-        IP_MESSAGE_TYPE head = ipr_policy::ident(), cur = ipr_policy::ident();
-        int head_has = 0, cur_has = 0;
-        uint32_t nb = 0;
-        IP_MESSAGE_TYPE fv[K];
-        int fh[K];
+            /* head: the part of the list open on entry (it began in an earlier
+             * lane); fv/fh: lists that begin and end in these K arcs */
+            IP_MESSAGE_TYPE head = ipr_policy::ident(), cur = ipr_policy::ident();
+            int head_has = 0, cur_has = 0;
+            uint32_t nb = 0;
+            IP_MESSAGE_TYPE fv[K];
+            int fh[K];
 #pragma unroll
-        for (int j = 0; j < K; j++) {
-            const bool live = t0 + j < w1;
-            if (live && ok[j]) {
-                IP_MESSAGE_TYPE m = mv[j];
+            for (int j = 0; j < K; j++) {
+                const bool live = t0 + j < w1;
+                if (live && ok[j]) {
+                    IP_MESSAGE_TYPE m = mv[j];
 #ifdef IP_NEEDS_OUT_NEIGHBOUR_WEIGHTS
-                if (ok[j] & 2u) m = (IP_MESSAGE_TYPE)(m + (IP_MESSAGE_TYPE)wt[j]);
+                    if (ok[j] & 2u) m = (IP_MESSAGE_TYPE)(m + (IP_MESSAGE_TYPE)wt[j]);
 #endif
-                ipr_policy::fold(&cur, m);
-                cur_has = 1;
+                    ipr_policy::fold(&cur, m);
+                    cur_has = 1;
+                }
+                fv[j] = cur;
+                fh[j] = 0;
+                if (live && (a[j] & IPR_GATHER_LAST)) {
+                    if (nb == 0) { head = cur; head_has = cur_has; }
+                    else fh[j] = cur_has;
+                    nb++;
+                    cur = ipr_policy::ident();
+                    cur_has = 0;
+                }
             }
-            fv[j] = cur;
-            fh[j] = 0;
-            if (live && (a[j] & IPR_GATHER_LAST)) {
-                if (nb == 0) { head = cur; head_has = cur_has; }
-                else fh[j] = cur_has;
-                nb++;
-                cur = ipr_policy::ident();
-                cur_has = 0;
-            }
-        }
-        uint32_t k = 0;
+            uint32_t k = 0;
 #pragma unroll
-        for (int j = 0; j < K; j++) {
-            if (t0 + j < w1 && (a[j] & IPR_GATHER_LAST)) {
-                if (fh[j]) ipr_gather_flush(d, s, d.g_dst[o + k], fv[j]);
-                k++;
+            for (int j = 0; j < K; j++) {
+                if (t0 + j < w1 && (a[j] & IPR_GATHER_LAST)) {
+                    if (fh[j]) ipr_gather_flush(d, s, d.g_dst[o + k], fv[j]);
+                    k++;
+                }
             }
+            /* lanes without a list end extend the list open before them */
+            const int f = nb > 0;
+            if (lane == 0 && !f) { ipr_policy::fold(&cur, carry); cur_has |= carry_has; }
+            IP_MESSAGE_TYPE S = cur;
+            int Sh = cur_has, F = f;
+            for (int dd = 1; dd < IPR_WAVE; dd <<= 1) {
+                const IP_MESSAGE_TYPE y = __shfl_up(S, dd);
+                const int yh = __shfl_up(Sh, dd), yf = __shfl_up(F, dd);
+                if (lane >= (uint32_t)dd && !F) { ipr_policy::fold(&S, y); Sh |= yh; F |= yf; }
+            }
+            IP_MESSAGE_TYPE in = __shfl_up(S, 1);
+            int inh = __shfl_up(Sh, 1);
+            if (lane == 0) { in = carry; inh = carry_has; }
+            if (f && (head_has | inh)) {
+                ipr_policy::fold(&head, in);
+                ipr_gather_flush(d, s, d.g_dst[o], head);
+            }
+            /* the last lane with arcs holds the list still open */
+            const uint64_t left = w1 - base;
+            const int last = left >= (uint64_t)IPR_WAVE * K ? IPR_WAVE - 1 : (int)((left - 1) / K);
+            carry = __shfl(S, last);
+            carry_has = __shfl(Sh, last);
+            carry_o = __shfl(o + nb, last);
         }
-        /* lanes without a list end extend the list open before them */
-        const int f = nb > 0;
-        if (lane == 0 && !f) { ipr_policy::fold(&cur, carry); cur_has |= carry_has; }
-        IP_MESSAGE_TYPE S = cur;
-        int Sh = cur_has, F = f;
-        for (int dd = 1; dd < IPR_WAVE; dd <<= 1) {
-            const IP_MESSAGE_TYPE y = __shfl_up(S, dd);
-            const int yh = __shfl_up(Sh, dd), yf = __shfl_up(F, dd);
-            if (lane >= (uint32_t)dd && !F) { ipr_policy::fold(&S, y); Sh |= yh; F |= yf; }
-        }
-        IP_MESSAGE_TYPE in = __shfl_up(S, 1);
-        int inh = __shfl_up(Sh, 1);
-        if (lane == 0) { in = carry; inh = carry_has; }
-        if (f && (head_has | inh)) {
-            ipr_policy::fold(&head, in);
-            ipr_gather_flush(d, s, d.g_dst[o], head);
-        }
-        /* the last lane with arcs holds the list still open */
-        const uint64_t left = w1 - base;
-        const int last = left >= (uint64_t)IPR_WAVE * K ? IPR_WAVE - 1 : (int)((left - 1) / K);
-        carry = __shfl(S, last);
-        carry_has = __shfl(Sh, last);
-        carry_o = __shfl(o + nb, last);
+        if (lane == 0 && carry_has) ipr_gather_flush(d, s, d.g_dst[carry_o], carry);
     }
-    if (lane == 0 && carry_has) ipr_gather_flush(d, s, d.g_dst[carry_o], carry);
 }
 
 /* PUSH: src/combiner_postamble.h:263-278 - a delivered mailbox becomes the

@@ -367,12 +367,16 @@ static void dev_setup(void)
          * the init kernel */
         if (env_u32("IPR_GATHER_RECORD4", 1) == 1)
             d->brec4 = dalloc(((vv + 15) / 16) * 64, "gather records");
-        /* wavefronts (IPR_GATHER_WAVES per workgroup) own equal arc ranges,
-         * whole passes */
-        d->gather_blocks = env_u32("IPR_GATHER_BLOCKS", 32768 / IPR_GATHER_WAVES);
+        /* the arcs split into ranges of about 2048 arcs (whole passes; four
+         * passes keep the range counters' atomics rare at any E while the
+         * ranges still balance); the wavefronts of one resident round of
+         * workgroups (5 per CU: 32 KB of LDS and 96 VGPRs each) take them in
+         * turn */
+        uint32_t cus = 1;
+        hip_check(ipr_hip_cu_count(&cus), "compute units");
+        d->gather_blocks = env_u32("IPR_GATHER_BLOCKS", cus * 5);
         if (d->gather_blocks < 1) d->gather_blocks = 1;
-        const uint64_t nw = (uint64_t)d->gather_blocks * IPR_GATHER_WAVES;
-        uint64_t per = (E + nw - 1) / nw;
+        uint64_t per = env_u32("IPR_GATHER_RANGE_ARCS", 2048);
         per = (per + pass - 1) / pass * pass;
         d->g_per_wave = per ? per : pass;
     }

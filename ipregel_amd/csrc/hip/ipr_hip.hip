@@ -44,6 +44,15 @@ extern "C" int ipr_hip_init(int device, char* name, size_t name_len, uint64_t* h
     return 0;
 }
 
+extern "C" int ipr_hip_cu_count(uint32_t* cus)
+{
+    int dev = 0, n = 0;
+    CK(hipGetDevice(&dev));
+    CK(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+    *cus = n > 0 ? (uint32_t)n : 1u;
+    return 0;
+}
+
 extern "C" int ipr_hip_wallclock_hz(uint64_t* hz)
 {
     int dev = 0, khz = 0;
@@ -700,6 +709,9 @@ __global__ void k_finalize(ipr_dev d, int variant)
                                              : c->fsize[(s + 1) & 1];
         edges = ipr_take(d, IPR_C_EDGES, true);
         computed = ipr_take(d, IPR_C_WORKV, true);
+        /* the gather's range counters start from 0 in the next superstep */
+        d.cnt[IPR_C_GNEXT * IPR_CSLOTS + threadIdx.x] = 0;
+        d.cnt[(IPR_C_GNEXT + 1) * IPR_CSLOTS + threadIdx.x] = 0;
     }
     if (threadIdx.x != 0) return;
     if (variant == IPR_VARIANT_PULL) {

@@ -112,15 +112,22 @@ def test_cc_tuning_does_not_change_results(tune):
     assert np.array_equal(r.values, ref)
 
 
-@pytest.mark.parametrize("blocks,order,live", [("1", "1", "60"), ("3", "0", "0"), ("8192", "1", "100"),
-                                               ("8192", "0", "60")])
-def test_gather_every_superstep(blocks, order, live, monkeypatch):
+@pytest.mark.parametrize("blocks,order,live,ranges", [("1", "1", "60", "2048"), ("3", "0", "0", "100000"),
+                                                      ("8192", "1", "100", "512"), ("8192", "0", "60", "4000000"),
+                                                      ("", "1", "0", "")])
+def test_gather_every_superstep(blocks, order, live, ranges, monkeypatch):
     """Dense-superstep gather forced on every superstep (dir_alpha huge), with
-    1, 3 and 8192 workgroups: in-lists split across lanes, passes and
-    wavefronts, hubs, isolated vertices; symmetric and directed transposes,
+    1, 3, 8192 or the default number of workgroups, and arc ranges of one
+    pass (512 arcs) up to the whole graph, taken from the range counters:
+    in-lists split across lanes, passes and ranges, hubs, isolated vertices;
+    symmetric and directed transposes,
     with and without weights; broadcast records by vertex id or by degree;
     the broadcasters' bitmap off, below 60% live arcs, or always."""
-    monkeypatch.setenv("IPR_GATHER_BLOCKS", blocks)
+    for k, v in (("IPR_GATHER_BLOCKS", blocks), ("IPR_GATHER_RANGE_ARCS", ranges)):
+        if v:
+            monkeypatch.setenv(k, v)
+        else:
+            monkeypatch.delenv(k, raising=False)
     monkeypatch.setenv("IPR_RECORD_ORDER", order)
     monkeypatch.setenv("IPR_GATHER_LIVE", live)
     g = G.rmat(13, 16, seed=5, symmetrise=True)
