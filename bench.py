@@ -98,9 +98,9 @@ TUNING = {"pr16": {"pull_ordered": 2}}
 # mean over the dense launches); commands and per-launch figures are in
 # profiles/ipr_k_gather_cc_rmat22.md, ipr_k_gather_cc_rmat24.md and
 # ipr_k_gather_sssp_rmat24.md
-TRAFFIC = {("cc22", "ipr_k_gather"): (755_178 * 2 + 49_022) * 1024,
-           ("cc24", "ipr_k_gather"): (7_926_598 * 2 + 212_748) * 1024,
-           ("ss24a", "ipr_k_gather"): (4_051_680 * 2 + 159_614) * 1024}
+TRAFFIC = {("cc22", "ipr_k_gather"): (767_391 * 2 + 57_213) * 1024,
+           ("cc24", "ipr_k_gather"): (7_922_027 * 2 + 246_882) * 1024,
+           ("ss24a", "ipr_k_gather"): (4_068_136 * 2 + 182_677) * 1024}
 
 
 def wl_ss24a():

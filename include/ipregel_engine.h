@@ -89,7 +89,9 @@ typedef struct ipr_ctl {
 
 #define IPR_LPACK_SHIFT 40
 /* arc-balanced gather: arcs per lane per pass, and the end-of-list mark */
+#ifndef IPR_GATHER_K
 #define IPR_GATHER_K 8
+#endif
 #define IPR_GATHER_LAST 0x80000000u
 #define IPR_LPACK_MASK ((1ull << IPR_LPACK_SHIFT) - 1)
 
