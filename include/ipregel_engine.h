@@ -135,7 +135,7 @@ typedef struct ipr_dev {
     uint32_t* brec4;
     uint64_t g_per_wave;         /* arcs per range (whole passes) */
     uint32_t gather_blocks;      /* resident gather workgroups; their wavefronts take ranges in turn */
-    uint32_t g_pad;
+    uint32_t compact_rounds;     /* frontier compaction rounds per workgroup: 1, 2, 4 or 8 (0: by V) */
     void* val;
     void* msg;
     uint8_t* act;

@@ -389,6 +389,8 @@ static void dev_setup(void)
         for (int b = 0; b < IPR_NBINS; b++) d->pbins[b] = dalloc(vv * 4, "pull bins");
         hip_check(ipr_hip_build_pull_bins(d, ps, pm, st), "pull bins");
     }
+    /* frontier compaction rounds per workgroup (0: picked from V) */
+    d->compact_rounds = env_u32("IPR_COMPACT_ROUNDS", 0);
     uint32_t ug = c && c->use_graph ? (uint32_t)c->use_graph : env_u32("IPR_GRAPH", 1);
     D.use_graph = ug == 1;
     D.max_chunk = c && c->max_chunk ? c->max_chunk : env_u32("IPR_MAX_CHUNK", 64);

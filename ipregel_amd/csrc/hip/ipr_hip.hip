@@ -611,6 +611,8 @@ extern "C" int ipr_hip_compact_frontier(const ipr_dev* dev, void* stream)
      * more (fewer workgroups: fewer counter atomics, longer serial work) */
     int T = 8;
     while (T > 1 && groups < 1024ull * 256 * T) T >>= 1;
+    if (dev->compact_rounds == 1 || dev->compact_rounds == 2 || dev->compact_rounds == 4 || dev->compact_rounds == 8)
+        T = (int)dev->compact_rounds;
     const uint64_t span = 256ull * T;
     uint64_t blocks = (groups + span - 1) / span;
     if (blocks < 1) blocks = 1;

@@ -147,6 +147,26 @@ def test_gather_every_superstep(blocks, order, live, ranges, monkeypatch):
     assert np.array_equal(r.values, O.min_ancestor(star))
 
 
+@pytest.mark.parametrize("rounds", ["1", "2", "4", "8"])
+def test_frontier_compaction_rounds(rounds, monkeypatch):
+    """The frontier compaction with 1, 2, 4 and 8 rounds of 4096 flags per
+    workgroup (the engine picks the count from V; small graphs get 1): CC on
+    R-MAT 16 (dense and sparse frontiers over 2-16 workgroups) and BFS on a
+    333x301 grid (V not a multiple of 16, 629 supersteps), with the
+    active-count traces against the oracle's."""
+    monkeypatch.setenv("IPR_COMPACT_ROUNDS", rounds)
+    g = G.rmat(16, 16, seed=2, symmetrise=True)
+    ref = O.run(g, O.CC, O.SPREAD)
+    r = EN.run_app("cc_spread", g)
+    assert np.array_equal(r.values, ref.values)
+    assert list(r.active) == list(ref.active[: ref.supersteps])
+    gg = G.grid(333, 301)
+    r = EN.run_app("sssp_spread", gg, source=5)
+    assert np.array_equal(r.values, O.bfs(gg, 5))
+    ref = O.run(gg, O.SSSP, O.SPREAD, source=5)
+    assert list(r.active) == list(ref.active[: ref.supersteps])
+
+
 @pytest.mark.parametrize("record4", ["1", "0"])
 def test_gather_stamp_window_and_escapes(record4, monkeypatch):
     """The gather forced on every superstep for more supersteps than the
