@@ -1119,6 +1119,37 @@ __device__ inline bool ipr_fetch_ordered_wave(const ipr_dev& d, uint32_t v, cons
  * Otherwise (declared combiner, reassociating): bin 0 the lane-cooperative
  * segmented fold, bin 1 a wavefront and bin 2 a workgroup per vertex with
  * tree reductions. */
+/* reassociating fold of one in-list slice: this thread takes the arcs
+ * k0, k0 + STRIDE, ... below end, four at a time: their source ids first,
+ * then the four records, so that no load waits on another of its kind */
+template <uint32_t STRIDE>
+__device__ inline void ipr_pull_chunks(const ipr_dev& d, const ipr_bslot_t* pr, uint32_t want, uint64_t k0,
+                                       uint64_t end, IP_MESSAGE_TYPE* acc, bool* has)
+{
+    constexpr int U = 4;
+    for (uint64_t k = k0; k < end; k += (uint64_t)STRIDE * U) {
+        uint32_t src[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const uint64_t ku = k + (uint64_t)u * STRIDE;
+            src[u] = ku < end ? d.in_adj[ku] : 0u;
+        }
+        ipr_bslot_t r[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            r[u].step = 0;
+            if (k + (uint64_t)u * STRIDE < end) r[u] = pr[src[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (r[u].step == want) {
+                ipr_policy::fold(acc, r[u].msg);
+                *has = true;
+            }
+        }
+    }
+}
+
 template <bool ORDERED>
 __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_pull_step(ipr_dev d, uint32_t nb0, uint32_t nb1)
 {
@@ -1182,10 +1213,7 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_pull_step(ipr_dev d, uint32_t
             bool has = false;
             if (s > 0) {
                 uint64_t beg = d.in_idx[v], end = d.in_idx[v + 1];
-                for (uint64_t k = beg + lane; k < end; k += IPR_WAVE) {
-                    const ipr_bslot_t r = pr[d.in_adj[k]];
-                    if (r.step == want) { ipr_policy::fold(&acc, r.msg); has = true; }
-                }
+                ipr_pull_chunks<IPR_WAVE>(d, pr, want, beg + lane, end, &acc, &has);
                 for (int o = 32; o > 0; o >>= 1) {
                     IP_MESSAGE_TYPE y = __shfl_xor(acc, o);
                     ipr_policy::fold(&acc, y);
@@ -1202,10 +1230,7 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_pull_step(ipr_dev d, uint32_t
             bool has = false;
             if (s > 0) {
                 uint64_t beg = d.in_idx[v], end = d.in_idx[v + 1];
-                for (uint64_t k = beg + threadIdx.x; k < end; k += IPR_BLOCK) {
-                    const ipr_bslot_t r = pr[d.in_adj[k]];
-                    if (r.step == want) { ipr_policy::fold(&acc, r.msg); has = true; }
-                }
+                ipr_pull_chunks<IPR_BLOCK>(d, pr, want, beg + threadIdx.x, end, &acc, &has);
                 for (int o = 32; o > 0; o >>= 1) {
                     IP_MESSAGE_TYPE y = __shfl_xor(acc, o);
                     ipr_policy::fold(&acc, y);
