@@ -514,9 +514,16 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_compute_push(ipr_dev d)
                 x.active = true;
             }
             if (run) {
-                /* the broadcast bookkeeping's loads go out with the vertex's own */
-                const uint64_t o0 = d.out_idx[v], o1 = d.out_idx[v + 1];
-                const uint32_t rp = ipr_brec(d, v);
+                /* a frontier vertex's broadcast bookkeeping is loaded with its
+                 * own state (most frontier vertices broadcast); a full sweep
+                 * loads it only for the vertices that do */
+                uint64_t o0 = 0, o1 = 0;
+                uint32_t rp = 0;
+                if (list) {
+                    o0 = d.out_idx[v];
+                    o1 = d.out_idx[v + 1];
+                    rp = ipr_brec(d, v);
+                }
                 ipr_fill_vertex(d, &x, v);
                 const IP_VALUE_TYPE before = x.value;
                 const bool hm_before = x.has_message;
@@ -530,6 +537,11 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_compute_push(ipr_dev d)
                     n_active += x.active;
                 }
                 if (x.ipr_bcast) {
+                    if (!list) {
+                        o0 = d.out_idx[v];
+                        o1 = d.out_idx[v + 1];
+                        rp = ipr_brec(d, v);
+                    }
                     /* recorded even without out-arcs: with directed=false a
                      * SPREAD_PULL in-list may still name this vertex */
                     uint64_t deg = o1 - o0;
