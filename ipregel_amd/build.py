@@ -131,7 +131,7 @@ def build_all(jobs: int | None = None, verbose: bool = True) -> None:
                 print(f"built {futs[f]}_32", flush=True)
         f_batch.result()
     # the reference's own applications against this engine (needs the
-    # reference tree; the GPU box uses what was built here)
+    # reference tree; a host without it uses the binaries built here)
     from . import dropin
     dropin.build()
     # the oracle is test infrastructure with its own recipe
