@@ -515,8 +515,8 @@ extern "C" int ipr_hip_build_pull_bins(ipr_dev* dev, uint32_t small_max, uint32_
  * thread) order, which is ascending vertex order: one workgroup scan per
  * round and one atomic per workgroup on the list counter. Each round's
  * entries are staged in LDS and written out by the whole workgroup, 64
- * consecutive entries per wavefront store. T is picked per graph so that the
- * grid has about 1024 workgroups or more. */
+ * consecutive entries per wavefront store. T is 4, or less for graphs that
+ * would otherwise get under 256 workgroups. */
 template <int T>
 __global__ void __launch_bounds__(256) k_compact(ipr_dev d)
 {
@@ -607,10 +607,11 @@ __global__ void __launch_bounds__(256) k_compact(ipr_dev d)
 extern "C" int ipr_hip_compact_frontier(const ipr_dev* dev, void* stream)
 {
     const uint64_t groups = (dev->V + 15) / 16;
-    /* rounds per workgroup: as many as keep the grid at 1024 workgroups or
-     * more (fewer workgroups: fewer counter atomics, longer serial work) */
-    int T = 8;
-    while (T > 1 && groups < 1024ull * 256 * T) T >>= 1;
+    /* rounds per workgroup: 4, or fewer where that leaves under 256
+     * workgroups (fewer workgroups: fewer atomics on the one list counter,
+     * longer serial work; 4 measured best at V = 4M and 16M) */
+    int T = 4;
+    while (T > 1 && groups < 256ull * 256 * T) T >>= 1;
     if (dev->compact_rounds == 1 || dev->compact_rounds == 2 || dev->compact_rounds == 4 || dev->compact_rounds == 8)
         T = (int)dev->compact_rounds;
     const uint64_t span = 256ull * T;
