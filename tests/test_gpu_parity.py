@@ -112,17 +112,20 @@ def test_cc_tuning_does_not_change_results(tune):
     assert np.array_equal(r.values, ref)
 
 
-@pytest.mark.parametrize("blocks,order,live,ranges", [("1", "1", "60", "2048"), ("3", "0", "0", "100000"),
-                                                      ("8192", "1", "100", "512"), ("8192", "0", "60", "4000000"),
-                                                      ("", "1", "0", "")])
-def test_gather_every_superstep(blocks, order, live, ranges, monkeypatch):
+@pytest.mark.parametrize("blocks,order,live,ranges,record4", [
+    ("1", "1", "60", "2048", "1"), ("3", "0", "0", "100000", "1"), ("8192", "1", "100", "512", "1"),
+    ("8192", "0", "60", "4000000", "1"), ("", "1", "0", "", "1"), ("", "1", "60", "", "0"),
+    ("5", "0", "100", "1024", "0")])
+def test_gather_every_superstep(blocks, order, live, ranges, record4, monkeypatch):
     """Dense-superstep gather forced on every superstep (dir_alpha huge), with
     1, 3, 8192 or the default number of workgroups, and arc ranges of one
     pass (512 arcs) up to the whole graph, taken from the range counters:
     in-lists split across lanes, passes and ranges, hubs, isolated vertices;
     symmetric and directed transposes,
     with and without weights; broadcast records by vertex id or by degree;
-    the broadcasters' bitmap off, below 60% live arcs, or always."""
+    the broadcasters' bitmap off, below 60% live arcs, or always; 4-byte or
+    8-byte broadcast records."""
+    monkeypatch.setenv("IPR_GATHER_RECORD4", record4)
     for k, v in (("IPR_GATHER_BLOCKS", blocks), ("IPR_GATHER_RANGE_ARCS", ranges)):
         if v:
             monkeypatch.setenv(k, v)
