@@ -227,7 +227,7 @@ def run_workload(key: str, steps: int, warmup: int, with_cpu: bool) -> dict:
     phase_ms = (st2.kernel_ms - k0_ms) / steps
     phase_launch = (st2.kernel_launches - k0_n) / steps
     per_arc = 12 if "weighted" in app else 8
-    alpha = TUNING.get(key, {}).get("dir_alpha", 6)
+    alpha = TUNING.get(key, {}).get("dir_alpha", EN.DIR_ALPHA_DEFAULT)
     gathered = [i for i in range(len(sedges)) if int(sedges[i]) * alpha > g.E]
     if app.startswith("pagerank"):
         k_name, k_launch = "ipr_k_pull_step", phase_launch

@@ -18,6 +18,9 @@ from .graph import Graph
 
 PKG = Path(__file__).resolve().parent
 LIB = PKG / "lib"
+# the driver's push-or-gather threshold when none is given: gather when the
+# broadcasters' out-arcs times this exceed E
+DIR_ALPHA_DEFAULT = 4
 BIN = PKG / "bin"
 
 # application name -> (value dtype, takes a source argument, needs weights)
