@@ -816,7 +816,17 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_gather_live(ipr_dev d)
  * range. Gathering is
  * only enabled for integral MIN/MAX/SUM combiners, whose result does not
  * depend on the order or grouping of the folds. */
-__global__ void __launch_bounds__(IPR_GATHER_WAVES * IPR_WAVE) ipr_k_gather(ipr_dev d)
+/* at least 5 wavefronts per SIMD (at most 96 VGPRs): the occupancy its 32 KB
+ * LDS copy allows, which the launch's 5 workgroups per CU assume. With arc
+ * weights or messages wider than 4 bytes, 96 VGPRs would spill, and 4
+ * wavefronts per SIMD are kept (apps whose messages are not integral never
+ * launch it). */
+#ifdef IP_NEEDS_OUT_NEIGHBOUR_WEIGHTS
+constexpr int ipr_gather_min_waves = 4;
+#else
+constexpr int ipr_gather_min_waves = sizeof(IP_MESSAGE_TYPE) <= 4 ? 5 : 4;
+#endif
+__global__ void __launch_bounds__(IPR_GATHER_WAVES * IPR_WAVE, ipr_gather_min_waves) ipr_k_gather(ipr_dev d)
 {
     constexpr int K = IPR_GATHER_K;
     static_assert(K % 4 == 0, "sources are read four at a time");
