@@ -134,3 +134,14 @@ def test_bench_refuses_more_than_one_gpu():
                        timeout=300)
     assert r.returncode == 2
     assert "one GPU" in json.loads(r.stdout.strip().splitlines()[-1])["error"]
+
+
+def test_every_engine_variable_is_documented():
+    """Each IPR_* environment variable the native code reads has a line in
+    INTEGRATION.md's tables."""
+    read = set()
+    for f in list((ROOT / "ipregel_amd" / "csrc").rglob("*")) + list(INC.glob("*.h")):
+        if f.suffix in (".c", ".cpp", ".hip", ".h"):
+            read |= set(re.findall(r'(?:env_\w+|getenv)\("(IPR_[A-Z0-9_]+)"', f.read_text()))
+    documented = set(re.findall(r"IPR_[A-Z0-9_]+", (ROOT / "INTEGRATION.md").read_text()))
+    assert read and not read - documented, sorted(read - documented)
