@@ -93,6 +93,10 @@ typedef struct ipr_ctl {
 #define IPR_GATHER_K 8
 #endif
 #define IPR_GATHER_LAST 0x80000000u
+/* gather split by record line: classes (one per XCD), positions per class
+ * unit (the 4-byte records of one 128-byte line) */
+#define IPR_GATHER_CLASSES 8
+#define IPR_REC4_LINE 32
 #define IPR_LPACK_MASK ((1ull << IPR_LPACK_SHIFT) - 1)
 
 typedef struct ipr_dev {
@@ -134,6 +138,16 @@ typedef struct ipr_dev {
      * read by the gather in place of the 8-byte record (NULL: off) */
     uint32_t* brec4;
     uint64_t g_per_wave;         /* arcs per range (whole passes) */
+    /* g_xcd set: the gather arcs are split into IPR_GATHER_CLASSES classes by
+     * the line of the source's record; class c's arcs are g_arc[g_cbeg[c],
+     * g_cend[c]) (g_cbeg a multiple of IPR_GATHER_K), its g_own entries index
+     * g_dst + g_cdst[c], and workgroup b starts with class b mod classes */
+    uint32_t g_xcd;
+    uint32_t g_xpad;
+    const uint32_t* g_wgt;       /* g_xcd with weights: the arcs' weights in g_arc's order */
+    uint64_t g_cbeg[IPR_GATHER_CLASSES];
+    uint64_t g_cend[IPR_GATHER_CLASSES];
+    uint64_t g_cdst[IPR_GATHER_CLASSES];
     uint32_t gather_blocks;      /* resident gather workgroups; their wavefronts take ranges in turn */
     uint32_t compact_rounds;     /* frontier compaction rounds per workgroup: 1, 2, 4 or 8 (0: by V) */
     void* val;

@@ -88,6 +88,17 @@ int ipr_hip_build_gather(uint64_t V, uint64_t E, const uint64_t* t_idx, const ui
                          const uint32_t* bpos, uint32_t K, uint32_t* g_arc, uint32_t* g_own,
                          uint32_t* g_dst, uint64_t* n_dst, void* stream);
 
+/* the same index split into IPR_GATHER_CLASSES classes by the 128-byte line of
+ * the source's 4-byte record (line mod classes); per class c: arcs at
+ * g_arc[cbeg[c], cend[c]) with cbeg[c] a multiple of K, chunk owners at
+ * g_own[cbeg[c]/K ...] indexing g_dst + cdst[c]; with t_wgt and g_wgt, the
+ * arcs' weights in g_wgt at the same positions as g_arc. g_arc and g_wgt need
+ * E + classes*K entries, g_own (E + classes*K)/K, g_dst min(E, classes*V). */
+int ipr_hip_build_gather_classes(uint64_t V, uint64_t E, const uint64_t* t_idx, const uint32_t* t_adj,
+                                 const uint32_t* t_wgt, const uint32_t* bpos, uint32_t K, uint32_t* g_arc,
+                                 uint32_t* g_wgt, uint32_t* g_own, uint32_t* g_dst, uint64_t* cbeg, uint64_t* cend,
+                                 uint64_t* cdst, void* stream);
+
 /* byte-wise equality of two device buffers (multiple of 4 bytes) */
 int ipr_hip_equal(const void* a, const void* b, uint64_t bytes, int* equal, void* stream);
 

@@ -847,6 +847,10 @@ __global__ void __launch_bounds__(IPR_GATHER_WAVES * IPR_WAVE, ipr_gather_min_wa
     const uint32_t lane = ipr_lane();
     const ipr_bslot_t* bs = (const ipr_bslot_t*)d.bslot;
     const uint32_t* b4 = d.brec4;
+#ifdef IP_NEEDS_OUT_NEIGHBOUR_WEIGHTS
+    /* the arcs' weights in g_arc's order */
+    const uint32_t* tw = d.g_wgt ? d.g_wgt : d.t_wgt;
+#endif
     /* 4-byte records: twice as many of them in the same LDS */
     const uint32_t cap = b4 ? 2 * HOT : HOT;
     const uint32_t nhot = d.bpos ? (uint32_t)(d.V < cap ? d.V : cap) : 0u;
@@ -865,14 +869,41 @@ __global__ void __launch_bounds__(IPR_GATHER_WAVES * IPR_WAVE, ipr_gather_min_wa
     const uint32_t G = gridDim.x < IPR_GATHER_CTRS ? gridDim.x : IPR_GATHER_CTRS;
     const uint32_t gc = blockIdx.x % G;
     unsigned long long* ctr = (unsigned long long*)&d.cnt[IPR_C_GNEXT * IPR_CSLOTS + gc * 16];
+    /* arcs split by record line (g_xcd): workgroup b takes the ranges of class
+     * b mod classes from that class's counter (workgroups reach the XCDs in
+     * turn, so an XCD reads one share of the record lines), then helps with
+     * the other classes once its own is done */
+    uint32_t cls = d.g_xcd ? blockIdx.x % IPR_GATHER_CLASSES : 0u, ctried = 0;
     for (;;) {
-        unsigned long long k = 0;
-        if (lane == 0) k = atomicAdd(ctr, 1ull);
-        k = __shfl(k, 0);
-        const uint64_t rg = (uint64_t)k * G + gc;
-        if (rg >= nranges) break;
-        const uint64_t w0 = (uint64_t)rg * per;
-        const uint64_t w1 = w0 + per < d.E ? w0 + per : d.E;
+        uint64_t w0 = 0, w1 = 0;
+        const uint32_t* gd = d.g_dst;
+        if (d.g_xcd) {
+            bool got = false;
+            while (ctried < IPR_GATHER_CLASSES) {
+                unsigned long long k = 0;
+                if (lane == 0)
+                    k = atomicAdd((unsigned long long*)&d.cnt[IPR_C_GNEXT * IPR_CSLOTS + cls * 16], 1ull);
+                k = __shfl(k, 0);
+                w0 = d.g_cbeg[cls] + (uint64_t)k * per;
+                if (w0 < d.g_cend[cls]) {
+                    w1 = w0 + per < d.g_cend[cls] ? w0 + per : d.g_cend[cls];
+                    gd = d.g_dst + d.g_cdst[cls];
+                    got = true;
+                    break;
+                }
+                cls = (cls + 1) % IPR_GATHER_CLASSES;
+                ctried++;
+            }
+            if (!got) break;
+        } else {
+            unsigned long long k = 0;
+            if (lane == 0) k = atomicAdd(ctr, 1ull);
+            k = __shfl(k, 0);
+            const uint64_t rg = (uint64_t)k * G + gc;
+            if (rg >= nranges) break;
+            w0 = (uint64_t)rg * per;
+            w1 = w0 + per < d.E ? w0 + per : d.E;
+        }
         IP_MESSAGE_TYPE carry = ipr_policy::ident();
         int carry_has = 0;
         uint32_t carry_o = 0;
@@ -938,12 +969,12 @@ __global__ void __launch_bounds__(IPR_GATHER_WAVES * IPR_WAVE, ipr_gather_min_wa
             if (t0 + K <= w1) {
 #pragma unroll
                 for (int j = 0; j < K; j += 4) {
-                    const ipr_u32x4 q = __builtin_nontemporal_load((const ipr_u32x4*)(d.t_wgt + t0 + j));
+                    const ipr_u32x4 q = __builtin_nontemporal_load((const ipr_u32x4*)(tw + t0 + j));
                     wt[j] = q[0]; wt[j + 1] = q[1]; wt[j + 2] = q[2]; wt[j + 3] = q[3];
                 }
             } else {
 #pragma unroll
-                for (int j = 0; j < K; j++) wt[j] = t0 + j < w1 ? d.t_wgt[t0 + j] : 0u;
+                for (int j = 0; j < K; j++) wt[j] = t0 + j < w1 ? tw[t0 + j] : 0u;
             }
 #endif
             /* head: the part of the list open on entry (it began in an earlier
@@ -978,7 +1009,7 @@ __global__ void __launch_bounds__(IPR_GATHER_WAVES * IPR_WAVE, ipr_gather_min_wa
 #pragma unroll
             for (int j = 0; j < K; j++) {
                 if (t0 + j < w1 && (a[j] & IPR_GATHER_LAST)) {
-                    if (fh[j]) ipr_gather_flush(d, s, d.g_dst[o + k], fv[j]);
+                    if (fh[j]) ipr_gather_flush(d, s, gd[o + k], fv[j]);
                     k++;
                 }
             }
@@ -997,7 +1028,7 @@ __global__ void __launch_bounds__(IPR_GATHER_WAVES * IPR_WAVE, ipr_gather_min_wa
             if (lane == 0) { in = carry; inh = carry_has; }
             if (f && (head_has | inh)) {
                 ipr_policy::fold(&head, in);
-                ipr_gather_flush(d, s, d.g_dst[o], head);
+                ipr_gather_flush(d, s, gd[o], head);
             }
             /* the last lane with arcs holds the list still open */
             const uint64_t left = w1 - base;
@@ -1006,7 +1037,7 @@ __global__ void __launch_bounds__(IPR_GATHER_WAVES * IPR_WAVE, ipr_gather_min_wa
             carry_has = __shfl(Sh, last);
             carry_o = __shfl(o + nb, last);
         }
-        if (lane == 0 && carry_has) ipr_gather_flush(d, s, d.g_dst[carry_o], carry);
+        if (lane == 0 && carry_has) ipr_gather_flush(d, s, gd[carry_o], carry);
     }
 }
 

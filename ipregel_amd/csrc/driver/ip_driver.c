@@ -162,6 +162,7 @@ static void dev_free_all(void)
     ipr_hip_free((void*)d->g_arc);
     ipr_hip_free((void*)d->g_own);
     ipr_hip_free((void*)d->g_dst);
+    ipr_hip_free((void*)d->g_wgt);
     ipr_hip_free((void*)d->bpos);
     ipr_hip_free(d->g_live);
     ipr_hip_free(d->brec4);
@@ -340,10 +341,16 @@ static void dev_setup(void)
             d->t_wgt = t_wgt;
         }
         const uint64_t K = IPR_GATHER_K, pass = 64 * K;
-        uint64_t nch = (E + K - 1) / K, ndst = 0;
-        uint32_t* g_arc = dalloc(E * 4, "gather sources");
+        /* split by record line, one class per XCD (IPR_GATHER_XCD=1): each
+         * class's arcs start on a whole chunk, and its receivers get their own
+         * g_dst entries */
+        d->g_xcd = env_u32("IPR_GATHER_XCD", 0) == 1;
+        const uint64_t ea = d->g_xcd ? E + IPR_GATHER_CLASSES * K : E;
+        const uint64_t nd = d->g_xcd ? (E < IPR_GATHER_CLASSES * vv ? E : IPR_GATHER_CLASSES * vv) : vv;
+        uint64_t nch = (ea + K - 1) / K, ndst = 0;
+        uint32_t* g_arc = dalloc(ea * 4, "gather sources");
         uint32_t* g_own = dalloc(nch * 4, "gather chunk receivers");
-        uint32_t* g_dst = dalloc(vv * 4, "gather receivers");
+        uint32_t* g_dst = dalloc((nd ? nd : 1) * 4, "gather receivers");
         /* SPREAD reads records only for its frontier outside the gather, so
          * reordering them by out-degree costs little there (PUSH scans all V) */
         if (var == IPR_VARIANT_SPREAD && env_u32("IPR_RECORD_ORDER", 1) == 1) {
@@ -351,9 +358,17 @@ static void dev_setup(void)
             hip_check(ipr_hip_build_record_order(V, d->out_idx, bpos, st), "record order");
             d->bpos = bpos;
         }
-        hip_check(ipr_hip_build_gather(V, E, d->t_idx, d->t_adj, d->bpos, IPR_GATHER_K, g_arc, g_own, g_dst,
-                                       &ndst, st),
-                  "gather index");
+        if (d->g_xcd) {
+            uint32_t* g_wgt = d->t_wgt ? dalloc(ea * 4, "gather weights") : NULL;
+            hip_check(ipr_hip_build_gather_classes(V, E, d->t_idx, d->t_adj, d->t_wgt, d->bpos, IPR_GATHER_K, g_arc,
+                                                   g_wgt, g_own, g_dst, d->g_cbeg, d->g_cend, d->g_cdst, st),
+                      "gather class index");
+            d->g_wgt = g_wgt;
+        }
+        else
+            hip_check(ipr_hip_build_gather(V, E, d->t_idx, d->t_adj, d->bpos, IPR_GATHER_K, g_arc, g_own, g_dst,
+                                           &ndst, st),
+                      "gather index");
         d->g_arc = g_arc;
         d->g_own = g_own;
         d->g_dst = g_dst;

@@ -446,6 +446,133 @@ out:
     return rc;
 }
 
+/* gather split into IPR_GATHER_CLASSES classes by the cache line of the
+ * source's record (line mod classes), so that the workgroups of one XCD read a
+ * fixed share of the record lines: per class a transpose (counts, scans,
+ * scatter; in-lists keep their order), then the index above per class */
+__device__ __forceinline__ uint32_t ipr_arc_class(const uint32_t* bpos, uint32_t u)
+{
+    const uint32_t p = bpos ? bpos[u] : u;
+    return (p / IPR_REC4_LINE) % IPR_GATHER_CLASSES;
+}
+
+__global__ void k_class_count(uint64_t V, const uint64_t* t_idx, const uint32_t* t_adj, const uint32_t* bpos,
+                              uint64_t* cidx)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += stride) {
+        uint64_t n[IPR_GATHER_CLASSES] = {};
+        for (uint64_t e = t_idx[v]; e < t_idx[v + 1]; e++) n[ipr_arc_class(bpos, t_adj[e])]++;
+        for (int c = 0; c < IPR_GATHER_CLASSES; c++) cidx[(uint64_t)c * (V + 1) + v] = n[c];
+    }
+}
+
+__global__ void k_class_scatter(uint64_t V, const uint64_t* t_idx, const uint32_t* t_adj, const uint32_t* t_wgt,
+                                const uint32_t* bpos, const uint64_t* cidx, const uint64_t* abase, uint32_t* cadj,
+                                uint32_t* cwgt)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += stride) {
+        uint64_t o[IPR_GATHER_CLASSES];
+        for (int c = 0; c < IPR_GATHER_CLASSES; c++) o[c] = abase[c] + cidx[(uint64_t)c * (V + 1) + v];
+        for (uint64_t e = t_idx[v]; e < t_idx[v + 1]; e++) {
+            const uint32_t u = t_adj[e];
+            const uint64_t q = o[ipr_arc_class(bpos, u)]++;
+            cadj[q] = u;
+            if (t_wgt) cwgt[q] = t_wgt[e];
+        }
+    }
+}
+
+extern "C" int ipr_hip_build_gather_classes(uint64_t V, uint64_t E, const uint64_t* t_idx, const uint32_t* t_adj,
+                                            const uint32_t* t_wgt, const uint32_t* bpos, uint32_t K,
+                                            uint32_t* g_arc, uint32_t* g_wgt, uint32_t* g_own, uint32_t* g_dst,
+                                            uint64_t* cbeg, uint64_t* cend, uint64_t* cdst, void* stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    for (int c = 0; c < IPR_GATHER_CLASSES; c++) cbeg[c] = cend[c] = cdst[c] = 0;
+    if (E == 0 || V == 0) return 0;
+    if (K == 0 || K % 4) {
+        snprintf(g_err, sizeof(g_err), "gather classes: K=%u is not a multiple of 4", K);
+        return -1;
+    }
+    const uint64_t nc = IPR_GATHER_CLASSES;
+    uint64_t *ccnt = nullptr, *cidx = nullptr, *abase_d = nullptr;
+    uint32_t *cadj = nullptr, *cwgt = nullptr;
+    void* tmp = nullptr;
+    size_t tmp_bytes = 0;
+    uint64_t ec[IPR_GATHER_CLASSES], abase[IPR_GATHER_CLASSES];
+    int rc = 0;
+    hipError_t e;
+    uint64_t vb = (V + 255) / 256;
+    if (vb > 65536) vb = 65536;
+    if ((e = hipMalloc(&ccnt, nc * (V + 1) * 8)) != hipSuccess) { rc = fail(e, "gather class counts"); goto out; }
+    if ((e = hipMalloc(&cidx, nc * (V + 1) * 8)) != hipSuccess) { rc = fail(e, "gather class offsets"); goto out; }
+    if ((e = hipMalloc(&abase_d, nc * 8)) != hipSuccess) { rc = fail(e, "gather class bases"); goto out; }
+    if ((e = hipMalloc(&cadj, E * 4)) != hipSuccess) { rc = fail(e, "gather class arcs"); goto out; }
+    if (t_wgt && g_wgt && (e = hipMalloc(&cwgt, E * 4)) != hipSuccess) { rc = fail(e, "gather class weights"); goto out; }
+    /* the count slot V of every class is 0, so an exclusive scan over V+1
+     * entries leaves the class's arc count at V */
+    if ((e = hipMemsetAsync(ccnt, 0, nc * (V + 1) * 8, st)) != hipSuccess) { rc = fail(e, "gather class clear"); goto out; }
+    hipLaunchKernelGGL(k_class_count, dim3((unsigned)vb), dim3(256), 0, st, V, t_idx, t_adj, bpos, ccnt);
+    e = rocprim::exclusive_scan(nullptr, tmp_bytes, ccnt, cidx, (uint64_t)0, (size_t)(V + 1),
+                                rocprim::plus<uint64_t>(), st);
+    if (e != hipSuccess) { rc = fail(e, "gather class scan size"); goto out; }
+    if ((e = hipMalloc(&tmp, tmp_bytes ? tmp_bytes : 16)) != hipSuccess) { rc = fail(e, "gather class scan temp"); goto out; }
+    for (uint64_t c = 0; c < nc; c++) {
+        e = rocprim::exclusive_scan(tmp, tmp_bytes, ccnt + c * (V + 1), cidx + c * (V + 1), (uint64_t)0,
+                                    (size_t)(V + 1), rocprim::plus<uint64_t>(), st);
+        if (e != hipSuccess) { rc = fail(e, "gather class scan"); goto out; }
+        if ((e = hipMemcpyAsync(&ec[c], cidx + c * (V + 1) + V, 8, hipMemcpyDeviceToHost, st)) != hipSuccess) {
+            rc = fail(e, "gather class count"); goto out;
+        }
+    }
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) { rc = fail(e, "gather class sync"); goto out; }
+    {
+        /* class c's arcs start at a multiple of K in g_arc (whole chunks for
+         * g_own, 16-byte source loads) */
+        uint64_t a = 0, s = 0;
+        for (uint64_t c = 0; c < nc; c++) {
+            abase[c] = s;
+            s += ec[c];
+            cbeg[c] = a;
+            cend[c] = a + ec[c];
+            a = (cend[c] + K - 1) / K * K;
+        }
+        if (s != E) {
+            snprintf(g_err, sizeof(g_err), "gather classes: %llu of %llu arcs classed", (unsigned long long)s,
+                     (unsigned long long)E);
+            rc = -1;
+            goto out;
+        }
+    }
+    if ((e = hipMemcpyAsync(abase_d, abase, nc * 8, hipMemcpyHostToDevice, st)) != hipSuccess) { rc = fail(e, "gather class bases"); goto out; }
+    hipLaunchKernelGGL(k_class_scatter, dim3((unsigned)vb), dim3(256), 0, st, V, t_idx, t_adj, cwgt ? t_wgt : nullptr,
+                       bpos, cidx, abase_d, cadj, cwgt);
+    if ((e = hipGetLastError()) != hipSuccess) { rc = fail(e, "gather class scatter"); goto out; }
+    for (uint64_t c = 0, d = 0; c < nc; c++) {
+        uint64_t nd = 0;
+        cdst[c] = d;
+        if (ec[c]) {
+            rc = ipr_hip_build_gather(V, ec[c], cidx + c * (V + 1), cadj + abase[c], bpos, K, g_arc + cbeg[c],
+                                      g_own + cbeg[c] / K, g_dst + d, &nd, stream);
+            if (rc) goto out;
+            /* the weights in the same order as the class's sources */
+            if (cwgt && (e = hipMemcpyAsync(g_wgt + cbeg[c], cwgt + abase[c], ec[c] * 4, hipMemcpyDeviceToDevice,
+                                            st)) != hipSuccess) {
+                rc = fail(e, "gather class weights");
+                goto out;
+            }
+        }
+        d += nd;
+    }
+out:
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(ccnt); (void)hipFree(cidx); (void)hipFree(abase_d); (void)hipFree(cadj); (void)hipFree(cwgt);
+    (void)hipFree(tmp);
+    return rc;
+}
+
 /* ------------------------------------------------------------------------ */
 /* in-degree bins                                                            */
 /* ------------------------------------------------------------------------ */

@@ -150,6 +150,35 @@ def test_gather_every_superstep(blocks, order, live, ranges, record4, monkeypatc
     assert np.array_equal(r.values, O.min_ancestor(star))
 
 
+@pytest.mark.parametrize("blocks,ranges", [("", ""), ("3", "512"), ("1", "100000")])
+def test_gather_split_by_record_line(blocks, ranges, monkeypatch):
+    """IPR_GATHER_XCD=1: the gather's arcs split into 8 classes by the line of
+    their source's record, workgroups starting on their own class and then
+    helping with the others (with 1 or 3 workgroups, fewer than the classes,
+    every class is still taken); CC, BFS and weighted SSSP with the gather
+    forced on every superstep, and a hub star."""
+    monkeypatch.setenv("IPR_GATHER_XCD", "1")
+    for k, v in (("IPR_GATHER_BLOCKS", blocks), ("IPR_GATHER_RANGE_ARCS", ranges)):
+        if v:
+            monkeypatch.setenv(k, v)
+        else:
+            monkeypatch.delenv(k, raising=False)
+    g = G.rmat(13, 16, seed=5, symmetrise=True)
+    r = EN.run_app("cc_spread", g, dir_alpha=1 << 30)
+    assert np.array_equal(r.values, O.min_ancestor(g))
+    assert r.gathered > 0
+    gd = G.rmat(12, 16, seed=6)
+    r = EN.run_app("sssp_spread", gd, source=0, dir_alpha=1 << 30)
+    assert np.array_equal(r.values, O.bfs(gd, 0))
+    gw = gd.with_weights(2)
+    r = EN.run_app("sssp_weighted", gw, source=0, dir_alpha=1 << 30)
+    assert np.array_equal(r.values, O.dijkstra(gw, 0))
+    star = G.from_arcs(3100, np.r_[np.arange(1, 3000), np.zeros(5, int)],
+                       np.r_[np.full(2999, 2999), np.arange(3000, 3005)], keep_self_loops=False)
+    r = EN.run_app("cc_spread", star, dir_alpha=1 << 30)
+    assert np.array_equal(r.values, O.min_ancestor(star))
+
+
 @pytest.mark.parametrize("rounds", ["1", "2", "4", "8"])
 def test_frontier_compaction_rounds(rounds, monkeypatch):
     """The frontier compaction with 1, 2, 4 and 8 rounds of 4096 flags per
