@@ -54,6 +54,21 @@ def test_pagerank_reassociated_within_tolerance(case):
     assert rel.max() <= PR_RTOL, rel.max()
 
 
+@pytest.mark.parametrize("case", ["rmat10", "rmat8", "star65", "chain10", "rmat14"])
+def test_push_pagerank_within_tolerance(case):
+    """The push PageRank build (pagerank_32: messages combined by FP64 atomic
+    adds, in arrival order) against the oracle's push variant, which combines
+    them in program order: within 1e-12 relative per vertex, active counts
+    exact. (The reference's own push build truncates the messages to integers
+    in its compare-and-swap, so its outputs are not the comparison.)"""
+    g = G.rmat(14, 16, seed=7) if case == "rmat14" else G.read(GOLDEN / case)
+    ref = O.run(g, O.PR, O.PUSH)
+    r = EN.run_app("pagerank", g)
+    rel = np.abs(r.values - ref.values) / np.maximum(np.abs(ref.values), 1e-300)
+    assert rel.max() <= PR_RTOL, rel.max()
+    assert list(r.active) == list(ref.active[: ref.supersteps])
+
+
 def test_pagerank_scale16_config():
     """PageRank, 10 supersteps, FP64, pull, R-MAT scale 16 ef 16 seed 1."""
     g = G.rmat(16, 16, seed=1)
