@@ -74,3 +74,33 @@ def test_oracle_ppr_closed_forms():
     g3 = G.from_arcs(3, [0, 0], [1, 2])
     z = O.ppr(g3, 0, iters=1)
     assert np.allclose(z, [0.15, 0.425, 0.425], rtol=1e-15)
+
+
+def test_autotune_scan_stops_and_picks_the_fastest():
+    """A one-dimensional scan keeps the fastest candidate and stops at the
+    first one 1.5x slower than the best so far (later ones are not timed)."""
+    from ipregel_amd import autotune as AT
+    times = {1: 10.0, 2: 8.0, 4: 9.0, 8: 12.5, 16: 1.0}
+    timed = []
+
+    def measure(v):
+        timed.append(v)
+        return times[v]
+
+    best, seen = AT._scan([1, 2, 4, 8, 16], measure)
+    assert best == 2 and timed == [1, 2, 4, 8] and set(seen) == {"1", "2", "4", "8"}
+    best, seen = AT._scan([16, 1], lambda v: times[v])
+    assert best == 16 and list(seen) == ["16", "1"]
+
+
+def test_autotune_best_of_repeats_short_runs():
+    from ipregel_amd import autotune as AT
+    calls = []
+
+    def run(ts):
+        it = iter(ts)
+        return lambda: (calls.append(1), next(it))[1]
+
+    assert AT._best_of(run([100.0, 80.0])) == 80.0 and len(calls) == 2
+    calls.clear()
+    assert AT._best_of(run([900.0, 1.0])) == 900.0 and len(calls) == 1
