@@ -6,14 +6,16 @@
  *   - the type macros IP_VERTEX_ID_TYPE (uint32_t), IP_NEIGHBOUR_COUNT_TYPE,
  *     IP_MESSAGE_TYPE, IP_VALUE_TYPE (defined by the app before inclusion),
  *   - the variant switches IP_USE_SPREAD / IP_USE_SINGLE_BROADCAST
- *     (src/iPregel_preamble.h:338-350),
- *   - the hooks ip_compute / ip_combine (now __device__) and
- *     ip_serialise_vertex (host) (src/iPregel_preamble.h:281,294,302),
+ *     (src/iPregel_preamble.h:319-331),
+ *   - the hooks ip_combine / ip_compute (now __device__) and
+ *     ip_serialise_vertex (host) (src/iPregel_preamble.h:262,283,275),
  *   - ip_send_message, ip_broadcast, ip_get_next_message, ip_has_message,
- *     ip_vote_to_halt (device side, src/iPregel_preamble.h:141-185),
+ *     ip_vote_to_halt (device side, src/iPregel_preamble.h:122-166),
  *   - ip_get_superstep, ip_is_first_superstep, ip_get_vertices_count,
- *     ip_get_edges_count (host and device, src/iPregel_preamble.h:85-115),
- *   - ip_init, ip_run, ip_dump (host, src/iPregel_preamble.h:317,328,336).
+ *     ip_get_edges_count (host and device, src/iPregel_preamble.h:66-96),
+ *   - ip_get_vertex_by_location, ip_get_vertex_by_id (host, after ip_run,
+ *     src/iPregel_preamble.h:104,110),
+ *   - ip_init, ip_run, ip_dump (host, src/iPregel_preamble.h:298,309,317).
  * Messages are passed by value, as in the reference code (not its README).
  *
  * The edits an application needs are device qualifiers: __device__ on
@@ -54,6 +56,35 @@
 
 #ifndef IP_VERTEX_ID_TYPE
 #error "IP_VERTEX_ID_TYPE must be defined (e.g. -DIP_VERTEX_ID_TYPE=uint32_t)"
+#endif
+static_assert(sizeof(IP_VERTEX_ID_TYPE) == 4 || sizeof(IP_VERTEX_ID_TYPE) == 8,
+              "IP_VERTEX_ID_TYPE must be a 4- or 8-byte integer (the _32 and _64 builds)");
+static_assert(sizeof(IP_NEIGHBOUR_COUNT_TYPE) == 4 || sizeof(IP_NEIGHBOUR_COUNT_TYPE) == 8,
+              "IP_NEIGHBOUR_COUNT_TYPE must be a 4- or 8-byte integer (the width of the .idx offsets)");
+
+/* fields every vertex of a variant carries: out-neighbour ids and count in
+ * the push variants (src/combiner_preamble.h:21-27,
+ * src/combiner_spread_preamble.h:28-34), in-neighbour ids and count in the
+ * single-broadcast ones (src/combiner_single_broadcast_preamble.h:32-38),
+ * both in spread + single broadcast
+ * (src/combiner_spread_single_broadcast_preamble.h:21-35). Unused fields cost
+ * nothing on the device: the proxy's loads are dropped when nothing reads
+ * them. */
+#if !defined(IP_USE_SINGLE_BROADCAST) || defined(IP_USE_SPREAD)
+#ifndef IP_NEEDS_OUT_NEIGHBOUR_IDS
+#define IP_NEEDS_OUT_NEIGHBOUR_IDS
+#endif
+#ifndef IP_NEEDS_OUT_NEIGHBOUR_COUNT
+#define IP_NEEDS_OUT_NEIGHBOUR_COUNT
+#endif
+#endif
+#if defined(IP_USE_SINGLE_BROADCAST)
+#ifndef IP_NEEDS_IN_NEIGHBOUR_IDS
+#define IP_NEEDS_IN_NEIGHBOUR_IDS
+#endif
+#ifndef IP_NEEDS_IN_NEIGHBOUR_COUNT
+#define IP_NEEDS_IN_NEIGHBOUR_COUNT
+#endif
 #endif
 
 /* implied needs, src/iPregel_preamble.h:32-38 */
@@ -105,7 +136,7 @@
 #define IPR_STRINGIFY(x) IPR_STRINGIFY_LITERAL(x)
 #define IPR_STRINGIFY_LITERAL(x) #x
 
-/* Host globals of the reference preamble (src/iPregel_preamble.h:63-76),
+/* Host globals of the reference preamble (src/iPregel_preamble.h:45-57),
  * mirrored from the driver after ip_init and ip_run. */
 size_t ip_superstep = 0;
 size_t ip_edges_count = 0;
@@ -154,6 +185,10 @@ void ip_init(const char* file_path, int number_of_threads, const char* schedule,
              int chunk_size, bool directed, bool weighted);
 int ip_run(void);
 void ip_dump(FILE* f);
+
+/* host-side vertex lookup after ip_run (src/iPregel_preamble.h:104,110) */
+struct ip_vertex_t* ip_get_vertex_by_location(size_t location);
+struct ip_vertex_t* ip_get_vertex_by_id(IP_VERTEX_ID_TYPE id);
 }
 
 #include "ipregel_kernels.h"
@@ -248,7 +283,8 @@ __device__ inline void ip_broadcast(struct ip_vertex_t* v, IP_MESSAGE_TYPE messa
 __device__ inline void ip_broadcast_weighted(struct ip_vertex_t* v, IP_MESSAGE_TYPE message)
 {
 #if IPR_IS_PULL
-    ipr_raise_error(IPR_ERR_SEND_IN_PULL);
+    (void)v; (void)message;
+    ipr_raise_error(IPR_ERR_WEIGHTED_IN_PULL);
 #else
     if (v->ipr_bcast) ipr_raise_error(IPR_ERR_MIXED_BROADCAST);
     v->ipr_bcast = true;
@@ -259,16 +295,23 @@ __device__ inline void ip_broadcast_weighted(struct ip_vertex_t* v, IP_MESSAGE_T
 #endif
 
 /* Push variants: immediate delivery into the next superstep's mailbox
- * (src/combiner_postamble.h:149-174, src/combiner_spread_postamble.h:202-227).
+ * (src/combiner_postamble.h:54-79, src/combiner_spread_postamble.h:73-98).
  * Pull variants: the reference prints and exits
- * (src/combiner_single_broadcast_postamble.h:146-153); here the kernel flags
- * the error and the driver prints the same text and exits after the launch. */
+ * (src/combiner_single_broadcast_postamble.h:42-49,
+ * src/combiner_spread_single_broadcast_postamble.h:55-62); here the kernel
+ * flags the error and the driver prints the same text and exits after the
+ * launch. An id outside [0, V) is flagged too (the reference would write out
+ * of bounds) and the message is dropped. */
 __device__ inline void ip_send_message(IP_VERTEX_ID_TYPE id, IP_MESSAGE_TYPE message)
 {
 #if IPR_IS_PULL
     (void)id; (void)message;
     ipr_raise_error(IPR_ERR_SEND_IN_PULL);
 #else
+    if ((uint64_t)id >= ipr_d_dev.V) {
+        ipr_raise_error(IPR_ERR_SEND_OUT_OF_RANGE);
+        return;
+    }
     ipr_send_immediate(id, message);
 #endif
 }
@@ -302,9 +345,12 @@ static void ipr_app_serialise(void* file, const void* val, const uint64_t* out_i
     }
 }
 
+static void ipr_host_view_free(void);
+
 void ip_init(const char* file_path, int number_of_threads, const char* schedule,
              int chunk_size, bool directed, bool weighted)
 {
+    ipr_host_view_free();
     ipr_app_desc d;
     memset(&d, 0, sizeof(d));
     d.variant = IPR_VARIANT;
@@ -314,8 +360,17 @@ void ip_init(const char* file_path, int number_of_threads, const char* schedule,
     d.msg_size = sizeof(IP_MESSAGE_TYPE);
     d.mbox_size = sizeof(ipr_policy::slot_t);
     d.bslot_size = sizeof(ipr_bslot_t);
-#if IPR_IS_PULL
+    d.idx_size = sizeof(IP_NEIGHBOUR_COUNT_TYPE);
+#if IPR_IS_PULL || defined(IP_NEEDS_IN_NEIGHBOUR_IDS) || defined(IP_NEEDS_IN_NEIGHBOUR_COUNT)
+    /* in-lists are mirrored from the out-lists when the graph is directed
+     * (src/iPregel_postamble.h:429-487) */
     d.needs_in_lists = 1;
+#endif
+#ifdef IP_NEEDS_OUT_NEIGHBOUR_IDS
+    d.needs_out_ids = 1;
+#endif
+#ifdef IP_NEEDS_IN_NEIGHBOUR_IDS
+    d.needs_in_ids = 1;
 #endif
 #ifdef IP_NEEDS_OUT_NEIGHBOUR_WEIGHTS
     d.needs_weights = 1;
@@ -341,17 +396,165 @@ void ip_init(const char* file_path, int number_of_threads, const char* schedule,
     ip_superstep = 0;
 }
 
+/* ------------------------------------------------------------------------ */
+/* host view of the vertices: ip_get_vertex_by_location / ip_get_vertex_by_id */
+/* ------------------------------------------------------------------------ */
+
+/* The reference keeps every vertex in one host array and returns pointers
+ * into it (src/iPregel_postamble.h:78-101). Here the vertex state lives in
+ * HBM as structure of arrays, so the first lookup after ip_run builds a host
+ * array of ip_vertex_t from it: id, value, active, has_message, message,
+ * neighbour counts and neighbour id / weight pointers into host copies of the
+ * adjacency. The pointers stay valid until the next ip_init or ip_run; ip_dump
+ * serialises this array while it is current, so edits made through the
+ * pointers show in the dump as they would in the reference. Host code only:
+ * device code has no vertex array to point into. */
+static struct ip_vertex_t* ipr_host_vertices = NULL;
+static uint64_t ipr_host_vertices_run = ~0ull;
+static IP_VERTEX_ID_TYPE* ipr_host_ids[2] = {NULL, NULL};   /* out, in: IP_VERTEX_ID_TYPE copies */
+static uint64_t* ipr_host_in_idx = NULL;
+
+static void ipr_host_view_free(void)
+{
+    free(ipr_host_vertices);
+    ipr_host_vertices = NULL;
+    if (ipr_host_ids[1] != ipr_host_ids[0]) free(ipr_host_ids[1]);
+    free(ipr_host_ids[0]);
+    ipr_host_ids[0] = ipr_host_ids[1] = NULL;
+    free(ipr_host_in_idx);
+    ipr_host_in_idx = NULL;
+    ipr_host_vertices_run = ~0ull;
+}
+
+static bool ipr_host_view_current(void)
+{
+    return ipr_host_vertices != NULL && ipr_host_vertices_run == ipr_drv_runs();
+}
+
+static void* ipr_host_alloc(size_t n)
+{
+    void* p = malloc(n ? n : 1);
+    if (!p) {
+        printf("Failed to allocate %zu bytes for the host view of the vertices.\n", n);
+        exit(-1);
+    }
+    return p;
+}
+
+static void ipr_host_view_build(void)
+{
+    ipr_host_view_free();
+    const uint64_t V = ipr_drv_vertices_count(), E = ipr_drv_edges_count();
+    const uint64_t* idx = ipr_drv_host_out_idx();
+    IP_VALUE_TYPE* val = (IP_VALUE_TYPE*)ipr_host_alloc(V * sizeof(IP_VALUE_TYPE));
+    IP_MESSAGE_TYPE* msg = (IP_MESSAGE_TYPE*)ipr_host_alloc(V * sizeof(IP_MESSAGE_TYPE));
+    uint8_t* act = (uint8_t*)ipr_host_alloc(V);
+    uint8_t* hm = (uint8_t*)ipr_host_alloc(V);
+    if (ipr_drv_vertex_state(val, act, hm, msg)) {
+        printf("Failed to read the vertex state back from the device.\n");
+        exit(-1);
+    }
+#if defined(IP_NEEDS_OUT_NEIGHBOUR_IDS) || defined(IP_NEEDS_IN_NEIGHBOUR_IDS)
+    /* out-neighbour ids in the application's id type */
+    {
+        const uint32_t* adj = ipr_drv_host_out_adj();
+        ipr_host_ids[0] = (IP_VERTEX_ID_TYPE*)ipr_host_alloc(E * sizeof(IP_VERTEX_ID_TYPE));
+        for (uint64_t e = 0; e < E; e++) ipr_host_ids[0][e] = (IP_VERTEX_ID_TYPE)adj[e];
+    }
+#endif
+#if defined(IP_NEEDS_IN_NEIGHBOUR_IDS) || defined(IP_NEEDS_IN_NEIGHBOUR_COUNT)
+    /* in-lists: the out-lists when undirected, the device's mirror otherwise */
+    if (ipr_drv_directed()) {
+        ipr_host_in_idx = (uint64_t*)ipr_host_alloc((V + 1) * sizeof(uint64_t));
+        uint32_t* in_adj = (uint32_t*)ipr_host_alloc(E * sizeof(uint32_t));
+        if (ipr_drv_in_lists(ipr_host_in_idx, in_adj)) {
+            printf("Failed to read the in-neighbour lists back from the device.\n");
+            exit(-1);
+        }
+#ifdef IP_NEEDS_IN_NEIGHBOUR_IDS
+        ipr_host_ids[1] = (IP_VERTEX_ID_TYPE*)ipr_host_alloc(E * sizeof(IP_VERTEX_ID_TYPE));
+        for (uint64_t e = 0; e < E; e++) ipr_host_ids[1][e] = (IP_VERTEX_ID_TYPE)in_adj[e];
+#endif
+        free(in_adj);
+    } else {
+        ipr_host_ids[1] = ipr_host_ids[0];
+    }
+#endif
+    const bool spread = IPR_VARIANT == IPR_VARIANT_SPREAD || IPR_VARIANT == IPR_VARIANT_SPREAD_PULL;
+    ipr_host_vertices = (struct ip_vertex_t*)ipr_host_alloc(V * sizeof(struct ip_vertex_t));
+    for (uint64_t v = 0; v < V; v++) {
+        struct ip_vertex_t* x = &ipr_host_vertices[v];
+        memset(x, 0, sizeof(*x));
+        x->id = (IP_VERTEX_ID_TYPE)v;
+        x->value = val[v];
+        /* the spread variants keep no active flags (ip_vote_to_halt is a no-op
+         * there) and consume every message within its superstep */
+        x->active = !spread && act[v] != 0;
+        x->has_message = !spread && hm[v] != 0;
+        if (x->has_message) x->message = msg[v];
+#ifdef IP_NEEDS_OUT_NEIGHBOUR_COUNT
+        x->out_neighbour_count = (IP_NEIGHBOUR_COUNT_TYPE)(idx[v + 1] - idx[v]);
+#endif
+#ifdef IP_NEEDS_OUT_NEIGHBOUR_IDS
+        x->out_neighbours = ipr_host_ids[0] + idx[v];
+#endif
+#ifdef IP_NEEDS_IN_NEIGHBOUR_COUNT
+        {
+            const uint64_t* in_idx = ipr_host_in_idx ? ipr_host_in_idx : idx;
+            x->in_neighbour_count = (IP_NEIGHBOUR_COUNT_TYPE)(in_idx[v + 1] - in_idx[v]);
+        }
+#endif
+#ifdef IP_NEEDS_IN_NEIGHBOUR_IDS
+        x->in_neighbours = ipr_host_ids[1] + (ipr_host_in_idx ? ipr_host_in_idx : idx)[v];
+#endif
+#ifdef IP_NEEDS_OUT_NEIGHBOUR_WEIGHTS
+        x->out_neighbour_weights = (IP_EDGE_WEIGHT_TYPE*)ipr_drv_host_out_wgt() + idx[v];
+#endif
+    }
+    (void)E;
+    free(val);
+    free(msg);
+    free(act);
+    free(hm);
+    ipr_host_vertices_run = ipr_drv_runs();
+}
+
+/* src/iPregel_postamble.h:78-81; the location is the id here (the reference's
+ * IP_FORCE_DIRECT_MAPPING / IP_ID_OFFSET == 0 case, :83-95) */
+struct ip_vertex_t* ip_get_vertex_by_location(size_t location)
+{
+    if (!ipr_host_view_current()) ipr_host_view_build();
+    return &ipr_host_vertices[location];
+}
+
+struct ip_vertex_t* ip_get_vertex_by_id(IP_VERTEX_ID_TYPE id)
+{
+    return ip_get_vertex_by_location((size_t)id);
+}
+
 int ip_run(void)
 {
+    ipr_host_view_free();
     int rc = ipr_drv_run();
     ip_superstep = (size_t)ipr_drv_superstep();
     ip_active_vertices = 0;
     return rc;
 }
 
+/* src/iPregel_postamble.h:103-115 */
 void ip_dump(FILE* f)
 {
-    ipr_drv_dump(f);
+    if (!ipr_host_view_current()) {
+        ipr_drv_dump(f);
+        return;
+    }
+    struct timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    const uint64_t V = ipr_drv_vertices_count();
+    for (uint64_t v = 0; v < V; v++) ip_serialise_vertex(f, &ipr_host_vertices[v]);
+    fflush(f);
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    printf("DumpingTime:%f\n", (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec));
 }
 
 } /* extern "C" */

@@ -29,6 +29,20 @@ uint64_t ipr_drv_vertices_count(void);
 uint64_t ipr_drv_edges_count(void);
 uint64_t ipr_drv_superstep(void);
 
+/* Host view of the vertices after ip_run, behind ip_get_vertex_by_id and
+ * ip_get_vertex_by_location (src/iPregel_postamble.h:78-101), which iPregel.h
+ * builds from these: */
+uint64_t ipr_drv_runs(void);                 /* superstep loops run so far */
+int ipr_drv_directed(void);                  /* ip_init's `directed` */
+const uint64_t* ipr_drv_host_out_idx(void);  /* V+1 offsets (host copy) */
+const uint32_t* ipr_drv_host_out_adj(void);  /* E out-neighbour ids (host copy) */
+const uint32_t* ipr_drv_host_out_wgt(void);  /* E weights or NULL */
+/* copy the device's values, active and has_message flags and messages (any
+ * may be NULL); 0 on success */
+int ipr_drv_vertex_state(void* val, uint8_t* act, uint8_t* hm, void* msg);
+/* copy the device's in-lists (V+1 offsets, E ids); -ENOENT without them */
+int ipr_drv_in_lists(uint64_t* idx, uint32_t* adj);
+
 /* ---- harness (not part of the reference surface) ---------------------- */
 
 typedef struct ipr_harness_cfg {

@@ -109,6 +109,12 @@ typedef struct ipr_dev {
     const uint64_t* in_idx;
     const uint32_t* in_adj;
     const uint32_t* in_wgt;
+    /* the adjacency in the application's IP_VERTEX_ID_TYPE, for the
+     * out_neighbours / in_neighbours pointers of ip_vertex_t: the 32-bit
+     * arrays above in _32 builds, 64-bit copies in _64 builds that ask for
+     * neighbour ids (IP_NEEDS_{OUT,IN}_NEIGHBOUR_IDS), NULL otherwise */
+    const void* out_ids;
+    const void* in_ids;
     /* push variants that gather dense supersteps: the exact transpose of the
      * out-edge CSR (aliases it when the graph is symmetric) */
     const uint64_t* t_idx;
@@ -211,7 +217,10 @@ typedef struct ipr_app_desc {
     int variant;
     int combiner;
     uint32_t id_size, val_size, msg_size, mbox_size, bslot_size;
-    int needs_in_lists;          /* pull variants */
+    uint32_t idx_size;           /* sizeof(IP_NEIGHBOUR_COUNT_TYPE): the width of the .idx offsets */
+    int needs_in_lists;          /* pull variants, IP_NEEDS_IN_NEIGHBOUR_{IDS,COUNT} */
+    int needs_out_ids;           /* IP_NEEDS_OUT_NEIGHBOUR_IDS */
+    int needs_in_ids;            /* IP_NEEDS_IN_NEIGHBOUR_IDS (pull variants imply it) */
     int needs_weights;           /* IP_NEEDS_OUT_NEIGHBOUR_WEIGHTS */
     int can_gather;              /* push app whose combiner makes gather exact */
     const char* application;     /* IP_APPLICATION */

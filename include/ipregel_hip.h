@@ -70,6 +70,10 @@ int ipr_hip_build_csc(uint64_t V, uint64_t E, const uint64_t* out_idx,
                       uint64_t* in_idx, uint32_t* in_adj, uint32_t* in_wgt,
                       void* stream);
 
+/* dst[i] = src[i] widened to 64 bits (the neighbour-id arrays that _64
+ * applications read through ip_vertex_t, src/iPregel_postamble.h:361-381) */
+int ipr_hip_widen_ids(const uint32_t* src, uint64_t* dst, uint64_t n, void* stream);
+
 /* static in-degree bins for the pull gather; sizes returned in dev->pbin_size */
 int ipr_hip_build_pull_bins(ipr_dev* dev, uint32_t small_max, uint32_t medium_max,
                             void* stream);

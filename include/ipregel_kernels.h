@@ -61,7 +61,14 @@ __constant__ ipr_dev ipr_d_dev;
  * dependent global loads */
 __shared__ uint64_t ipr_sm_superstep;
 
-enum { IPR_ERR_SEND_IN_PULL = 1, IPR_ERR_MIXED_BROADCAST = 2 };
+/* device-side API misuse, reported by the driver after the launch (bits of
+ * ipr_ctl.error; the driver's report_device_error prints one message each) */
+enum {
+    IPR_ERR_SEND_IN_PULL = 1,        /* ip_send_message in a single-broadcast variant */
+    IPR_ERR_MIXED_BROADCAST = 2,     /* ip_broadcast_weighted with another broadcast */
+    IPR_ERR_WEIGHTED_IN_PULL = 4,    /* ip_broadcast_weighted in a single-broadcast variant */
+    IPR_ERR_SEND_OUT_OF_RANGE = 8    /* ip_send_message to an id outside [0, V) */
+};
 
 __device__ inline void ipr_raise_error(uint32_t code)
 {
@@ -411,13 +418,15 @@ __device__ inline void ipr_fill_vertex(const ipr_dev& d, struct ip_vertex_t* x, 
     x->out_neighbour_count = (IP_NEIGHBOUR_COUNT_TYPE)(d.out_idx[v + 1] - d.out_idx[v]);
 #endif
 #ifdef IP_NEEDS_OUT_NEIGHBOUR_IDS
-    x->out_neighbours = (IP_VERTEX_ID_TYPE*)(d.out_adj + d.out_idx[v]);
+    /* out_ids holds IP_VERTEX_ID_TYPE entries: the 32-bit adjacency itself in
+     * _32 builds, a 64-bit copy in _64 builds */
+    x->out_neighbours = (IP_VERTEX_ID_TYPE*)d.out_ids + d.out_idx[v];
 #endif
 #ifdef IP_NEEDS_IN_NEIGHBOUR_COUNT
     x->in_neighbour_count = (IP_NEIGHBOUR_COUNT_TYPE)(d.in_idx[v + 1] - d.in_idx[v]);
 #endif
 #ifdef IP_NEEDS_IN_NEIGHBOUR_IDS
-    x->in_neighbours = (IP_VERTEX_ID_TYPE*)(d.in_adj + d.in_idx[v]);
+    x->in_neighbours = (IP_VERTEX_ID_TYPE*)d.in_ids + d.in_idx[v];
 #endif
 #ifdef IP_NEEDS_OUT_NEIGHBOUR_WEIGHTS
     x->out_neighbour_weights = (IP_EDGE_WEIGHT_TYPE*)(d.out_wgt + d.out_idx[v]);

@@ -18,6 +18,17 @@ typedef IP_MESSAGE_TYPE IP_VALUE_TYPE;
 
 #define PR_ROUNDS 10u
 
+/* The reference application loads the graph with directed=false
+ * (benchmarks/pagerank.c:91), so every vertex pulls over its own out-list. A
+ * build with -DPAGERANK_DIRECTED passes directed=true instead: the in-lists
+ * are mirrored from the out-lists in ascending source order
+ * (src/iPregel_postamble.h:429-487) and the result is PageRank of G. */
+#ifdef PAGERANK_DIRECTED
+#define PR_DIRECTED true
+#else
+#define PR_DIRECTED false
+#endif
+
 /* set by main after ip_init, read by ip_compute on the device */
 __device__ double pr_teleport;
 __device__ double pr_start;
@@ -57,7 +68,7 @@ int main(int argc, char* argv[])
         return -1;
     }
     printf("ApplicationConfiguration:maxSuperstepCount=10\n");
-    ip_init(argv[1], atoi(argv[3]), argv[4], atoi(argv[5]), false, false);
+    ip_init(argv[1], atoi(argv[3]), argv[4], atoi(argv[5]), PR_DIRECTED, false);
     pr_teleport = 0.15 / ip_get_vertices_count();
     pr_start = 1.0 / ip_get_vertices_count();
     IP_PUBLISH(pr_teleport);

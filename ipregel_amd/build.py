@@ -7,7 +7,7 @@ Produces, in-tree (git-ignored, shipped with the tree):
   ipregel_amd/lib/ip_driver.o           C host driver (linked into every app)
   ipregel_amd/lib/libipr_<app>_32.so    app TU + driver, loadable over ctypes
   ipregel_amd/bin/<app>_32              the same app as a command-line program
-and the oracle's own library (oracle/build.py).
+(and _64 builds, IP_VERTEX_ID_TYPE uint64_t, of the API-coverage apps).
 
 Every HIP translation unit is compiled for gfx950 only, with packed-FP32
 instructions disabled (-Xclang -target-feature -Xclang -packed-fp32-ops).
@@ -18,9 +18,7 @@ from __future__ import annotations
 
 import concurrent.futures as cf
 import os
-import shutil
 import subprocess
-import sys
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
@@ -44,6 +42,9 @@ C_FLAGS = ["-std=c11", "-O3", "-fPIC", "-fopenmp", "-Wall", "-Wextra"]
 APPS = [
     ("pagerank_single_broadcast", "pagerank.c", ["-DIP_USE_SINGLE_BROADCAST", "-DIP_COMBINER_SUM"]),
     ("pagerank", "pagerank.c", ["-DIP_COMBINER_SUM"]),
+    # the same with directed=true: PageRank over the mirrored in-lists
+    ("pagerank_directed_single_broadcast", "pagerank.c",
+     ["-DIP_USE_SINGLE_BROADCAST", "-DIP_COMBINER_SUM", "-DPAGERANK_DIRECTED"]),
     ("cc_spread", "cc.c", ["-DIP_USE_SPREAD", "-DIP_COMBINER_MIN"]),
     ("cc", "cc.c", ["-DIP_COMBINER_MIN"]),
     ("cc_single_broadcast", "cc.c", ["-DIP_USE_SINGLE_BROADCAST", "-DIP_COMBINER_MIN"]),
@@ -57,6 +58,16 @@ APPS = [
      ["-DIP_USE_SINGLE_BROADCAST", "-DIP_USE_SPREAD", "-DIP_COMBINER_MIN"]),
     ("sssp_weighted_spread", "sssp_weighted.c", ["-DIP_USE_SPREAD", "-DIP_COMBINER_MIN"]),
     ("sssp_weighted", "sssp_weighted.c", ["-DIP_COMBINER_MIN"]),
+    # API-coverage applications, in both id widths: ip_send_message to each
+    # out-neighbour (and its abort in a single-broadcast build), 4-byte .idx
+    # offsets; neighbour-id lists and the host-side vertex lookup
+    ("cc_send", "cc_send.c", ["-DIP_COMBINER_MIN"], (32, 64)),
+    ("cc_send_spread", "cc_send.c", ["-DIP_USE_SPREAD", "-DIP_COMBINER_MIN"], (32, 64)),
+    ("cc_send_single_broadcast", "cc_send.c", ["-DIP_USE_SINGLE_BROADCAST", "-DIP_COMBINER_MIN"], (32,)),
+    ("nbr_view", "nbr_view.c", ["-DIP_COMBINER_MIN"], (32, 64)),
+    ("nbr_view_single_broadcast", "nbr_view.c", ["-DIP_USE_SINGLE_BROADCAST", "-DIP_COMBINER_MIN"], (32, 64)),
+    ("nbr_view_single_broadcast_spread", "nbr_view.c",
+     ["-DIP_USE_SINGLE_BROADCAST", "-DIP_USE_SPREAD", "-DIP_COMBINER_MIN"], (32, 64)),
 ]
 
 
@@ -97,15 +108,19 @@ def build_driver() -> Path:
     return out
 
 
-def build_app(name: str, src: str, defines: list[str], driver_obj: Path) -> tuple[Path, Path]:
-    obj = OBJ / f"{name}_32.o"
+def build_app(name: str, src: str, defines: list[str], driver_obj: Path, width: int = 32) -> tuple[Path, Path]:
+    """One application variant: libipr_<name>_<width>.so and bin/<name>_<width>
+    (IP_VERTEX_ID_TYPE uint32_t or uint64_t, as the reference makefile's _32
+    and _64 builds, makefile:16-17)."""
+    obj = OBJ / f"{name}_{width}.o"
     app_label = name
+    id_type = "uint32_t" if width == 32 else "uint64_t"
     _run([HIPCC, *HIP_FLAGS, "-x", "hip", "-ffp-contract=off", f"-I{INC}",
-          "-DIP_VERTEX_ID_TYPE=uint32_t", f'-DIP_APPLICATION="{app_label}"',
+          f"-DIP_VERTEX_ID_TYPE={id_type}", f'-DIP_APPLICATION="{app_label}"',
           '-DIP_MACHINE="MI355X"', '-DVERSION="1.0.0-mi355x"',
           *defines, "-c", str(PKG / "apps" / src), "-o", str(obj)])
-    so = LIB / f"libipr_{name}_32.so"
-    exe = BIN / f"{name}_32"
+    so = LIB / f"libipr_{name}_{width}.so"
+    exe = BIN / f"{name}_{width}"
     link = [str(obj), str(driver_obj), f"-L{LIB}", "-lipregel_hip", "-fopenmp"]
     _run([HIPCC, "-shared", "-fPIC", "-Wl,-Bsymbolic", "-Wl,-rpath,$ORIGIN", *link, "-o", str(so)])
     _run([HIPCC, "-Wl,-rpath,$ORIGIN/../lib", *link, "-o", str(exe)])
@@ -124,20 +139,20 @@ def build_all(jobs: int | None = None, verbose: bool = True) -> None:
         f_hip.result()
         f_graph.result()
         f_batch = ex.submit(build_batch_lib)
-        futs = {ex.submit(build_app, n, s, d, drv): n for (n, s, d) in APPS}
+        futs = {}
+        for entry in APPS:
+            n, s, d = entry[:3]
+            for w in (entry[3] if len(entry) > 3 else (32,)):
+                futs[ex.submit(build_app, n, s, d, drv, w)] = f"{n}_{w}"
         for f in cf.as_completed(futs):
             f.result()
             if verbose:
-                print(f"built {futs[f]}_32", flush=True)
+                print(f"built {futs[f]}", flush=True)
         f_batch.result()
     # the reference's own applications against this engine (needs the
     # reference tree; a host without it uses the binaries built here)
     from . import dropin
     dropin.build()
-    # the oracle is test infrastructure with its own recipe
-    sys.path.insert(0, str(ROOT))
-    from oracle import build as oracle_build  # noqa: E402
-    oracle_build.build()
 
 
 if __name__ == "__main__":

@@ -156,6 +156,8 @@ static void dev_free_all(void)
         D.graphs[i] = NULL;
     }
     ipr_dev* d = &D.dev;
+    if (d->in_ids && d->in_ids != d->in_adj && d->in_ids != d->out_ids) ipr_hip_free((void*)d->in_ids);
+    if (d->out_ids && d->out_ids != d->out_adj) ipr_hip_free((void*)d->out_ids);
     if (d->t_idx && d->t_idx != d->out_idx) ipr_hip_free((void*)d->t_idx);
     if (d->t_adj && d->t_adj != d->out_adj) ipr_hip_free((void*)d->t_adj);
     if (d->t_wgt && d->t_wgt != d->out_wgt) ipr_hip_free((void*)d->t_wgt);
@@ -263,6 +265,28 @@ static void dev_setup(void)
         d->in_idx = d->out_idx;
         d->in_adj = d->out_adj;
         d->in_wgt = d->out_wgt;
+    }
+    /* neighbour ids as the application reads them (IP_VERTEX_ID_TYPE): the
+     * 32-bit arrays in _32 builds, widened copies in _64 builds */
+    if (a->needs_out_ids) {
+        if (a->id_size == 8) {
+            uint64_t* w = dalloc(E * 8, "out-neighbour ids");
+            hip_check(ipr_hip_widen_ids(d->out_adj, w, E, st), "widen out-neighbour ids");
+            d->out_ids = w;
+        } else {
+            d->out_ids = d->out_adj;
+        }
+    }
+    if (a->needs_in_ids) {
+        if (d->in_adj == d->out_adj && d->out_ids) {
+            d->in_ids = d->out_ids;
+        } else if (a->id_size == 8) {
+            uint64_t* w = dalloc(E * 8, "in-neighbour ids");
+            hip_check(ipr_hip_widen_ids(d->in_adj, w, E, st), "widen in-neighbour ids");
+            d->in_ids = w;
+        } else {
+            d->in_ids = d->in_adj;
+        }
     }
 
     d->val = dalloc(vv * a->val_size, "values");
@@ -433,7 +457,10 @@ static void dev_setup(void)
 /* loader                                                                    */
 /* ------------------------------------------------------------------------ */
 
-static void load_from_files(const char* root, int want_weights, uint32_t id_size)
+/* .idx holds V offsets of IP_NEIGHBOUR_COUNT_TYPE (src/iPregel_postamble.h:
+ * 294-323 reads sizeof(IP_NEIGHBOUR_COUNT_TYPE) per entry; the reference's
+ * applications use 8 bytes, benchmarks/template.c 4): widened to 64 bits */
+static void load_from_files(const char* root, int want_weights, uint32_t id_size, uint32_t idx_size)
 {
     char* p = with_ext(root, ".config");
     printf("\t- Loading configuration file from: \"%s\".\n", p);
@@ -454,7 +481,15 @@ static void load_from_files(const char* root, int want_weights, uint32_t id_size
     D.h_adj = safe_malloc((E ? E : 1) * sizeof(uint32_t));
     p = with_ext(root, ".idx");
     printf("\t- Loading offset file from: \"%s\".\n", p);
-    read_elements(p, D.h_idx, sizeof(uint64_t), V);
+    if (idx_size == 4) {
+        uint32_t* narrow = safe_malloc((V ? V : 1) * sizeof(uint32_t));
+        read_elements(p, narrow, sizeof(uint32_t), V);
+        #pragma omp parallel for schedule(static)
+        for (uint64_t v = 0; v < V; v++) D.h_idx[v] = narrow[v];
+        free(narrow);
+    } else {
+        read_elements(p, D.h_idx, sizeof(uint64_t), V);
+    }
     free(p);
     D.h_idx[V] = E;
     /* .adj holds IP_VERTEX_ID_TYPE ids: 4 bytes in _32 builds, 8 in _64 ones;
@@ -573,7 +608,7 @@ void ipr_drv_init(const ipr_app_desc* app, const char* file_path, int number_of_
         if (want_w && !D.h_wgt) die("The application needs edge weights but none were given.\n");
         printf("\t\t- %llu vertices\n\t\t- %llu edges\n", (unsigned long long)D.V, (unsigned long long)D.E);
     } else {
-        load_from_files(file_path, want_w, D.app.id_size);
+        load_from_files(file_path, want_w, D.app.id_size, D.app.idx_size ? D.app.idx_size : 8);
     }
     validate_graph();
 
@@ -611,6 +646,35 @@ void ipr_drv_init(const ipr_app_desc* app, const char* file_path, int number_of_
 uint64_t ipr_drv_vertices_count(void) { return D.V; }
 uint64_t ipr_drv_edges_count(void) { return D.E; }
 uint64_t ipr_drv_superstep(void) { return D.stats.supersteps; }
+uint64_t ipr_drv_runs(void) { return D.stats.runs; }
+int ipr_drv_directed(void) { return D.directed; }
+const uint64_t* ipr_drv_host_out_idx(void) { return D.h_idx; }
+const uint32_t* ipr_drv_host_out_adj(void) { return D.h_adj; }
+const uint32_t* ipr_drv_host_out_wgt(void) { return D.h_wgt; }
+
+int ipr_drv_vertex_state(void* val, uint8_t* act, uint8_t* hm, void* msg)
+{
+    if (!D.initialised) return -EINVAL;
+    const uint64_t V = D.V;
+    int rc = 0;
+    if (val && ipr_hip_memcpy_d2h(val, D.dev.val, V * D.app.val_size, D.stream)) rc = -EIO;
+    if (act && ipr_hip_memcpy_d2h(act, D.dev.act, V, D.stream)) rc = -EIO;
+    if (hm && ipr_hip_memcpy_d2h(hm, D.dev.hm, V, D.stream)) rc = -EIO;
+    if (msg && ipr_hip_memcpy_d2h(msg, D.dev.msg, V * D.app.msg_size, D.stream)) rc = -EIO;
+    if (ipr_hip_stream_sync(D.stream)) rc = -EIO;
+    return rc;
+}
+
+int ipr_drv_in_lists(uint64_t* idx, uint32_t* adj)
+{
+    if (!D.initialised) return -EINVAL;
+    if (!D.dev.in_idx || !D.dev.in_adj) return -ENOENT;
+    int rc = 0;
+    if (idx && ipr_hip_memcpy_d2h(idx, D.dev.in_idx, (D.V + 1) * 8, D.stream)) rc = -EIO;
+    if (adj && ipr_hip_memcpy_d2h(adj, D.dev.in_adj, D.E * 4, D.stream)) rc = -EIO;
+    if (ipr_hip_stream_sync(D.stream)) rc = -EIO;
+    return rc;
+}
 
 /* ------------------------------------------------------------------------ */
 /* ip_run                                                                    */
@@ -710,13 +774,21 @@ static void enqueue_chunk(uint32_t n)
     }
 }
 
+/* flags raised by the device-side API (IPR_ERR_* of ipregel_kernels.h) */
 static void report_device_error(uint32_t err)
 {
+    /* src/combiner_single_broadcast_postamble.h:42-49 prints this and exits */
     if (err & 1u)
         die("The function send_message should not be used in the SINGLE_BROADCAST "
             "version; only broadcast() should be called, and once per superstep maximum.\n");
+    if (err & 4u)
+        die("The function ip_broadcast_weighted should not be used in the SINGLE_BROADCAST "
+            "version; weighted broadcasts are delivered by the push variants only.\n");
     if (err & 2u)
         die("ip_broadcast_weighted cannot be combined with another broadcast in one superstep.\n");
+    if (err & 8u)
+        die("ip_send_message was given a vertex identifier outside [0, %llu).\n",
+            (unsigned long long)D.V);
     die("Device-side error flags 0x%x.\n", err);
 }
 

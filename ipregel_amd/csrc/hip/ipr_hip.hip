@@ -294,6 +294,22 @@ out:
     return rc;
 }
 
+__global__ void k_widen_ids(const uint32_t* src, uint64_t* dst, uint64_t n)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+extern "C" int ipr_hip_widen_ids(const uint32_t* src, uint64_t* dst, uint64_t n, void* stream)
+{
+    if (n == 0) return 0;
+    uint64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_widen_ids, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, src, dst, n);
+    CK(hipGetLastError());
+    return 0;
+}
+
 /* ------------------------------------------------------------------------ */
 /* arc-balanced gather index                                                 */
 /* ------------------------------------------------------------------------ */

@@ -17,7 +17,21 @@ BINS := $(OUT)/bin/cc_32 $(OUT)/bin/cc_spread_32 $(OUT)/bin/cc_single_broadcast_
 # the _64 builds (IP_VERTEX_ID_TYPE=uint64_t, makefile:17)
 BINS += $(patsubst %_32,%_64,$(BINS))
 
+# benchmarks/pagerank.c with directed=true (its line 91 set to true), built
+# from a copy under /tmp that is deleted afterwards: PageRank over the
+# in-lists the reference mirrors in ascending source order
+# (src/iPregel_postamble.h:429-487)
+BINS += $(OUT)/bin/pagerank_directed_single_broadcast_32
+
 all: $(BINS)
+
+$(OUT)/bin/pagerank_directed_single_broadcast_32: | $(OUT)/bin
+	tmp=$$(mktemp -d /tmp/ipr_ref_directed_XXXXXX) && \
+	sed 's/bool directed = false;/bool directed = true;/' $(REF)/benchmarks/pagerank.c > $$tmp/pagerank.c && \
+	grep -q 'bool directed = true;' $$tmp/pagerank.c && \
+	gcc $(CFLAGS) $(DEFS) -DIP_APPLICATION=\"pagerank_directed_single_broadcast\" -DIP_USE_SINGLE_BROADCAST \
+	    -DIP_VERTEX_ID_TYPE=uint32_t -I$(REF)/src $$tmp/pagerank.c -o $@; \
+	rc=$$?; rm -rf $$tmp; exit $$rc
 
 $(OUT)/bin:
 	mkdir -p $@

@@ -7,6 +7,8 @@ reference binary that applies, its ip_dump output (<case>__<bin>.out) and its
 per-superstep ActiveVertexCount trace (<case>__<bin>.trace, one number a line).
 Four cases are also written with 64-bit ids (<case>_w64.*) and run by the
 reference's _64 builds (<case>_w64__<bin>_64.out / .trace).
+pagerank_directed_single_broadcast_32 is benchmarks/pagerank.c with
+directed=true (oracle/ref.mk), run on every case.
 SSSP runs from vertex 0: the reference cannot run from another source
 (benchmarks/sssp.c:101 reads the schedule string).
 
@@ -29,7 +31,7 @@ from ipregel_amd import graph as G  # noqa: E402
 REF_BIN = ROOT / "oracle" / "_ref" / "bin"
 
 BINS = {
-    "pagerank": ["pagerank_single_broadcast_32"],
+    "pagerank": ["pagerank_single_broadcast_32", "pagerank_directed_single_broadcast_32"],
     "cc": ["cc_32", "cc_spread_32", "cc_single_broadcast_32", "cc_single_broadcast_spread_32"],
     "sssp": ["sssp_32", "sssp_spread_32", "sssp_single_broadcast_32", "sssp_single_broadcast_spread_32"],
 }
@@ -67,6 +69,8 @@ def run_ref(binname: str, root: Path, out: Path) -> list[int]:
 # cases also written with 64-bit ids (<case>_w64.*: the .adj holds uint64
 # entries, as the _64 builds read it) and run by the _64 builds
 WIDE_CASES = ["chain10", "star65", "twocomp10", "rmat10"]
+# reference binaries built in the _32 width only (oracle/ref.mk)
+NARROW_ONLY = {"pagerank_directed_single_broadcast_32"}
 
 
 def write_wide(g: G.Graph, root: Path) -> None:
@@ -91,6 +95,8 @@ def main() -> None:
             write_wide(g, wroot)
             for app, bins in BINS.items():
                 for b in bins:
+                    if b in NARROW_ONLY:
+                        continue
                     b64 = b.replace("_32", "_64")
                     out = HERE / f"{name}_w64__{b64}.out"
                     trace = run_ref(b64, wroot, out)

@@ -45,9 +45,11 @@ def test_app_libs_export_driver_and_ipregel_api(lib):
 
 def test_all_apps_built():
     from ipregel_amd.build import APPS
-    for name, _, _ in APPS:
-        assert (LIB / f"libipr_{name}_32.so").exists(), name
-        assert (ROOT / "ipregel_amd" / "bin" / f"{name}_32").exists(), name
+    for entry in APPS:
+        name = entry[0]
+        for width in (entry[3] if len(entry) > 3 else (32,)):
+            assert (LIB / f"libipr_{name}_{width}.so").exists(), name
+            assert (ROOT / "ipregel_amd" / "bin" / f"{name}_{width}").exists(), name
 
 
 def test_batch_lib_exports():

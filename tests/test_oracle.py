@@ -12,16 +12,21 @@ APP = {"pagerank": O.PR, "cc": O.CC, "sssp": O.SSSP}
 
 
 def parse_bin(binname):
+    """reference binary name -> (app, variant, directed); a "_directed" infix
+    is benchmarks/pagerank.c built with directed=true (oracle/ref.mk)"""
     app = binname.split("_")[0]
     variant = binname[len(app):-3]
-    return APP[app], O.VARIANTS[variant]
+    directed = variant.startswith("_directed")
+    if directed:
+        variant = variant[len("_directed"):]
+    return APP[app], O.VARIANTS[variant], directed
 
 
 @pytest.mark.parametrize("case,binname", golden_runs())
 def test_oracle_matches_reference_binary(case, binname, tmp_path):
     g = G.read(GOLDEN / case)
-    app, variant = parse_bin(binname)
-    r = O.run(g, app, variant, directed=False, source=0)
+    app, variant, directed = parse_bin(binname)
+    r = O.run(g, app, variant, directed=directed, source=0)
     out = tmp_path / "o.txt"
     O.dump(app, r.values, str(out))
     assert out.read_text() == (GOLDEN / f"{case}__{binname}.out").read_text()
@@ -87,19 +92,53 @@ def test_dijkstra_closed_form():
     assert O.dijkstra(g, 0).tolist() == [0, 5, 1, 6]
 
 
-def dense_pagerank(g, rounds=10):
+def dense_pagerank(g, rounds=10, transpose=False):
     """x_s = 0.15/V + 0.85 * A (x_{s-1}/outdeg), A[v, w] = 1 for w in adj(v):
-    pull over the vertex's own list, as the reference does with directed=false."""
+    pull over the vertex's own list, as the reference does with directed=false.
+    transpose=True uses A^T (v sums over the u with v in adj(u)): PageRank of
+    the directed graph, what a push of x/outdeg along the out-arcs computes
+    and what a pull over mirrored in-lists (directed=true) computes."""
     V = g.V
     deg = np.diff(g.idx).astype(np.float64)
     A = np.zeros((V, V))
     for v in range(V):
         A[v, g.adj[g.idx[v]:g.idx[v + 1]]] = 1.0
+    if transpose:
+        A = A.T.copy()
     inv = np.where(deg > 0, 1.0 / np.where(deg > 0, deg, 1), 0.0)
     x = np.full(V, 1.0 / V)
     for _ in range(rounds):
         x = 0.15 / V + 0.85 * (A @ (x * inv))
     return x
+
+
+def test_dense_pagerank_transpose_on_a_hand_case():
+    # 0 -> 1, 0 -> 2, 1 -> 2: with A^T vertex 2 gathers x0/2 + x1/1
+    g = G.from_arcs(3, [0, 0, 1], [1, 2, 2])
+    x = dense_pagerank(g, rounds=1, transpose=True)
+    v = 1.0 / 3
+    assert x[0] == 0.15 / 3
+    assert abs(x[1] - (0.15 / 3 + 0.85 * v / 2)) < 1e-15
+    assert abs(x[2] - (0.15 / 3 + 0.85 * (v / 2 + v))) < 1e-15
+
+
+def test_push_pagerank_matches_dense_transposed_recurrence():
+    """The oracle's push variant (messages x/outdeg along the out-arcs, summed
+    in program order) is PageRank of G: the A^T recurrence."""
+    g = G.rmat(9, 16, seed=7)
+    r = O.run(g, O.PR, O.PUSH)
+    np.testing.assert_allclose(r.values, dense_pagerank(g, transpose=True), rtol=1e-12, atol=0)
+    assert r.supersteps == 11
+
+
+def test_directed_pull_pagerank_matches_dense_transposed_recurrence():
+    """With directed=true the pull reads mirrored in-lists: the same A^T
+    recurrence as the push, and a result that differs from directed=false."""
+    g = G.rmat(9, 16, seed=7)
+    r = O.run(g, O.PR, O.PULL, directed=True)
+    ref = dense_pagerank(g, transpose=True)
+    np.testing.assert_allclose(r.values, ref, rtol=1e-12, atol=0)
+    assert not np.allclose(O.run(g, O.PR, O.PULL).values, ref, rtol=1e-6)
 
 
 def test_pagerank_matches_dense_recurrence():
