@@ -86,6 +86,7 @@ typedef struct ipr_run_stats {
     double kernel_ms;           /* dominant kernel: summed event time of live launches */
     uint64_t kernel_launches;   /* dominant kernel: live launches timed */
     char device_name[128];
+    uint64_t fused_steps;       /* supersteps of the last run run by the fused sparse kernel */
 } ipr_run_stats;
 
 void ipr_harness_configure(const ipr_harness_cfg* cfg);

@@ -64,7 +64,13 @@ enum {
     IPR_C_BCAST1 = 5,    /* ... parity 1 */
     IPR_C_GNEXT = 6,     /* gather range counters: IPR_GATHER_CTRS of them, 128 bytes apart
                           * (spans this id and the next; zeroed at every superstep's end) */
-    IPR_NCNT = 8
+    IPR_C_ARRIVE = 8,    /* workgroups of the running kernel that have finished, by
+                          * blockIdx mod 64, for kernels whose last workgroup ends
+                          * the superstep ... */
+    IPR_C_ARRIVE_TOP = 9,/* ... slot 0: the slots whose workgroups have all finished */
+    IPR_C_FARCS = 10,    /* spread: out-arcs of the frontier being compacted */
+    IPR_C_FMAX = 11,     /* spread: its largest out-degree (per-slot maxima) */
+    IPR_NCNT = 12
 };
 
 /* device-resident control block: every kernel reads the superstep and the
@@ -84,7 +90,19 @@ typedef struct ipr_ctl {
     uint64_t executed;           /* supersteps executed so far */
     uint64_t last_active;        /* ActiveVertexCount of the last finalized superstep */
     uint64_t gathered;           /* supersteps delivered by gather */
-    uint64_t pad[4];
+    /* spread variant, by superstep parity: the out-arcs of the frontier's
+     * vertices and the largest of their out-degrees, summed while the frontier
+     * is built; they decide whether a superstep can run as one fused sparse
+     * kernel, and are cleared when that frontier has been computed */
+    uint64_t farcs[2];
+    uint64_t fmaxdeg[2];
+    uint32_t bail;               /* a fused sparse kernel found its superstep too dense and stopped */
+    uint32_t fused_steps;        /* supersteps run by the fused sparse kernel */
+    /* the superstep a chunk of fused sparse kernels starts at, set before the
+     * chunk is launched; kernel k of the chunk runs superstep chunk_base + k
+     * (never ctl->superstep, which its own last workgroup advances) */
+    uint64_t chunk_base;
+    uint64_t pad[1];
 } ipr_ctl;
 
 #define IPR_LPACK_SHIFT 40
@@ -180,6 +198,19 @@ typedef struct ipr_dev {
     uint32_t pull_ordered;       /* 1: fold in in-list order (bit-exact) */
     uint32_t dir_opt;            /* push apps: allow gathering dense supersteps */
     uint32_t dir_alpha;          /* gather when bcast out-arcs * alpha > E */
+    /* spread variant: supersteps whose frontier has at most fused_max_arcs
+     * out-arcs, none above fused_max_deg, and would not gather run as the
+     * fused compute + expansion kernel and a compaction that ends the
+     * superstep (0: off) */
+    uint32_t fused;
+    uint32_t fused_blocks;       /* workgroups of the fused kernel */
+    uint64_t fused_max_arcs;
+    uint64_t fused_max_deg;
+    /* the last workgroup of the fused pull kernel (pull) or of the frontier
+     * compaction (spread) ends the superstep: no separate finalize launch */
+    uint32_t fin_in_kernel;
+    /* position of a fused sparse kernel in its chunk (set per launch) */
+    uint32_t step_in_chunk;
 } ipr_dev;
 
 /* 4-byte broadcast record: bits 31-26 the superstep's stamp (superstep mod
@@ -210,6 +241,7 @@ enum {
     IPR_K_FETCH,         /* SPREAD_PULL: fetch for targets */
     IPR_K_GATHER,        /* push apps, dense superstep: receivers gather */
     IPR_K_GATHER_LIVE,   /* push apps, dense superstep with few broadcasters: their bitmap */
+    IPR_K_SPARSE_STEP,   /* SPREAD, sparse superstep: compute, expansion, next frontier, end */
     IPR_K_COUNT
 };
 

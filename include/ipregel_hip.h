@@ -48,6 +48,9 @@ int ipr_hip_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream);
 int ipr_hip_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream);
 /* fill with a byte value by kernel (safe to record in a HIP graph) */
 int ipr_hip_fill(void* dst, int byte, size_t bytes, void* stream);
+/* store one 64-bit value in device memory, in stream order (a kernel, so it
+ * may also be recorded into a HIP graph) */
+int ipr_hip_set_u64(void* dst, uint64_t value, void* stream);
 
 int ipr_hip_stream_create(void** stream);
 int ipr_hip_stream_destroy(void* stream);
@@ -108,7 +111,9 @@ int ipr_hip_equal(const void* a, const void* b, uint64_t bytes, int* equal, void
 
 /* push variants: the next frontier (claim bitmap of superstep s+1) compacted
  * into front[(s+1)&1], ascending within runs; the bitmap is cleared */
-int ipr_hip_compact_frontier(const ipr_dev* dev, void* stream);
+/* finalize_variant >= 0: the compaction's last workgroup also ends the
+ * superstep of that variant (ipregel_control.h), with no finalize launch */
+int ipr_hip_compact_frontier(const ipr_dev* dev, int finalize_variant, void* stream);
 
 /* superstep control (kernels on `stream`, graph-capturable) */
 int ipr_hip_begin(const ipr_dev* dev, void* stream);

@@ -41,6 +41,8 @@
 #include <limits>
 #include <type_traits>
 
+#include "ipregel_control.h"
+
 #define IPR_BLOCK 256
 #define IPR_WAVE 64
 #define IPR_WAVES (IPR_BLOCK / IPR_WAVE)
@@ -199,7 +201,9 @@ template <typename T> struct ipr_comb<T, IPR_COMBINER_MIN> {
     __device__ static slot_t empty() { return ident(); }
     __device__ static T value(slot_t s) { return s; }
     __device__ static void fold(T* acc, T m) { if (m < *acc) *acc = m; }
-    /* returns true when this delivery may be the first one */
+    /* returns true when this delivery may be the first one (waiting for the
+     * atomic's old value measured faster than raising the receiver's flag on
+     * every lowering delivery: fewer flag reads) */
     __device__ static bool deliver(slot_t* p, T m)
     {
         T cur = *p;                       /* stale reads only overestimate */
@@ -393,7 +397,7 @@ __device__ inline void ipr_emit(const ipr_dev& d, uint64_t s, uint32_t u, IP_MES
 __device__ inline void ipr_send_immediate(IP_VERTEX_ID_TYPE id, IP_MESSAGE_TYPE m)
 {
     const ipr_dev& d = ipr_d_dev;
-    ipr_deliver(d, d.ctl->superstep, (uint32_t)id, m);
+    ipr_deliver(d, ipr_sm_superstep, (uint32_t)id, m);
 }
 
 __device__ inline IP_MESSAGE_TYPE ipr_edge_msg(const uint32_t* wgt, IP_MESSAGE_TYPE m, bool weighted, uint64_t e)
@@ -691,6 +695,88 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_expand(ipr_dev d)
         __syncthreads();
     }
 }
+
+/* ------------------------------------------------------------------------ */
+/* spread variant: one fused kernel per sparse superstep                     */
+/* ------------------------------------------------------------------------ */
+
+#if IPR_VARIANT == 2
+/* The compute and the expansion of a sparse superstep of the spread variant
+ * in one launch (src/combiner_spread_postamble.h:160-431 phases 1-2): every
+ * vertex of the (ascending) frontier list takes its combined message from the
+ * mailbox and runs ip_compute; the broadcasts of a tile of 256 vertices are
+ * laid end to end by a workgroup prefix sum and delivered 256 arcs at a time,
+ * raising the receivers' flags. The frontier compaction that follows builds
+ * the next list in ascending order and its last workgroup ends the superstep
+ * (no finalize launch). No broadcast record is written: nothing reads one in
+ * a sparse superstep. Large broadcasters are not split into chunks here: a
+ * frontier with a vertex above fused_max_deg out-arcs, or with enough arcs to
+ * gather (or above fused_max_arcs), is left to the general kernels: the kernel
+ * raises ctl->bail and returns, and so does the compaction after it. */
+__global__ void __launch_bounds__(IPR_BLOCK) ipr_k_sparse_step(ipr_dev d)
+{
+    __shared__ uint32_t s_ex[IPR_BLOCK];
+    __shared__ uint64_t s_off[IPR_BLOCK];
+    __shared__ IP_MESSAGE_TYPE s_msg[IPR_BLOCK];
+    __shared__ uint8_t s_wf[IPR_BLOCK];
+    __shared__ uint32_t scan[IPR_WAVES];
+    ipr_ctl* ctl = d.ctl;
+    /* the superstep: the chunk's base plus this kernel's place in it (a
+     * kernel after a bail in the same chunk finds ctl->superstep behind) */
+    const uint64_t s = ctl->chunk_base + d.step_in_chunk;
+    if (ctl->done || ctl->bail || ctl->superstep != s) return;
+    const uint64_t n = ctl->fsize[s & 1];
+    const uint64_t arcs = ctl->farcs[s & 1], maxdeg = ctl->fmaxdeg[s & 1];
+    if (s == 0 || arcs > d.E || arcs > d.fused_max_arcs || maxdeg > d.fused_max_deg ||
+        (d.dir_opt && arcs * (uint64_t)d.dir_alpha > d.E)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) ctl->bail = 1;
+        return;
+    }
+    ipr_sm_superstep = s;
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->fused_steps++;
+    const uint32_t* list = d.front[s & 1];
+    ipr_slot_t* mcur = (ipr_slot_t*)d.mbox[s & 1];
+    uint64_t n_computed = 0, n_edges = 0;
+    for (uint64_t base = (uint64_t)blockIdx.x * IPR_BLOCK; base < n; base += (uint64_t)gridDim.x * IPR_BLOCK) {
+        const uint64_t i = base + threadIdx.x;
+        uint32_t deg = 0;
+        if (i < n) {
+            const uint32_t v = list[i];
+            struct ip_vertex_t x;
+            x.active = true;
+            x.has_message = true;
+            x.message = ipr_policy::value(mcur[v]);
+            mcur[v] = ipr_policy::empty();
+            const uint64_t o0 = d.out_idx[v], o1 = d.out_idx[v + 1];
+            ipr_fill_vertex(d, &x, v);
+            const IP_VALUE_TYPE before = x.value;
+            ip_compute(&x);
+            ipr_store_value(d, v, before, x.value);
+            n_computed++;
+            if (x.ipr_bcast) {
+                n_edges += o1 - o0;
+                deg = (uint32_t)(o1 - o0);
+                s_off[threadIdx.x] = o0;
+                s_msg[threadIdx.x] = x.ipr_bcast_msg;
+                s_wf[threadIdx.x] = x.ipr_bcast_weighted;
+            }
+        }
+        uint32_t total = 0;
+        const uint32_t ex = ipr_block_excl_scan(deg, scan, &total);
+        s_ex[threadIdx.x] = ex;
+        __syncthreads();
+        for (uint32_t t = threadIdx.x; t < total; t += IPR_BLOCK) {
+            uint32_t lo = 0;                     /* last owner with start <= t */
+            for (uint32_t step = IPR_BLOCK / 2; step > 0; step >>= 1)
+                if (s_ex[lo + step] <= t) lo += step;
+            const uint64_t e = s_off[lo] + (t - s_ex[lo]);
+            ipr_deliver(d, s, d.out_adj[e], ipr_edge_msg(d.out_wgt, s_msg[lo], s_wf[lo] != 0, e));
+        }
+        __syncthreads();
+    }
+    ipr_block_count3(d, -1, 0, IPR_C_WORKV, n_computed, IPR_C_EDGES, n_edges);
+}
+#endif /* IPR_VARIANT == 2 (spread) */
 
 /* ------------------------------------------------------------------------ */
 /* push variants: gather (dense supersteps)                                  */
@@ -1314,6 +1400,9 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_pull_step(ipr_dev d, uint32_t
         }
     }
     ipr_pull_counts(d, s, n_react, n_active, n_bcast);
+    /* the last workgroup ends superstep s-1: no finalize launch */
+    if (d.fin_in_kernel && ipr_last_workgroup(d, gridDim.x) && threadIdx.x < IPR_WAVE)
+        ipr_finalize_step(d, IPR_VARIANT_PULL);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1457,6 +1546,14 @@ static int ipr_app_launch(int kernel, const ipr_dev* dp, void* stream)
                            dim3(IPR_BLOCK), 0, st, d);
         break;
     }
+    case IPR_K_SPARSE_STEP:
+#if IPR_VARIANT == 2
+        hipLaunchKernelGGL(ipr_k_sparse_step, dim3(d.fused_blocks ? d.fused_blocks : 1024), dim3(IPR_BLOCK), 0, st,
+                           d);
+        break;
+#else
+        return -1;
+#endif
     case IPR_K_PULL_STEP: {
         uint32_t nb[3];
         ipr_bin_blocks(d, nb);

@@ -48,10 +48,12 @@ static struct {
     void* ev0;
     void* ev1;
     void* graphs[IPR_MAX_GRAPHS];
+    void* fgraphs[IPR_MAX_GRAPHS];   /* chunks of fused sparse supersteps */
     uint64_t wall_hz;
     ipr_run_stats stats;
     int use_graph;
     uint32_t max_chunk;
+    uint32_t first_chunk;
     int quiet;
     int kev_on;              /* events around the dominant kernel */
     void* kev[3][64];        /* start / split / stop per superstep slot of a chunk */
@@ -153,7 +155,8 @@ static void dev_free_all(void)
 {
     for (int i = 0; i < IPR_MAX_GRAPHS; i++) {
         if (D.graphs[i]) ipr_hip_graph_destroy(D.graphs[i]);
-        D.graphs[i] = NULL;
+        if (D.fgraphs[i]) ipr_hip_graph_destroy(D.fgraphs[i]);
+        D.graphs[i] = D.fgraphs[i] = NULL;
     }
     ipr_dev* d = &D.dev;
     if (d->in_ids && d->in_ids != d->in_adj && d->in_ids != d->out_ids) ipr_hip_free((void*)d->in_ids);
@@ -430,9 +433,30 @@ static void dev_setup(void)
     }
     /* frontier compaction rounds per workgroup (0: picked from V) */
     d->compact_rounds = env_u32("IPR_COMPACT_ROUNDS", 0);
+    /* spread: a sparse superstep is two launches, the fused compute and
+     * expansion, then the frontier compaction whose last workgroup ends the
+     * superstep; a frontier vertex above large_min out-arcs keeps the
+     * superstep on the general kernels, which split it in chunks */
+    d->fused = var == IPR_VARIANT_SPREAD && env_u32("IPR_FUSED", 1) == 1;
+    d->fused_blocks = env_u32("IPR_FUSED_BLOCKS", 1024);
+    if (d->fused_blocks < 1) d->fused_blocks = 1;
+    d->fused_max_deg = env_u32("IPR_FUSED_MAX_DEG", d->large_min);
+    {
+        const char* fa = getenv("IPR_FUSED_MAX_ARCS");
+        d->fused_max_arcs = fa && *fa ? strtoull(fa, NULL, 10) : ~0ull;
+    }
+    /* pull and spread: the last workgroup of the fused pull kernel or of the
+     * frontier compaction ends the superstep (no finalize launch) */
+    d->fin_in_kernel = (pull || var == IPR_VARIANT_SPREAD) && env_u32("IPR_FIN_IN_KERNEL", 1) == 1;
     uint32_t ug = c && c->use_graph ? (uint32_t)c->use_graph : env_u32("IPR_GRAPH", 1);
     D.use_graph = ug == 1;
     D.max_chunk = c && c->max_chunk ? c->max_chunk : env_u32("IPR_MAX_CHUNK", 64);
+    /* a power of two; the pull kernel's no-op launches are cheap */
+    {
+        uint32_t fc = env_u32("IPR_FIRST_CHUNK", pull ? 16 : 1), p2 = 1;
+        while (p2 < fc && p2 < 64) p2 *= 2;
+        D.first_chunk = p2;
+    }
     if (D.max_chunk < 1) D.max_chunk = 1;
     hip_check(ipr_hip_host_alloc((void**)&D.h_ctl, sizeof(ipr_ctl)), "pinned ctl");
     hip_check(ipr_hip_event_create(&D.ev0), "event");
@@ -703,10 +727,31 @@ static void pmark(int which)
     hip_check(ipr_hip_event_record(D.pev[which][D.kev_slot], D.stream), "profile event");
 }
 
-/* one superstep of the variant's BSP loop, in phase order */
-static void enqueue_superstep(void)
+/* one superstep of the variant's BSP loop, in phase order; `fused`: the
+ * spread variant's sparse superstep as one kernel, which ends the superstep
+ * itself (its time counts as the delivery phase of the kernel events and as
+ * compute in the phase profile) */
+static void enqueue_superstep(int fused, uint32_t step_in_chunk)
 {
     pmark(0);
+    if (fused) {
+        /* the fused kernel takes its superstep from the chunk's base plus its
+         * place in the chunk, passed in its copy of the device state */
+        ipr_dev dk = D.dev;
+        dk.step_in_chunk = step_in_chunk;
+        kev_mark(0);
+        kev_mark(1);
+        int rc = D.app.launch(IPR_K_SPARSE_STEP, &dk, D.stream);
+        if (rc) die("HIP failure launching kernel %d (%d)\n", IPR_K_SPARSE_STEP, rc);
+        kev_mark(2);
+        pmark(1);
+        pmark(2);
+        /* the ascending next frontier; its last workgroup ends the superstep */
+        hip_check(ipr_hip_compact_frontier(&D.dev, IPR_VARIANT_SPREAD, D.stream), "compact");
+        pmark(3);
+        if ((D.kev_on || D.prof_on) && D.kev_slot < 64) D.kev_slot++;
+        return;
+    }
     switch (D.app.variant) {
     case IPR_VARIANT_PULL:
         /* the fused kernel is fetch(s-1) + compute(s): counted as compute */
@@ -728,7 +773,7 @@ static void enqueue_superstep(void)
         kev_mark(1);
         launch_app(IPR_K_EXPAND);
         kev_mark(2);
-        hip_check(ipr_hip_compact_frontier(&D.dev, D.stream), "compact");
+        hip_check(ipr_hip_compact_frontier(&D.dev, -1, D.stream), "compact");
         launch_app(IPR_K_FETCH);
         pmark(2);
         break;
@@ -743,8 +788,10 @@ static void enqueue_superstep(void)
         launch_app(IPR_K_EXPAND);
         kev_mark(2);
         pmark(2);
+        /* spread: the compaction's last workgroup ends the superstep */
         if (D.app.variant == IPR_VARIANT_SPREAD)
-            hip_check(ipr_hip_compact_frontier(&D.dev, D.stream), "compact");
+            hip_check(ipr_hip_compact_frontier(&D.dev, D.dev.fin_in_kernel ? IPR_VARIANT_SPREAD : -1, D.stream),
+                      "compact");
         else
             launch_app(IPR_K_SWAP);
         break;
@@ -752,26 +799,47 @@ static void enqueue_superstep(void)
         die("Engine variant %d is not available in this build.\n", D.app.variant);
     }
     pmark(3);
-    hip_check(ipr_hip_finalize(&D.dev, D.app.variant, D.stream), "finalize");
+    /* the fused pull kernel's or the spread compaction's last workgroup ends
+     * the superstep itself */
+    if (!((D.app.variant == IPR_VARIANT_PULL || D.app.variant == IPR_VARIANT_SPREAD) && D.dev.fin_in_kernel))
+        hip_check(ipr_hip_finalize(&D.dev, D.app.variant, D.stream), "finalize");
     /* the next superstep of the chunk records into the next event slot */
     if ((D.kev_on || D.prof_on) && D.kev_slot < 64) D.kev_slot++;
 }
 
-static void enqueue_chunk(uint32_t n)
+/* n supersteps (a power of two up to 64): replayed from a cached HIP graph
+ * per (size, kind of chunk) when graphs are on. A fused chunk starts at
+ * superstep `base`, stored in the control block before the chunk. */
+static void enqueue_chunk(uint32_t n, int fused, uint64_t base)
 {
     int slot = 0;
     while ((1u << slot) < n) slot++;
     D.kev_slot = 0;
+    if (fused) hip_check(ipr_hip_set_u64(&D.dev.ctl->chunk_base, base, D.stream), "chunk base");
+    void** cache = fused ? D.fgraphs : D.graphs;
     if (D.use_graph && !D.kev_on && !D.prof_on && slot < IPR_MAX_GRAPHS) {
-        if (!D.graphs[slot]) {
+        if (!cache[slot]) {
             hip_check(ipr_hip_graph_begin(D.stream), "graph capture");
-            for (uint32_t i = 0; i < n; i++) enqueue_superstep();
-            hip_check(ipr_hip_graph_end(D.stream, &D.graphs[slot]), "graph capture");
+            for (uint32_t i = 0; i < n; i++) enqueue_superstep(fused, i);
+            hip_check(ipr_hip_graph_end(D.stream, &cache[slot]), "graph capture");
         }
-        hip_check(ipr_hip_graph_launch(D.graphs[slot], D.stream), "graph launch");
+        hip_check(ipr_hip_graph_launch(cache[slot], D.stream), "graph launch");
     } else {
-        for (uint32_t i = 0; i < n; i++) enqueue_superstep();
+        for (uint32_t i = 0; i < n; i++) enqueue_superstep(fused, i);
     }
+}
+
+/* the superstep about to run can be a fused sparse one: its frontier's
+ * out-arcs (summed while the frontier was built) would not gather, and none
+ * of its vertices has more than fused_max_deg out-arcs */
+static int fusable(const ipr_ctl* c)
+{
+    const ipr_dev* d = &D.dev;
+    const uint64_t s = c->superstep, arcs = c->farcs[s & 1];
+    if (!d->fused || s == 0) return 0;
+    /* after a dense superstep the figures are left unknown (all ones) */
+    if (arcs > d->E || arcs > d->fused_max_arcs || c->fmaxdeg[s & 1] > d->fused_max_deg) return 0;
+    return !(d->dir_opt && arcs * (uint64_t)d->dir_alpha > d->E);
 }
 
 /* flags raised by the device-side API (IPR_ERR_* of ipregel_kernels.h) */
@@ -798,10 +866,16 @@ static void run_once(void)
     hip_check(ipr_hip_begin(&D.dev, st), "begin");
     launch_app(IPR_K_INIT);
     hip_check(ipr_hip_event_record(D.ev0, st), "event");
-    uint32_t chunk = 1;
+    /* supersteps of the first chunk: each chunk ends with a host check of the
+     * control block (about 30 us), each superstep launched after the loop has
+     * halted exits at once (one launch for pull, five for the others) */
+    uint32_t chunk = D.first_chunk < D.max_chunk ? D.first_chunk : D.max_chunk;
     uint64_t launched = 0, prev_exec = 0;
+    int fused = 0;
     for (;;) {
-        enqueue_chunk(chunk);
+        /* a fused chunk starts at the superstep the last readback showed
+         * (fused mode is only entered after one) */
+        enqueue_chunk(chunk, fused, fused ? D.h_ctl->superstep : 0);
         launched += chunk;
         hip_check(ipr_hip_memcpy_d2h(D.h_ctl, D.dev.ctl, sizeof(ipr_ctl), st), "ctl readback");
         hip_check(ipr_hip_stream_sync(st), "superstep chunk");
@@ -839,6 +913,19 @@ static void run_once(void)
         }
         prev_exec = D.h_ctl->executed;
         if (D.h_ctl->done) break;
+        if (fused && D.h_ctl->bail) {
+            /* back to the general kernels (the compaction after every fused
+             * superstep kept the 4-byte records' stamp-window clears) */
+            hip_check(ipr_hip_fill(&D.dev.ctl->bail, 0, sizeof(uint32_t), st), "bail clear");
+            fused = 0;
+            chunk = 1;
+            continue;
+        }
+        if (!fused && fusable(D.h_ctl)) {
+            fused = 1;
+            chunk = D.first_chunk < D.max_chunk ? D.first_chunk : D.max_chunk;
+            continue;
+        }
         if (chunk < D.max_chunk) chunk *= 2;
         if (chunk > D.max_chunk) chunk = D.max_chunk;
     }
@@ -854,6 +941,7 @@ static void run_once(void)
     D.stats.work_edges = D.h_ctl->work_edges;
     D.stats.work_vertices = D.h_ctl->work_vertices;
     D.stats.gathered = D.h_ctl->gathered;
+    D.stats.fused_steps = D.h_ctl->fused_steps;
 }
 
 static int run_and_report(int reps);

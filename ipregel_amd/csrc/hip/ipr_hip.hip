@@ -12,6 +12,7 @@
 #include <rocprim/device/device_scan.hpp>
 
 #include "ipregel_hip.h"
+#include "ipregel_control.h"
 
 static thread_local char g_err[512];
 
@@ -126,6 +127,18 @@ extern "C" int ipr_hip_fill(void* dst, int byte, size_t bytes, void* stream)
     if (blocks > 8192) blocks = 8192;
     hipLaunchKernelGGL(k_fill, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
                        (uint8_t*)dst, (uint8_t)byte, bytes);
+    CK(hipGetLastError());
+    return 0;
+}
+
+__global__ void k_set_u64(uint64_t* p, uint64_t v)
+{
+    *p = v;
+}
+
+extern "C" int ipr_hip_set_u64(void* dst, uint64_t value, void* stream)
+{
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, (hipStream_t)stream, (uint64_t*)dst, value);
     CK(hipGetLastError());
     return 0;
 }
@@ -661,18 +674,31 @@ extern "C" int ipr_hip_build_pull_bins(ipr_dev* dev, uint32_t small_max, uint32_
  * consecutive entries per wavefront store. T is 4, or less for graphs that
  * would otherwise get under 256 workgroups. */
 template <int T>
-__global__ void __launch_bounds__(256) k_compact(ipr_dev d)
+__global__ void __launch_bounds__(256) k_compact(ipr_dev d, int fin_variant)
 {
     __shared__ uint32_t wsum[T][4];
     __shared__ uint32_t stage[256 * 16];
     __shared__ unsigned long long blk_base;
     ipr_ctl* c = d.ctl;
-    if (c->done) return;
+    /* a fused sparse kernel that stopped (bail) leaves its superstep to the
+     * general kernels: no compaction either */
+    if (c->done || c->bail) return;
     const uint64_t s = c->superstep;
     uint4* flags = (uint4*)d.recv[(s + 1) & 1];
     uint32_t* out = d.front[(s + 1) & 1];
     unsigned long long* counter = (unsigned long long*)&c->fsize[(s + 1) & 1];
     const uint64_t groups = (d.V + 15) / 16;           /* flag arrays are padded to 16 */
+    /* with the fused sparse kernel on, after a superstep that did not gather:
+     * out-arcs of the new frontier (sum and largest out-degree). After a
+     * dense superstep they are not read (the end of the superstep marks them
+     * unknown), which spares a read per entry of a large frontier. */
+    uint64_t f_arcs = 0, f_max = 0;
+    bool want_f = d.fused;
+    if (want_f && d.dir_opt) {
+        uint64_t w = d.cnt[IPR_C_EDGES * IPR_CSLOTS + (threadIdx.x & 63)];
+        for (int o = 32; o > 0; o >>= 1) w += __shfl_xor(w, o);
+        want_f = w * (uint64_t)d.dir_alpha <= d.E;
+    }
     const uint64_t span = 256ull * T;
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     /* the last superstep of a stamp window clears the 4-byte gather records
@@ -735,19 +761,54 @@ __global__ void __launch_bounds__(256) k_compact(ipr_dev d)
                     uint32_t w = words[k];
                     while (w) {
                         uint32_t bit = __ffs(w) - 1;
-                        stage[pos++] = (uint32_t)(g * 16 + k * 4 + bit / 8);
+                        const uint32_t id = (uint32_t)(g * 16 + k * 4 + bit / 8);
+                        stage[pos++] = id;
                         w &= w - 1;
                     }
                 }
                 __syncthreads();
-                for (uint32_t i = threadIdx.x; i < cnt[t]; i += 256) out[base + first[t] + i] = stage[i];
+                /* coalesced write-out; the out-degrees of consecutive entries
+                 * are read here, by consecutive threads */
+                for (uint32_t i = threadIdx.x; i < cnt[t]; i += 256) {
+                    const uint32_t id = stage[i];
+                    out[base + first[t] + i] = id;
+                    if (want_f) {
+                        const uint64_t dg = d.out_idx[id + 1] - d.out_idx[id];
+                        f_arcs += dg;
+                        f_max = dg > f_max ? dg : f_max;
+                    }
+                }
                 __syncthreads();
             }
         }
     }
+    if (want_f) {
+        /* one atomic per workgroup and figure */
+        __shared__ uint64_t r_arcs[4], r_max[4];
+        for (int o = 32; o > 0; o >>= 1) {
+            f_arcs += __shfl_xor(f_arcs, o);
+            const uint64_t m = __shfl_xor(f_max, o);
+            f_max = m > f_max ? m : f_max;
+        }
+        if (lane == 0) { r_arcs[wave] = f_arcs; r_max[wave] = f_max; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint64_t a = 0, m = 0;
+            for (int w = 0; w < 4; w++) { a += r_arcs[w]; m = r_max[w] > m ? r_max[w] : m; }
+            /* spread over the 64 slots; the end of the superstep collapses
+             * them into ctl->farcs / fmaxdeg */
+            const uint32_t slot = blockIdx.x % IPR_CSLOTS;
+            if (a) atomicAdd((unsigned long long*)&d.cnt[IPR_C_FARCS * IPR_CSLOTS + slot], (unsigned long long)a);
+            if (m) atomicMax((unsigned long long*)&d.cnt[IPR_C_FMAX * IPR_CSLOTS + slot], (unsigned long long)m);
+        }
+    }
+    /* after a fused sparse superstep: the last workgroup ends the superstep
+     * (every workgroup counts, so none reads the control block after it) */
+    if (fin_variant >= 0 && ipr_last_workgroup(d, gridDim.x) && threadIdx.x < 64)
+        ipr_finalize_step(d, fin_variant);
 }
 
-extern "C" int ipr_hip_compact_frontier(const ipr_dev* dev, void* stream)
+extern "C" int ipr_hip_compact_frontier(const ipr_dev* dev, int finalize_variant, void* stream)
 {
     const uint64_t groups = (dev->V + 15) / 16;
     /* rounds per workgroup: 4, or fewer where that leaves under 256
@@ -764,10 +825,10 @@ extern "C" int ipr_hip_compact_frontier(const ipr_dev* dev, void* stream)
     const dim3 grid((unsigned)blocks), block(256);
     hipStream_t st = (hipStream_t)stream;
     switch (T) {
-    case 8: hipLaunchKernelGGL(k_compact<8>, grid, block, 0, st, *dev); break;
-    case 4: hipLaunchKernelGGL(k_compact<4>, grid, block, 0, st, *dev); break;
-    case 2: hipLaunchKernelGGL(k_compact<2>, grid, block, 0, st, *dev); break;
-    default: hipLaunchKernelGGL(k_compact<1>, grid, block, 0, st, *dev); break;
+    case 8: hipLaunchKernelGGL(k_compact<8>, grid, block, 0, st, *dev, finalize_variant); break;
+    case 4: hipLaunchKernelGGL(k_compact<4>, grid, block, 0, st, *dev, finalize_variant); break;
+    case 2: hipLaunchKernelGGL(k_compact<2>, grid, block, 0, st, *dev, finalize_variant); break;
+    default: hipLaunchKernelGGL(k_compact<1>, grid, block, 0, st, *dev, finalize_variant); break;
     }
     CK(hipGetLastError());
     return 0;
@@ -818,71 +879,11 @@ __global__ void k_begin(ipr_dev d)
     if (cap) d.trace_ts[0] = wall_clock64();
 }
 
-/* sum (and optionally clear) one spread counter; 64 threads */
-__device__ uint64_t ipr_take(const ipr_dev& d, int id, bool clear)
-{
-    uint64_t x = d.cnt[id * IPR_CSLOTS + threadIdx.x];
-    if (clear) d.cnt[id * IPR_CSLOTS + threadIdx.x] = 0;
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
-    return x;
-}
-
-/* End of one superstep: active count, halting test, superstep increment.
- * PULL finalizes superstep s-1 after the fused fetch(s-1)+compute(s) kernel
- * (src/combiner_single_broadcast_postamble.h:267 loop test `active != 0`);
- * SPREAD: src/combiner_spread_postamble.h:341 `first || active > 0` with
- * active = |next frontier| (:410-411); PUSH: src/combiner_postamble.h:229. */
+/* End of one superstep (ipregel_control.h): one wavefront. Launched after
+ * the superstep's kernels when none of them ends the superstep itself. */
 __global__ void k_finalize(ipr_dev d, int variant)
 {
-    /* one wavefront: lane k owns counter slot k */
-    ipr_ctl* c = d.ctl;
-    if (c->done) return;
-    const uint64_t s = c->superstep;
-    uint64_t fin, active, edges, computed;
-    if (variant == IPR_VARIANT_PULL) {
-        if (s == 0) {
-            if (threadIdx.x == 0) c->superstep = 1;
-            return;
-        }
-        fin = s - 1;
-        active = ipr_take(d, IPR_C_ACT0 + (int)(fin & 1), true);
-        uint64_t bc = ipr_take(d, IPR_C_BCAST0 + (int)(fin & 1), true);
-        computed = ipr_take(d, IPR_C_WORKV, true);
-        edges = bc ? d.in_idx[d.V] : 0;
-    } else {
-        fin = s;
-        active = variant == IPR_VARIANT_PUSH ? ipr_take(d, IPR_C_ACT0 + (int)(s & 1), true)
-                                             : c->fsize[(s + 1) & 1];
-        edges = ipr_take(d, IPR_C_EDGES, true);
-        computed = ipr_take(d, IPR_C_WORKV, true);
-        /* the gather's range counters start from 0 in the next superstep */
-        d.cnt[IPR_C_GNEXT * IPR_CSLOTS + threadIdx.x] = 0;
-        d.cnt[(IPR_C_GNEXT + 1) * IPR_CSLOTS + threadIdx.x] = 0;
-    }
-    if (threadIdx.x != 0) return;
-    if (variant == IPR_VARIANT_PULL) {
-        c->done = active == 0;
-        c->executed = fin + 1;
-        c->superstep = c->done ? s : s + 1;
-    } else {
-        if (variant != IPR_VARIANT_PUSH) c->fsize[s & 1] = 0;
-        c->done = active == 0;
-        c->executed = s + 1;
-        c->superstep = s + 1;
-    }
-    c->lpack = 0;
-    c->step_edges = edges;
-    c->work_edges += edges;
-    c->work_vertices += computed;
-    /* the delivery kernels gathered this superstep by the same rule */
-    c->pull_mode = variant != IPR_VARIANT_PULL && d.dir_opt && edges * (uint64_t)d.dir_alpha > d.E;
-    if (c->pull_mode) c->gathered++;
-    c->last_active = active;
-    if (fin < c->trace_cap) {
-        d.trace_active[fin] = active;
-        d.trace_edges[fin] = edges;
-        d.trace_ts[fin + 1] = wall_clock64();
-    }
+    ipr_finalize_step(d, variant);
 }
 
 extern "C" int ipr_hip_begin(const ipr_dev* dev, void* stream)

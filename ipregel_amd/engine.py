@@ -60,7 +60,7 @@ class RunStats(C.Structure):
         ("last_ms", C.c_double), ("total_ms", C.c_double), ("min_ms", C.c_double),
         ("device_ms", C.c_double), ("load_s", C.c_double),
         ("kernel_ms", C.c_double), ("kernel_launches", C.c_uint64),
-        ("device_name", C.c_char * 128),
+        ("device_name", C.c_char * 128), ("fused_steps", C.c_uint64),
     ]
 
 
@@ -81,6 +81,7 @@ class RunResult:
     gathered: int
     launches: int
     device_name: str
+    fused_steps: int = 0
     extra: dict = field(default_factory=dict)
 
 
@@ -168,7 +169,8 @@ def run_app(name: str, graph: Graph, *, source: int = 0, out_path: str | None = 
                     work_vertices=int(st.work_vertices), active=active, step_edges=sedges,
                     step_ns=sns, last_ms=st.last_ms, min_ms=st.min_ms, total_ms=st.total_ms,
                     device_ms=st.device_ms, runs=int(st.runs), gathered=int(st.gathered),
-                    launches=int(st.launches), device_name=st.device_name.decode(errors="replace"))
+                    launches=int(st.launches), device_name=st.device_name.decode(errors="replace"),
+                    fused_steps=int(st.fused_steps))
     # the C driver keeps pointers to the host arrays until the next ip_init
     L.ipr_harness_configure(None)
     if release:

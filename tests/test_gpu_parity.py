@@ -243,6 +243,105 @@ def test_gather_stamp_window_and_escapes(record4, monkeypatch):
         assert r.gathered > 0
 
 
+@pytest.mark.parametrize("fused,blocks,maxdeg,alpha", [
+    ("1", "", "", 0), ("0", "", "", 0), ("1", "1", "", 0), ("1", "3", "", 0),
+    ("1", "", "4", 0), ("1", "2", "2", 40), ("1", "", "", 40)])
+def test_fused_sparse_supersteps(fused, blocks, maxdeg, alpha, monkeypatch):
+    """Sparse supersteps of the spread variant as one fused kernel (compute,
+    expansion, next frontier appended through an LDS buffer, the last
+    workgroup ending the superstep), against the general kernels and the
+    oracle: values and per-superstep active counts. With 1 or 3 workgroups
+    (many tiles each, buffer flushes), with a tiny out-degree limit or a low
+    gather threshold (supersteps alternate between the fused and the general
+    kernels, the fused one stopping itself when its superstep turns dense),
+    on CC (R-MAT 14), BFS and weighted SSSP on a 57x61 grid (a hundred sparse
+    supersteps, past the 4-byte records' 63-superstep stamp window), and
+    ip_send_message appending to the next frontier."""
+    monkeypatch.setenv("IPR_FUSED", fused)
+    for k, v in (("IPR_FUSED_BLOCKS", blocks), ("IPR_FUSED_MAX_DEG", maxdeg)):
+        if v:
+            monkeypatch.setenv(k, v)
+        else:
+            monkeypatch.delenv(k, raising=False)
+    tune = {"dir_alpha": alpha} if alpha else {}
+    g = G.rmat(14, 16, seed=9, symmetrise=True)
+    ref = O.run(g, O.CC, O.SPREAD)
+    print(f"cc_spread rmat14 fused={fused} blocks={blocks} maxdeg={maxdeg} alpha={alpha}", flush=True)
+    # one superstep per host check, so that the fused-or-general choice is
+    # made before every superstep of this short run
+    r = EN.run_app("cc_spread", g, max_chunk=1, **tune)
+    assert np.array_equal(r.values, ref.values)
+    assert list(r.active) == list(ref.active[: ref.supersteps])
+    if fused == "0":
+        assert r.fused_steps == 0
+    elif not maxdeg and not alpha:
+        assert 0 < r.fused_steps < r.supersteps
+    gg = G.grid(57, 61)
+    ref = O.run(gg, O.SSSP, O.SPREAD, source=3)
+    print("sssp_spread grid57x61", flush=True)
+    r = EN.run_app("sssp_spread", gg, source=3, **tune)
+    assert np.array_equal(r.values, ref.values)
+    assert list(r.active) == list(ref.active[: ref.supersteps])
+    if fused == "1" and not maxdeg:
+        assert r.fused_steps > 63
+    gw = gg.with_weights(5)
+    ref = O.run(gw, O.SSSP_W, O.SPREAD, source=3)
+    print("sssp_weighted_spread grid57x61", flush=True)
+    r = EN.run_app("sssp_weighted_spread", gw, source=3, **tune)
+    assert np.array_equal(r.values, ref.values)
+    assert np.array_equal(r.values, O.dijkstra(gw, 3))
+    assert list(r.active) == list(ref.active[: ref.supersteps])
+
+
+@pytest.mark.parametrize("fin,first", [("0", ""), ("1", ""), ("1", "1"), ("0", "64")])
+def test_end_of_superstep_in_kernel_or_launch(fin, first, monkeypatch):
+    """The end of a superstep done by the last workgroup of the pull kernel or
+    of the frontier compaction (IPR_FIN_IN_KERNEL=1, the default) or by its
+    own launch (0), with first chunks of 1, 16/4 (the defaults) or 64
+    supersteps: PageRank ordered (bit-exact), CC and BFS with their traces."""
+    monkeypatch.setenv("IPR_FIN_IN_KERNEL", fin)
+    if first:
+        monkeypatch.setenv("IPR_FIRST_CHUNK", first)
+    else:
+        monkeypatch.delenv("IPR_FIRST_CHUNK", raising=False)
+    g = G.rmat(12, 16, seed=1)
+    ref = O.run(g, O.PR, O.PULL)
+    r = EN.run_app("pagerank_single_broadcast", g, pull_ordered=1)
+    assert np.array_equal(r.values.view(np.uint64), ref.values.view(np.uint64))
+    assert list(r.active) == list(ref.active[: ref.supersteps])
+    gs = G.rmat(13, 16, seed=4, symmetrise=True)
+    ref = O.run(gs, O.CC, O.SPREAD)
+    r = EN.run_app("cc_spread", gs)
+    assert np.array_equal(r.values, ref.values)
+    assert list(r.active) == list(ref.active[: ref.supersteps])
+    gg = G.grid(31, 47)
+    ref = O.run(gg, O.SSSP, O.SPREAD, source=40)
+    r = EN.run_app("sssp_spread", gg, source=40)
+    assert np.array_equal(r.values, ref.values)
+    assert list(r.active) == list(ref.active[: ref.supersteps])
+
+
+def test_fused_sparse_supersteps_with_send_message(tmp_path, monkeypatch):
+    """cc_send_spread (ip_send_message to each out-neighbour) with the fused
+    kernel on and off: the reference cc_spread binary's output and trace."""
+    import subprocess
+    for fused in ("1", "0"):
+        for case in ("rmat10", "grid8x8", "chain10"):
+            env = dict(__import__("os").environ, IPR_FUSED=fused)
+            root = tmp_path / case
+            (tmp_path / f"{case}.config").write_text((GOLDEN / f"{case}.config").read_text())
+            (tmp_path / f"{case}.adj").write_bytes((GOLDEN / f"{case}.adj").read_bytes())
+            np.fromfile(GOLDEN / f"{case}.idx", dtype=np.uint64).astype(np.uint32).tofile(tmp_path / f"{case}.idx")
+            out = tmp_path / "o.txt"
+            print(f"cc_send_spread_32 {case} fused={fused}", flush=True)
+            p = subprocess.run([str(ROOT / "ipregel_amd" / "bin" / "cc_send_spread_32"), str(root), str(out),
+                                "1", "static", "0"], capture_output=True, text=True, timeout=120, env=env)
+            assert p.returncode == 0, p.stdout[-2000:]
+            assert out.read_text() == (GOLDEN / f"{case}__cc_spread_32.out").read_text()
+            trace = [int(x.split(":")[1]) for x in p.stdout.splitlines() if "ActiveVertexCount" in x]
+            assert trace == [int(x) for x in (GOLDEN / f"{case}__cc_spread_32.trace").read_text().split()]
+
+
 def test_tune_engine_keeps_results():
     from ipregel_amd import autotune as AT
     g = G.rmat(12, 16, seed=5, symmetrise=True)
