@@ -1,25 +1,41 @@
 """Benchmark of the MI355X iPregel engine on one GPU.
 
 A step is one complete superstep loop (ip_run: every superstep until the
-engine halts) of the application over its graph, which stays resident in HBM
-across steps. The headline line is hashmin connected components (cc_spread:
-push with an active frontier, min combiner) on a symmetrised R-MAT graph,
-scale 22, edge factor 16, a/b/c/d = .57/.19/.19/.05, seed 2.
+engine halts) of an application over its graph, which stays resident in HBM
+across steps. Six workloads are measured; the headline line is the first,
+hashmin connected components (cc_spread: push with an active frontier, min
+combiner) on a symmetrised R-MAT graph, scale 22, edge factor 16, a/b/c/d =
+.57/.19/.19/.05, seed 2:
 
-  value      GTEPS = W / t, W = sum over supersteps of the out-degrees of the
-             vertices that broadcast (deterministic under BSP), t = device time
-             of the K timed steps (HIP events on the engine's stream)
-  roofline   the dominant kernel: the gather of the dense supersteps, with
-             8 B per arc relaxed in them (4 B source id + 4 B mailbox update;
-             12 B with weights), over its HIP-event-timed launch durations
-             (plain launches), against the 8 TB/s HBM peak of MI355X; with no
-             dense superstep, the expansion (+ 4 B per next-frontier entry);
-             for PageRank the fused pull kernel. traffic = rocprofv3-measured
-             HBM bytes per launch where a profile exists
+  cc22   cc_spread, symmetrised R-MAT 22 (seed 2)
+  pr16   pagerank_single_broadcast FP64, R-MAT 16 (seed 1), 10 supersteps
+  ss24a  sssp_spread unit weights, R-MAT 24 (seed 3), source = highest out-degree
+  ss24b  sssp_weighted_spread, the same graph with weights in [1,255] (seed 3)
+  grid   sssp_weighted_spread, 4096x4096 grid, weights in [1,255] (seed 3), source 4097
+  cc24   cc_spread, symmetrised R-MAT 24 (seed 2)
+
+For each:
+  GTEPS      W / t, W = sum over supersteps of the out-degrees of the vertices
+             that broadcast (push) or E per message-bearing superstep (pull),
+             deterministic under BSP; t = device time of the timed runs (HIP
+             events on the engine's stream)
+  roofline   the dominant kernel: the gather of the dense supersteps (8 B per
+             arc relaxed in them, 12 B with weights), the pull kernel for
+             PageRank (12 B per in-arc + 40 B per vertex), or, without a dense
+             superstep, the fused sparse kernel (arcs as above + 24 B per
+             computed vertex) or the expansion; over its HIP-event-timed launch
+             durations (plain launches), against the 8 TB/s HBM peak of MI355X.
+             traffic = the rocprofv3-measured HBM bytes per launch where a
+             profile exists (profiles/)
   cpu_baseline  the reference's own OpenMP build (oracle/_ref, compiled from
-             the reference sources) on the same graph file, run again until
-             about 10 s of supersteps (at most 12 runs); the median of its
-             "Total time of supersteps", with its thread count
+             the reference sources) on the same graph file with the host's
+             threads (OMP_NUM_THREADS, else the CPUs this process may use), run
+             again until about 10 s of supersteps (at most 12 runs), the median
+             of its "Total time of supersteps"; its first run's dump is
+             compared with the GPU's results. The reference has no weighted
+             application: ss24b and the grid take the oracle's single-thread
+             C restatement on a bounded sample (its supersteps up to the first
+             boundary after 12 s), kind "port".
 
 Two more sections go into the same JSON line (libipregel_batch.so):
 
@@ -28,9 +44,10 @@ Two more sections go into the same JSON line (libipregel_batch.so):
              directed R-MAT scale 22 ef 16 seed 2 graph, each as one 64-lane
              batch, against the plain loop (the same queries one at a time on
              the one-query kernels); results checked identical; device time by
-             HIP events. Its cpu_baseline is the FP64 PPR restatement on a
-             sample of seeds, which also gives the FP32 result's L1 / max-rel /
-             top-100 agreement.
+             HIP events. Every PPR query's L1 / max-rel / top-100 agreement
+             with the FP64 oracle (all 64, computed by the oracle's multi-seed
+             restatement on the host's threads, which is also the section's
+             cpu_baseline).
   served     Poisson arrivals (seed 5) at several rates of a 40/40/20 SSSP /
              PPR / CC mix over six resident graphs (R-MAT scales 18, 20, 22,
              23, 24, ef 16, seed 2, weights seed 4; CC on their symmetrised
@@ -40,7 +57,9 @@ Two more sections go into the same JSON line (libipregel_batch.so):
              queries/s, and whether every result digest matched.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--apps cc22,pr16,...]
-                       [--no-batched] [--no-served] [--serve-rates 10,100,500,2000]
+                       [--no-cpu] [--no-batched] [--no-served] [--serve-rates 10,100,500,2000]
+       python bench.py --copy-probe MiB   (a plain device copy of that size, to
+                       calibrate rocprofv3's FETCH_SIZE / WRITE_SIZE counters)
 """
 from __future__ import annotations
 
@@ -67,30 +86,59 @@ from ipregel_amd import graph as G  # noqa: E402
 METRIC = "GTEPS & % HBM peak per app; served p50/p99 + queries/s vs plain loop, 1 GPU"
 HBM_PEAK_GBS = 8000.0
 REF_BIN = ROOT / "oracle" / "_ref" / "bin"
+DEFAULT_APPS = "cc22,pr16,ss24a,ss24b,grid,cc24"
 
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-# workload name -> (app, graph builder, description, reference binary, source)
-def wl_cc22():
-    g = G.rmat(22, 16, seed=2, symmetrise=True)
-    return ("cc_spread", g, "cc_spread hashmin CC, symmetrised R-MAT scale 22 ef 16 "
-            "a/b/c/d .57/.19/.19/.05 seed 2", "cc_spread_32", 0)
+def host_threads() -> int:
+    """The host threads a CPU baseline uses: OMP_NUM_THREADS when set (the
+    share of the machine this job is given), else the CPUs it may run on."""
+    cpus = len(os.sched_getaffinity(0))
+    n = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return min(n, cpus) if n > 0 else cpus
 
 
-def wl_pr16():
-    g = G.rmat(16, 16, seed=1)
-    return ("pagerank_single_broadcast", g, "pagerank_single_broadcast FP64 10 supersteps, "
-            "R-MAT scale 16 ef 16 seed 1 (pull over own list, directed=false), reassociated "
-            "fold (per-vertex relative error <= 1e-12 vs the in-order fold)",
-            "pagerank_single_broadcast_32", 0)
+# graphs, built once per process (the two SSSP workloads share theirs)
+_GRAPHS: dict[str, G.Graph] = {}
 
 
-# engine options per workload (the FP64 PageRank line uses the reassociating
-# fold; the ordered fold is bit-exact and is what the parity tests pin)
-TUNING = {"pr16": {"pull_ordered": 2}}
+def graph(key: str) -> G.Graph:
+    if key not in _GRAPHS:
+        make = {
+            "rmat22sym": lambda: G.rmat(22, 16, seed=2, symmetrise=True),
+            "rmat16": lambda: G.rmat(16, 16, seed=1),
+            "rmat24": lambda: G.rmat(24, 16, seed=3),
+            "rmat24w": lambda: graph("rmat24").with_weights(3),
+            "grid4096w": lambda: G.grid(4096, 4096).with_weights(3),
+            "rmat24sym": lambda: G.rmat(24, 16, seed=2, symmetrise=True),
+        }[key]
+        _GRAPHS[key] = make()
+    return _GRAPHS[key]
+
+
+# workload -> app, graph, description, reference binary (None: port), source,
+# engine options, step cap (the grid's run is 0.4 s)
+WORKLOADS = {
+    "cc22": dict(app="cc_spread", graph="rmat22sym", ref="cc_spread_32", source=0,
+                 desc="cc_spread hashmin CC, symmetrised R-MAT scale 22 ef 16 a/b/c/d .57/.19/.19/.05 seed 2"),
+    "pr16": dict(app="pagerank_single_broadcast", graph="rmat16", ref="pagerank_single_broadcast_32", source=0,
+                 tuning={"pull_ordered": 2},
+                 desc="pagerank_single_broadcast FP64 10 supersteps, R-MAT scale 16 ef 16 seed 1 (pull over own "
+                      "list, directed=false), reassociating fold (per-vertex relative error <= 1e-12 vs the "
+                      "in-order fold)"),
+    "ss24a": dict(app="sssp_spread", graph="rmat24", ref="sssp_spread_32", source="maxdeg",
+                  desc="sssp_spread unit weights, R-MAT scale 24 ef 16 seed 3, source = highest out-degree vertex"),
+    "ss24b": dict(app="sssp_weighted_spread", graph="rmat24w", ref=None, source="maxdeg",
+                  desc="sssp_weighted_spread uint32 weights [1,255], R-MAT scale 24 ef 16 seed 3"),
+    "grid": dict(app="sssp_weighted_spread", graph="grid4096w", ref=None, source=4097,
+                 steps_cap=5, warmup_cap=1,
+                 desc="sssp_weighted_spread 4096x4096 grid, weights [1,255] seed 3, source 4097"),
+    "cc24": dict(app="cc_spread", graph="rmat24sym", ref="cc_spread_32", source=0,
+                 desc="cc_spread hashmin CC, symmetrised R-MAT scale 24 ef 16 a/b/c/d .57/.19/.19/.05 seed 2"),
+}
 
 # HBM bytes per dense launch of the gather, from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE passes over `bench.py --apps <workload>` (FETCH_SIZE scaled by 2.0
@@ -103,34 +151,6 @@ TRAFFIC = {("cc22", "ipr_k_gather"): (767_391 * 2 + 57_213) * 1024,
            ("ss24a", "ipr_k_gather"): (4_068_136 * 2 + 182_677) * 1024}
 
 
-def wl_ss24a():
-    g = G.rmat(24, 16, seed=3)
-    return ("sssp_spread", g, "sssp_spread unit weights, R-MAT scale 24 ef 16 seed 3, "
-            "source = highest out-degree vertex", "sssp_spread_32", g.max_degree_vertex())
-
-
-def wl_ss24b():
-    g = G.rmat(24, 16, seed=3).with_weights(3)
-    return ("sssp_weighted_spread", g, "sssp_weighted_spread uint32 weights [1,255], R-MAT "
-            "scale 24 ef 16 seed 3", None, g.max_degree_vertex())
-
-
-def wl_grid():
-    g = G.grid(4096, 4096).with_weights(3)
-    return ("sssp_weighted_spread", g, "sssp_weighted_spread 4096x4096 grid, weights [1,255] "
-            "seed 3, source 4097", None, 4097)
-
-
-def wl_cc24():
-    g = G.rmat(24, 16, seed=2, symmetrise=True)
-    return ("cc_spread", g, "cc_spread hashmin CC, symmetrised R-MAT scale 24 ef 16 "
-            "a/b/c/d .57/.19/.19/.05 seed 2", "cc_spread_32", 0)
-
-
-WORKLOADS = {"cc22": wl_cc22, "pr16": wl_pr16, "ss24a": wl_ss24a, "ss24b": wl_ss24b,
-             "grid": wl_grid, "cc24": wl_cc24}
-
-
 def model_bytes(app: str, g: G.Graph, W: int, F: int, supersteps_with_bcast: int) -> int:
     """Algorithmic bytes of a whole run: push 8 B/relaxed arc (+4 B weight) +
     28 B/computed vertex; pull PR 12 B/in-arc + 40 B/vertex per superstep."""
@@ -140,62 +160,88 @@ def model_bytes(app: str, g: G.Graph, W: int, F: int, supersteps_with_bcast: int
     return per_arc * W + 28 * F
 
 
-def kernel_bytes(app: str, g: G.Graph, W: int, F_next: int, supersteps_with_bcast: int) -> int:
-    """Algorithmic bytes moved by the dominant kernel over the run."""
-    if app.startswith("pagerank"):
-        # fused fetch + compute: 4 B in-index + 8 B gathered message per in-arc,
-        # 8 B offset + 8 B value read/write + 8 B slot write per vertex
-        return supersteps_with_bcast * (12 * g.E + 40 * g.V)
-    per_arc = 12 if "weighted" in app else 8
-    return per_arc * W + 4 * F_next
-
-
-def cpu_baseline(binname: str | None, g: G.Graph, W: int, source: int,
-                 budget_s: float = 10.0, max_runs: int = 12) -> dict | None:
-    """The reference's own OpenMP binary on the same graph file, run again until
-    its supersteps have taken about `budget_s` seconds (at most `max_runs`
-    runs, each a full process: load, supersteps, dump to /dev/null); the
-    median of its 'Total time of supersteps' is the baseline."""
-    if binname is None or not (REF_BIN / binname).exists():
+def parse_dump_ints(path: str) -> np.ndarray | None:
+    """the values of a "%u: %u" dump (benchmarks/cc.c:67, sssp.c:83), None
+    unless its ids are 0..V-1 in order"""
+    a = np.fromstring(Path(path).read_text().replace(":", " "), dtype=np.uint64, sep=" ")
+    if a.size % 2 or not np.array_equal(a[0::2], np.arange(a.size // 2, dtype=np.uint64)):
         return None
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
-    with tempfile.TemporaryDirectory(dir="/tmp") as td:
-        root = os.path.join(td, "g")
-        g.write(root)
-        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="TRUE",
-                   OMP_PLACES="cores")
-        args = [str(REF_BIN / binname), root, "/dev/null", str(threads), "static", "0"]
+    return a[1::2]
+
+
+def pr_text(values: np.ndarray) -> str:
+    """PageRank's ip_dump text (benchmarks/pagerank.c:75, "%lu: %0.20f")"""
+    return "".join(f"{i}: {v:.20f}\n" for i, v in enumerate(values.tolist()))
+
+
+def cpu_reference(binname: str, g: G.Graph, W: int, source: int, budget_s: float = 10.0,
+                  max_runs: int = 12) -> tuple[dict, str | None, tempfile.TemporaryDirectory]:
+    """The reference's own OpenMP binary on the same graph file, run again
+    until its supersteps have taken about `budget_s` seconds (at most
+    `max_runs` runs, each a full process: load, supersteps, dump); the median
+    of its 'Total time of supersteps' is the baseline. Its first run writes
+    its dump to a file (returned) for comparison with the GPU's results."""
+    threads = host_threads()
+    td = tempfile.TemporaryDirectory(dir="/tmp")
+    root = os.path.join(td.name, "g")
+    g.write(root)
+    dump = os.path.join(td.name, "ref_dump.txt")
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="TRUE", OMP_PLACES="cores")
+    times, trace, t0 = [], None, time.time()
+    while len(times) < max_runs and (not times or sum(times) < budget_s):
+        args = [str(REF_BIN / binname), root, dump if not times else "/dev/null", str(threads), "static", "0"]
         if binname.startswith("sssp"):
             args.append(str(source))
-        times, trace, t0 = [], None, time.time()
-        while len(times) < max_runs and (not times or sum(times) < budget_s):
-            try:
-                p = subprocess.run(args, env=env, capture_output=True, text=True, timeout=150)
-            except subprocess.TimeoutExpired:
-                return {"value": None, "unit": "GTEPS", "cores": threads, "kind": "reference",
-                        "sample": f"{binname} did not finish within 150 s"}
-            m = re.search(r"Total time of supersteps: ([0-9.]+)s", p.stdout)
-            if p.returncode != 0 or not m:
-                return {"value": None, "unit": "GTEPS", "cores": threads, "kind": "reference",
-                        "sample": f"{binname} failed rc={p.returncode}"}
-            times.append(float(m.group(1)))
-            if trace is None:
-                trace = [int(x.split(":")[1]) for x in p.stdout.splitlines() if "ActiveVertexCount" in x]
-        wall = time.time() - t0
-        t = float(np.median(times))
-        return {"value": W / t / 1e9, "unit": "GTEPS", "cores": threads, "kind": "reference",
-                "sample": f"{len(times)} full runs of the reference {binname} (OpenMP, {threads} threads, "
-                          f"static schedule) on the same graph file; median 'Total time of supersteps' "
-                          f"{t:.3f} s (min {min(times):.3f}, max {max(times):.3f}; {sum(times):.1f} s of "
-                          f"supersteps, {wall:.1f} s of process wall time)",
-                "seconds": t, "runs": len(times), "trace": trace}
+        try:
+            p = subprocess.run(args, env=env, capture_output=True, text=True, timeout=150)
+        except subprocess.TimeoutExpired:
+            return ({"value": None, "unit": "GTEPS", "cores": threads, "kind": "reference",
+                     "sample": f"{binname} did not finish within 150 s"}, None, td)
+        m = re.search(r"Total time of supersteps: ([0-9.]+)s", p.stdout)
+        if p.returncode != 0 or not m:
+            return ({"value": None, "unit": "GTEPS", "cores": threads, "kind": "reference",
+                     "sample": f"{binname} failed rc={p.returncode}"}, None, td)
+        times.append(float(m.group(1)))
+        if trace is None:
+            trace = [int(x.split(":")[1]) for x in p.stdout.splitlines() if "ActiveVertexCount" in x]
+    wall = time.time() - t0
+    t = float(np.median(times))
+    out = {"value": W / t / 1e9, "unit": "GTEPS", "cores": threads, "kind": "reference",
+           "sample": f"{len(times)} full runs of the reference {binname} (OpenMP, {threads} threads, static "
+                     f"schedule) on the same graph file; median 'Total time of supersteps' {t:.3f} s (min "
+                     f"{min(times):.3f}, max {max(times):.3f}; {sum(times):.1f} s of supersteps, {wall:.1f} s of "
+                     f"process wall time)",
+           "seconds": t, "runs": len(times), "trace": trace}
+    return out, dump, td
+
+
+def cpu_port(g: G.Graph, source: int, n_super: int, budget_s: float = 12.0) -> dict:
+    """The oracle's single-thread C restatement of the spread variant running
+    weighted SSSP, a bounded sample: its supersteps up to the first boundary
+    after `budget_s` seconds; GTEPS by the sample's own W."""
+    from oracle import oracle as O
+    t0 = time.perf_counter()
+    r = O.run(g, O.SSSP_W, O.SPREAD, source=source, stop_seconds=budget_s, max_supersteps=1)
+    dt = time.perf_counter() - t0
+    whole = r.supersteps >= n_super
+    return {"value": r.work_edges / dt / 1e9, "unit": "GTEPS", "cores": 1, "kind": "port",
+            "sample": f"the oracle's single-thread C restatement (oracle/ipr_oracle.c, spread variant, weighted "
+                      f"SSSP), {'the whole run' if whole else f'its first {r.supersteps} of {n_super} supersteps'}"
+                      f": W = {r.work_edges} relaxed arcs in {dt:.2f} s (graph conversion included)",
+            "seconds": dt, "supersteps": r.supersteps, "W": r.work_edges}
 
 
 def run_workload(key: str, steps: int, warmup: int, with_cpu: bool) -> dict:
+    w = WORKLOADS[key]
     t0 = time.time()
-    app, g, desc, refbin, source = WORKLOADS[key]()
-    log(f"{key}: graph V={g.V} E={g.E} built in {time.time() - t0:.1f}s")
-    sess = EN.Session(app, g, source=source, **TUNING.get(key, {}))
+    g = graph(w["graph"])
+    app = w["app"]
+    source = g.max_degree_vertex() if w["source"] == "maxdeg" else int(w["source"])
+    tuning = w.get("tuning", {})
+    steps = min(steps, w.get("steps_cap", steps))
+    warmup = min(warmup, w.get("warmup_cap", warmup))
+    log(f"{key}: graph V={g.V} E={g.E} ready in {time.time() - t0:.1f}s")
+    sess = EN.Session(app, g, source=source, **tuning)
     log(f"{key}: resident on {sess.stats().device_name.decode()} (load {sess.stats().load_s:.2f}s)")
     sess.run(warmup)
     st = sess.stats()
@@ -209,87 +255,100 @@ def run_workload(key: str, steps: int, warmup: int, with_cpu: bool) -> dict:
     ms = (st.total_ms - before_ms) / steps
     W = int(st.work_edges)
     F = int(st.work_vertices)
+    fused = int(st.fused_steps)
     active, sedges, sns = sess.trace()
+    values = sess.values()
     n_super = int(st.supersteps)
     n_bcast_steps = int(np.count_nonzero(sedges))
     gteps = W / (ms * 1e-3) / 1e9
     step_bytes = model_bytes(app, g, W, F, n_bcast_steps)
-    log(f"{key}: {n_super} supersteps, W={W}, {ms:.3f} ms/step, {gteps:.2f} GTEPS")
+    log(f"{key}: {n_super} supersteps ({fused} fused), W={W}, {ms:.3f} ms/step, {gteps:.2f} GTEPS")
     # the dominant kernel, timed by HIP events around each of its launches
-    # (plain launches; per superstep: the gather or fused pull kernel, then
-    # the expansion)
+    # (plain launches; per superstep: the gather or the pull kernel, then the
+    # expansion or the fused sparse kernel)
     sess.kernel_events(True)
     k0_ms, k0_n = st.kernel_ms, st.kernel_launches
-    sess.run(steps)
+    ksteps = min(steps, 3)
+    sess.run(ksteps)
     st2 = sess.stats()
     t_first, t_second = sess.kernel_times()
     sess.kernel_events(False)
-    phase_ms = (st2.kernel_ms - k0_ms) / steps
-    phase_launch = (st2.kernel_launches - k0_n) / steps
+    phase_ms = (st2.kernel_ms - k0_ms) / ksteps
+    phase_launch = (st2.kernel_launches - k0_n) / ksteps
     per_arc = 12 if "weighted" in app else 8
-    alpha = TUNING.get(key, {}).get("dir_alpha", EN.DIR_ALPHA_DEFAULT)
+    alpha = tuning.get("dir_alpha", EN.DIR_ALPHA_DEFAULT)
     gathered = [i for i in range(len(sedges)) if int(sedges[i]) * alpha > g.E]
     if app.startswith("pagerank"):
-        k_name, k_launch = "ipr_k_pull_step", phase_launch
-        k_ms = phase_ms
-        k_bytes = kernel_bytes(app, g, W, 0, n_bcast_steps)
+        # 4 B in-index + 8 B gathered record per in-arc; 8 B offset + 8 B
+        # out-offset + 8 B value read + 8 B value write + 8 B record per vertex
+        k_name, k_launch, k_ms = "ipr_k_pull_step", phase_launch, phase_ms
+        k_bytes = n_bcast_steps * (12 * g.E + 40 * g.V)
     elif gathered and int(st.gathered) == len(gathered):
-        # dense supersteps: the gather kernel alone, 8 (12 weighted) B per arc
-        # relaxed in those supersteps
         k_name, k_launch = "ipr_k_gather", len(gathered)
         k_ms = float(sum(t_first[i] for i in gathered))
         k_bytes = per_arc * int(sum(int(sedges[i]) for i in gathered))
+    elif fused:
+        # compute + expansion: arcs as above, 24 B per computed vertex
+        # (frontier id, two offsets, value read and write, message)
+        k_name, k_launch = "ipr_k_sparse_step", phase_launch
+        k_ms = float(t_second.sum())
+        k_bytes = per_arc * W + 24 * F
     else:
         k_name, k_launch = "ipr_k_expand", phase_launch
         k_ms = float(t_second.sum())
-        k_bytes = kernel_bytes(app, g, W, int(active.sum()), n_bcast_steps)
+        k_bytes = per_arc * W + 4 * int(active.sum())
     achieved = k_bytes / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
-    values_digest = int(np.bitwise_xor.reduce(sess.values().view(np.uint32))) if g.V else 0
-    sess.close()
+    traffic = TRAFFIC.get((key, k_name))
+    ordered_values = None
+    if app.startswith("pagerank"):
+        # the bit-exact ordered fold, for the comparison with the reference's dump
+        sess.close()
+        so = EN.Session(app, g, source=source, pull_ordered=1)
+        so.run(1)
+        ordered_values = so.values()
+        so.close()
+    else:
+        sess.close()
     out = {
-        "app": app, "workload": desc, "V": g.V, "E": g.E, "supersteps": n_super, "W": W,
-        "vertices_computed": F, "ms_per_step": ms, "wall_ms_per_step": wall_ms, "GTEPS": gteps,
-        "model_bytes_per_step": step_bytes,
+        "app": app, "workload": w["desc"], "V": g.V, "E": g.E, "supersteps": n_super, "fused_supersteps": fused,
+        "W": W, "vertices_computed": F, "steps": steps, "ms_per_step": ms, "wall_ms_per_step": wall_ms,
+        "GTEPS": gteps, "model_bytes_per_step": step_bytes,
         "model_GBps": step_bytes / (ms * 1e-3) / 1e9,
         "hbm_frac_model": step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-        "kernel": {"name": k_name,
-                   "ms_per_step": k_ms, "launches_per_step": k_launch,
-                   "avg_launch_ms": k_ms / k_launch if k_launch else None,
-                   "algorithmic_bytes_per_step": k_bytes, "achieved_GBps": achieved,
-                   "traffic_per_launch": TRAFFIC.get((key, k_name)),
-                   "delivery_phase_ms_per_step": phase_ms,
-                   "per_superstep_ms": {"first": [round(float(x), 4) for x in t_first[:64]],
-                                        "expansion": [round(float(x), 4) for x in t_second[:64]]},
-                   "ms_per_step_plain_launches": (st2.total_ms - st.total_ms) / steps},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": k_name,
+                     "avg_launch_ms": k_ms / k_launch if k_launch else None,
+                     "algorithmic_bytes_per_step": k_bytes, "ms_per_step": k_ms,
+                     "launches_per_step": k_launch},
+        "kernel_events": {"delivery_phase_ms_per_step": phase_ms,
+                          "per_superstep_ms": {"first": [round(float(x), 4) for x in t_first[:64]],
+                                               "second": [round(float(x), 4) for x in t_second[:64]]},
+                          "ms_per_step_plain_launches": (st2.total_ms - st.total_ms) / ksteps},
         "active_trace": [int(x) for x in active[:64]],
-        "values_xor": values_digest,
     }
     if with_cpu:
-        cb = cpu_baseline(refbin, g, W, source)
-        if cb:
+        if w["ref"]:
+            cb, dump, td = cpu_reference(w["ref"], g, W, source)
             if cb.get("trace") is not None:
                 cb["trace_matches_gpu"] = cb["trace"] == [int(x) for x in active]
                 cb.pop("trace")
-            out["cpu_baseline"] = cb
-            log(f"{key}: cpu baseline {cb}")
+            if dump and os.path.exists(dump):
+                if app.startswith("pagerank"):
+                    ref_text = Path(dump).read_text()
+                    cb["ref_dump_identical_ordered"] = pr_text(ordered_values) == ref_text
+                    ref_vals = np.array([float(x.split(":")[1]) for x in ref_text.splitlines()])
+                    rel = np.abs(values - ref_vals) / np.maximum(np.abs(ref_vals), 1e-300)
+                    cb["timed_fold_max_rel_vs_ref_dump"] = float(rel.max())
+                else:
+                    ref_vals = parse_dump_ints(dump)
+                    cb["ref_dump_identical"] = ref_vals is not None and bool(
+                        np.array_equal(ref_vals, values.astype(np.uint64)))
+            td.cleanup()
+        else:
+            cb = cpu_port(g, source, n_super)
+        out["cpu_baseline"] = cb
+        log(f"{key}: cpu baseline {cb}")
     return out
-
-
-def ppr_cpu_sample(g: G.Graph, seeds, x32: np.ndarray, iters: int = 20) -> dict:
-    """cpu_baseline of the batched PPR line: the FP64 restatement (oracle,
-    one thread) on a few seeds, timed, and the FP32 results compared with it."""
-    from oracle import oracle as O
-    t0 = time.perf_counter()
-    worst = {"l1": 0.0, "max_rel": 0.0, "top_overlap": 100}
-    for i, s in enumerate(seeds):
-        ref = O.ppr(g, int(s), iters)
-        m = BA.ppr_agreement(x32[i], ref)
-        worst = {"l1": max(worst["l1"], m["l1"]), "max_rel": max(worst["max_rel"], m["max_rel"]),
-                 "top_overlap": min(worst["top_overlap"], m["top_overlap"])}
-    t = time.perf_counter() - t0
-    return {"value": len(seeds) / t, "unit": "queries/s", "cores": 1, "kind": "port",
-            "sample": f"FP64 PPR restatement, {len(seeds)} seeds x {iters} iterations, "
-                      f"{t:.2f} s", "fp32_vs_fp64_worst": worst}
 
 
 def run_batched(with_cpu: bool) -> dict:
@@ -320,6 +379,9 @@ def run_batched(with_cpu: bool) -> dict:
     E = g.E
     # pull iteration, 64 live slots: 4 B id + 4 B weight + 256 B row per in-arc, 256 B row written per vertex
     ppr_bytes_iter = E * (8 + 64 * 4) + g.V * 64 * 4
+    # the plain loop's one-query pull iteration: 4 B id + 4 B weight + 4 B value per in-arc, 8 B per vertex
+    plain_bytes_iter = E * 12 + g.V * 8
+    plain_ms_iter = pplain_ms / 64 / 20
     out = {
         "workload": "directed R-MAT scale 22 ef 16 seed 2, weights [1,255] seed 4; 64 queries per kind "
                     "(seed 4)", "V": g.V, "E": E,
@@ -334,6 +396,9 @@ def run_batched(with_cpu: bool) -> dict:
                 "shared_bytes_per_iteration": ppr_bytes_iter,
                 "shared_GBps": ppr_bytes_iter / (pb["ms"] / 20 * 1e-3) / 1e9,
                 "hbm_frac_shared": ppr_bytes_iter / (pb["ms"] / 20 * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                "plain_ms_per_iteration": plain_ms_iter,
+                "plain_bytes_per_iteration": plain_bytes_iter,
+                "plain_hbm_frac": plain_bytes_iter / (plain_ms_iter * 1e-3) / 1e9 / HBM_PEAK_GBS,
                 "identical_to_plain": psame,
                 "residual_max": float(res.max())},
         "mix_speedup": (plain_ms + pplain_ms) / (sb["ms"] + pb["ms"]),
@@ -342,8 +407,21 @@ def run_batched(with_cpu: bool) -> dict:
     log(f"batched: sssp {sb['ms']:.1f} ms vs plain {plain_ms:.1f} ms; ppr {pb['ms']:.1f} ms vs "
         f"plain {pplain_ms:.1f} ms; identical {same} {psame}")
     if with_cpu:
-        out["cpu_baseline"] = ppr_cpu_sample(g, seeds[:3], x[:3])
-        log(f"batched: cpu baseline {out['cpu_baseline']}")
+        # every PPR query against the FP64 oracle (multi-seed, the host's threads)
+        from oracle import oracle as O
+        t1 = time.perf_counter()
+        x64 = O.ppr_many(g, seeds, 20)
+        dt = time.perf_counter() - t1
+        acc = [BA.ppr_agreement(x[q], x64[q]) for q in range(64)]
+        out["ppr"]["fp32_vs_fp64"] = {
+            "l1": [a["l1"] for a in acc], "max_rel": [a["max_rel"] for a in acc],
+            "top100_overlap": [a["top_overlap"] for a in acc],
+            "worst": {"l1": max(a["l1"] for a in acc), "max_rel": max(a["max_rel"] for a in acc),
+                      "top100_overlap": min(a["top_overlap"] for a in acc)}}
+        out["cpu_baseline"] = {"value": 64 / dt, "unit": "queries/s", "cores": host_threads(), "kind": "port",
+                               "sample": f"FP64 PPR restatement (oracle ppr_many, eight seeds per pass, "
+                                         f"OpenMP over vertices), all 64 seeds x 20 iterations, {dt:.1f} s"}
+        log(f"batched: PPR accuracy worst {out['ppr']['fp32_vs_fp64']['worst']}; cpu {dt:.1f}s")
     rg.close()
     return out
 
@@ -404,6 +482,37 @@ def run_served(rates: list[float], counts: list[int], budget: int) -> dict:
             "hbm_budget_bytes": budget, "ppr_iterations": 20, "sweep": sweep, "state_bytes": slot}
 
 
+def copy_probe(mib: int) -> None:
+    """A device-to-device copy of `mib` MiB through libipregel_hip.so: under
+    rocprofv3 --pmc FETCH_SIZE (or WRITE_SIZE) it gives the bytes those
+    counters report per byte copied, the factor the gather's traffic is
+    scaled by."""
+    import ctypes as C
+    L = C.CDLL(str(ROOT / "ipregel_amd" / "lib" / "libipregel_hip.so"))
+    L.ipr_hip_init.argtypes = [C.c_int, C.c_char_p, C.c_size_t, C.POINTER(C.c_uint64)]
+    L.ipr_hip_malloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    L.ipr_hip_memcpy_d2d.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]
+    L.ipr_hip_fill.argtypes = [C.c_void_p, C.c_int, C.c_size_t, C.c_void_p]
+    L.ipr_hip_stream_create.argtypes = [C.POINTER(C.c_void_p)]
+    L.ipr_hip_stream_sync.argtypes = [C.c_void_p]
+    L.ipr_hip_free.argtypes = [C.c_void_p]
+    name, hbm = C.create_string_buffer(128), C.c_uint64()
+    if L.ipr_hip_init(0, name, 128, C.byref(hbm)):
+        raise SystemExit("no device")
+    n = mib << 20
+    a, b, st = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    for p in (a, b):
+        if L.ipr_hip_malloc(C.byref(p), n):
+            raise SystemExit("allocation failed")
+    L.ipr_hip_stream_create(C.byref(st))
+    L.ipr_hip_fill(a, 1, n, st)
+    L.ipr_hip_memcpy_d2d(b, a, n, st)
+    L.ipr_hip_stream_sync(st)
+    L.ipr_hip_free(a)
+    L.ipr_hip_free(b)
+    print(json.dumps({"copy_probe_bytes": n, "device": name.value.decode()}))
+
+
 def _plain(o):
     """numpy scalars in the JSON line as plain numbers"""
     if isinstance(o, np.generic):
@@ -416,22 +525,27 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--apps", default="cc22,pr16")
+    ap.add_argument("--apps", default=DEFAULT_APPS)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--serve-rates", default="10,100,500,2000")
     ap.add_argument("--serve-counts", default="40,200,300,400")
     ap.add_argument("--serve-budget-gib", type=float, default=192.0)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MiB")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is a design "
                                    "(see the design notes), --gpus must be 1"}))
         return 2
+    if a.copy_probe:
+        copy_probe(a.copy_probe)
+        return 0
     keys = [k for k in a.apps.split(",") if k]
     results = {}
     for k in keys:
         results[k] = run_workload(k, a.steps, a.warmup, not a.no_cpu)
+    _GRAPHS.clear()
     batched = None if a.no_batched else run_batched(not a.no_cpu)
     served = None
     if not a.no_served:
@@ -439,7 +553,6 @@ def main() -> int:
         counts = [int(x) for x in a.serve_counts.split(",") if x]
         served = run_served(rates, counts, int(a.serve_budget_gib * 2**30))
     head = results[keys[0]]
-    k = head["kernel"]
     line = {
         "metric": METRIC,
         "value": head["GTEPS"],
@@ -455,10 +568,8 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": head["workload"], "V": head["V"], "E": head["E"],
                    "supersteps": head["supersteps"], "W": head["W"]},
-        "roofline": {"bound": "hbm", "achieved": k["achieved_GBps"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": k["achieved_GBps"] / HBM_PEAK_GBS,
-                     "traffic": k.get("traffic_per_launch"), "kernel": k["name"],
-                     "avg_launch_ms": k["avg_launch_ms"]},
+        "roofline": {x: head["roofline"][x] for x in ("bound", "achieved", "peak", "unit", "frac", "traffic",
+                                                      "kernel", "avg_launch_ms")},
         "cpu_baseline": head.get("cpu_baseline"),
         "hbm_frac_model": head["hbm_frac_model"],
         "apps": {kk: {x: v for x, v in r.items() if x not in ("active_trace",)}

@@ -21,6 +21,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 typedef union msg { double f; uint32_t u; uint64_t bits; } msg_t;
 
@@ -63,6 +64,7 @@ struct o_engine {
     /* spread_pull: broadcast targets */
     unsigned char* tgt;
     ipr_oracle_result* res;
+    double t0;              /* start of the run, for p->stop_seconds */
 };
 
 /* ---------------------------------------------------------------- API ---- */
@@ -245,10 +247,25 @@ static void record(o_engine* e, uint64_t* trace, uint64_t active)
 }
 
 /* Push, full sweep: src/combiner_postamble.h:218-294. */
+static double o_now(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+/* a bounded sample: the loop ends after p->stop_after supersteps, or at the
+ * first superstep boundary after p->stop_seconds of run time */
+static int o_stopped(const o_engine* e)
+{
+    if (e->p->stop_after && e->superstep >= e->p->stop_after) return 1;
+    return e->p->stop_seconds > 0.0 && e->superstep > 0 && o_now() - e->t0 >= e->p->stop_seconds;
+}
+
 static void run_push(o_engine* e, uint64_t* trace)
 {
     uint64_t active = e->V; /* ip_active_vertices = V at load (iPregel_postamble.h:519) */
-    while (active != 0) {
+    while (active != 0 && !o_stopped(e)) {
         active = 0;
         for (uint64_t i = 0; i < e->V; i++) {
             o_vertex* v = &e->vx[i];
@@ -277,7 +294,7 @@ static void run_push(o_engine* e, uint64_t* trace)
 static void run_pull(o_engine* e, uint64_t* trace)
 {
     uint64_t active = e->V;
-    while (active != 0) {
+    while (active != 0 && !o_stopped(e)) {
         active = 0;
         int any_bcast = 0;
         for (uint64_t i = 0; i < e->V; i++) {
@@ -307,7 +324,7 @@ static void run_spread(o_engine* e, uint64_t* trace)
 {
     uint32_t* f = malloc(sizeof(uint32_t) * (e->V ? e->V : 1));
     uint64_t fsize = 0, active = 0;
-    while (e->superstep == 0 || active > 0) {
+    while ((e->superstep == 0 || active > 0) && !o_stopped(e)) {
         e->nf_size = 0;
         if (e->superstep == 0) {
             for (uint64_t i = 0; i < e->V; i++) { e->res->work_vertices++; e->compute(e, &e->vx[i]); }
@@ -336,7 +353,7 @@ static void run_spread_pull(o_engine* e, uint64_t* trace)
     uint32_t* t = malloc(sizeof(uint32_t) * (e->V ? e->V : 1));
     uint64_t tsize = e->V;
     for (uint64_t i = 0; i < e->V; i++) t[i] = (uint32_t)i; /* targets = all V at start */
-    while (e->superstep == 0 || tsize > 0) {
+    while ((e->superstep == 0 || tsize > 0) && !o_stopped(e)) {
         int any_bcast = 0;
         for (uint64_t k = 0; k < tsize; k++) { e->res->work_vertices++; e->compute(e, &e->vx[t[k]]); }
         tsize = 0;
@@ -409,6 +426,7 @@ int ipr_oracle_run(const ipr_csr* g, int directed, int variant, int app,
         v->has_message = 0;
     }
     if (V > 0) {
+        e.t0 = o_now();
         switch (variant) {
         case IPR_O_PUSH: run_push(&e, active_counts); break;
         case IPR_O_PULL: run_pull(&e, active_counts); break;
@@ -476,6 +494,64 @@ int ipr_oracle_ppr(const ipr_csr* g, uint32_t seed, uint32_t iters, double* x)
         for (uint64_t v = 0; v < V; v++) x[v] = y[v] / total;
     }
     free(y); free(inv);
+    ipr_graph_free(&t);
+    return 0;
+}
+
+/* The same for n seeds at once, bit for bit what ipr_oracle_ppr gives for
+ * each: the transpose is built once, eight seeds share a cache line per
+ * vertex, and the vertices of an iteration are spread over the OpenMP
+ * threads. Each y(v) is the same in-list-order sum as above; the dangling and
+ * total sums stay serial in vertex order. x gets n rows of V doubles. */
+int ipr_oracle_ppr_many(const ipr_csr* g, const uint32_t* seeds, uint32_t n, uint32_t iters, double* x)
+{
+    const uint64_t V = g->V;
+    for (uint32_t j = 0; j < n; j++)
+        if (seeds[j] >= V) return -EINVAL;
+    ipr_csr t;
+    int rc = ipr_graph_transpose(g, &t);
+    if (rc) return rc;
+    enum { B = 8 };
+    double* xs = malloc(sizeof(double) * V * B);
+    double* ys = malloc(sizeof(double) * V * B);
+    double* inv = malloc(sizeof(double) * (V ? V : 1));
+    if (!xs || !ys || !inv) { free(xs); free(ys); free(inv); ipr_graph_free(&t); return -ENOMEM; }
+    for (uint64_t v = 0; v < V; v++) {
+        uint64_t d = g->idx[v + 1] - g->idx[v];
+        inv[v] = d ? 1.0 / (double)d : 0.0;
+    }
+    for (uint32_t j0 = 0; j0 < n; j0 += B) {
+        const uint32_t nb = n - j0 < B ? n - j0 : B;
+        for (uint64_t v = 0; v < V; v++)
+            for (uint32_t j = 0; j < B; j++) xs[v * B + j] = j < nb && v == seeds[j0 + j] ? 1.0 : 0.0;
+        for (uint32_t it = 0; it < iters; it++) {
+            double dangling[B] = {0}, total[B] = {0};
+            for (uint64_t v = 0; v < V; v++)
+                if (g->idx[v + 1] == g->idx[v])
+                    for (uint32_t j = 0; j < B; j++) dangling[j] += xs[v * B + j];
+            #pragma omp parallel for schedule(dynamic, 4096)
+            for (int64_t vi = 0; vi < (int64_t)V; vi++) {
+                const uint64_t v = (uint64_t)vi;
+                double s[B] = {0};
+                for (uint64_t k = t.idx[v]; k < t.idx[v + 1]; k++) {
+                    const uint32_t u = t.adj[k];
+                    for (uint32_t j = 0; j < B; j++) s[j] += xs[(uint64_t)u * B + j] * inv[u];
+                }
+                for (uint32_t j = 0; j < B; j++) {
+                    double r = 0.85 * s[j];
+                    if (j < nb && v == seeds[j0 + j]) r += 0.85 * dangling[j] + 0.15;
+                    ys[v * B + j] = r;
+                }
+            }
+            for (uint64_t v = 0; v < V; v++)
+                for (uint32_t j = 0; j < B; j++) total[j] += ys[v * B + j];
+            for (uint64_t v = 0; v < V; v++)
+                for (uint32_t j = 0; j < B; j++) xs[v * B + j] = total[j] != 0.0 ? ys[v * B + j] / total[j] : 0.0;
+        }
+        for (uint32_t j = 0; j < nb; j++)
+            for (uint64_t v = 0; v < V; v++) x[(uint64_t)(j0 + j) * V + v] = xs[v * B + j];
+    }
+    free(xs); free(ys); free(inv);
     ipr_graph_free(&t);
     return 0;
 }

@@ -38,6 +38,8 @@ typedef struct ipr_oracle_params {
     uint32_t rounds;        /* PR / PPR broadcasting supersteps (PR: 10) */
     double   damping;       /* PR: 0.85 */
     uint64_t max_supersteps;/* capacity of active_counts (0: unlimited, no trace) */
+    uint64_t stop_after;    /* end after this many supersteps, a bounded sample (0: run to the end) */
+    double   stop_seconds;  /* end at the first superstep boundary after this much run time (0: none) */
 } ipr_oracle_params;
 
 typedef struct ipr_oracle_result {
@@ -67,6 +69,9 @@ int ipr_oracle_dump(int app, uint64_t V, const void* values, const char* path);
  * g is the out-edge CSR; in-sums run over the exact transpose in ascending
  * source order. `iters` applications; x gets V doubles. */
 int ipr_oracle_ppr(const ipr_csr* g, uint32_t seed, uint32_t iters, double* x);
+/* n seeds, bit-identical to n calls of ipr_oracle_ppr, OpenMP over vertices;
+ * x gets n rows of V doubles. */
+int ipr_oracle_ppr_many(const ipr_csr* g, const uint32_t* seeds, uint32_t n, uint32_t iters, double* x);
 
 /* Closed forms. */
 /* label[v] = min{u : u reaches v}; on a symmetric graph the component minimum. */

@@ -25,7 +25,8 @@ class _Csr(C.Structure):
 
 class _Params(C.Structure):
     _fields_ = [("source", C.c_uint32), ("rounds", C.c_uint32), ("damping", C.c_double),
-                ("max_supersteps", C.c_uint64)]
+                ("max_supersteps", C.c_uint64), ("stop_after", C.c_uint64),
+                ("stop_seconds", C.c_double)]
 
 
 class _Result(C.Structure):
@@ -49,6 +50,7 @@ def lib() -> C.CDLL:
         for f in ("ipr_oracle_min_ancestor",):
             getattr(L, f).argtypes = [C.POINTER(_Csr), C.c_void_p]
         L.ipr_oracle_ppr.argtypes = [C.POINTER(_Csr), C.c_uint32, C.c_uint32, C.c_void_p]
+        L.ipr_oracle_ppr_many.argtypes = [C.POINTER(_Csr), C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]
         for f in ("ipr_oracle_bfs", "ipr_oracle_dijkstra"):
             getattr(L, f).argtypes = [C.POINTER(_Csr), C.c_uint32, C.c_void_p]
         _lib = L
@@ -73,9 +75,13 @@ class OracleResult:
 
 
 def run(g, app: int, variant: int, directed: bool = False, source: int = 0,
-        rounds: int = 10, max_supersteps: int = 1 << 20) -> OracleResult:
+        rounds: int = 10, max_supersteps: int = 1 << 20, stop_after: int = 0,
+        stop_seconds: float = 0.0) -> OracleResult:
+    """A bounded sample: stop_after > 0 ends the run after that many
+    supersteps, stop_seconds > 0 at the first superstep boundary after that
+    much run time."""
     c, keep = _csr(g)
-    p = _Params(source, rounds, 0.85, max_supersteps)
+    p = _Params(source, rounds, 0.85, max_supersteps, stop_after, stop_seconds)
     vals = np.zeros(g.V, dtype=np.float64 if app == PR else np.uint32)
     trace = np.zeros(max_supersteps, dtype=np.uint64)
     r = _Result()
@@ -127,4 +133,15 @@ def ppr(g, seed: int, iters: int = 20) -> np.ndarray:
     rc = lib().ipr_oracle_ppr(C.byref(c), seed, iters, out.ctypes.data)
     if rc:
         raise RuntimeError(f"ppr failed ({rc})")
+    return out
+
+
+def ppr_many(g, seeds, iters: int = 20) -> np.ndarray:
+    """ppr for every seed (rows), bit for bit, with the host's OpenMP threads"""
+    c, keep = _csr(g)
+    s = np.ascontiguousarray(seeds, dtype=np.uint32)
+    out = np.zeros((s.size, g.V), np.float64)
+    rc = lib().ipr_oracle_ppr_many(C.byref(c), s.ctypes.data, s.size, iters, out.ctypes.data)
+    if rc:
+        raise RuntimeError(f"ppr_many failed ({rc})")
     return out

@@ -76,6 +76,17 @@ def test_oracle_ppr_closed_forms():
     assert np.allclose(z, [0.15, 0.425, 0.425], rtol=1e-15)
 
 
+def test_oracle_ppr_many_is_ppr_bit_for_bit():
+    """The multi-seed FP64 PPR (eight seeds per cache line, OpenMP over
+    vertices) gives every seed's ppr bits, with a repeated seed, a sink and
+    a seed count that is not a multiple of eight."""
+    g = G.rmat(10, 16, seed=5)
+    seeds = np.array([0, 3, 3, 700, 1023, 17, 512, 9, 100, 44, 2], np.uint32)
+    many = O.ppr_many(g, seeds, 20)
+    for i, s in enumerate(seeds):
+        assert np.array_equal(many[i].view(np.uint64), O.ppr(g, int(s), 20).view(np.uint64)), i
+
+
 def test_autotune_scan_stops_and_picks_the_fastest():
     """A one-dimensional scan keeps the fastest candidate and stops at the
     first one 1.5x slower than the best so far (later ones are not timed)."""

@@ -155,3 +155,20 @@ def test_pagerank_no_inneighbours_gets_teleport_only():
     g = G.from_arcs(3, [0, 1], [1, 0])
     r = O.run(g, O.PR, O.PULL)
     assert r.values[2] == 0.15 / 3
+
+
+@pytest.mark.parametrize("variant", [O.PUSH, O.SPREAD])
+def test_bounded_sample_is_a_prefix_of_the_run(variant):
+    # the CPU baseline's bounded samples: the first k supersteps of the run,
+    # with the same per-superstep active counts
+    g = G.rmat(12, 8, seed=3).with_weights(3)
+    src = g.max_degree_vertex()
+    full = O.run(g, O.SSSP_W, variant, source=src)
+    assert full.supersteps > 3
+    part = O.run(g, O.SSSP_W, variant, source=src, stop_after=3)
+    assert part.supersteps == 3
+    assert np.array_equal(part.active, full.active[:3])
+    assert 0 < part.work_edges <= full.work_edges
+    timed = O.run(g, O.SSSP_W, variant, source=src, stop_seconds=1e-9)
+    assert timed.supersteps == 1
+    assert np.array_equal(timed.active, full.active[:1])
