@@ -96,8 +96,10 @@ __device__ inline bool ipr_last_workgroup(const ipr_dev& d, uint32_t n)
 }
 
 /* End of superstep s (PULL: of s-1, after the fused fetch(s-1)+compute(s)),
- * run by lanes 0..63 of one wavefront. */
-__device__ inline void ipr_finalize_step(const ipr_dev& d, int variant)
+ * run by lanes 0..63 of one wavefront. SPREAD: `flags_front` is 1 when the
+ * next frontier was left as raised receive flags (the fused sparse kernel),
+ * 0 when it was compacted into a list. */
+__device__ inline void ipr_finalize_step(const ipr_dev& d, int variant, int flags_front = 0)
 {
     ipr_ctl* c = d.ctl;
     if (c->done) return;
@@ -115,8 +117,11 @@ __device__ inline void ipr_finalize_step(const ipr_dev& d, int variant)
         edges = bc ? d.in_idx[d.V] : 0;
     } else {
         fin = s;
+        /* spread: the next frontier's size, from the compaction's list counter
+         * or from the fused kernel's count of activated receivers */
         active = variant == IPR_VARIANT_PUSH ? ipr_take(d, IPR_C_ACT0 + (int)(s & 1), true)
-                                             : ipr_coherent_load(&c->fsize[(s + 1) & 1]);
+                                             : ipr_coherent_load(&c->fsize[(s + 1) & 1]) +
+                                                   ipr_take(d, IPR_C_FNEXT, true);
         edges = ipr_take(d, IPR_C_EDGES, true);
         computed = ipr_take(d, IPR_C_WORKV, true);
         if (variant != IPR_VARIANT_PUSH) {
@@ -148,6 +153,8 @@ __device__ inline void ipr_finalize_step(const ipr_dev& d, int variant)
             c->fsize[s & 1] = 0;
             c->farcs[s & 1] = 0;
             c->fmaxdeg[s & 1] = 0;
+            c->fsize[(s + 1) & 1] = active;
+            c->front_flags = (uint64_t)flags_front;
         }
         c->done = active == 0;
         c->executed = s + 1;

@@ -68,9 +68,11 @@ enum {
                           * blockIdx mod 64, for kernels whose last workgroup ends
                           * the superstep ... */
     IPR_C_ARRIVE_TOP = 9,/* ... slot 0: the slots whose workgroups have all finished */
-    IPR_C_FARCS = 10,    /* spread: out-arcs of the frontier being compacted */
+    IPR_C_FARCS = 10,    /* spread: out-arcs of the next frontier */
     IPR_C_FMAX = 11,     /* spread: its largest out-degree (per-slot maxima) */
-    IPR_NCNT = 12
+    IPR_C_FNEXT = 12,    /* spread: the next frontier's size, counted by the fused
+                          * sparse kernel as its deliveries activate receivers */
+    IPR_NCNT = 13
 };
 
 /* device-resident control block: every kernel reads the superstep and the
@@ -102,7 +104,10 @@ typedef struct ipr_ctl {
      * chunk is launched; kernel k of the chunk runs superstep chunk_base + k
      * (never ctl->superstep, which its own last workgroup advances) */
     uint64_t chunk_base;
-    uint64_t pad[1];
+    /* spread: 1 when the current superstep's frontier exists only as raised
+     * receive flags (a fused sparse kernel built it, no list), 0 when it is
+     * the compacted list front[superstep & 1] */
+    uint64_t front_flags;
 } ipr_ctl;
 
 #define IPR_LPACK_SHIFT 40
@@ -199,9 +204,9 @@ typedef struct ipr_dev {
     uint32_t dir_opt;            /* push apps: allow gathering dense supersteps */
     uint32_t dir_alpha;          /* gather when bcast out-arcs * alpha > E */
     /* spread variant: supersteps whose frontier has at most fused_max_arcs
-     * out-arcs, none above fused_max_deg, and would not gather run as the
-     * fused compute + expansion kernel and a compaction that ends the
-     * superstep (0: off) */
+     * out-arcs, none above fused_max_deg, and would not gather run as one
+     * fused kernel (frontier scan, compute, expansion, end of superstep)
+     * (0: off) */
     uint32_t fused;
     uint32_t fused_blocks;       /* workgroups of the fused kernel */
     uint64_t fused_max_arcs;

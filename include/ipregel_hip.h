@@ -112,8 +112,11 @@ int ipr_hip_equal(const void* a, const void* b, uint64_t bytes, int* equal, void
 /* push variants: the next frontier (claim bitmap of superstep s+1) compacted
  * into front[(s+1)&1], ascending within runs; the bitmap is cleared */
 /* finalize_variant >= 0: the compaction's last workgroup also ends the
- * superstep of that variant (ipregel_control.h), with no finalize launch */
-int ipr_hip_compact_frontier(const ipr_dev* dev, int finalize_variant, void* stream);
+ * superstep of that variant (ipregel_control.h), with no finalize launch.
+ * relist != 0: the current superstep's frontier instead (claim bitmap of
+ * superstep s, left by a fused sparse kernel that stopped) into front[s&1],
+ * counted from fsize[s&1], which the caller zeroes first; no finalize */
+int ipr_hip_compact_frontier(const ipr_dev* dev, int finalize_variant, int relist, void* stream);
 
 /* superstep control (kernels on `stream`, graph-capturable) */
 int ipr_hip_begin(const ipr_dev* dev, void* stream);

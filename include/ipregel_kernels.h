@@ -197,6 +197,10 @@ template <typename T> __device__ inline T ipr_atomic_max(T* p, T m)
 template <typename T> struct ipr_comb<T, IPR_COMBINER_MIN> {
     typedef T slot_t;
     static constexpr int exact_gather = std::is_integral<T>::value;
+    /* deliver() answers true only for the first delivery to a slot (and for
+     * a message equal to the empty value), so the receiver's flag is
+     * raised without reading it first */
+    static constexpr int first_once = 1;
     __host__ __device__ static T ident() { return std::numeric_limits<T>::has_infinity ? std::numeric_limits<T>::infinity() : std::numeric_limits<T>::max(); }
     __device__ static slot_t empty() { return ident(); }
     __device__ static T value(slot_t s) { return s; }
@@ -217,6 +221,10 @@ template <typename T> struct ipr_comb<T, IPR_COMBINER_MIN> {
 template <typename T> struct ipr_comb<T, IPR_COMBINER_MAX> {
     typedef T slot_t;
     static constexpr int exact_gather = std::is_integral<T>::value;
+    /* deliver() answers true only for the first delivery to a slot (and for
+     * a message equal to the empty value), so the receiver's flag is
+     * raised without reading it first */
+    static constexpr int first_once = 1;
     __host__ __device__ static T ident() { return std::numeric_limits<T>::has_infinity ? -std::numeric_limits<T>::infinity() : std::numeric_limits<T>::lowest(); }
     __device__ static slot_t empty() { return ident(); }
     __device__ static T value(slot_t s) { return s; }
@@ -235,6 +243,8 @@ template <typename T> struct ipr_comb<T, IPR_COMBINER_MAX> {
 template <typename T> struct ipr_comb<T, IPR_COMBINER_SUM> {
     typedef T slot_t;
     static constexpr int exact_gather = std::is_integral<T>::value;
+    /* deliver() answers true for every delivery */
+    static constexpr int first_once = 0;
     __host__ __device__ static T ident() { return (T)0; }
     __device__ static slot_t empty() { return (T)0; }
     __device__ static T value(slot_t s) { return s; }
@@ -256,6 +266,10 @@ template <typename T> struct ipr_comb<T, IPR_COMBINER_GENERIC> {
                   "without -DIP_COMBINER_MIN/_MAX/_SUM, messages must be 4 bytes");
     typedef unsigned long long slot_t;
     static constexpr int exact_gather = 0;
+    /* deliver() answers true only for the first delivery to a slot (and for
+     * a message equal to the empty value), so the receiver's flag is
+     * raised without reading it first */
+    static constexpr int first_once = 1;
     __host__ __device__ static T ident() { return T(); }
     __device__ static slot_t empty() { return 0ull; }
     __device__ static T value(slot_t s) { uint32_t b = (uint32_t)s; T t; memcpy(&t, &b, 4); return t; }
@@ -393,11 +407,41 @@ __device__ inline void ipr_emit(const ipr_dev& d, uint64_t s, uint32_t u, IP_MES
 #endif
 }
 
+/* deliver one message to u in superstep s; true when this delivery is the
+ * one that raised u's received flag (u joins the next frontier: the atomic
+ * OR's old word tells, so exactly one delivery per receiver answers true) */
+__device__ inline bool ipr_deliver_activates(const ipr_dev& d, uint64_t s, uint32_t u, IP_MESSAGE_TYPE m)
+{
+    ipr_slot_t* mb = (ipr_slot_t*)d.mbox[(s + 1) & 1];
+    if (!ipr_policy::deliver(mb + u, m)) return false;
+    uint8_t* recv = d.recv[(s + 1) & 1];
+    if (!ipr_policy::first_once && __atomic_load_n(recv + u, __ATOMIC_RELAXED)) return false;
+    const uint32_t bit = 1u << (8 * (u & 3u));
+    return (atomicOr((uint32_t*)(recv + (u & ~3u)), bit) & bit) == 0;
+}
+
+/* 1 in the fused sparse kernel, whose deliveries count the receivers they
+ * activate (the next frontier's size and out-arcs, kept in the workgroup's
+ * ipr_sm_fcount: count, arcs, largest out-degree); 0 in every other kernel
+ * that runs application code. Each thread stores it before ip_compute. */
+__shared__ uint32_t ipr_sm_counted;
+__shared__ unsigned long long ipr_sm_fcount[3];
+
 /* ip_send_message in push variants: immediate delivery */
 __device__ inline void ipr_send_immediate(IP_VERTEX_ID_TYPE id, IP_MESSAGE_TYPE m)
 {
     const ipr_dev& d = ipr_d_dev;
-    ipr_deliver(d, ipr_sm_superstep, (uint32_t)id, m);
+    if (!ipr_sm_counted) {
+        ipr_deliver(d, ipr_sm_superstep, (uint32_t)id, m);
+        return;
+    }
+    const uint32_t u = (uint32_t)id;
+    if (ipr_deliver_activates(d, ipr_sm_superstep, u, m)) {
+        const unsigned long long deg = d.out_idx[u + 1] - d.out_idx[u];
+        atomicAdd(&ipr_sm_fcount[0], 1ull);
+        atomicAdd(&ipr_sm_fcount[1], deg);
+        atomicMax(&ipr_sm_fcount[2], deg);
+    }
 }
 
 __device__ inline IP_MESSAGE_TYPE ipr_edge_msg(const uint32_t* wgt, IP_MESSAGE_TYPE m, bool weighted, uint64_t e)
@@ -489,6 +533,7 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_compute_push(ipr_dev d)
     if (ctl->done) return;
     const uint64_t s = ctl->superstep;
     ipr_sm_superstep = s;
+    ipr_sm_counted = 0;
     uint64_t n = d.V;
     const uint32_t* list = nullptr;
     if ((IPR_VARIANT == IPR_VARIANT_SPREAD || IPR_VARIANT == IPR_VARIANT_SPREAD_PULL) && s > 0) {
@@ -617,6 +662,7 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_expand(ipr_dev d)
     if (ipr_gathers(d)) return;
     const uint64_t s = ctl->superstep;
     ipr_sm_superstep = s;
+    ipr_sm_counted = 0;
     const uint32_t want = (uint32_t)(s + 1);
     const uint32_t want4 = (uint32_t)(s % IPR_REC4_WINDOW) + 1u;
 
@@ -701,18 +747,94 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_expand(ipr_dev d)
 /* ------------------------------------------------------------------------ */
 
 #if IPR_VARIANT == 2
-/* The compute and the expansion of a sparse superstep of the spread variant
- * in one launch (src/combiner_spread_postamble.h:160-431 phases 1-2): every
- * vertex of the (ascending) frontier list takes its combined message from the
- * mailbox and runs ip_compute; the broadcasts of a tile of 256 vertices are
- * laid end to end by a workgroup prefix sum and delivered 256 arcs at a time,
- * raising the receivers' flags. The frontier compaction that follows builds
- * the next list in ascending order and its last workgroup ends the superstep
- * (no finalize launch). No broadcast record is written: nothing reads one in
- * a sparse superstep. Large broadcasters are not split into chunks here: a
- * frontier with a vertex above fused_max_deg out-arcs, or with enough arcs to
- * gather (or above fused_max_arcs), is left to the general kernels: the kernel
- * raises ctl->bail and returns, and so does the compaction after it. */
+/* vertices whose receive flags one workgroup scans per pass of the fused
+ * kernel (one 16-byte load of flags per thread) */
+#define IPR_SCAN_SPAN (IPR_BLOCK * 16)
+
+/* One tile of the fused sparse kernel: thread t computes vertex v (when
+ * `valid`), the tile's broadcasts are laid end to end by a workgroup prefix
+ * sum and delivered 256 arcs at a time; receivers a delivery activates are
+ * counted with their out-degrees in *n_new / *f_arcs / *f_max. Every thread of
+ * the workgroup calls it. */
+__device__ inline void ipr_sparse_tile(const ipr_dev& d, uint64_t s, ipr_slot_t* mcur, uint32_t v, bool valid,
+                                       uint32_t* s_ex, uint64_t* s_off, IP_MESSAGE_TYPE* s_msg, uint8_t* s_wf,
+                                       uint32_t* scan, uint64_t* n_computed, uint64_t* n_edges, uint64_t* n_new,
+                                       uint64_t* f_arcs, uint64_t* f_max)
+{
+    uint32_t deg = 0;
+    if (valid) {
+        struct ip_vertex_t x;
+        x.active = true;
+        x.has_message = true;
+        x.message = ipr_policy::value(mcur[v]);
+        mcur[v] = ipr_policy::empty();
+        const uint64_t o0 = d.out_idx[v], o1 = d.out_idx[v + 1];
+        ipr_fill_vertex(d, &x, v);
+        const IP_VALUE_TYPE before = x.value;
+        ip_compute(&x);
+        ipr_store_value(d, v, before, x.value);
+        (*n_computed)++;
+        if (x.ipr_bcast) {
+            *n_edges += o1 - o0;
+            deg = (uint32_t)(o1 - o0);
+            s_off[threadIdx.x] = o0;
+            s_msg[threadIdx.x] = x.ipr_bcast_msg;
+            s_wf[threadIdx.x] = x.ipr_bcast_weighted;
+        }
+    }
+    uint32_t total = 0;
+    const uint32_t ex = ipr_block_excl_scan(deg, scan, &total);
+    s_ex[threadIdx.x] = ex;
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < total; t += IPR_BLOCK) {
+        uint32_t lo = 0;                         /* last owner with start <= t */
+        for (uint32_t step = IPR_BLOCK / 2; step > 0; step >>= 1)
+            if (s_ex[lo + step] <= t) lo += step;
+        const uint64_t e = s_off[lo] + (t - s_ex[lo]);
+        const uint32_t u = d.out_adj[e];
+        IP_MESSAGE_TYPE m = s_msg[lo];
+#ifdef IP_NEEDS_OUT_NEIGHBOUR_WEIGHTS
+        /* loaded with the arc's target whether the broadcast is weighted or
+         * not: a load under the branch would wait for the target first */
+        const uint32_t w = d.out_wgt[e];
+        if (s_wf[lo]) m = (IP_MESSAGE_TYPE)(m + (IP_MESSAGE_TYPE)w);
+#endif
+        /* u's offsets are loaded with the delivery, not after it: the
+         * out-degree is wanted only if this delivery activates u, but waiting
+         * for the activation first would add a round trip */
+        const uint64_t u0 = d.out_idx[u], u1 = d.out_idx[u + 1];
+        if (ipr_deliver_activates(d, s, u, m)) {
+            const uint64_t du = u1 - u0;
+            (*n_new)++;
+            *f_arcs += du;
+            *f_max = du > *f_max ? du : *f_max;
+        }
+    }
+    __syncthreads();
+}
+
+/* A whole sparse superstep of the spread variant in one launch
+ * (src/combiner_spread_postamble.h:160-431 phases 1-3): every vertex of the
+ * frontier takes its combined message from the mailbox and runs ip_compute;
+ * the broadcasts of a tile of 256 vertices are laid end to end by a workgroup
+ * prefix sum and delivered 256 arcs at a time, raising the receivers' flags;
+ * the deliveries that activate a receiver count it, with its out-degree, so
+ * the next frontier's size and out-arc figures are known without a
+ * compaction; the last workgroup ends the superstep.
+ *
+ * The frontier is the list front[s & 1] after a superstep of the general
+ * kernels (ctl->front_flags == 0), else the raised flags recv[s & 1] left by
+ * the previous fused kernel: each workgroup scans IPR_SCAN_SPAN vertices at a
+ * time, clears the flags it finds and computes those vertices in ascending
+ * order from LDS. No broadcast record is written (nothing reads one in a
+ * sparse superstep); the 4-byte records are cleared at the end of each stamp
+ * window as the compaction does.
+ *
+ * Large broadcasters are not split into chunks here: a frontier with a vertex
+ * above fused_max_deg out-arcs, or with enough arcs to gather (or above
+ * fused_max_arcs), is left to the general kernels: the kernel raises
+ * ctl->bail and returns before touching anything, and the host rebuilds the
+ * list from the flags if the frontier has no list. */
 __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_sparse_step(ipr_dev d)
 {
     __shared__ uint32_t s_ex[IPR_BLOCK];
@@ -720,61 +842,109 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_sparse_step(ipr_dev d)
     __shared__ IP_MESSAGE_TYPE s_msg[IPR_BLOCK];
     __shared__ uint8_t s_wf[IPR_BLOCK];
     __shared__ uint32_t scan[IPR_WAVES];
+    __shared__ uint32_t s_ids[IPR_SCAN_SPAN];
     ipr_ctl* ctl = d.ctl;
     /* the superstep: the chunk's base plus this kernel's place in it (a
      * kernel after a bail in the same chunk finds ctl->superstep behind) */
     const uint64_t s = ctl->chunk_base + d.step_in_chunk;
     if (ctl->done || ctl->bail || ctl->superstep != s) return;
-    const uint64_t n = ctl->fsize[s & 1];
     const uint64_t arcs = ctl->farcs[s & 1], maxdeg = ctl->fmaxdeg[s & 1];
     if (s == 0 || arcs > d.E || arcs > d.fused_max_arcs || maxdeg > d.fused_max_deg ||
         (d.dir_opt && arcs * (uint64_t)d.dir_alpha > d.E)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) ctl->bail = 1;
         return;
     }
+    const bool from_flags = ctl->front_flags != 0;
     ipr_sm_superstep = s;
+    ipr_sm_counted = 1;
+    if (threadIdx.x < 3) ipr_sm_fcount[threadIdx.x] = 0;
+    __syncthreads();
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl->fused_steps++;
-    const uint32_t* list = d.front[s & 1];
     ipr_slot_t* mcur = (ipr_slot_t*)d.mbox[s & 1];
-    uint64_t n_computed = 0, n_edges = 0;
-    for (uint64_t base = (uint64_t)blockIdx.x * IPR_BLOCK; base < n; base += (uint64_t)gridDim.x * IPR_BLOCK) {
-        const uint64_t i = base + threadIdx.x;
-        uint32_t deg = 0;
-        if (i < n) {
-            const uint32_t v = list[i];
-            struct ip_vertex_t x;
-            x.active = true;
-            x.has_message = true;
-            x.message = ipr_policy::value(mcur[v]);
-            mcur[v] = ipr_policy::empty();
-            const uint64_t o0 = d.out_idx[v], o1 = d.out_idx[v + 1];
-            ipr_fill_vertex(d, &x, v);
-            const IP_VALUE_TYPE before = x.value;
-            ip_compute(&x);
-            ipr_store_value(d, v, before, x.value);
-            n_computed++;
-            if (x.ipr_bcast) {
-                n_edges += o1 - o0;
-                deg = (uint32_t)(o1 - o0);
-                s_off[threadIdx.x] = o0;
-                s_msg[threadIdx.x] = x.ipr_bcast_msg;
-                s_wf[threadIdx.x] = x.ipr_bcast_weighted;
+    uint64_t n_computed = 0, n_edges = 0, n_new = 0, f_arcs = 0, f_max = 0;
+    const uint64_t groups = (d.V + 15) / 16;     /* flag arrays are padded to 16 */
+    /* the last superstep of a stamp window clears the 4-byte gather records
+     * (padded to 16 like the flags) */
+    if (d.brec4 && (s + 1) % IPR_REC4_WINDOW == 0) {
+        uint4* clr = (uint4*)d.brec4;
+        const uint4 z = make_uint4(0, 0, 0, 0);
+        for (uint64_t i = (uint64_t)blockIdx.x * IPR_BLOCK + threadIdx.x; i < groups * 4;
+             i += (uint64_t)gridDim.x * IPR_BLOCK)
+            clr[i] = z;
+    }
+    if (!from_flags) {
+        const uint64_t n = ctl->fsize[s & 1];
+        const uint32_t* list = d.front[s & 1];
+        for (uint64_t base = (uint64_t)blockIdx.x * IPR_BLOCK; base < n; base += (uint64_t)gridDim.x * IPR_BLOCK) {
+            const uint64_t i = base + threadIdx.x;
+            ipr_sparse_tile(d, s, mcur, i < n ? list[i] : 0u, i < n, s_ex, s_off, s_msg, s_wf, scan,
+                            &n_computed, &n_edges, &n_new, &f_arcs, &f_max);
+        }
+    } else {
+        uint4* flags = (uint4*)d.recv[s & 1];
+        const uint64_t stride = (uint64_t)gridDim.x * IPR_BLOCK;
+        uint4 qn = make_uint4(0, 0, 0, 0);
+        if ((uint64_t)blockIdx.x * IPR_BLOCK + threadIdx.x < groups)
+            qn = flags[(uint64_t)blockIdx.x * IPR_BLOCK + threadIdx.x];
+        for (uint64_t g0 = (uint64_t)blockIdx.x * IPR_BLOCK; g0 < groups; g0 += stride) {
+            const uint64_t g = g0 + threadIdx.x;
+            const uint4 q = qn;
+            /* the next pass's flags are loaded while this pass runs */
+            qn = make_uint4(0, 0, 0, 0);
+            if (g + stride < groups) qn = flags[g + stride];
+            /* flags are 0 or 1, so each raised flag is one bit (0, 8, 16 or 24) */
+            const uint32_t c = __popc(q.x) + __popc(q.y) + __popc(q.z) + __popc(q.w);
+            if (c) flags[g] = make_uint4(0, 0, 0, 0);
+            uint32_t total = 0;
+            uint32_t pos = ipr_block_excl_scan(c, scan, &total);
+            if (!total) continue;                /* uniform across the workgroup */
+            const uint32_t words[4] = {q.x, q.y, q.z, q.w};
+            for (int k = 0; k < 4; k++) {
+                uint32_t w = words[k];
+                while (w) {
+                    const uint32_t bit = __ffs(w) - 1;
+                    s_ids[pos++] = (uint32_t)(g * 16 + k * 4 + bit / 8);
+                    w &= w - 1;
+                }
+            }
+            __syncthreads();
+            for (uint32_t t0 = 0; t0 < total; t0 += IPR_BLOCK) {
+                const uint32_t t = t0 + threadIdx.x;
+                ipr_sparse_tile(d, s, mcur, t < total ? s_ids[t] : 0u, t < total, s_ex, s_off, s_msg, s_wf, scan,
+                                &n_computed, &n_edges, &n_new, &f_arcs, &f_max);
             }
         }
-        uint32_t total = 0;
-        const uint32_t ex = ipr_block_excl_scan(deg, scan, &total);
-        s_ex[threadIdx.x] = ex;
-        __syncthreads();
-        for (uint32_t t = threadIdx.x; t < total; t += IPR_BLOCK) {
-            uint32_t lo = 0;                     /* last owner with start <= t */
-            for (uint32_t step = IPR_BLOCK / 2; step > 0; step >>= 1)
-                if (s_ex[lo + step] <= t) lo += step;
-            const uint64_t e = s_off[lo] + (t - s_ex[lo]);
-            ipr_deliver(d, s, d.out_adj[e], ipr_edge_msg(d.out_wgt, s_msg[lo], s_wf[lo] != 0, e));
-        }
-        __syncthreads();
     }
-    ipr_block_count3(d, -1, 0, IPR_C_WORKV, n_computed, IPR_C_EDGES, n_edges);
+    /* the workgroup's figures: one atomic per counter, spread over the slots
+     * (ip_send_message deliveries counted theirs in LDS) */
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        n_new += ipr_sm_fcount[0];
+        f_arcs += ipr_sm_fcount[1];
+        f_max = ipr_sm_fcount[2] > f_max ? ipr_sm_fcount[2] : f_max;
+    }
+    ipr_block_count3(d, IPR_C_FNEXT, n_new, IPR_C_WORKV, n_computed, IPR_C_EDGES, n_edges);
+    {
+        __shared__ uint64_t r_arcs[IPR_WAVES], r_max[IPR_WAVES];
+        for (int o = IPR_WAVE / 2; o > 0; o >>= 1) {
+            f_arcs += __shfl_xor(f_arcs, o);
+            const uint64_t m = __shfl_xor(f_max, o);
+            f_max = m > f_max ? m : f_max;
+        }
+        if (ipr_lane() == 0) { r_arcs[threadIdx.x / IPR_WAVE] = f_arcs; r_max[threadIdx.x / IPR_WAVE] = f_max; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint64_t a = 0, m = 0;
+            for (int w = 0; w < IPR_WAVES; w++) { a += r_arcs[w]; m = r_max[w] > m ? r_max[w] : m; }
+            const uint32_t slot = blockIdx.x % IPR_CSLOTS;
+            if (a) atomicAdd((unsigned long long*)&d.cnt[IPR_C_FARCS * IPR_CSLOTS + slot], (unsigned long long)a);
+            if (m) atomicMax((unsigned long long*)&d.cnt[IPR_C_FMAX * IPR_CSLOTS + slot], (unsigned long long)m);
+        }
+    }
+    /* the last workgroup to finish ends the superstep; the next frontier is
+     * left as raised flags */
+    if (ipr_last_workgroup(d, gridDim.x) && threadIdx.x < IPR_WAVE)
+        ipr_finalize_step(d, IPR_VARIANT_SPREAD, 1);
 }
 #endif /* IPR_VARIANT == 2 (spread) */
 
@@ -1144,6 +1314,7 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_swap(ipr_dev d)
     if (ctl->done) return;
     const uint64_t s = ctl->superstep;
     ipr_sm_superstep = s;
+    ipr_sm_counted = 0;
     ipr_slot_t* mb = (ipr_slot_t*)d.mbox[(s + 1) & 1];
     uint8_t* rv = d.recv[(s + 1) & 1];
     uint64_t n_react = 0;
@@ -1307,6 +1478,7 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_pull_step(ipr_dev d, uint32_t
     if (ctl->done) return;
     const uint64_t s = ctl->superstep;
     ipr_sm_superstep = s;
+    ipr_sm_counted = 0;
     /* superstep s-1's broadcast records, stamped s */
     const ipr_bslot_t* pr = (const ipr_bslot_t*)d.bcm[(s + 1) & 1];
     const uint32_t want = (uint32_t)s;
@@ -1450,6 +1622,7 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_fetch_targets(ipr_dev d)
     if (ctl->done) return;
     const uint64_t s = ctl->superstep;
     ipr_sm_superstep = s;
+    ipr_sm_counted = 0;
     const uint32_t want = (uint32_t)(s + 1);
     const uint64_t n = ctl->fsize[(s + 1) & 1];
     const uint32_t* list = d.front[(s + 1) & 1];
@@ -1582,6 +1755,7 @@ static int ipr_app_bind(const ipr_dev* d)
 __global__ void ipr_k_check_combiner(const IP_MESSAGE_TYPE* a, const IP_MESSAGE_TYPE* b, int n, int* bad)
 {
     ipr_sm_superstep = 0;   /* ip_combine may ask (it runs before the first superstep) */
+    ipr_sm_counted = 0;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     IP_MESSAGE_TYPE x = a[i], y = x;

@@ -736,7 +736,8 @@ static void enqueue_superstep(int fused, uint32_t step_in_chunk)
     pmark(0);
     if (fused) {
         /* the fused kernel takes its superstep from the chunk's base plus its
-         * place in the chunk, passed in its copy of the device state */
+         * place in the chunk, passed in its copy of the device state; its last
+         * workgroup ends the superstep and leaves the next frontier as flags */
         ipr_dev dk = D.dev;
         dk.step_in_chunk = step_in_chunk;
         kev_mark(0);
@@ -746,8 +747,6 @@ static void enqueue_superstep(int fused, uint32_t step_in_chunk)
         kev_mark(2);
         pmark(1);
         pmark(2);
-        /* the ascending next frontier; its last workgroup ends the superstep */
-        hip_check(ipr_hip_compact_frontier(&D.dev, IPR_VARIANT_SPREAD, D.stream), "compact");
         pmark(3);
         if ((D.kev_on || D.prof_on) && D.kev_slot < 64) D.kev_slot++;
         return;
@@ -773,7 +772,7 @@ static void enqueue_superstep(int fused, uint32_t step_in_chunk)
         kev_mark(1);
         launch_app(IPR_K_EXPAND);
         kev_mark(2);
-        hip_check(ipr_hip_compact_frontier(&D.dev, -1, D.stream), "compact");
+        hip_check(ipr_hip_compact_frontier(&D.dev, -1, 0, D.stream), "compact");
         launch_app(IPR_K_FETCH);
         pmark(2);
         break;
@@ -790,7 +789,7 @@ static void enqueue_superstep(int fused, uint32_t step_in_chunk)
         pmark(2);
         /* spread: the compaction's last workgroup ends the superstep */
         if (D.app.variant == IPR_VARIANT_SPREAD)
-            hip_check(ipr_hip_compact_frontier(&D.dev, D.dev.fin_in_kernel ? IPR_VARIANT_SPREAD : -1, D.stream),
+            hip_check(ipr_hip_compact_frontier(&D.dev, D.dev.fin_in_kernel ? IPR_VARIANT_SPREAD : -1, 0, D.stream),
                       "compact");
         else
             launch_app(IPR_K_SWAP);
@@ -914,9 +913,15 @@ static void run_once(void)
         prev_exec = D.h_ctl->executed;
         if (D.h_ctl->done) break;
         if (fused && D.h_ctl->bail) {
-            /* back to the general kernels (the compaction after every fused
-             * superstep kept the 4-byte records' stamp-window clears) */
+            /* back to the general kernels (the fused kernels kept the 4-byte
+             * records' stamp-window clears); a frontier that a fused kernel
+             * left as flags is compacted into its list first */
             hip_check(ipr_hip_fill(&D.dev.ctl->bail, 0, sizeof(uint32_t), st), "bail clear");
+            if (D.h_ctl->front_flags) {
+                const uint64_t s = D.h_ctl->superstep;
+                hip_check(ipr_hip_set_u64(&D.dev.ctl->fsize[s & 1], 0, st), "relist counter");
+                hip_check(ipr_hip_compact_frontier(&D.dev, -1, 1, st), "relist");
+            }
             fused = 0;
             chunk = 1;
             continue;

@@ -674,7 +674,7 @@ extern "C" int ipr_hip_build_pull_bins(ipr_dev* dev, uint32_t small_max, uint32_
  * consecutive entries per wavefront store. T is 4, or less for graphs that
  * would otherwise get under 256 workgroups. */
 template <int T>
-__global__ void __launch_bounds__(256) k_compact(ipr_dev d, int fin_variant)
+__global__ void __launch_bounds__(256) k_compact(ipr_dev d, int fin_variant, int relist)
 {
     __shared__ uint32_t wsum[T][4];
     __shared__ uint32_t stage[256 * 16];
@@ -684,16 +684,20 @@ __global__ void __launch_bounds__(256) k_compact(ipr_dev d, int fin_variant)
      * general kernels: no compaction either */
     if (c->done || c->bail) return;
     const uint64_t s = c->superstep;
-    uint4* flags = (uint4*)d.recv[(s + 1) & 1];
-    uint32_t* out = d.front[(s + 1) & 1];
-    unsigned long long* counter = (unsigned long long*)&c->fsize[(s + 1) & 1];
+    /* relist: the current superstep's frontier, left as flags by a fused
+     * sparse kernel, into its list (its size is already known; the host
+     * zeroes the counter first) */
+    const uint64_t par = relist ? (s & 1) : ((s + 1) & 1);
+    uint4* flags = (uint4*)d.recv[par];
+    uint32_t* out = d.front[par];
+    unsigned long long* counter = (unsigned long long*)&c->fsize[par];
     const uint64_t groups = (d.V + 15) / 16;           /* flag arrays are padded to 16 */
     /* with the fused sparse kernel on, after a superstep that did not gather:
      * out-arcs of the new frontier (sum and largest out-degree). After a
      * dense superstep they are not read (the end of the superstep marks them
      * unknown), which spares a read per entry of a large frontier. */
     uint64_t f_arcs = 0, f_max = 0;
-    bool want_f = d.fused;
+    bool want_f = d.fused && !relist;
     if (want_f && d.dir_opt) {
         uint64_t w = d.cnt[IPR_C_EDGES * IPR_CSLOTS + (threadIdx.x & 63)];
         for (int o = 32; o > 0; o >>= 1) w += __shfl_xor(w, o);
@@ -703,7 +707,7 @@ __global__ void __launch_bounds__(256) k_compact(ipr_dev d, int fin_variant)
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     /* the last superstep of a stamp window clears the 4-byte gather records
      * (padded to 16 like the flags) */
-    uint4* clr = d.brec4 && (s + 1) % IPR_REC4_WINDOW == 0 ? (uint4*)d.brec4 : nullptr;
+    uint4* clr = !relist && d.brec4 && (s + 1) % IPR_REC4_WINDOW == 0 ? (uint4*)d.brec4 : nullptr;
     for (uint64_t b0 = (uint64_t)blockIdx.x * span; b0 < groups; b0 += (uint64_t)gridDim.x * span) {
         uint4 q[T];
         uint32_t n[T], incl[T];
@@ -804,11 +808,11 @@ __global__ void __launch_bounds__(256) k_compact(ipr_dev d, int fin_variant)
     }
     /* after a fused sparse superstep: the last workgroup ends the superstep
      * (every workgroup counts, so none reads the control block after it) */
-    if (fin_variant >= 0 && ipr_last_workgroup(d, gridDim.x) && threadIdx.x < 64)
+    if (fin_variant >= 0 && !relist && ipr_last_workgroup(d, gridDim.x) && threadIdx.x < 64)
         ipr_finalize_step(d, fin_variant);
 }
 
-extern "C" int ipr_hip_compact_frontier(const ipr_dev* dev, int finalize_variant, void* stream)
+extern "C" int ipr_hip_compact_frontier(const ipr_dev* dev, int finalize_variant, int relist, void* stream)
 {
     const uint64_t groups = (dev->V + 15) / 16;
     /* rounds per workgroup: 4, or fewer where that leaves under 256
@@ -825,10 +829,10 @@ extern "C" int ipr_hip_compact_frontier(const ipr_dev* dev, int finalize_variant
     const dim3 grid((unsigned)blocks), block(256);
     hipStream_t st = (hipStream_t)stream;
     switch (T) {
-    case 8: hipLaunchKernelGGL(k_compact<8>, grid, block, 0, st, *dev, finalize_variant); break;
-    case 4: hipLaunchKernelGGL(k_compact<4>, grid, block, 0, st, *dev, finalize_variant); break;
-    case 2: hipLaunchKernelGGL(k_compact<2>, grid, block, 0, st, *dev, finalize_variant); break;
-    default: hipLaunchKernelGGL(k_compact<1>, grid, block, 0, st, *dev, finalize_variant); break;
+    case 8: hipLaunchKernelGGL(k_compact<8>, grid, block, 0, st, *dev, finalize_variant, relist); break;
+    case 4: hipLaunchKernelGGL(k_compact<4>, grid, block, 0, st, *dev, finalize_variant, relist); break;
+    case 2: hipLaunchKernelGGL(k_compact<2>, grid, block, 0, st, *dev, finalize_variant, relist); break;
+    default: hipLaunchKernelGGL(k_compact<1>, grid, block, 0, st, *dev, finalize_variant, relist); break;
     }
     CK(hipGetLastError());
     return 0;

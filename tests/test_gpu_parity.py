@@ -247,16 +247,17 @@ def test_gather_stamp_window_and_escapes(record4, monkeypatch):
     ("1", "", "", 0), ("0", "", "", 0), ("1", "1", "", 0), ("1", "3", "", 0),
     ("1", "", "4", 0), ("1", "2", "2", 40), ("1", "", "", 40)])
 def test_fused_sparse_supersteps(fused, blocks, maxdeg, alpha, monkeypatch):
-    """Sparse supersteps of the spread variant as one fused kernel (compute,
-    expansion, next frontier appended through an LDS buffer, the last
-    workgroup ending the superstep), against the general kernels and the
-    oracle: values and per-superstep active counts. With 1 or 3 workgroups
-    (many tiles each, buffer flushes), with a tiny out-degree limit or a low
-    gather threshold (supersteps alternate between the fused and the general
-    kernels, the fused one stopping itself when its superstep turns dense),
-    on CC (R-MAT 14), BFS and weighted SSSP on a 57x61 grid (a hundred sparse
-    supersteps, past the 4-byte records' 63-superstep stamp window), and
-    ip_send_message appending to the next frontier."""
+    """Sparse supersteps of the spread variant as one fused kernel (frontier
+    from the list or from the raised flags, compute, expansion, the next
+    frontier counted as receivers are activated, the last workgroup ending the
+    superstep), against the general kernels and the oracle: values and
+    per-superstep active counts. With 1 or 3 workgroups (many scan passes and
+    tiles each), with a tiny out-degree limit or a low gather threshold
+    (supersteps alternate between the fused and the general kernels, the fused
+    one stopping itself when its superstep turns dense and the host rebuilding
+    the list from the flags), on CC (R-MAT 14), BFS and weighted SSSP on a
+    57x61 grid (a hundred sparse supersteps, past the 4-byte records'
+    63-superstep stamp window)."""
     monkeypatch.setenv("IPR_FUSED", fused)
     for k, v in (("IPR_FUSED_BLOCKS", blocks), ("IPR_FUSED_MAX_DEG", maxdeg)):
         if v:
