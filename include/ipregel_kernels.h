@@ -214,6 +214,10 @@ template <typename T> struct ipr_comb<T, IPR_COMBINER_MIN> {
         if (m < cur) return ipr_atomic_min(p, m) == ident();
         return cur == ident();            /* m == identity on an empty slot */
     }
+    /* the same without reading the slot first: every delivery is an atomic,
+     * one round trip less for those that lower the slot (measured faster in
+     * the fused sparse kernel, where a receiver's flag is raised after) */
+    __device__ static bool deliver_direct(slot_t* p, T m) { return ipr_atomic_min(p, m) == ident(); }
     /* single writer (gather): no atomics */
     __device__ static void merge(slot_t* p, T m) { if (m < *p) *p = m; }
 };
@@ -235,6 +239,7 @@ template <typename T> struct ipr_comb<T, IPR_COMBINER_MAX> {
         if (m > cur) return ipr_atomic_max(p, m) == ident();
         return cur == ident();
     }
+    __device__ static bool deliver_direct(slot_t* p, T m) { return ipr_atomic_max(p, m) == ident(); }
     __device__ static void merge(slot_t* p, T m) { if (m > *p) *p = m; }
 };
 
@@ -254,6 +259,7 @@ template <typename T> struct ipr_comb<T, IPR_COMBINER_SUM> {
         atomicAdd(p, m);
         return true;
     }
+    __device__ static bool deliver_direct(slot_t* p, T m) { return deliver(p, m); }
     __device__ static void merge(slot_t* p, T m) { *p += m; }
 };
 
@@ -274,6 +280,7 @@ template <typename T> struct ipr_comb<T, IPR_COMBINER_GENERIC> {
     __device__ static slot_t empty() { return 0ull; }
     __device__ static T value(slot_t s) { uint32_t b = (uint32_t)s; T t; memcpy(&t, &b, 4); return t; }
     __device__ static void fold(T* acc, T m) { ip_combine(acc, m); }
+    __device__ static bool deliver_direct(slot_t* p, T m) { return deliver(p, m); }
     __device__ static bool deliver(slot_t* p, T m)
     {
         slot_t old = *p;
@@ -413,7 +420,7 @@ __device__ inline void ipr_emit(const ipr_dev& d, uint64_t s, uint32_t u, IP_MES
 __device__ inline bool ipr_deliver_activates(const ipr_dev& d, uint64_t s, uint32_t u, IP_MESSAGE_TYPE m)
 {
     ipr_slot_t* mb = (ipr_slot_t*)d.mbox[(s + 1) & 1];
-    if (!ipr_policy::deliver(mb + u, m)) return false;
+    if (!ipr_policy::deliver_direct(mb + u, m)) return false;
     uint8_t* recv = d.recv[(s + 1) & 1];
     if (!ipr_policy::first_once && __atomic_load_n(recv + u, __ATOMIC_RELAXED)) return false;
     const uint32_t bit = 1u << (8 * (u & 3u));
@@ -842,7 +849,7 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_sparse_step(ipr_dev d)
     __shared__ IP_MESSAGE_TYPE s_msg[IPR_BLOCK];
     __shared__ uint8_t s_wf[IPR_BLOCK];
     __shared__ uint32_t scan[IPR_WAVES];
-    __shared__ uint32_t s_ids[IPR_SCAN_SPAN];
+    __shared__ uint16_t s_ids[IPR_SCAN_SPAN];  /* offsets within the pass */
     ipr_ctl* ctl = d.ctl;
     /* the superstep: the chunk's base plus this kernel's place in it (a
      * kernel after a bail in the same chunk finds ctl->superstep behind) */
@@ -903,15 +910,15 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_sparse_step(ipr_dev d)
                 uint32_t w = words[k];
                 while (w) {
                     const uint32_t bit = __ffs(w) - 1;
-                    s_ids[pos++] = (uint32_t)(g * 16 + k * 4 + bit / 8);
+                    s_ids[pos++] = (uint16_t)(threadIdx.x * 16 + k * 4 + bit / 8);
                     w &= w - 1;
                 }
             }
             __syncthreads();
             for (uint32_t t0 = 0; t0 < total; t0 += IPR_BLOCK) {
                 const uint32_t t = t0 + threadIdx.x;
-                ipr_sparse_tile(d, s, mcur, t < total ? s_ids[t] : 0u, t < total, s_ex, s_off, s_msg, s_wf, scan,
-                                &n_computed, &n_edges, &n_new, &f_arcs, &f_max);
+                ipr_sparse_tile(d, s, mcur, t < total ? (uint32_t)(g0 * 16 + s_ids[t]) : 0u, t < total, s_ex, s_off,
+                                s_msg, s_wf, scan, &n_computed, &n_edges, &n_new, &f_arcs, &f_max);
             }
         }
     }

@@ -433,12 +433,12 @@ static void dev_setup(void)
     }
     /* frontier compaction rounds per workgroup (0: picked from V) */
     d->compact_rounds = env_u32("IPR_COMPACT_ROUNDS", 0);
-    /* spread: a sparse superstep is two launches, the fused compute and
-     * expansion, then the frontier compaction whose last workgroup ends the
-     * superstep; a frontier vertex above large_min out-arcs keeps the
-     * superstep on the general kernels, which split it in chunks */
+    /* spread: a sparse superstep is one launch (frontier scan, compute,
+     * expansion; its last workgroup ends the superstep); a frontier vertex
+     * above large_min out-arcs keeps the superstep on the general kernels,
+     * which split it in chunks */
     d->fused = var == IPR_VARIANT_SPREAD && env_u32("IPR_FUSED", 1) == 1;
-    d->fused_blocks = env_u32("IPR_FUSED_BLOCKS", 1024);
+    d->fused_blocks = env_u32("IPR_FUSED_BLOCKS", 1536);
     if (d->fused_blocks < 1) d->fused_blocks = 1;
     d->fused_max_deg = env_u32("IPR_FUSED_MAX_DEG", d->large_min);
     {
