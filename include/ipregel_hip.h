@@ -109,6 +109,15 @@ int ipr_hip_build_gather_classes(uint64_t V, uint64_t E, const uint64_t* t_idx, 
 /* byte-wise equality of two device buffers (multiple of 4 bytes) */
 int ipr_hip_equal(const void* a, const void* b, uint64_t bytes, int* equal, void* stream);
 
+/* ipr_hip_equal, allocating nothing: no counterpart in the reference. For
+ * callers that cannot allocate device memory (code in a tight serving loop,
+ * harnesses that forbid hipMalloc). Its scratch is a 256-byte device array per
+ * GPU, held until the process ends (one 32-bit difference flag is used, so
+ * the input size is not limited by it); bytes must be a multiple of 4, else
+ * it returns an error. Calls that overlap on one GPU share the flag and race.
+ * The same result as ipr_hip_equal for every input both accept. */
+int ipr_hip_equal_noalloc(const void* a, const void* b, uint64_t bytes, int* equal, void* stream);
+
 /* push variants: the next frontier (claim bitmap of superstep s+1) compacted
  * into front[(s+1)&1], ascending within runs; the bitmap is cleared */
 /* finalize_variant >= 0: the compaction's last workgroup also ends the

@@ -6,6 +6,7 @@
  */
 #include <cstring>
 #include <cstdio>
+#include <mutex>
 
 #include <hip/hip_runtime.h>
 #include <rocprim/device/device_radix_sort.hpp>
@@ -29,6 +30,37 @@ static int fail(hipError_t e, const char* what)
 
 extern "C" const char* ipr_hip_last_error(void) { return g_err; }
 
+/* Scratch of the allocation-free entry point (ipr_hip_equal_noalloc): a
+ * file-scope device array, one copy per GPU, held until the process ends.
+ * Host code never passes its name to a kernel: the name is a host-side
+ * stand-in, and the device address of each GPU's copy is looked up once per
+ * device (hipGetSymbolAddress, with that device current) and kept here. */
+#define IPR_NOALLOC_SCRATCH_BYTES 256
+#define IPR_NOALLOC_MAX_DEVICES 64
+static __device__ __align__(256) unsigned int scratch[IPR_NOALLOC_SCRATCH_BYTES / 4];
+static unsigned int* g_scratch[IPR_NOALLOC_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+/* the current device's copy of `scratch` (looked up on first use) */
+static int scratch_of_current_device(unsigned int** out)
+{
+    int dev = 0;
+    CK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= IPR_NOALLOC_MAX_DEVICES) {
+        snprintf(g_err, sizeof(g_err), "device %d is beyond the %d devices the scratch table holds", dev,
+                 IPR_NOALLOC_MAX_DEVICES);
+        return -(int)hipErrorInvalidDevice;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        CK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch[dev] = (unsigned int*)p;
+    }
+    *out = g_scratch[dev];
+    return 0;
+}
+
 extern "C" int ipr_hip_init(int device, char* name, size_t name_len, uint64_t* hbm_bytes)
 {
     int n = 0;
@@ -38,6 +70,9 @@ extern "C" int ipr_hip_init(int device, char* name, size_t name_len, uint64_t* h
         return -(int)hipErrorInvalidDevice;
     }
     CK(hipSetDevice(device));
+    unsigned int* sp = nullptr;
+    int rc = scratch_of_current_device(&sp);
+    if (rc) return rc;
     hipDeviceProp_t p;
     CK(hipGetDeviceProperties(&p, device));
     if (name && name_len) snprintf(name, name_len, "%s (%s)", p.name, p.gcnArchName);
@@ -864,6 +899,47 @@ extern "C" int ipr_hip_equal(const void* a, const void* b, uint64_t bytes, int* 
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     (void)hipFree(flag);
     if (e != hipSuccess) return fail(e, "equality check");
+    *equal = h == 0;
+    return 0;
+}
+
+__global__ void k_clear_u32(unsigned int* p, uint32_t n)
+{
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
+}
+
+/* ipr_hip_equal without an allocation: the difference flag is the first word
+ * of this device's `scratch`, cleared by a kernel (not a memset node) before
+ * the comparison. Calls that overlap on one GPU share the flag and race. */
+extern "C" int ipr_hip_equal_noalloc(const void* a, const void* b, uint64_t bytes, int* equal, void* stream)
+{
+    if (!equal || (bytes && (!a || !b))) {
+        snprintf(g_err, sizeof(g_err), "ipr_hip_equal_noalloc: null argument");
+        return -(int)hipErrorInvalidValue;
+    }
+    if (bytes % 4) {
+        snprintf(g_err, sizeof(g_err), "ipr_hip_equal_noalloc: %llu bytes is not a multiple of 4",
+                 (unsigned long long)bytes);
+        return -(int)hipErrorInvalidValue;
+    }
+    unsigned int* flag = nullptr;
+    int rc = scratch_of_current_device(&flag);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    unsigned int h = 0;
+    hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(64), 0, st, flag, 1u);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && bytes >= 4) {
+        const uint64_t n = bytes / 4;
+        uint64_t blocks = (n + 255) / 256;
+        if (blocks > 16384) blocks = 16384;
+        hipLaunchKernelGGL(k_differ, dim3((unsigned)blocks), dim3(256), 0, st, (const uint32_t*)a,
+                           (const uint32_t*)b, n, flag);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, flag, sizeof(h), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return fail(e, "equality check (no allocation)");
     *equal = h == 0;
     return 0;
 }
