@@ -140,15 +140,18 @@ WORKLOADS = {
                  desc="cc_spread hashmin CC, symmetrised R-MAT scale 24 ef 16 a/b/c/d .57/.19/.19/.05 seed 2"),
 }
 
-# HBM bytes per dense launch of the gather, from rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE passes over `bench.py --apps <workload>` (FETCH_SIZE scaled by 2.0
-# and WRITE_SIZE by 1.0, the factors a plain 1 GiB device copy gives; KiB,
-# mean over the dense launches); commands and per-launch figures are in
-# profiles/ipr_k_gather_cc_rmat22.md, ipr_k_gather_cc_rmat24.md and
-# ipr_k_gather_sssp_rmat24.md
-TRAFFIC = {("cc22", "ipr_k_gather"): (767_391 * 2 + 57_213) * 1024,
-           ("cc24", "ipr_k_gather"): (7_922_027 * 2 + 246_882) * 1024,
-           ("ss24a", "ipr_k_gather"): (4_068_136 * 2 + 182_677) * 1024}
+# HBM bytes per launch of each workload's dominant kernel, from rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE passes over `bench.py --apps <workload>` with every
+# kernel launched directly (FETCH_SIZE scaled by 2.0 and WRITE_SIZE by 1.0, the
+# factors a plain 1 GiB device copy gives; KiB, mean over the launches that do
+# work: the dense gathers, the pull launches of supersteps 0-11, every fused
+# sparse launch); commands and per-launch figures are in profiles/
+TRAFFIC = {("cc22", "ipr_k_gather"): (767_103 * 2 + 56_781) * 1024,
+           ("cc24", "ipr_k_gather"): (7_919_949 * 2 + 245_533) * 1024,
+           ("ss24a", "ipr_k_gather"): (4_066_499 * 2 + 181_607) * 1024,
+           ("ss24b", "ipr_k_gather"): (4_645_812 * 2 + 205_246) * 1024,
+           ("pr16", "ipr_k_pull_step"): (5_975 * 2 + 1_385) * 1024,
+           ("grid", "ipr_k_sparse_step"): (34_089 * 2 + 31_473) * 1024}
 
 
 def model_bytes(app: str, g: G.Graph, W: int, F: int, supersteps_with_bcast: int) -> int:
