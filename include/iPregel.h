@@ -245,7 +245,7 @@ __device__ inline bool ip_has_message(struct ip_vertex_t* v)
     return v->has_message;
 }
 
-/* src/combiner_postamble.h:124-134 (identical in all variants) */
+/* src/combiner_postamble.h:29-39 (identical in all variants) */
 __device__ inline bool ip_get_next_message(struct ip_vertex_t* v, IP_MESSAGE_TYPE* message_value)
 {
     if (v->has_message) {
@@ -260,7 +260,7 @@ __device__ inline bool ip_get_next_message(struct ip_vertex_t* v, IP_MESSAGE_TYP
  * dense superstep) delivers it to every out-neighbour. Two broadcasts in one
  * ip_compute reach the same neighbours, so they are combined here.
  * Pull variants write the vertex's broadcast slot (last one wins),
- * src/combiner_single_broadcast_postamble.h:155-159. */
+ * src/combiner_single_broadcast_postamble.h:51-55. */
 __device__ inline void ip_broadcast(struct ip_vertex_t* v, IP_MESSAGE_TYPE message)
 {
 #if IPR_IS_PULL
@@ -318,7 +318,7 @@ __device__ inline void ip_send_message(IP_VERTEX_ID_TYPE id, IP_MESSAGE_TYPE mes
 
 __device__ inline void ip_vote_to_halt(struct ip_vertex_t* v)
 {
-    /* a no-op in the spread variants (src/combiner_spread_postamble.h:562-565);
+    /* a no-op in the spread variants (src/combiner_spread_postamble.h:433-436);
      * the spread kernels ignore the flag */
     v->active = false;
 }
@@ -520,7 +520,7 @@ static void ipr_host_view_build(void)
 }
 
 /* src/iPregel_postamble.h:78-81; the location is the id here (the reference's
- * IP_FORCE_DIRECT_MAPPING / IP_ID_OFFSET == 0 case, :83-95) */
+ * IP_FORCE_DIRECT_MAPPING / IP_ID_OFFSET == 0 case, :85-93) */
 struct ip_vertex_t* ip_get_vertex_by_location(size_t location)
 {
     if (!ipr_host_view_current()) ipr_host_view_build();

@@ -5,10 +5,10 @@
  * the application-TU kernels that end their superstep themselves: there, the
  * last workgroup to finish does it, so the superstep needs no extra launch.
  *
- * Reference loop tests: src/combiner_postamble.h:131 (full-sweep push,
- * `while active > 0`), src/combiner_single_broadcast_postamble.h:134 (pull),
- * src/combiner_spread_postamble.h:170 (spread: `first || active > 0` with
- * active = |next frontier|, :424-425).
+ * Reference loop tests: src/combiner_postamble.h:134 (full-sweep push,
+ * `while active > 0`), src/combiner_single_broadcast_postamble.h:163 (pull),
+ * src/combiner_spread_postamble.h:212 (spread: `first || active > 0` with
+ * active = |next frontier|, :278-285).
  *
  * Coherence: the eight L2 caches of an MI355X are not coherent with one
  * another within a kernel. Counters are only ever changed by device-scope

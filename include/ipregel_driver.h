@@ -5,7 +5,7 @@
  * Reference interfaces replaced (file:line in capellil/iPregel):
  *   ipr_drv_init   ip_init         src/iPregel_postamble.h:195-242
  *                  ip_load_graph   src/iPregel_postamble.h:508-546
- *   ipr_drv_run    ip_run          e.g. src/combiner_spread_postamble.h:289-560
+ *   ipr_drv_run    ip_run          e.g. src/combiner_spread_postamble.h:160-431
  *   ipr_drv_dump   ip_dump         src/iPregel_postamble.h:103-115
  * Error behaviour is the reference's: a message on stdout and exit(-1).
  */

@@ -32,7 +32,7 @@ extern "C" {
 #endif
 
 /* engine variants: IP_USE_SPREAD x IP_USE_SINGLE_BROADCAST
- * (src/iPregel_preamble.h:338-350) */
+ * (src/iPregel_preamble.h:319-331) */
 enum {
     IPR_VARIANT_PUSH = 0,        /* combiner_{pre,post}amble.h */
     IPR_VARIANT_PULL = 1,        /* combiner_single_broadcast_{pre,post}amble.h */

@@ -15,7 +15,7 @@
  *                                  the superstep loop control of ip_run: counter
  *                                  reset, active count, halting test, superstep
  *                                  increment (e.g. src/combiner_spread_postamble.h
- *                                  :341-354, :410-414, :476-532)
+ *                                  :212, :221-225, :278-285, :347-403)
  *   ipr_hip_build_pull_bins        (new) in-degree bins of the pull gather
  *   ipr_hip_build_gather           (new) arc-balanced index of the push
  *                                  applications' dense-superstep gather

@@ -25,7 +25,7 @@
  *            skewed R-MAT degrees). A delivery combines into mbox[(s+1)&1]
  *            with a hardware atomic (declared combiner) or a CAS loop around
  *            ip_combine and raises recv[(s+1)&1][u]
- *            (src/combiner_spread_postamble.h:202-227).
+ *            (src/combiner_spread_postamble.h:46-98).
  *   The next frontier is compacted from recv (libipregel_hip.so) in
  *   ascending runs; PUSH's swap turns recv into has_message instead.
  * Pull variant (PULL): one kernel per superstep fetches superstep s-1's
@@ -265,7 +265,7 @@ template <typename T> struct ipr_comb<T, IPR_COMBINER_SUM> {
 
 /* GENERIC: 32-bit message packed with a presence flag in one 64-bit slot,
  * combined with the application's ip_combine in a CAS loop
- * (src/combiner_postamble.h:136-147 combines the same way, under a lock for
+ * (src/combiner_postamble.h:41-52 and :54-79 combine the same way, under a lock for
  * the first message). */
 template <typename T> struct ipr_comb<T, IPR_COMBINER_GENERIC> {
     static_assert(sizeof(T) == 4,
@@ -505,7 +505,7 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_init(ipr_dev d)
     uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < d.V; v += stride) {
         ((IP_VALUE_TYPE*)d.val)[v] = IP_VALUE_TYPE();
-        d.act[v] = 1;  /* ip_init_vertex_range, e.g. src/combiner_postamble.h:189 */
+        d.act[v] = 1;  /* ip_init_vertex_range, e.g. src/combiner_postamble.h:94 */
         d.hm[v] = 0;
 #if IPR_VARIANT == 1
         ((ipr_bslot_t*)d.bcm[0])[v].step = 0;
@@ -529,10 +529,10 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_init(ipr_dev d)
 /* push variants: compute                                                    */
 /* ------------------------------------------------------------------------ */
 
-/* SPREAD: src/combiner_spread_postamble.h:364-401 (superstep 0 over all V,
- * then the frontier) and :459-471 (the frontier's mailboxes become messages:
+/* SPREAD: src/combiner_spread_postamble.h:227-276 (superstep 0 over all V,
+ * then the frontier) and :322-340 (the frontier's mailboxes become messages:
  * read straight from the mailbox, which is reset for reuse two supersteps
- * on). PUSH: src/combiner_postamble.h:246-259 (every vertex with active or
+ * on). PUSH: src/combiner_postamble.h:151-164 (every vertex with active or
  * has_message). */
 __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_compute_push(ipr_dev d)
 {
@@ -1313,7 +1313,7 @@ __global__ void __launch_bounds__(IPR_GATHER_WAVES * IPR_WAVE, ipr_gather_min_wa
     }
 }
 
-/* PUSH: src/combiner_postamble.h:263-278 - a delivered mailbox becomes the
+/* PUSH: src/combiner_postamble.h:166-183 - a delivered mailbox becomes the
  * vertex's message and reactivates it. */
 __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_swap(ipr_dev d)
 {
@@ -1346,7 +1346,7 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_swap(ipr_dev d)
 /* ------------------------------------------------------------------------ */
 
 /* fold of the broadcasting in-neighbours of v in in-list order
- * (src/combiner_single_broadcast_postamble.h:161-193); pr holds superstep
+ * (src/combiner_single_broadcast_postamble.h:57-89); pr holds superstep
  * s-1's broadcast records, stamped want = s */
 __device__ inline bool ipr_fetch_ordered(const ipr_dev& d, uint32_t v, const ipr_bslot_t* pr, uint32_t want,
                                          IP_MESSAGE_TYPE* out)

@@ -6,7 +6,7 @@
  *   ip_init        schedule check, provenance banner, graph load   (:176-242)
  *   loader         .config / .idx / .adj (+ .wgt)                    (:244-546)
  *   ip_run         the superstep loop; per-superstep log lines       (e.g.
- *                  src/combiner_spread_postamble.h:341-536)
+ *                  src/combiner_spread_postamble.h:160-431)
  *   ip_dump        values in id order through ip_serialise_vertex    (:103-115)
  * All device work goes through libipregel_hip.so (include/ipregel_hip.h) and
  * the kernel table of the application (ipr_app_desc.launch). No superstep
@@ -1000,7 +1000,7 @@ int ipr_harness_kernel_times(double* first, double* second, uint64_t cap)
 }
 
 /* the reference's IP_ENABLE_THREAD_PROFILING table
- * (src/combiner_spread_postamble.h:352-399), with the device as its one
+ * (src/combiner_spread_postamble.h:354-400), with the device as its one
  * column: phase times of the superstep in seconds, and its share of the
  * superstep's relaxed arcs (the device's: 100%) with their count */
 static void print_phase_table(uint64_t s, uint64_t arcs)

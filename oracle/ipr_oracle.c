@@ -72,20 +72,20 @@ struct o_engine {
 static int o_get_next_message(o_engine* e, o_vertex* v, msg_t* m)
 {
     (void)e;
-    /* every variant: src/combiner_postamble.h:124-134 and its three twins */
+    /* every variant: src/combiner_postamble.h:29-39 and its three twins */
     if (v->has_message) { *m = v->message; v->has_message = 0; return 1; }
     return 0;
 }
 
 static void o_vote_to_halt(o_engine* e, o_vertex* v)
 {
-    /* no-op in the two spread variants (combiner_spread_postamble.h:562-565) */
+    /* no-op in the two spread variants (combiner_spread_postamble.h:433-436) */
     if (e->variant == IPR_O_PUSH || e->variant == IPR_O_PULL) v->active = 0;
 }
 
 /* Push delivery: first message of the superstep is stored, the rest combined
- * (combiner_postamble.h:149-174; spread version also appends the receiver to
- * the next frontier, combiner_spread_postamble.h:202-227). */
+ * (combiner_postamble.h:41-79; spread version also appends the receiver to
+ * the next frontier, combiner_spread_postamble.h:46-98). */
 static void o_send(o_engine* e, uint32_t id, msg_t m)
 {
     if (e->variant == IPR_O_PULL || e->variant == IPR_O_SPREAD_PULL) {
@@ -106,7 +106,7 @@ static void o_broadcast(o_engine* e, o_vertex* v, msg_t m)
 {
     e->res->work_edges += (e->variant == IPR_O_PULL) ? 0 : v->out_count;
     if (e->variant == IPR_O_PULL) {
-        /* combiner_single_broadcast_postamble.h:155-159 */
+        /* combiner_single_broadcast_postamble.h:51-55 */
         e->bcf[v->id] = 1; e->bcm[v->id] = m;
     } else if (e->variant == IPR_O_SPREAD_PULL) {
         /* combiner_spread_single_broadcast_postamble.h ip_broadcast */
@@ -129,7 +129,7 @@ static void o_broadcast_weighted(o_engine* e, o_vertex* v, msg_t m)
 }
 
 /* Pull fetch: first broadcasting in-neighbour's message, then the others
- * combined in in-list order (combiner_single_broadcast_postamble.h:161-193).
+ * combined in in-list order (combiner_single_broadcast_postamble.h:57-89).
  * Returns 1 when a message was found. */
 static int o_fetch(o_engine* e, uint32_t v)
 {
@@ -246,7 +246,7 @@ static void record(o_engine* e, uint64_t* trace, uint64_t active)
         trace[e->superstep] = active;
 }
 
-/* Push, full sweep: src/combiner_postamble.h:218-294. */
+/* Push, full sweep: src/combiner_postamble.h:123-199. */
 static double o_now(void)
 {
     struct timespec ts;
@@ -290,7 +290,7 @@ static void run_push(o_engine* e, uint64_t* trace)
     }
 }
 
-/* Pull (single broadcast): src/combiner_single_broadcast_postamble.h:229-363. */
+/* Pull (single broadcast): src/combiner_single_broadcast_postamble.h:125-259. */
 static void run_pull(o_engine* e, uint64_t* trace)
 {
     uint64_t active = e->V;
@@ -317,7 +317,7 @@ static void run_pull(o_engine* e, uint64_t* trace)
     }
 }
 
-/* Push with frontier (spread): src/combiner_spread_postamble.h:289-560.
+/* Push with frontier (spread): src/combiner_spread_postamble.h:160-431.
  * The frontier order of the reference depends on threads; it changes nothing
  * observable, so here it is the order of first delivery. */
 static void run_spread(o_engine* e, uint64_t* trace)
@@ -422,7 +422,7 @@ int ipr_oracle_run(const ipr_csr* g, int directed, int variant, int app,
         v->out_count = g->idx[i + 1] - g->idx[i];
         v->out = g->adj + g->idx[i];
         v->out_w = g->wgt ? g->wgt + g->idx[i] : NULL;
-        v->active = 1;       /* ip_init_vertex_range, e.g. combiner_postamble.h:189 */
+        v->active = 1;       /* ip_init_vertex_range, e.g. combiner_postamble.h:94 */
         v->has_message = 0;
     }
     if (V > 0) {

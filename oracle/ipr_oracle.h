@@ -7,9 +7,9 @@
  *
  * It restates, single-threaded and in plain C, the BSP semantics of the four
  * engine variants of iPregel and of its three benchmark applications:
- *   variant PUSH         src/combiner_postamble.h:218-294
- *   variant PULL         src/combiner_single_broadcast_postamble.h:229-363
- *   variant SPREAD       src/combiner_spread_postamble.h:289-560
+ *   variant PUSH         src/combiner_postamble.h:123-199
+ *   variant PULL         src/combiner_single_broadcast_postamble.h:125-259
+ *   variant SPREAD       src/combiner_spread_postamble.h:160-431
  *   variant SPREAD_PULL  src/combiner_spread_single_broadcast_postamble.h (ip_run)
  *   app PR               benchmarks/pagerank.c:36-71
  *   app CC               benchmarks/cc.c:31-63
