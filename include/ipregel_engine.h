@@ -201,6 +201,7 @@ typedef struct ipr_dev {
     uint32_t large_chunk;        /* arcs per workgroup task of a large broadcaster */
     uint32_t grid_blocks;        /* persistent-style grids */
     uint32_t pull_ordered;       /* 1: fold in in-list order (bit-exact) */
+    uint32_t pull_wave_blocks;   /* pull: most workgroups of the wavefront bin (its waves loop beyond) */
     uint32_t dir_opt;            /* push apps: allow gathering dense supersteps */
     uint32_t dir_alpha;          /* gather when bcast out-arcs * alpha > E */
     /* spread variant: supersteps whose frontier has at most fused_max_arcs

@@ -1492,8 +1492,8 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_pull_step(ipr_dev d, uint32_t
     uint64_t n_react = 0, n_active = 0, n_bcast = 0;
     const uint32_t lane = ipr_lane();
     const uint32_t wave_in_block = threadIdx.x / IPR_WAVE;
-    const uint32_t b = blockIdx.x;
     const uint32_t nb2 = gridDim.x - nb0 - nb1;
+    const uint32_t b = blockIdx.x;
     if (b < nb0) {
         const uint64_t n = d.pbin_size[0];
         if (ORDERED) {
@@ -1684,8 +1684,13 @@ static void ipr_bin_blocks(const ipr_dev& d, uint32_t nb[3])
     uint64_t b0 = (d.pbin_size[0] + IPR_BLOCK - 1) / IPR_BLOCK;
     uint64_t b1 = (d.pbin_size[1] + IPR_WAVES - 1) / IPR_WAVES;
     uint64_t b2 = d.pbin_size[2];
+    /* the wavefront bin loops when it has more vertices than pull_wave_blocks
+     * workgroups hold (3 per CU by default): PageRank scale 16 launches 1850
+     * workgroups without the cap, 1016 with it, and a run takes 0.235 ms
+     * instead of 0.256 (a cap of 400: 0.26 ms, 1024: 0.245 ms) */
+    const uint64_t cap1 = d.pull_wave_blocks ? d.pull_wave_blocks : 8192;
     nb[0] = (uint32_t)(b0 > 8192 ? 8192 : b0);
-    nb[1] = (uint32_t)(b1 > 8192 ? 8192 : b1);
+    nb[1] = (uint32_t)(b1 > cap1 ? cap1 : b1);
     nb[2] = (uint32_t)(b2 > 8192 ? 8192 : b2);
     if (nb[0] + nb[1] + nb[2] == 0) nb[0] = 1;
 }

@@ -28,7 +28,7 @@
 #include <time.h>
 
 #define IPR_TRACE_CAP (1u << 20)
-#define IPR_MAX_GRAPHS 8
+#define IPR_MAX_GRAPHS 65   /* graph cache: one per chunk length 1..64 */
 
 static struct {
     ipr_app_desc app;
@@ -54,6 +54,13 @@ static struct {
     int use_graph;
     uint32_t max_chunk;
     uint32_t first_chunk;
+    /* the launches the previous run of this graph needed, and whether it ran
+     * fused sparse supersteps: a run that repeats one without them starts
+     * with a chunk of exactly that many launches (no launches past the
+     * halting superstep, one host check) */
+    int adapt_chunk;
+    uint32_t prev_launches;
+    int prev_fused;
     int quiet;
     int kev_on;              /* events around the dominant kernel */
     void* kev[3][64];        /* start / split / stop per superstep slot of a chunk */
@@ -333,6 +340,11 @@ static void dev_setup(void)
     const ipr_harness_cfg* c = D.have_cfg ? &D.cfg : NULL;
     d->large_min = c && c->bin_medium_max ? c->bin_medium_max : env_u32("IPR_LARGE_MIN", 2048);
     d->grid_blocks = c && c->grid_blocks ? c->grid_blocks : env_u32("IPR_GRID_BLOCKS", 2048);
+    {
+        uint32_t cus = 256;
+        hip_check(ipr_hip_cu_count(&cus), "CU count");
+        d->pull_wave_blocks = env_u32("IPR_PULL_WAVE_BLOCKS", 3 * cus);
+    }
     uint32_t po = c && c->pull_ordered ? (uint32_t)c->pull_ordered : env_u32("IPR_PULL_ORDERED", 1);
     d->pull_ordered = po == 1;
     uint32_t dopt = c && c->dir_opt ? (uint32_t)c->dir_opt : env_u32("IPR_DIR_OPT", 1);
@@ -458,6 +470,9 @@ static void dev_setup(void)
         D.first_chunk = p2;
     }
     if (D.max_chunk < 1) D.max_chunk = 1;
+    D.adapt_chunk = env_u32("IPR_ADAPT_CHUNK", 1) == 1;
+    D.prev_launches = 0;
+    D.prev_fused = 0;
     hip_check(ipr_hip_host_alloc((void**)&D.h_ctl, sizeof(ipr_ctl)), "pinned ctl");
     hip_check(ipr_hip_event_create(&D.ev0), "event");
     hip_check(ipr_hip_event_create(&D.ev1), "event");
@@ -806,13 +821,12 @@ static void enqueue_superstep(int fused, uint32_t step_in_chunk)
     if ((D.kev_on || D.prof_on) && D.kev_slot < 64) D.kev_slot++;
 }
 
-/* n supersteps (a power of two up to 64): replayed from a cached HIP graph
- * per (size, kind of chunk) when graphs are on. A fused chunk starts at
- * superstep `base`, stored in the control block before the chunk. */
+/* n supersteps (1 to 64): replayed from a cached HIP graph per (length,
+ * kind of chunk) when graphs are on. A fused chunk starts at superstep
+ * `base`, stored in the control block before the chunk. */
 static void enqueue_chunk(uint32_t n, int fused, uint64_t base)
 {
-    int slot = 0;
-    while ((1u << slot) < n) slot++;
+    const uint32_t slot = n;
     D.kev_slot = 0;
     if (fused) hip_check(ipr_hip_set_u64(&D.dev.ctl->chunk_base, base, D.stream), "chunk base");
     void** cache = fused ? D.fgraphs : D.graphs;
@@ -869,6 +883,8 @@ static void run_once(void)
      * control block (about 30 us), each superstep launched after the loop has
      * halted exits at once (one launch for pull, five for the others) */
     uint32_t chunk = D.first_chunk < D.max_chunk ? D.first_chunk : D.max_chunk;
+    if (D.adapt_chunk && D.prev_launches && !D.prev_fused)
+        chunk = D.prev_launches < D.max_chunk ? D.prev_launches : D.max_chunk;
     uint64_t launched = 0, prev_exec = 0;
     int fused = 0;
     for (;;) {
@@ -947,6 +963,9 @@ static void run_once(void)
     D.stats.work_vertices = D.h_ctl->work_vertices;
     D.stats.gathered = D.h_ctl->gathered;
     D.stats.fused_steps = D.h_ctl->fused_steps;
+    /* pull: launch s finalizes superstep s-1, so one launch more than supersteps */
+    D.prev_launches = (uint32_t)(D.h_ctl->executed + (D.app.variant == IPR_VARIANT_PULL ? 1 : 0));
+    D.prev_fused = D.h_ctl->fused_steps > 0;
 }
 
 static int run_and_report(int reps);

@@ -389,6 +389,34 @@ def test_repeat_runs_identical():
     assert np.array_equal(r.values, O.bfs(g, 0))
 
 
+@pytest.mark.parametrize("adapt", ["1", "0"])
+def test_adaptive_first_chunk_keeps_results(adapt, monkeypatch):
+    """Runs that repeat a run of the same graph start with a chunk of exactly
+    the launches it needed (IPR_ADAPT_CHUNK=1, the default): PageRank ordered
+    (bit-exact, 12 launches for 11 supersteps), CC and push SSSP give the
+    oracle's values and active counts on every run, as with the rule off."""
+    monkeypatch.setenv("IPR_ADAPT_CHUNK", adapt)
+    cases = [("pagerank_single_broadcast", G.rmat(12, 16, seed=1), O.PR, O.PULL, {"pull_ordered": 1}),
+             ("cc_spread", G.rmat(13, 16, seed=4, symmetrise=True), O.CC, O.SPREAD, {}),
+             ("sssp", G.rmat(12, 8, seed=5), O.SSSP, O.PUSH, {})]
+    for app, g, oapp, ovar, opts in cases:
+        ref = O.run(g, oapp, ovar, source=0)
+        s = EN.Session(app, g, source=0, **opts)
+        try:
+            for k in range(3):
+                print(f"{app} adapt={adapt} run {k}", flush=True)
+                s.run(1)
+                v = s.values()
+                if oapp == O.PR:
+                    assert np.array_equal(v.view(np.uint64), ref.values.view(np.uint64))
+                else:
+                    assert np.array_equal(v, ref.values)
+                active, _, _ = s.trace()
+                assert list(active) == list(ref.active[: ref.supersteps])
+        finally:
+            s.close()
+
+
 def test_empty_and_edgeless_graphs():
     g = G.from_arcs(5, [], [])
     r = EN.run_app("cc_spread", g)
