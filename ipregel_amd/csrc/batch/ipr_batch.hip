@@ -1257,26 +1257,6 @@ __device__ __forceinline__ float yval64(const float* yin, uint32_t u, int lane, 
         }                                                                                           \
     }
 
-/* scalar: lanes gather and multiply 64 arcs, every lane runs the Kahan chain */
-#define PPR_STREAM1(RA, RB, EJ, CLOSE)                                                              \
-    for (uint64_t e_ = (RA); e_ < (RB); e_ += 64) {                                               \
-        const uint32_t cnt_ = (uint32_t)min<uint64_t>(64, (RB) - e_);                              \
-        float p_ = 0.f;                                                                             \
-        if ((uint32_t)lane < cnt_) {                                                                \
-            const uint32_t u_ = a.iadj[e_ + lane];                                                  \
-            const float w_ = a.inw[e_ + lane];                                                      \
-            p_ = (fresh ? (u_ == seed ? 1.f : 0.f) : yin[u_]) * w_;                                 \
-        }                                                                                           \
-        if (e_ + cnt_ <= (EJ)) {                                                                    \
-            for (uint32_t i_ = 0; i_ < cnt_; ++i_) kahan(s, c, rlf(p_, i_));                        \
-        } else {                                                                                    \
-            for (uint32_t i_ = 0; i_ < cnt_; ++i_) {                                                \
-                while (e_ + i_ >= (EJ)) { CLOSE; }                                                  \
-                kahan(s, c, rlf(p_, i_));                                                           \
-            }                                                                                       \
-        }                                                                                           \
-    }
-
 /* batched pull. Work items: the hub chunks first, then the 64-vertex segments;
  * y and the segment sums follow id order. */
 __global__ __launch_bounds__(256) void kp_pull_b(PprArgs a, int cur)
@@ -1343,7 +1323,40 @@ __global__ __launch_bounds__(256) void kp_pull_b(PprArgs a, int cur)
     }
 }
 
-/* scalar pull: same work items and order, lanes over arcs */
+/* scalar pull: the batched kernel's work items, order and arithmetic for one
+ * query. Arcs are taken 256 at a time: each lane loads four (source ids and
+ * weights coalesced, then the four sources' y), so an item of 2048 arcs waits
+ * on 16 round trips instead of 64. A hub chunk is one Kahan chain, run by
+ * every lane over the 256 products in list order (as each lane of the batched
+ * kernel runs it for its slot). In a segment, the 256 products go to LDS
+ * and lane v runs vertex v's own chain over its products among them, in list
+ * order; a window costs as many fold steps as its longest vertex run instead
+ * of 256. Each vertex's sum is the same chain over the same products in the
+ * same order, so the result is bit-identical to the batched kernel's lane. */
+struct PprQuad {
+    float p[4];
+};
+
+__device__ __forceinline__ PprQuad ppr_load4(const PprArgs& a, const float* yin, uint64_t e, uint64_t end, int lane,
+                                             bool fresh, uint32_t seed)
+{
+    uint32_t u[4];
+    float w[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const uint64_t x = e + (uint64_t)k * 64 + (uint64_t)lane;
+        u[k] = x < end ? a.iadj[x] : 0u;
+        w[k] = x < end ? a.inw[x] : 0.f;
+    }
+    PprQuad q;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const uint64_t x = e + (uint64_t)k * 64 + (uint64_t)lane;
+        q.p[k] = x < end ? (fresh ? (u[k] == seed ? 1.f : 0.f) : yin[u[k]]) * w[k] : 0.f;
+    }
+    return q;
+}
+
 __global__ __launch_bounds__(256) void kp_pull_s(PprArgs a, int cur)
 {
     const int lane = threadIdx.x & 63;
@@ -1356,6 +1369,8 @@ __global__ __launch_bounds__(256) void kp_pull_s(PprArgs a, int cur)
     const double invS = sl->invS[0], Dn = sl->Dn[0];
     const float* yin = cur ? a.y1 : a.y0;
     float* yout = cur ? a.y0 : a.y1;
+    __shared__ float wins[4][256];
+    float* win = wins[threadIdx.x >> 6];
     const uint64_t nitems = (uint64_t)a.nchunk + a.nseg;
     for (uint64_t it = wid; it < nitems; it += nw) {
         if (it < a.nchunk) {
@@ -1364,7 +1379,16 @@ __global__ __launch_bounds__(256) void kp_pull_s(PprArgs a, int cur)
             const uint64_t e0 = a.iidx[v] + (uint64_t)(it - a.hub_c0[h]) * HUB;
             const uint64_t e1 = min<uint64_t>(e0 + HUB, a.iidx[v + 1]);
             float s = 0.f, c = 0.f;
-            PPR_STREAM1(e0, e1, ~0ull, (void)0)
+            for (uint64_t e = e0; e < e1; e += 256) {
+                const PprQuad q = ppr_load4(a, yin, e, e1, lane, fresh, seed);
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const uint64_t ek = e + (uint64_t)k * 64;
+                    if (ek >= e1) break;
+                    const uint32_t cnt = (uint32_t)min<uint64_t>(64, e1 - ek);
+                    for (uint32_t i = 0; i < cnt; ++i) kahan(s, c, rlf(q.p[k], (int)i));
+                }
+            }
             if (lane == 0) a.hubpart[it] = s - c;
             continue;
         }
@@ -1376,8 +1400,9 @@ __global__ __launch_bounds__(256) void kp_pull_s(PprArgs a, int cur)
         const float dg = in ? a.invdeg[base + lane] : 1.f;
         const bool hub = in && E1 - E0 > HUB;
         const unsigned long long hubm = __ballot(hub), dangm = __ballot(in && dg == 0.f);
-        double sS = 0.0, sD = 0.0;
-        float mine = 0.f;
+        /* the non-hub vertices' lists lie end to end between hubs: stream
+         * each run of them once */
+        float s = 0.f, c = 0.f;
         uint32_t j = 0;
         while (j < nv) {
             if ((hubm >> j) & 1ull) {
@@ -1387,24 +1412,36 @@ __global__ __launch_bounds__(256) void kp_pull_s(PprArgs a, int cur)
             const unsigned long long rest = hubm >> j;
             const uint32_t h = rest ? j + (uint32_t)__builtin_ctzll(rest) : nv;
             const uint64_t ra = rl64(E0, j), rb = rl64(E1, h - 1);
-            float s = 0.f, c = 0.f;
-            uint64_t ej = rl64(E1, j);
-#define CLOSE_S                                                    \
-    {                                                              \
-        const float y_ = ppr_y(s - c, invS, Dn, base + j == seed); \
-        if ((uint32_t)lane == j) mine = y_;                        \
-        sS += (double)y_;                                          \
-        if ((dangm >> j) & 1ull) sD += (double)y_;                 \
-        s = 0.f;                                                   \
-        c = 0.f;                                                   \
-        j++;                                                       \
-        ej = j < h ? rl64(E1, j) : ~0ull;                          \
-    }
-            PPR_STREAM1(ra, rb, ej, CLOSE_S)
-            while (j < h) CLOSE_S
-#undef CLOSE_S
+            const bool mine = (uint32_t)lane >= j && (uint32_t)lane < h;
+            for (uint64_t e = ra; e < rb; e += 256) {
+                const PprQuad q = ppr_load4(a, yin, e, rb, lane, fresh, seed);
+                /* the window's 256 products in LDS, in arc order */
+#pragma unroll
+                for (int k = 0; k < 4; k++) win[k * 64 + lane] = q.p[k];
+                __builtin_amdgcn_wave_barrier();
+                const uint64_t en = min<uint64_t>(e + 256, rb);
+                /* this lane's vertex's products in the window: [lo, hi) */
+                uint32_t lo = 0, hi = 0;
+                if (mine) {
+                    const uint64_t b0 = E0 > e ? E0 : e, b1 = E1 < en ? E1 : en;
+                    if (b1 > b0) { lo = (uint32_t)(b0 - e); hi = (uint32_t)(b1 - e); }
+                }
+                for (uint32_t t = lo; t < hi; t++) kahan(s, c, win[t]);
+                __builtin_amdgcn_wave_barrier();
+            }
+            j = h;
         }
-        if (in && !hub) yout[base + lane] = mine;
+        /* y of the lane's vertex, then the segment's sums in vertex order */
+        float y = 0.f;
+        if (in && !hub) y = ppr_y(s - c, invS, Dn, base + (uint32_t)lane == seed);
+        double sS = 0.0, sD = 0.0;
+        for (uint32_t k = 0; k < nv; k++) {
+            if ((hubm >> k) & 1ull) continue;
+            const float yk = rlf(y, (int)k);
+            sS += (double)yk;
+            if ((dangm >> k) & 1ull) sD += (double)yk;
+        }
+        if (in && !hub) yout[base + lane] = y;
         if (lane == 0) {
             a.partS[seg] = sS;
             a.partD[seg] = sD;
