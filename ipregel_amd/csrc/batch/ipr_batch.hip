@@ -983,6 +983,8 @@ __global__ void k_push_out(PushArgs a, int nxt, int batched)
         a.ctl->relax_total += r;
         /* running queries: in the next frontier, or with parked entries */
         a.ctl->out.orm = (batched ? o : (ns + nl ? 1ull : 0ull)) | pleft;
+        if (a.ctl->out.orm) a.ctl->busy_total += 1;
+        a.ctl->out.busy = a.ctl->busy_total;
         a.ctl->out.relax = a.ctl->relax_total;
         a.ctl->out.nsmall = ns;
         a.ctl->out.nlarge = nl;
@@ -1040,6 +1042,7 @@ __global__ __launch_bounds__(256) void k_start_cc_s(PushArgs a, int cur)
 __global__ void k_ctl_zero(PushArgs a, int cur)
 {
     a.ctl->relax_total = 0;
+    a.ctl->busy_total = 0;
     a.ctl->thr[0] = INF;
     a.ctl->pleft = 0;
     a.ctl->react = 0;
@@ -2005,6 +2008,10 @@ PushState::~PushState()
     for (int w = 0; w < 2; w++)
         for (int p = 0; p < 2; p++)
             if (gx[w][p]) (void)hipGraphExecDestroy(gx[w][p]);
+    for (int w = 0; w < 2; w++)
+        for (int p = 0; p < 2; p++)
+            for (int i = 0; i < 7; i++)
+                if (gxn[w][p][i]) (void)hipGraphExecDestroy(gxn[w][p][i]);
     void* ps[] = {val, mbox, act, vmask, touched, tgran, pend, fst, small[0], small[1], large[0], large[1], lpre, ctl, dsum};
     for (void* p : ps)
         if (p) (void)hipFree(p);
@@ -2131,34 +2138,62 @@ int PushState::release(unsigned long long mask)
     return 0;
 }
 
-int PushState::step(int* parity)
+int PushState::steps(uint64_t k, int* parity)
 {
-    const int p = cur;
-    if (use_graph == 1) {
-        hipGraphExec_t& x = gx[wzero][p];
-        if (!x) {
-            hipGraph_t graph = nullptr;
-            HCK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "capture superstep");
-            const int rc = enqueue_superstep(this, p, wzero);
-            const hipError_t e = hipStreamEndCapture(st, &graph);
-            if (rc) {
-                if (graph) (void)hipGraphDestroy(graph);
-                return rc;
+    if (k == 0) return 0;
+    while (k) {
+        uint32_t i = 0;
+        while (i < 6 && (2ull << i) <= k) i++;
+        const uint32_t n = 1u << i;
+        const int p0 = cur;
+        if (use_graph == 1 && n > 1) {
+            hipGraphExec_t& x = gxn[wzero][p0][i];
+            if (!x) {
+                hipGraph_t graph = nullptr;
+                HCK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "capture supersteps");
+                int rc = 0;
+                for (uint32_t j = 0; j < n && !rc; j++) rc = enqueue_superstep(this, p0 ^ (int)(j & 1), wzero);
+                const hipError_t e = hipStreamEndCapture(st, &graph);
+                if (rc) {
+                    if (graph) (void)hipGraphDestroy(graph);
+                    return rc;
+                }
+                HCK(e, "end capture");
+                const hipError_t ei = hipGraphInstantiate(&x, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                HCK(ei, "instantiate supersteps");
             }
-            HCK(e, "end capture");
-            const hipError_t ei = hipGraphInstantiate(&x, graph, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(graph);
-            HCK(ei, "instantiate superstep");
+            HCK(hipGraphLaunch(x, st), "superstep graph");
+        } else if (use_graph == 1) {
+            hipGraphExec_t& x = gx[wzero][p0];
+            if (!x) {
+                hipGraph_t graph = nullptr;
+                HCK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "capture superstep");
+                const int rc = enqueue_superstep(this, p0, wzero);
+                const hipError_t e = hipStreamEndCapture(st, &graph);
+                if (rc) {
+                    if (graph) (void)hipGraphDestroy(graph);
+                    return rc;
+                }
+                HCK(e, "end capture");
+                const hipError_t ei = hipGraphInstantiate(&x, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                HCK(ei, "instantiate superstep");
+            }
+            HCK(hipGraphLaunch(x, st), "superstep graph");
+        } else {
+            for (uint32_t j = 0; j < n; j++) {
+                const int rc = enqueue_superstep(this, p0 ^ (int)(j & 1), wzero);
+                if (rc) return rc;
+            }
         }
-        HCK(hipGraphLaunch(x, st), "superstep graph");
-    } else {
-        const int rc = enqueue_superstep(this, p, wzero);
-        if (rc) return rc;
+        cur = p0 ^ (int)(n & 1);
+        supersteps += n;
+        k -= n;
     }
+    const int p = cur ^ 1;   /* parity of the last superstep */
     HCK(hipMemcpyAsync(&hout[p], &ctl->out, sizeof(TickOut), hipMemcpyDeviceToHost, st), "tick readback");
     HCK(hipEventRecord(ev[p], st), "tick event");
-    cur ^= 1;
-    supersteps++;
     if (parity) *parity = p;
     return 0;
 }
@@ -2174,10 +2209,8 @@ int PushState::run_to_end(uint64_t max_supersteps)
     uint64_t k = 1, done = 0;
     for (;;) {
         int p = 0;
-        for (uint64_t i = 0; i < k; i++) {
-            const int rc = step(&p);
-            if (rc) return rc;
-        }
+        const int rc0 = steps(k, &p);
+        if (rc0) return rc0;
         done += k;
         const int rc = wait(p);
         if (rc) return rc;
@@ -2523,11 +2556,15 @@ extern "C" int ipr_batch_sssp(ipr_bgraph* g, const uint32_t* sources, int nq, ui
     const unsigned long long live = nq == 64 ? ~0ull : ((1ull << nq) - 1);
     Timer tm(s->st);
     uint64_t relax = 0;
+    uint64_t ran = 0;
     if ((rc = s->admit(nq, slots, sources)) == 0) {
-        for (;;) {
+        /* doubling runs of supersteps between host checks; the supersteps
+         * past the end find nothing to do and change nothing */
+        for (uint64_t k = 1;; k = std::min<uint64_t>(k * 2, 64)) {
             int p = 0;
-            if ((rc = s->step(&p)) != 0 || (rc = s->wait(p)) != 0) break;
+            if ((rc = s->steps(k, &p)) != 0 || (rc = s->wait(p)) != 0) break;
             relax = s->hout[p].relax;
+            ran = s->hout[p].busy + 1;
             if (!(s->hout[p].orm & live)) break;
             if (s->supersteps > 64 * g->V + 64) {
                 set_error("ipr_batch_sssp: no convergence");
@@ -2539,7 +2576,7 @@ extern "C" int ipr_batch_sssp(ipr_bgraph* g, const uint32_t* sources, int nq, ui
     const double ms = tm.stop();
     if (!rc) rc = s->columns(nq, out);
     if (!rc && st) {
-        st->supersteps = s->supersteps;
+        st->supersteps = ran;
         st->relaxations = relax;
         st->ms = ms;
     }
@@ -2562,7 +2599,7 @@ extern "C" int ipr_single_sssp(ipr_bgraph* g, uint32_t source, uint32_t* out, ip
     const double ms = tm.stop();
     if (!rc) rc = s->values(out);
     if (!rc && st) {
-        st->supersteps = s->supersteps;
+        st->supersteps = s->hout[(s->cur ^ 1)].busy + 1;
         st->relaxations = s->hout[(s->cur ^ 1)].relax;
         st->ms = ms;
     }
@@ -2585,7 +2622,7 @@ extern "C" int ipr_batch_cc(ipr_bgraph* g, uint32_t* out, ipr_batch_stats* st)
     const double ms = tm.stop();
     if (!rc) rc = s->values(out);
     if (!rc && st) {
-        st->supersteps = s->supersteps;
+        st->supersteps = g->tune.cc_mode == 1 ? s->supersteps : s->hout[(s->cur ^ 1)].busy + 1;
         st->relaxations = g->tune.cc_mode == 1 ? g->E : s->hout[(s->cur ^ 1)].relax + g->E;
         st->ms = ms;
     }

@@ -97,6 +97,7 @@ struct TickOut {
     unsigned long long orm;       /* batched: OR of next-frontier masks (slots still running) */
     unsigned long long relax;     /* (query, arc) relaxations the next superstep issues */
     uint32_t nsmall, nlarge;      /* next frontier size */
+    unsigned long long busy;      /* supersteps so far that left work for another */
     unsigned long long dig[64];   /* batched: per-slot digest of dist (telescoped) */
 };
 
@@ -116,6 +117,7 @@ struct PushCtl {
     uint32_t pscan[SPREAD][64];       /* smallest distance left parked after a wake-up scan */
     uint32_t step, pad;               /* superstep counter (frontier stamps) */
     unsigned long long relax_total;   /* cumulative over the run */
+    unsigned long long busy_total;    /* supersteps of the run that left work for another */
     TickOut out;                      /* out.relax: relax_total after the superstep */
 };
 
@@ -139,6 +141,8 @@ struct PushState {
     hipStream_t st = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
     hipGraphExec_t gx[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  /* [weight-0][parity] */
+    /* runs of 2^i supersteps in one graph (i = 1..6): [weight-0][first parity][i] */
+    hipGraphExec_t gxn[2][2][7] = {};
     unsigned long long* dsum = nullptr;
     int cur = 0;                       /* frontier parity of the next superstep */
     uint32_t wzero = 0;                /* 1 while running CC (every arc weighs 0) */
@@ -157,8 +161,9 @@ struct PushState {
     /* batched: inject distance 0 at sources for slots; clear released slots */
     int admit(int n, const int* slots, const uint32_t* sources);
     int release(unsigned long long mask);
-    /* one superstep; its TickOut lands in hout[parity] when ev[parity] completes */
-    int step(int* parity);
+    /* k supersteps, replayed as graphs of 64, 32, ... 1 supersteps; the last
+     * one's TickOut lands in hout[*parity] when ev[*parity] completes */
+    int steps(uint64_t k, int* parity);
     int wait(int parity);
     /* copy results to the host: scalar values (V) or one batched column */
     int values(uint32_t* host) ;

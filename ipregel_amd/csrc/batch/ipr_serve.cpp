@@ -396,8 +396,7 @@ struct Server {
             /* one superstep per tick while queries wait for a slot, else a
              * doubling run of supersteps between host checks */
             b.run = (n || !b.waiting.empty()) ? 1 : std::min<uint64_t>(b.run * 2, 32);
-            for (uint64_t i = 0; i < b.run; i++)
-                if ((rc = b.ps->step(&b.parity)) != 0) return rc;
+            if ((rc = b.ps->steps(b.run, &b.parity)) != 0) return rc;
         } else if (b.kind == IPR_Q_PPR) {
             int slots[64];
             uint32_t seeds[64];
@@ -466,8 +465,7 @@ struct Server {
             }
             /* label propagation: doubling runs of supersteps per tick */
             if (b.g->tune.cc_mode == 0)
-                for (uint64_t i = 0; i < b.cc_chunk; i++)
-                    if ((rc = b.ps->step(&b.parity)) != 0) return rc;
+                if ((rc = b.ps->steps(b.cc_chunk, &b.parity)) != 0) return rc;
         }
         b.inflight = true;
         b.ticks++;
