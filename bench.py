@@ -287,7 +287,8 @@ def run_workload(key: str, steps: int, warmup: int, with_cpu: bool) -> dict:
         k_name, k_launch, k_ms = "ipr_k_pull_step", phase_launch, phase_ms
         k_bytes = n_bcast_steps * (12 * g.E + 40 * g.V)
     elif gathered and int(st.gathered) == len(gathered):
-        k_name, k_launch = "ipr_k_gather", len(gathered)
+        k_name = "ipr_k_gather_tiled" if int(st.gather_tiles) else "ipr_k_gather"
+        k_launch = len(gathered)
         k_ms = float(sum(t_first[i] for i in gathered))
         k_bytes = per_arc * int(sum(int(sedges[i]) for i in gathered))
     elif fused:

@@ -87,6 +87,7 @@ typedef struct ipr_run_stats {
     uint64_t kernel_launches;   /* dominant kernel: live launches timed */
     char device_name[128];
     uint64_t fused_steps;       /* supersteps of the last run run by the fused sparse kernel */
+    uint64_t gather_tiles;      /* tiles of the receiver-blocked gather (0: the arc-balanced gather) */
 } ipr_run_stats;
 
 void ipr_harness_configure(const ipr_harness_cfg* cfg);

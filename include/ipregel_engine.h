@@ -122,6 +122,18 @@ typedef struct ipr_ctl {
 #define IPR_REC4_LINE 32
 #define IPR_LPACK_MASK ((1ull << IPR_LPACK_SHIFT) - 1)
 
+/* receiver-blocked gather: one block of consecutive receivers and its
+ * in-arcs, sorted by the source's record position; a block whose arcs are
+ * split between several tiles (a receiver with more in-arcs than a tile
+ * holds) has `shared` set, and its tiles deliver with the combiner's atomic */
+typedef struct ipr_tblock {
+    uint64_t a0;                 /* first arc in g_tpos / g_trcv / g_twgt */
+    uint32_t na;                 /* arcs */
+    uint32_t v0;                 /* first receiver */
+    uint32_t nr;                 /* receivers (at most 65536: 16-bit offsets) */
+    uint32_t shared;
+} ipr_tblock;
+
 typedef struct ipr_dev {
     ipr_ctl* ctl;
     uint64_t* cnt;               /* IPR_NCNT x IPR_CSLOTS counters */
@@ -178,6 +190,16 @@ typedef struct ipr_dev {
     uint64_t g_cend[IPR_GATHER_CLASSES];
     uint64_t g_cdst[IPR_GATHER_CLASSES];
     uint32_t gather_blocks;      /* resident gather workgroups; their wavefronts take ranges in turn */
+    /* receiver-blocked gather (g_tn > 0): the tiles, largest first, taken in
+     * turn by the resident workgroups, each folding its block's messages in
+     * LDS; g_tpos the arcs' source record positions, g_trcv their receivers
+     * as offsets in the block, g_twgt their weights (weighted apps) */
+    const ipr_tblock* g_tblk;
+    uint32_t g_tn;
+    uint32_t tile_blocks;        /* resident workgroups of the receiver-blocked gather */
+    const uint32_t* g_tpos;
+    const uint16_t* g_trcv;
+    const uint32_t* g_twgt;
     uint32_t compact_rounds;     /* frontier compaction rounds per workgroup: 1, 2, 4 or 8 (0: by V) */
     void* val;
     void* msg;
@@ -277,6 +299,11 @@ typedef struct ipr_app_desc {
     int (*check_combiner)(void);
     /* built with IP_ENABLE_THREAD_PROFILING: a per-superstep phase table */
     int thread_profiling;
+    /* the receiver-blocked gather applies: an integral MIN/MAX/SUM combiner
+     * on 4- or 8-byte slots (folded by LDS atomics); its LDS holds
+     * tile_receivers receivers per tile */
+    int can_tile;
+    uint32_t tile_receivers;
 } ipr_app_desc;
 
 #ifdef __cplusplus

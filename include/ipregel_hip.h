@@ -19,6 +19,8 @@
  *   ipr_hip_build_pull_bins        (new) in-degree bins of the pull gather
  *   ipr_hip_build_gather           (new) arc-balanced index of the push
  *                                  applications' dense-superstep gather
+ *   ipr_hip_build_gather_tiles     (new) receiver-blocked, source-sorted index
+ *                                  of the same gather
  *   ipr_hip_graph_*                (new) replay of a chunk of supersteps
  */
 #ifndef IPREGEL_HIP_H
@@ -105,6 +107,20 @@ int ipr_hip_build_gather_classes(uint64_t V, uint64_t E, const uint64_t* t_idx, 
                                  const uint32_t* t_wgt, const uint32_t* bpos, uint32_t K, uint32_t* g_arc,
                                  uint32_t* g_wgt, uint32_t* g_own, uint32_t* g_dst, uint64_t* cbeg, uint64_t* cend,
                                  uint64_t* cdst, void* stream);
+
+/* receiver-blocked gather index over the transpose (t_idx/t_adj/t_wgt).
+ * Receivers split into blocks of consecutive ids with at most max_recv
+ * (<= 65536) receivers and at most max_arcs in-arcs; a receiver with more
+ * in-arcs forms a block alone, split into tiles of max_arcs arcs whose
+ * deliveries are atomic (shared). Within a block the arcs are sorted by the
+ * source's record position (bpos, or the id when bpos is NULL): t_pos[E] the
+ * positions, t_rcv[E] the receivers as offsets in the block, t_w[E] (when
+ * t_wgt and t_w are given) the weights. *tiles receives a device array of
+ * *n_tiles ipr_tblock entries, largest first (free with ipr_hip_free). */
+int ipr_hip_build_gather_tiles(uint64_t V, uint64_t E, const uint64_t* t_idx, const uint32_t* t_adj,
+                               const uint32_t* t_wgt, const uint32_t* bpos, uint32_t max_recv, uint64_t max_arcs,
+                               uint32_t* t_pos, uint16_t* t_rcv, uint32_t* t_w, void** tiles, uint32_t* n_tiles,
+                               void* stream);
 
 /* byte-wise equality of two device buffers (multiple of 4 bytes) */
 int ipr_hip_equal(const void* a, const void* b, uint64_t bytes, int* equal, void* stream);

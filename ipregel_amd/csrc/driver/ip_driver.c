@@ -175,6 +175,10 @@ static void dev_free_all(void)
     ipr_hip_free((void*)d->g_own);
     ipr_hip_free((void*)d->g_dst);
     ipr_hip_free((void*)d->g_wgt);
+    ipr_hip_free((void*)d->g_tpos);
+    ipr_hip_free((void*)d->g_trcv);
+    ipr_hip_free((void*)d->g_twgt);
+    ipr_hip_free((void*)d->g_tblk);
     ipr_hip_free((void*)d->bpos);
     ipr_hip_free(d->g_live);
     ipr_hip_free(d->brec4);
@@ -384,12 +388,15 @@ static void dev_setup(void)
          * class's arcs start on a whole chunk, and its receivers get their own
          * g_dst entries */
         d->g_xcd = env_u32("IPR_GATHER_XCD", 0) == 1;
+        /* receiver-blocked tiles (IPR_GATHER_TILED=1, the default where the
+         * combiner folds by LDS atomics) replace the arc-balanced index */
+        const int tiled = !d->g_xcd && a->can_tile && env_u32("IPR_GATHER_TILED", 1) == 1;
         const uint64_t ea = d->g_xcd ? E + IPR_GATHER_CLASSES * K : E;
         const uint64_t nd = d->g_xcd ? (E < IPR_GATHER_CLASSES * vv ? E : IPR_GATHER_CLASSES * vv) : vv;
         uint64_t nch = (ea + K - 1) / K, ndst = 0;
-        uint32_t* g_arc = dalloc(ea * 4, "gather sources");
-        uint32_t* g_own = dalloc(nch * 4, "gather chunk receivers");
-        uint32_t* g_dst = dalloc((nd ? nd : 1) * 4, "gather receivers");
+        uint32_t* g_arc = tiled ? NULL : dalloc(ea * 4, "gather sources");
+        uint32_t* g_own = tiled ? NULL : dalloc(nch * 4, "gather chunk receivers");
+        uint32_t* g_dst = tiled ? NULL : dalloc((nd ? nd : 1) * 4, "gather receivers");
         /* SPREAD reads records only for its frontier outside the gather, so
          * reordering them by out-degree costs little there (PUSH scans all V) */
         if (var == IPR_VARIANT_SPREAD && env_u32("IPR_RECORD_ORDER", 1) == 1) {
@@ -397,7 +404,39 @@ static void dev_setup(void)
             hip_check(ipr_hip_build_record_order(V, d->out_idx, bpos, st), "record order");
             d->bpos = bpos;
         }
-        if (d->g_xcd) {
+        if (tiled) {
+            /* two resident workgroups per CU (64 KB of receiver slots each);
+             * tiles of at most tile_receivers receivers and about one
+             * workgroup's share of the arcs: larger tiles share more record
+             * lines between their arcs (measured: CC R-MAT 22 gather 0.36 ms
+             * with a quarter share, 0.28 ms with a whole one; CC R-MAT 24
+             * 1.52 -> 1.29 ms), and the blocks that the receiver limit keeps
+             * smaller fill in behind them, largest first */
+            uint32_t cus = 1;
+            hip_check(ipr_hip_cu_count(&cus), "compute units");
+            d->tile_blocks = env_u32("IPR_TILE_BLOCKS", cus * 2);
+            if (d->tile_blocks < 1) d->tile_blocks = 1;
+            uint32_t max_recv = env_u32("IPR_TILE_RECV", a->tile_receivers);
+            if (max_recv < 32 || max_recv > a->tile_receivers) max_recv = a->tile_receivers;
+            uint64_t max_arcs = env_u32("IPR_TILE_ARCS", 0);
+            if (max_arcs == 0) {
+                max_arcs = E / (uint64_t)d->tile_blocks;
+                if (max_arcs < 65536) max_arcs = 65536;
+            }
+            uint32_t* tpos = dalloc(E * 4, "gather tile sources");
+            uint16_t* trcv = (uint16_t*)dalloc(E * 2, "gather tile receivers");
+            uint32_t* twgt = d->t_wgt ? dalloc(E * 4, "gather tile weights") : NULL;
+            void* tiles = NULL;
+            uint32_t ntiles = 0;
+            hip_check(ipr_hip_build_gather_tiles(V, E, d->t_idx, d->t_adj, d->t_wgt, d->bpos, max_recv, max_arcs,
+                                                 tpos, trcv, twgt, &tiles, &ntiles, st),
+                      "gather tiles");
+            d->g_tpos = tpos;
+            d->g_trcv = trcv;
+            d->g_twgt = twgt;
+            d->g_tblk = (const ipr_tblock*)tiles;
+            d->g_tn = ntiles;
+        } else if (d->g_xcd) {
             uint32_t* g_wgt = d->t_wgt ? dalloc(ea * 4, "gather weights") : NULL;
             hip_check(ipr_hip_build_gather_classes(V, E, d->t_idx, d->t_adj, d->t_wgt, d->bpos, IPR_GATHER_K, g_arc,
                                                    g_wgt, g_own, g_dst, d->g_cbeg, d->g_cend, d->g_cdst, st),
@@ -963,6 +1002,7 @@ static void run_once(void)
     D.stats.work_vertices = D.h_ctl->work_vertices;
     D.stats.gathered = D.h_ctl->gathered;
     D.stats.fused_steps = D.h_ctl->fused_steps;
+    D.stats.gather_tiles = D.dev.g_tn;
     /* pull: launch s finalizes superstep s-1, so one launch more than supersteps */
     D.prev_launches = (uint32_t)(D.h_ctl->executed + (D.app.variant == IPR_VARIANT_PULL ? 1 : 0));
     D.prev_fused = D.h_ctl->fused_steps > 0;

@@ -4,6 +4,8 @@
  * do not depend on the application's types: superstep control, CSC build,
  * in-degree binning.
  */
+#include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <cstdio>
 #include <mutex>
@@ -634,6 +636,160 @@ out:
     (void)hipStreamSynchronize(st);
     (void)hipFree(ccnt); (void)hipFree(cidx); (void)hipFree(abase_d); (void)hipFree(cadj); (void)hipFree(cwgt);
     (void)hipFree(tmp);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* receiver-blocked gather index                                             */
+/* ------------------------------------------------------------------------ */
+
+/* sort keys of block b's arcs, (b << kshift) | (source record position << 16)
+ * | receiver offset in the block; workgroups take blocks in turn, a wavefront
+ * per receiver */
+__global__ void k_tile_keys(const uint64_t* t_idx, const uint32_t* t_adj, const uint32_t* bpos,
+                            const uint32_t* bv0, const uint32_t* bnr, uint64_t nblk, unsigned kshift,
+                            uint64_t* keys)
+{
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (uint64_t b = blockIdx.x; b < nblk; b += gridDim.x) {
+        const uint32_t v0 = bv0[b], nr = bnr[b];
+        for (uint32_t r = wv; r < nr; r += nw) {
+            const uint64_t e0 = t_idx[v0 + r], e1 = t_idx[(uint64_t)v0 + r + 1];
+            for (uint64_t k = e0 + lane; k < e1; k += 64) {
+                const uint32_t u = t_adj[k];
+                const uint64_t p = bpos ? bpos[u] : u;
+                keys[k] = (b << kshift) | (p << 16) | r;
+            }
+        }
+    }
+}
+
+__global__ void k_tile_finish(uint64_t E, const uint64_t* keys, const uint32_t* perm, const uint32_t* t_wgt,
+                              uint64_t pmask, uint32_t* tpos, uint16_t* trcv, uint32_t* twgt)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < E; i += stride) {
+        const uint64_t k = keys[i];
+        tpos[i] = (uint32_t)((k >> 16) & pmask);
+        trcv[i] = (uint16_t)(k & 0xFFFFu);
+        if (twgt) twgt[i] = t_wgt[perm[i]];
+    }
+}
+
+extern "C" int ipr_hip_build_gather_tiles(uint64_t V, uint64_t E, const uint64_t* t_idx, const uint32_t* t_adj,
+                                          const uint32_t* t_wgt, const uint32_t* bpos, uint32_t max_recv,
+                                          uint64_t max_arcs, uint32_t* t_pos, uint16_t* t_rcv, uint32_t* t_w,
+                                          void** tiles, uint32_t* n_tiles, void* stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    *tiles = nullptr;
+    *n_tiles = 0;
+    if (V >= (1ull << 32) || max_recv == 0 || max_recv > 65536 || max_arcs == 0) {
+        snprintf(g_err, sizeof(g_err), "gather tiles: needs V < 2^32, 1 <= receivers per block <= 65536 and "
+                 "arcs per tile >= 1 (V=%llu receivers=%u arcs=%llu)", (unsigned long long)V, max_recv,
+                 (unsigned long long)max_arcs);
+        return -1;
+    }
+    if (E == 0 || V == 0) return 0;
+    int rc = 0;
+    hipError_t e;
+    uint64_t* hidx = (uint64_t*)malloc((V + 1) * 8);
+    uint32_t *hv0 = nullptr, *hnr = nullptr, *dv0 = nullptr, *dnr = nullptr, *pin = nullptr, *pout = nullptr;
+    uint64_t *ha0 = nullptr, *kin = nullptr, *kout = nullptr;
+    ipr_tblock* ht = nullptr;
+    void* dt = nullptr;
+    void* tmp = nullptr;
+    size_t tmp_bytes = 0;
+    uint64_t nb = 0, nt = 0;
+    unsigned kshift = 0, end_bit = 0;
+    if (!hidx) { snprintf(g_err, sizeof(g_err), "gather tiles: host memory"); return -1; }
+    if ((e = hipMemcpyAsync(hidx, t_idx, (V + 1) * 8, hipMemcpyDeviceToHost, st)) != hipSuccess ||
+        (e = hipStreamSynchronize(st)) != hipSuccess) { rc = fail(e, "gather tiles offsets"); goto out; }
+    /* blocks of consecutive receivers: at most max_recv of them and at most
+     * max_arcs in-arcs, except a receiver with more in-arcs, which forms a
+     * block alone; blocks without arcs are left out */
+    hv0 = (uint32_t*)malloc(V * 4);
+    hnr = (uint32_t*)malloc(V * 4);
+    ha0 = (uint64_t*)malloc(V * 8);
+    if (!hv0 || !hnr || !ha0) { snprintf(g_err, sizeof(g_err), "gather tiles: host memory"); rc = -1; goto out; }
+    for (uint64_t v = 0; v < V;) {
+        const uint64_t v0 = v, a0 = hidx[v];
+        while (v < V && v - v0 < max_recv && (v == v0 || hidx[v + 1] - a0 <= max_arcs)) v++;
+        if (hidx[v] > a0) {
+            hv0[nb] = (uint32_t)v0;
+            hnr[nb] = (uint32_t)(v - v0);
+            ha0[nb] = a0;
+            nt += (hidx[v] - a0 + max_arcs - 1) / max_arcs;
+            nb++;
+        }
+    }
+    kshift = 16 + (unsigned)nbits(V - 1);
+    end_bit = kshift + (unsigned)nbits(nb - 1);
+    if (end_bit > 64 || nt >= (1ull << 31)) {
+        snprintf(g_err, sizeof(g_err), "gather tiles: %llu blocks do not fit the sort key",
+                 (unsigned long long)nb);
+        rc = -1;
+        goto out;
+    }
+    /* tiles, largest first: a block above max_arcs (one receiver) is split
+     * into tiles of max_arcs that deliver by atomics */
+    ht = (ipr_tblock*)malloc(nt * sizeof(ipr_tblock));
+    if (!ht) { snprintf(g_err, sizeof(g_err), "gather tiles: host memory"); rc = -1; goto out; }
+    {
+        uint64_t k = 0;
+        for (uint64_t b = 0; b < nb; b++) {
+            const uint64_t a0 = ha0[b], na = hidx[(uint64_t)hv0[b] + hnr[b]] - a0;
+            const uint32_t shared = na > max_arcs ? 1u : 0u;
+            for (uint64_t o = 0; o < na; o += max_arcs) {
+                const uint64_t n = na - o < max_arcs ? na - o : max_arcs;
+                ht[k++] = ipr_tblock{a0 + o, (uint32_t)n, hv0[b], hnr[b], shared};
+            }
+        }
+        std::stable_sort(ht, ht + nt, [](const ipr_tblock& x, const ipr_tblock& y) { return x.na > y.na; });
+    }
+    if ((e = hipMalloc(&dv0, nb * 4)) != hipSuccess) { rc = fail(e, "gather tile blocks"); goto out; }
+    if ((e = hipMalloc(&dnr, nb * 4)) != hipSuccess) { rc = fail(e, "gather tile blocks"); goto out; }
+    if ((e = hipMalloc(&dt, nt * sizeof(ipr_tblock))) != hipSuccess) { rc = fail(e, "gather tiles"); goto out; }
+    if ((e = hipMalloc(&kin, E * 8)) != hipSuccess) { rc = fail(e, "gather tile keys"); goto out; }
+    if ((e = hipMalloc(&kout, E * 8)) != hipSuccess) { rc = fail(e, "gather tile keys"); goto out; }
+    if ((e = hipMemcpyAsync(dv0, hv0, nb * 4, hipMemcpyHostToDevice, st)) != hipSuccess ||
+        (e = hipMemcpyAsync(dnr, hnr, nb * 4, hipMemcpyHostToDevice, st)) != hipSuccess ||
+        (e = hipMemcpyAsync(dt, ht, nt * sizeof(ipr_tblock), hipMemcpyHostToDevice, st)) != hipSuccess) {
+        rc = fail(e, "gather tile upload");
+        goto out;
+    }
+    hipLaunchKernelGGL(k_tile_keys, dim3((unsigned)(nb < 8192 ? nb : 8192)), dim3(256), 0, st, t_idx, t_adj, bpos,
+                       dv0, dnr, nb, kshift, kin);
+    if (t_w) {
+        if ((e = hipMalloc(&pin, E * 4)) != hipSuccess) { rc = fail(e, "gather tile perm"); goto out; }
+        if ((e = hipMalloc(&pout, E * 4)) != hipSuccess) { rc = fail(e, "gather tile perm"); goto out; }
+        hipLaunchKernelGGL(k_iota_u32, dim3(4096), dim3(256), 0, st, pin, E);
+        e = rocprim::radix_sort_pairs(nullptr, tmp_bytes, kin, kout, pin, pout, (size_t)E, 0, end_bit, st);
+    } else {
+        e = rocprim::radix_sort_keys(nullptr, tmp_bytes, kin, kout, (size_t)E, 0, end_bit, st);
+    }
+    if (e != hipSuccess) { rc = fail(e, "gather tile sort size"); goto out; }
+    if ((e = hipMalloc(&tmp, tmp_bytes ? tmp_bytes : 16)) != hipSuccess) { rc = fail(e, "gather tile temp"); goto out; }
+    if (t_w)
+        e = rocprim::radix_sort_pairs(tmp, tmp_bytes, kin, kout, pin, pout, (size_t)E, 0, end_bit, st);
+    else
+        e = rocprim::radix_sort_keys(tmp, tmp_bytes, kin, kout, (size_t)E, 0, end_bit, st);
+    if (e != hipSuccess) { rc = fail(e, "gather tile sort"); goto out; }
+    {
+        uint64_t eb = (E + 255) / 256;
+        if (eb > 65536) eb = 65536;
+        hipLaunchKernelGGL(k_tile_finish, dim3((unsigned)eb), dim3(256), 0, st, E, kout, pout, t_wgt,
+                           (1ull << (kshift - 16)) - 1, t_pos, t_rcv, t_w);
+    }
+    if ((e = hipGetLastError()) != hipSuccess) { rc = fail(e, "gather tile index"); goto out; }
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) { rc = fail(e, "gather tile sync"); goto out; }
+    *tiles = dt;
+    *n_tiles = (uint32_t)nt;
+    dt = nullptr;
+out:
+    free(hidx); free(hv0); free(hnr); free(ha0); free(ht);
+    (void)hipFree(dv0); (void)hipFree(dnr); (void)hipFree(kin); (void)hipFree(kout);
+    (void)hipFree(pin); (void)hipFree(pout); (void)hipFree(tmp); (void)hipFree(dt);
     return rc;
 }
 

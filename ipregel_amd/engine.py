@@ -60,7 +60,7 @@ class RunStats(C.Structure):
         ("last_ms", C.c_double), ("total_ms", C.c_double), ("min_ms", C.c_double),
         ("device_ms", C.c_double), ("load_s", C.c_double),
         ("kernel_ms", C.c_double), ("kernel_launches", C.c_uint64),
-        ("device_name", C.c_char * 128), ("fused_steps", C.c_uint64),
+        ("device_name", C.c_char * 128), ("fused_steps", C.c_uint64), ("gather_tiles", C.c_uint64),
     ]
 
 

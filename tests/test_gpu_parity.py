@@ -139,7 +139,9 @@ def test_gather_every_superstep(blocks, order, live, ranges, record4, monkeypatc
     symmetric and directed transposes,
     with and without weights; broadcast records by vertex id or by degree;
     the broadcasters' bitmap off, below 60% live arcs, or always; 4-byte or
-    8-byte broadcast records."""
+    8-byte broadcast records. The receiver-blocked gather is off here
+    (IPR_GATHER_TILED=0); test_tiled_gather_every_superstep covers it."""
+    monkeypatch.setenv("IPR_GATHER_TILED", "0")
     monkeypatch.setenv("IPR_GATHER_RECORD4", record4)
     for k, v in (("IPR_GATHER_BLOCKS", blocks), ("IPR_GATHER_RANGE_ARCS", ranges)):
         if v:
@@ -163,6 +165,48 @@ def test_gather_every_superstep(blocks, order, live, ranges, record4, monkeypatc
                        np.r_[np.full(2999, 2999), np.arange(3000, 3005)], keep_self_loops=False)
     r = EN.run_app("cc_spread", star, dir_alpha=1 << 30)
     assert np.array_equal(r.values, O.min_ancestor(star))
+
+
+@pytest.mark.parametrize("blocks,recv,arcs,order,record4", [
+    ("", "", "", "1", "1"), ("1", "", "", "1", "1"), ("3", "32", "100", "0", "1"),
+    ("", "64", "1000", "1", "0"), ("7", "16384", "5000", "1", "1"), ("2", "33", "65536", "0", "0")])
+def test_tiled_gather_every_superstep(blocks, recv, arcs, order, record4, monkeypatch):
+    """The receiver-blocked gather (tiles of consecutive receivers, arcs sorted
+    by source record, folds in LDS) forced on every superstep, with 1, 2, 3,
+    7 or the default number of resident workgroups; blocks of 32, 33, 64,
+    16384 or the default number of receivers; tiles of 100 arcs up to the
+    default, so that hub in-lists split into many tiles that deliver by
+    atomics; records by id or by degree, 4- or 8-byte; SPREAD and PUSH, with
+    and without weights; a hub star."""
+    monkeypatch.setenv("IPR_GATHER_TILED", "1")
+    monkeypatch.setenv("IPR_GATHER_RECORD4", record4)
+    monkeypatch.setenv("IPR_RECORD_ORDER", order)
+    for k, v in (("IPR_TILE_BLOCKS", blocks), ("IPR_TILE_RECV", recv), ("IPR_TILE_ARCS", arcs)):
+        if v:
+            monkeypatch.setenv(k, v)
+        else:
+            monkeypatch.delenv(k, raising=False)
+    g = G.rmat(13, 16, seed=5, symmetrise=True)
+    for name in ("cc_spread", "cc"):
+        print(f"app={name} rmat13 tiles {blocks}/{recv}/{arcs}", flush=True)
+        r = EN.run_app(name, g, dir_alpha=1 << 30)
+        assert np.array_equal(r.values, O.min_ancestor(g))
+        assert r.gathered > 0
+    gd = G.rmat(12, 16, seed=6)
+    r = EN.run_app("sssp_spread", gd, source=0, dir_alpha=1 << 30)
+    assert np.array_equal(r.values, O.bfs(gd, 0))
+    gw = gd.with_weights(2)
+    for name in ("sssp_weighted_spread", "sssp_weighted"):
+        print(f"app={name} rmat12 tiles {blocks}/{recv}/{arcs}", flush=True)
+        r = EN.run_app(name, gw, source=0, dir_alpha=1 << 30)
+        assert np.array_equal(r.values, O.dijkstra(gw, 0))
+        assert r.gathered > 0
+    star = G.from_arcs(3100, np.r_[np.arange(1, 3000), np.zeros(5, int)],
+                       np.r_[np.full(2999, 2999), np.arange(3000, 3005)], keep_self_loops=False)
+    ref = O.run(star, O.CC, O.SPREAD)
+    r = EN.run_app("cc_spread", star, dir_alpha=1 << 30)
+    assert np.array_equal(r.values, ref.values)
+    assert list(r.active) == list(ref.active[: ref.supersteps])
 
 
 @pytest.mark.parametrize("blocks,ranges", [("", ""), ("3", "512"), ("1", "100000")])
