@@ -127,12 +127,33 @@ typedef struct ipr_ctl {
  * split between several tiles (a receiver with more in-arcs than a tile
  * holds) has `shared` set, and its tiles deliver with the combiner's atomic */
 typedef struct ipr_tblock {
-    uint64_t a0;                 /* first arc in g_tpos / g_trcv / g_twgt */
-    uint32_t na;                 /* arcs */
+    uint64_t a0;                 /* first arc slot in g_tq / g_twgt (a multiple of 64) */
+    uint32_t na;                 /* arc slots (a multiple of 64; unused slots hold IPR_TQ_NONE) */
     uint32_t v0;                 /* first receiver */
     uint32_t nr;                 /* receivers (at most 65536: 16-bit offsets) */
     uint32_t shared;
 } ipr_tblock;
+
+/* The receiver-blocked gather's index: each block's arcs, sorted by source
+ * record position, fill whole groups of 64 slots. Slot i holds
+ * (position - base) << 16 | receiver offset in g_tq[i]; g_thead[i / 64] is the
+ * group's base position, or IPR_TQ_ESC | e when its positions span more than
+ * 16 bits, and then the positions are g_tesc[64 e + i % 64] (the slot's high
+ * half is unused). A slot whose receiver offset is IPR_TQ_NONE is empty. */
+#define IPR_TQ_ESC 0x80000000u
+#define IPR_TQ_NONE 0xFFFFu
+
+typedef struct ipr_gather_tiles {
+    void* tiles;                 /* device ipr_tblock[n_tiles], largest first */
+    uint32_t n_tiles;
+    uint32_t wgt_bytes;          /* width of wgt: 1, 2 or 4 (0: no weights) */
+    uint64_t slots;              /* entries of q (a multiple of 64) */
+    uint64_t escaped;            /* groups whose positions are in esc */
+    uint32_t* q;                 /* device, slots */
+    uint32_t* head;              /* device, slots / 64 */
+    uint32_t* esc;               /* device, 64 per escaped group (NULL when none) */
+    void* wgt;                   /* device, slots x wgt_bytes (NULL without weights) */
+} ipr_gather_tiles;
 
 typedef struct ipr_dev {
     ipr_ctl* ctl;
@@ -192,14 +213,18 @@ typedef struct ipr_dev {
     uint32_t gather_blocks;      /* resident gather workgroups; their wavefronts take ranges in turn */
     /* receiver-blocked gather (g_tn > 0): the tiles, largest first, taken in
      * turn by the resident workgroups, each folding its block's messages in
-     * LDS; g_tpos the arcs' source record positions, g_trcv their receivers
-     * as offsets in the block, g_twgt their weights (weighted apps) */
+     * LDS; the index g_tq / g_thead / g_tesc (layout at IPR_TQ_ESC), the
+     * arcs' weights g_twgt in the same slots, g_twsz bytes each (weighted
+     * apps: 1 or 2 when every weight fits) */
     const ipr_tblock* g_tblk;
     uint32_t g_tn;
     uint32_t tile_blocks;        /* resident workgroups of the receiver-blocked gather */
-    const uint32_t* g_tpos;
-    const uint16_t* g_trcv;
-    const uint32_t* g_twgt;
+    const uint32_t* g_tq;
+    const uint32_t* g_thead;
+    const uint32_t* g_tesc;
+    const void* g_twgt;
+    uint32_t g_twsz;
+    uint32_t g_tpad;
     uint32_t compact_rounds;     /* frontier compaction rounds per workgroup: 1, 2, 4 or 8 (0: by V) */
     void* val;
     void* msg;

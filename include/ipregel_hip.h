@@ -110,17 +110,16 @@ int ipr_hip_build_gather_classes(uint64_t V, uint64_t E, const uint64_t* t_idx, 
 
 /* receiver-blocked gather index over the transpose (t_idx/t_adj/t_wgt).
  * Receivers split into blocks of consecutive ids with at most max_recv
- * (<= 65536) receivers and at most max_arcs in-arcs; a receiver with more
- * in-arcs forms a block alone, split into tiles of max_arcs arcs whose
+ * (< 65535) receivers and at most max_arcs in-arcs; a receiver with more
+ * in-arcs forms a block alone, split into tiles of whole 64-slot groups whose
  * deliveries are atomic (shared). Within a block the arcs are sorted by the
- * source's record position (bpos, or the id when bpos is NULL): t_pos[E] the
- * positions, t_rcv[E] the receivers as offsets in the block, t_w[E] (when
- * t_wgt and t_w are given) the weights. *tiles receives a device array of
- * *n_tiles ipr_tblock entries, largest first (free with ipr_hip_free). */
+ * source's record position (bpos, or the id when bpos is NULL) and laid out
+ * in groups of 64 slots as IPR_TQ_ESC describes; weights (when t_wgt is
+ * given) at 1, 2 or 4 bytes, the narrowest that holds the largest. All of
+ * *out is allocated here (device memory; free each array with ipr_hip_free). */
 int ipr_hip_build_gather_tiles(uint64_t V, uint64_t E, const uint64_t* t_idx, const uint32_t* t_adj,
                                const uint32_t* t_wgt, const uint32_t* bpos, uint32_t max_recv, uint64_t max_arcs,
-                               uint32_t* t_pos, uint16_t* t_rcv, uint32_t* t_w, void** tiles, uint32_t* n_tiles,
-                               void* stream);
+                               ipr_gather_tiles* out, void* stream);
 
 /* byte-wise equality of two device buffers (multiple of 4 bytes) */
 int ipr_hip_equal(const void* a, const void* b, uint64_t bytes, int* equal, void* stream);

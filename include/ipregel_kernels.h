@@ -1393,17 +1393,36 @@ __global__ void __launch_bounds__(IPR_TILE_THREADS) ipr_k_gather_tiled(ipr_dev d
 #ifdef IP_NEEDS_OUT_NEIGHBOUR_WEIGHTS
                 uint32_t wt[K];
 #endif
+                /* slot group j of this pass (whole groups: tiles start and
+                 * end on them): its base position (one scalar load), then
+                 * each lane's slot; the index is read once per launch, so it
+                 * is kept out of L2, which holds the records */
+                uint32_t h[K], qv[K];
 #pragma unroll
                 for (int j = 0; j < (int)K; j++) {
-                    const uint64_t i = base + (uint64_t)j * IPR_WAVE + lane;
-                    const bool in = i < a1;
-                    /* the index is read once per launch: keep it out of
-                     * L2, which holds the records */
-                    p[j] = in ? __builtin_nontemporal_load(d.g_tpos + i) : 0xFFFFFFFFu;
-                    r[j] = in ? (uint32_t)__builtin_nontemporal_load(d.g_trcv + i) : 0u;
+                    const uint64_t gs = base + (uint64_t)j * IPR_WAVE;
+                    h[j] = 0;
+                    qv[j] = 0xFFFFFFFFu;
+                    if (gs < a1) {
+                        h[j] = d.g_thead[gs >> 6];
+                        qv[j] = __builtin_nontemporal_load(d.g_tq + gs + lane);
+                    }
 #ifdef IP_NEEDS_OUT_NEIGHBOUR_WEIGHTS
-                    wt[j] = in ? __builtin_nontemporal_load(d.g_twgt + i) : 0u;
+                    wt[j] = 0;
+                    if (gs < a1) {
+                        if (d.g_twsz == 1) wt[j] = __builtin_nontemporal_load((const uint8_t*)d.g_twgt + gs + lane);
+                        else if (d.g_twsz == 2)
+                            wt[j] = __builtin_nontemporal_load((const uint16_t*)d.g_twgt + gs + lane);
+                        else wt[j] = __builtin_nontemporal_load((const uint32_t*)d.g_twgt + gs + lane);
+                    }
 #endif
+                }
+#pragma unroll
+                for (int j = 0; j < (int)K; j++) {
+                    r[j] = qv[j] & 0xFFFFu;
+                    if (h[j] & IPR_TQ_ESC) p[j] = d.g_tesc[(uint64_t)(h[j] & ~IPR_TQ_ESC) * IPR_WAVE + lane];
+                    else p[j] = h[j] + (qv[j] >> 16);
+                    if (r[j] == IPR_TQ_NONE) p[j] = 0xFFFFFFFFu;
                 }
                 if (live) {
                     uint64_t lw[K];

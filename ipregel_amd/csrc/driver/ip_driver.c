@@ -175,8 +175,9 @@ static void dev_free_all(void)
     ipr_hip_free((void*)d->g_own);
     ipr_hip_free((void*)d->g_dst);
     ipr_hip_free((void*)d->g_wgt);
-    ipr_hip_free((void*)d->g_tpos);
-    ipr_hip_free((void*)d->g_trcv);
+    ipr_hip_free((void*)d->g_tq);
+    ipr_hip_free((void*)d->g_thead);
+    ipr_hip_free((void*)d->g_tesc);
     ipr_hip_free((void*)d->g_twgt);
     ipr_hip_free((void*)d->g_tblk);
     ipr_hip_free((void*)d->bpos);
@@ -423,19 +424,17 @@ static void dev_setup(void)
                 max_arcs = E / (uint64_t)d->tile_blocks;
                 if (max_arcs < 65536) max_arcs = 65536;
             }
-            uint32_t* tpos = dalloc(E * 4, "gather tile sources");
-            uint16_t* trcv = (uint16_t*)dalloc(E * 2, "gather tile receivers");
-            uint32_t* twgt = d->t_wgt ? dalloc(E * 4, "gather tile weights") : NULL;
-            void* tiles = NULL;
-            uint32_t ntiles = 0;
+            ipr_gather_tiles gt;
             hip_check(ipr_hip_build_gather_tiles(V, E, d->t_idx, d->t_adj, d->t_wgt, d->bpos, max_recv, max_arcs,
-                                                 tpos, trcv, twgt, &tiles, &ntiles, st),
+                                                 &gt, st),
                       "gather tiles");
-            d->g_tpos = tpos;
-            d->g_trcv = trcv;
-            d->g_twgt = twgt;
-            d->g_tblk = (const ipr_tblock*)tiles;
-            d->g_tn = ntiles;
+            d->g_tq = gt.q;
+            d->g_thead = gt.head;
+            d->g_tesc = gt.esc;
+            d->g_twgt = gt.wgt;
+            d->g_twsz = gt.wgt_bytes;
+            d->g_tblk = (const ipr_tblock*)gt.tiles;
+            d->g_tn = gt.n_tiles;
         } else if (d->g_xcd) {
             uint32_t* g_wgt = d->t_wgt ? dalloc(ea * 4, "gather weights") : NULL;
             hip_check(ipr_hip_build_gather_classes(V, E, d->t_idx, d->t_adj, d->t_wgt, d->bpos, IPR_GATHER_K, g_arc,

@@ -664,28 +664,96 @@ __global__ void k_tile_keys(const uint64_t* t_idx, const uint32_t* t_adj, const 
     }
 }
 
-__global__ void k_tile_finish(uint64_t E, const uint64_t* keys, const uint32_t* perm, const uint32_t* t_wgt,
-                              uint64_t pmask, uint32_t* tpos, uint16_t* trcv, uint32_t* twgt)
+/* group g's block: the last b with pofs[b] <= 64 g */
+__device__ inline uint64_t tile_block_of(const uint64_t* pofs, uint64_t nblk, uint64_t slot)
+{
+    uint64_t lo = 0, hi = nblk;
+    while (hi - lo > 1) {
+        const uint64_t mid = lo + (hi - lo) / 2;
+        if (pofs[mid] <= slot) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+/* per group of 64 slots: its base position (the first arc's), or a mark
+ * (0xFFFFFFFF) when the group's positions span more than 16 bits, counted */
+__global__ void k_tile_heads(uint64_t ngroups, const uint64_t* keys, const uint64_t* pofs, const uint64_t* aofs,
+                             const uint32_t* narcs, uint64_t nblk, uint64_t pmask, uint32_t* head,
+                             unsigned long long* nesc)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < E; i += stride) {
-        const uint64_t k = keys[i];
-        tpos[i] = (uint32_t)((k >> 16) & pmask);
-        trcv[i] = (uint16_t)(k & 0xFFFFu);
-        if (twgt) twgt[i] = t_wgt[perm[i]];
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < ngroups; g += stride) {
+        const uint64_t slot = g * 64, b = tile_block_of(pofs, nblk, slot);
+        const uint64_t off = slot - pofs[b];
+        if (off >= narcs[b]) { head[g] = 0; continue; }   /* a whole group of padding */
+        const uint64_t k0 = aofs[b] + off;
+        const uint64_t k1 = aofs[b] + (off + 63 < narcs[b] ? off + 63 : narcs[b] - 1);
+        const uint64_t p0 = (keys[k0] >> 16) & pmask, p1 = (keys[k1] >> 16) & pmask;
+        if (p1 - p0 > 0xFFFFu) {
+            head[g] = 0xFFFFFFFFu;
+            atomicAdd(nesc, 1ull);
+        } else {
+            head[g] = (uint32_t)p0;
+        }
     }
+}
+
+/* the marked groups get their escape rows */
+__global__ void k_tile_escapes(uint64_t ngroups, uint32_t* head, unsigned long long* next)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < ngroups; g += stride)
+        if (head[g] == 0xFFFFFFFFu) head[g] = IPR_TQ_ESC | (uint32_t)atomicAdd(next, 1ull);
+}
+
+/* each sorted arc into its slot: position delta and receiver offset (or the
+ * position into the group's escape row), and its weight at the slot's width */
+__global__ void k_tile_slots(uint64_t E, const uint64_t* keys, const uint32_t* perm, const uint32_t* t_wgt,
+                             const uint64_t* pofs, const uint64_t* aofs, unsigned kshift, uint64_t pmask,
+                             const uint32_t* head, uint32_t* q, uint32_t* esc, void* wgt, uint32_t wbytes)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < E; k += stride) {
+        const uint64_t key = keys[k], b = key >> kshift;
+        const uint64_t slot = pofs[b] + (k - aofs[b]);
+        const uint32_t pos = (uint32_t)((key >> 16) & pmask), rcv = (uint32_t)(key & 0xFFFFu);
+        const uint32_t h = head[slot >> 6];
+        if (h & IPR_TQ_ESC) {
+            esc[(uint64_t)(h & ~IPR_TQ_ESC) * 64 + (slot & 63)] = pos;
+            q[slot] = rcv;
+        } else {
+            q[slot] = ((pos - h) << 16) | rcv;
+        }
+        if (wbytes) {
+            const uint32_t w = t_wgt[perm[k]];
+            if (wbytes == 1) ((uint8_t*)wgt)[slot] = (uint8_t)w;
+            else if (wbytes == 2) ((uint16_t*)wgt)[slot] = (uint16_t)w;
+            else ((uint32_t*)wgt)[slot] = w;
+        }
+    }
+}
+
+__global__ void k_max_u32(const uint32_t* a, uint64_t n, unsigned int* out)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    uint32_t m = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) m = a[i] > m ? a[i] : m;
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t y = __shfl_xor(m, o);
+        m = y > m ? y : m;
+    }
+    if ((threadIdx.x & 63) == 0) atomicMax(out, m);
 }
 
 extern "C" int ipr_hip_build_gather_tiles(uint64_t V, uint64_t E, const uint64_t* t_idx, const uint32_t* t_adj,
                                           const uint32_t* t_wgt, const uint32_t* bpos, uint32_t max_recv,
-                                          uint64_t max_arcs, uint32_t* t_pos, uint16_t* t_rcv, uint32_t* t_w,
-                                          void** tiles, uint32_t* n_tiles, void* stream)
+                                          uint64_t max_arcs, ipr_gather_tiles* out, void* stream)
 {
     hipStream_t st = (hipStream_t)stream;
-    *tiles = nullptr;
-    *n_tiles = 0;
-    if (V >= (1ull << 32) || max_recv == 0 || max_recv > 65536 || max_arcs == 0) {
-        snprintf(g_err, sizeof(g_err), "gather tiles: needs V < 2^32, 1 <= receivers per block <= 65536 and "
+    memset(out, 0, sizeof(*out));
+    if (V >= (1ull << 31) || max_recv == 0 || max_recv >= IPR_TQ_NONE || max_arcs == 0) {
+        snprintf(g_err, sizeof(g_err), "gather tiles: needs V < 2^31, 1 <= receivers per block < 65535 and "
                  "arcs per tile >= 1 (V=%llu receivers=%u arcs=%llu)", (unsigned long long)V, max_recv,
                  (unsigned long long)max_arcs);
         return -1;
@@ -694,35 +762,57 @@ extern "C" int ipr_hip_build_gather_tiles(uint64_t V, uint64_t E, const uint64_t
     int rc = 0;
     hipError_t e;
     uint64_t* hidx = (uint64_t*)malloc((V + 1) * 8);
-    uint32_t *hv0 = nullptr, *hnr = nullptr, *dv0 = nullptr, *dnr = nullptr, *pin = nullptr, *pout = nullptr;
-    uint64_t *ha0 = nullptr, *kin = nullptr, *kout = nullptr;
+    uint32_t *hv0 = nullptr, *hnr = nullptr, *hna = nullptr, *dv0 = nullptr, *dnr = nullptr, *dna = nullptr;
+    uint32_t *pin = nullptr, *pout = nullptr, *dq = nullptr, *dhead = nullptr, *desc = nullptr;
+    uint64_t *ha0 = nullptr, *hpo = nullptr, *da0 = nullptr, *dpo = nullptr, *kin = nullptr, *kout = nullptr;
+    unsigned long long* dctr = nullptr;
+    unsigned long long hctr[2] = {0, 0};
     ipr_tblock* ht = nullptr;
-    void* dt = nullptr;
-    void* tmp = nullptr;
+    void *dt = nullptr, *dw = nullptr, *tmp = nullptr;
     size_t tmp_bytes = 0;
-    uint64_t nb = 0, nt = 0;
+    uint64_t nb = 0, nt = 0, slots = 0, ngroups = 0;
     unsigned kshift = 0, end_bit = 0;
+    uint32_t wbytes = 0;
+    const uint64_t gpt = max_arcs / 64 ? max_arcs / 64 : 1;   /* groups per tile of a split receiver */
     if (!hidx) { snprintf(g_err, sizeof(g_err), "gather tiles: host memory"); return -1; }
     if ((e = hipMemcpyAsync(hidx, t_idx, (V + 1) * 8, hipMemcpyDeviceToHost, st)) != hipSuccess ||
         (e = hipStreamSynchronize(st)) != hipSuccess) { rc = fail(e, "gather tiles offsets"); goto out; }
     /* blocks of consecutive receivers: at most max_recv of them and at most
      * max_arcs in-arcs, except a receiver with more in-arcs, which forms a
-     * block alone; blocks without arcs are left out */
+     * block alone; blocks without arcs are left out. Each block's arcs start
+     * on a group of 64 slots. */
     hv0 = (uint32_t*)malloc(V * 4);
     hnr = (uint32_t*)malloc(V * 4);
+    hna = (uint32_t*)malloc(V * 4);
     ha0 = (uint64_t*)malloc(V * 8);
-    if (!hv0 || !hnr || !ha0) { snprintf(g_err, sizeof(g_err), "gather tiles: host memory"); rc = -1; goto out; }
+    hpo = (uint64_t*)malloc(V * 8);
+    if (!hv0 || !hnr || !hna || !ha0 || !hpo) {
+        snprintf(g_err, sizeof(g_err), "gather tiles: host memory");
+        rc = -1;
+        goto out;
+    }
     for (uint64_t v = 0; v < V;) {
         const uint64_t v0 = v, a0 = hidx[v];
         while (v < V && v - v0 < max_recv && (v == v0 || hidx[v + 1] - a0 <= max_arcs)) v++;
-        if (hidx[v] > a0) {
+        const uint64_t na = hidx[v] - a0;
+        if (na > 0) {
+            if (na >= (1ull << 32)) {
+                snprintf(g_err, sizeof(g_err), "gather tiles: a receiver with %llu in-arcs", (unsigned long long)na);
+                rc = -1;
+                goto out;
+            }
+            const uint64_t ng = (na + 63) / 64;
             hv0[nb] = (uint32_t)v0;
             hnr[nb] = (uint32_t)(v - v0);
+            hna[nb] = (uint32_t)na;
             ha0[nb] = a0;
-            nt += (hidx[v] - a0 + max_arcs - 1) / max_arcs;
+            hpo[nb] = slots;
+            slots += 64 * ng;
+            nt += na > max_arcs ? (ng + gpt - 1) / gpt : 1;
             nb++;
         }
     }
+    ngroups = slots / 64;
     kshift = 16 + (unsigned)nbits(V - 1);
     end_bit = kshift + (unsigned)nbits(nb - 1);
     if (end_bit > 64 || nt >= (1ull << 31)) {
@@ -732,35 +822,46 @@ extern "C" int ipr_hip_build_gather_tiles(uint64_t V, uint64_t E, const uint64_t
         goto out;
     }
     /* tiles, largest first: a block above max_arcs (one receiver) is split
-     * into tiles of max_arcs that deliver by atomics */
+     * into tiles of whole groups that deliver by atomics */
     ht = (ipr_tblock*)malloc(nt * sizeof(ipr_tblock));
     if (!ht) { snprintf(g_err, sizeof(g_err), "gather tiles: host memory"); rc = -1; goto out; }
     {
         uint64_t k = 0;
         for (uint64_t b = 0; b < nb; b++) {
-            const uint64_t a0 = ha0[b], na = hidx[(uint64_t)hv0[b] + hnr[b]] - a0;
-            const uint32_t shared = na > max_arcs ? 1u : 0u;
-            for (uint64_t o = 0; o < na; o += max_arcs) {
-                const uint64_t n = na - o < max_arcs ? na - o : max_arcs;
-                ht[k++] = ipr_tblock{a0 + o, (uint32_t)n, hv0[b], hnr[b], shared};
+            const uint64_t ns = 64 * ((hna[b] + 63ull) / 64);
+            if (hna[b] <= max_arcs) {
+                ht[k++] = ipr_tblock{hpo[b], (uint32_t)ns, hv0[b], hnr[b], 0u};
+                continue;
+            }
+            for (uint64_t o = 0; o < ns; o += 64 * gpt) {
+                const uint64_t n = ns - o < 64 * gpt ? ns - o : 64 * gpt;
+                ht[k++] = ipr_tblock{hpo[b] + o, (uint32_t)n, hv0[b], hnr[b], 1u};
             }
         }
         std::stable_sort(ht, ht + nt, [](const ipr_tblock& x, const ipr_tblock& y) { return x.na > y.na; });
     }
-    if ((e = hipMalloc(&dv0, nb * 4)) != hipSuccess) { rc = fail(e, "gather tile blocks"); goto out; }
-    if ((e = hipMalloc(&dnr, nb * 4)) != hipSuccess) { rc = fail(e, "gather tile blocks"); goto out; }
+    if ((e = hipMalloc(&dv0, nb * 4)) != hipSuccess || (e = hipMalloc(&dnr, nb * 4)) != hipSuccess ||
+        (e = hipMalloc(&dna, nb * 4)) != hipSuccess || (e = hipMalloc(&da0, nb * 8)) != hipSuccess ||
+        (e = hipMalloc(&dpo, nb * 8)) != hipSuccess || (e = hipMalloc(&dctr, 16)) != hipSuccess) {
+        rc = fail(e, "gather tile blocks");
+        goto out;
+    }
     if ((e = hipMalloc(&dt, nt * sizeof(ipr_tblock))) != hipSuccess) { rc = fail(e, "gather tiles"); goto out; }
     if ((e = hipMalloc(&kin, E * 8)) != hipSuccess) { rc = fail(e, "gather tile keys"); goto out; }
     if ((e = hipMalloc(&kout, E * 8)) != hipSuccess) { rc = fail(e, "gather tile keys"); goto out; }
     if ((e = hipMemcpyAsync(dv0, hv0, nb * 4, hipMemcpyHostToDevice, st)) != hipSuccess ||
         (e = hipMemcpyAsync(dnr, hnr, nb * 4, hipMemcpyHostToDevice, st)) != hipSuccess ||
+        (e = hipMemcpyAsync(dna, hna, nb * 4, hipMemcpyHostToDevice, st)) != hipSuccess ||
+        (e = hipMemcpyAsync(da0, ha0, nb * 8, hipMemcpyHostToDevice, st)) != hipSuccess ||
+        (e = hipMemcpyAsync(dpo, hpo, nb * 8, hipMemcpyHostToDevice, st)) != hipSuccess ||
+        (e = hipMemsetAsync(dctr, 0, 16, st)) != hipSuccess ||
         (e = hipMemcpyAsync(dt, ht, nt * sizeof(ipr_tblock), hipMemcpyHostToDevice, st)) != hipSuccess) {
         rc = fail(e, "gather tile upload");
         goto out;
     }
     hipLaunchKernelGGL(k_tile_keys, dim3((unsigned)(nb < 8192 ? nb : 8192)), dim3(256), 0, st, t_idx, t_adj, bpos,
                        dv0, dnr, nb, kshift, kin);
-    if (t_w) {
+    if (t_wgt) {
         if ((e = hipMalloc(&pin, E * 4)) != hipSuccess) { rc = fail(e, "gather tile perm"); goto out; }
         if ((e = hipMalloc(&pout, E * 4)) != hipSuccess) { rc = fail(e, "gather tile perm"); goto out; }
         hipLaunchKernelGGL(k_iota_u32, dim3(4096), dim3(256), 0, st, pin, E);
@@ -770,26 +871,63 @@ extern "C" int ipr_hip_build_gather_tiles(uint64_t V, uint64_t E, const uint64_t
     }
     if (e != hipSuccess) { rc = fail(e, "gather tile sort size"); goto out; }
     if ((e = hipMalloc(&tmp, tmp_bytes ? tmp_bytes : 16)) != hipSuccess) { rc = fail(e, "gather tile temp"); goto out; }
-    if (t_w)
+    if (t_wgt)
         e = rocprim::radix_sort_pairs(tmp, tmp_bytes, kin, kout, pin, pout, (size_t)E, 0, end_bit, st);
     else
         e = rocprim::radix_sort_keys(tmp, tmp_bytes, kin, kout, (size_t)E, 0, end_bit, st);
     if (e != hipSuccess) { rc = fail(e, "gather tile sort"); goto out; }
+    (void)hipFree(kin); kin = nullptr;
+    (void)hipFree(pin); pin = nullptr;
+    (void)hipFree(tmp); tmp = nullptr;
+    /* weights at the narrowest width that holds the largest */
+    if (t_wgt) {
+        unsigned int* dmax = (unsigned int*)(dctr + 1);
+        uint32_t wmax = 0;
+        hipLaunchKernelGGL(k_max_u32, dim3(1024), dim3(256), 0, st, t_wgt, E, dmax);
+        if ((e = hipMemcpyAsync(&wmax, dmax, 4, hipMemcpyDeviceToHost, st)) != hipSuccess ||
+            (e = hipStreamSynchronize(st)) != hipSuccess) { rc = fail(e, "gather tile weights"); goto out; }
+        wbytes = wmax < 256 ? 1 : (wmax < 65536 ? 2 : 4);
+        if ((e = hipMalloc(&dw, slots * wbytes)) != hipSuccess) { rc = fail(e, "gather tile weights"); goto out; }
+    }
+    if ((e = hipMalloc(&dq, slots * 4)) != hipSuccess) { rc = fail(e, "gather tile slots"); goto out; }
+    if ((e = hipMalloc(&dhead, ngroups * 4)) != hipSuccess) { rc = fail(e, "gather tile groups"); goto out; }
+    if ((e = hipMemsetAsync(dq, 0xFF, slots * 4, st)) != hipSuccess) { rc = fail(e, "gather tile slots"); goto out; }
+    if ((e = hipMemsetAsync(dctr, 0, 8, st)) != hipSuccess) { rc = fail(e, "gather tile groups"); goto out; }
     {
+        uint64_t gb = (ngroups + 255) / 256;
+        if (gb > 65536) gb = 65536;
+        hipLaunchKernelGGL(k_tile_heads, dim3((unsigned)gb), dim3(256), 0, st, ngroups, kout, dpo, da0, dna, nb,
+                           (1ull << (kshift - 16)) - 1, dhead, dctr);
+        if ((e = hipMemcpyAsync(&hctr[0], dctr, 8, hipMemcpyDeviceToHost, st)) != hipSuccess ||
+            (e = hipStreamSynchronize(st)) != hipSuccess) { rc = fail(e, "gather tile groups"); goto out; }
+        if (hctr[0]) {
+            if ((e = hipMalloc(&desc, hctr[0] * 64 * 4)) != hipSuccess) { rc = fail(e, "gather tile escapes"); goto out; }
+            if ((e = hipMemsetAsync(dctr, 0, 8, st)) != hipSuccess) { rc = fail(e, "gather tile escapes"); goto out; }
+            hipLaunchKernelGGL(k_tile_escapes, dim3((unsigned)gb), dim3(256), 0, st, ngroups, dhead, dctr);
+        }
         uint64_t eb = (E + 255) / 256;
         if (eb > 65536) eb = 65536;
-        hipLaunchKernelGGL(k_tile_finish, dim3((unsigned)eb), dim3(256), 0, st, E, kout, pout, t_wgt,
-                           (1ull << (kshift - 16)) - 1, t_pos, t_rcv, t_w);
+        hipLaunchKernelGGL(k_tile_slots, dim3((unsigned)eb), dim3(256), 0, st, E, kout, pout, t_wgt, dpo, da0,
+                           kshift, (1ull << (kshift - 16)) - 1, dhead, dq, desc, dw, wbytes);
     }
     if ((e = hipGetLastError()) != hipSuccess) { rc = fail(e, "gather tile index"); goto out; }
     if ((e = hipStreamSynchronize(st)) != hipSuccess) { rc = fail(e, "gather tile sync"); goto out; }
-    *tiles = dt;
-    *n_tiles = (uint32_t)nt;
-    dt = nullptr;
+    out->tiles = dt;
+    out->n_tiles = (uint32_t)nt;
+    out->wgt_bytes = wbytes;
+    out->slots = slots;
+    out->escaped = hctr[0];
+    out->q = dq;
+    out->head = dhead;
+    out->esc = desc;
+    out->wgt = dw;
+    dt = nullptr; dq = nullptr; dhead = nullptr; desc = nullptr; dw = nullptr;
 out:
-    free(hidx); free(hv0); free(hnr); free(ha0); free(ht);
-    (void)hipFree(dv0); (void)hipFree(dnr); (void)hipFree(kin); (void)hipFree(kout);
-    (void)hipFree(pin); (void)hipFree(pout); (void)hipFree(tmp); (void)hipFree(dt);
+    free(hidx); free(hv0); free(hnr); free(hna); free(ha0); free(hpo); free(ht);
+    (void)hipFree(dv0); (void)hipFree(dnr); (void)hipFree(dna); (void)hipFree(da0); (void)hipFree(dpo);
+    (void)hipFree(dctr); (void)hipFree(kin); (void)hipFree(kout); (void)hipFree(pin); (void)hipFree(pout);
+    (void)hipFree(tmp); (void)hipFree(dt); (void)hipFree(dw); (void)hipFree(dq); (void)hipFree(dhead);
+    (void)hipFree(desc);
     return rc;
 }
 

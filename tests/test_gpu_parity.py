@@ -169,7 +169,8 @@ def test_gather_every_superstep(blocks, order, live, ranges, record4, monkeypatc
 
 @pytest.mark.parametrize("blocks,recv,arcs,order,record4", [
     ("", "", "", "1", "1"), ("1", "", "", "1", "1"), ("3", "32", "100", "0", "1"),
-    ("", "64", "1000", "1", "0"), ("7", "16384", "5000", "1", "1"), ("2", "33", "65536", "0", "0")])
+    ("", "64", "1000", "1", "0"), ("7", "16384", "5000", "1", "1"), ("2", "33", "65536", "0", "0"),
+    ("", "64", "1000", "1", "1")])
 def test_tiled_gather_every_superstep(blocks, recv, arcs, order, record4, monkeypatch):
     """The receiver-blocked gather (tiles of consecutive receivers, arcs sorted
     by source record, folds in LDS) forced on every superstep, with 1, 2, 3,
@@ -177,7 +178,10 @@ def test_tiled_gather_every_superstep(blocks, recv, arcs, order, record4, monkey
     16384 or the default number of receivers; tiles of 100 arcs up to the
     default, so that hub in-lists split into many tiles that deliver by
     atomics; records by id or by degree, 4- or 8-byte; SPREAD and PUSH, with
-    and without weights; a hub star."""
+    and without weights; a hub star; sparse graphs of 160-200K vertices
+    whose small blocks have slot groups spanning more than 16 bits of record
+    positions (the escape rows: 155-294 groups with 32-64 receivers per block
+    and 100-1000 arcs per tile, by a count over the same layout on the CPU)."""
     monkeypatch.setenv("IPR_GATHER_TILED", "1")
     monkeypatch.setenv("IPR_GATHER_RECORD4", record4)
     monkeypatch.setenv("IPR_RECORD_ORDER", order)
@@ -207,6 +211,18 @@ def test_tiled_gather_every_superstep(blocks, recv, arcs, order, record4, monkey
     r = EN.run_app("cc_spread", star, dir_alpha=1 << 30)
     assert np.array_equal(r.values, ref.values)
     assert list(r.active) == list(ref.active[: ref.supersteps])
+    rng = np.random.default_rng(1)
+    gr = G.from_arcs(200000, rng.integers(0, 200000, 1000000), rng.integers(0, 200000, 1000000))
+    print(f"app=cc_spread random 200K tiles {blocks}/{recv}/{arcs}", flush=True)
+    r = EN.run_app("cc_spread", gr, dir_alpha=1 << 30)
+    assert np.array_equal(r.values, O.min_ancestor(gr))
+    grw = gr.with_weights(3)
+    r = EN.run_app("sssp_weighted_spread", grw, source=0, dir_alpha=1 << 30)
+    assert np.array_equal(r.values, O.dijkstra(grw, 0))
+    gg = G.grid(400, 400)
+    print(f"app=sssp_spread grid400 tiles {blocks}/{recv}/{arcs}", flush=True)
+    r = EN.run_app("sssp_spread", gg, source=0, dir_alpha=1 << 30)
+    assert np.array_equal(r.values, O.bfs(gg, 0))
 
 
 @pytest.mark.parametrize("blocks,ranges", [("", ""), ("3", "512"), ("1", "100000")])
