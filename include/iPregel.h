@@ -378,6 +378,7 @@ void ip_init(const char* file_path, int number_of_threads, const char* schedule,
     d.can_gather = ipr_policy::exact_gather;
     d.can_tile = ipr_tile_ok;
     d.tile_receivers = ipr_tile_r;
+    d.tile_wgs_per_cu = ipr_tile_wgs_per_cu;
     d.application = IP_APPLICATION;
     d.version = VERSION;
     d.machine = IP_MACHINE;

@@ -329,6 +329,7 @@ typedef struct ipr_app_desc {
      * tile_receivers receivers per tile */
     int can_tile;
     uint32_t tile_receivers;
+    uint32_t tile_wgs_per_cu;    /* resident workgroups per CU its LDS allows */
 } ipr_app_desc;
 
 #ifdef __cplusplus

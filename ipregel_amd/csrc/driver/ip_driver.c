@@ -406,16 +406,17 @@ static void dev_setup(void)
             d->bpos = bpos;
         }
         if (tiled) {
-            /* two resident workgroups per CU (64 KB of receiver slots each);
-             * tiles of at most tile_receivers receivers and about one
-             * workgroup's share of the arcs: larger tiles share more record
-             * lines between their arcs (measured: CC R-MAT 22 gather 0.36 ms
-             * with a quarter share, 0.28 ms with a whole one; CC R-MAT 24
-             * 1.52 -> 1.29 ms), and the blocks that the receiver limit keeps
-             * smaller fill in behind them, largest first */
+            /* the resident workgroups the LDS allows (one per CU with 128 KB
+             * of receiver slots); tiles of at most tile_receivers receivers
+             * and about one workgroup's share of the arcs: larger tiles share
+             * more record lines between their arcs (measured with two 64 KB
+             * workgroups per CU: CC R-MAT 22 gather 0.36 ms with a quarter
+             * share, 0.28 ms with a whole one; CC R-MAT 24 1.52 -> 1.29 ms),
+             * and the blocks that the receiver limit keeps smaller fill in
+             * behind them, largest first */
             uint32_t cus = 1;
             hip_check(ipr_hip_cu_count(&cus), "compute units");
-            d->tile_blocks = env_u32("IPR_TILE_BLOCKS", cus * 2);
+            d->tile_blocks = env_u32("IPR_TILE_BLOCKS", cus * (a->tile_wgs_per_cu ? a->tile_wgs_per_cu : 1));
             if (d->tile_blocks < 1) d->tile_blocks = 1;
             uint32_t max_recv = env_u32("IPR_TILE_RECV", a->tile_receivers);
             if (max_recv < 32 || max_recv > a->tile_receivers) max_recv = a->tile_receivers;
