@@ -1349,6 +1349,7 @@ constexpr bool ipr_tile_ok = ipr_policy::exact_gather && IPR_COMBINER != IPR_COM
 constexpr uint32_t ipr_tile_r = IPR_TILE_LDS_BYTES / sizeof(ipr_slot_t) < 65536u
                                     ? (uint32_t)(IPR_TILE_LDS_BYTES / sizeof(ipr_slot_t)) : 65536u;
 static_assert(ipr_tile_r % 32 == 0, "tile receivers: whole words of the message bits");
+static_assert(IPR_TILE_K >= 1 && IPR_TILE_K <= 64, "one lane loads each slot group's base");
 /* resident workgroups per CU that its LDS allows (160 KB per CU on MI355X) */
 constexpr uint32_t ipr_tile_wgs_per_cu =
     163840u / (uint32_t)(ipr_tile_r * sizeof(ipr_slot_t) + ipr_tile_r / 8 + 4) > 0
@@ -1386,12 +1387,16 @@ __global__ void __launch_bounds__(IPR_TILE_THREADS) ipr_k_gather_tiled(ipr_dev d
         ipr_slot_t* mb = (ipr_slot_t*)d.mbox[(s + 1) & 1];
         uint8_t* rv = d.recv[(s + 1) & 1];
         unsigned long long* ctr = (unsigned long long*)&d.cnt[IPR_C_GNEXT * IPR_CSLOTS];
-        const uint32_t lane = ipr_lane(), wv = threadIdx.x / IPR_WAVE;
+        /* the wavefront index and the tile number are uniform: read them
+         * through readfirstlane so that the tile's fields, the pass bounds and
+         * the group bases live in scalar registers (scalar loads, scalar
+         * branches on escape rows) */
+        const uint32_t lane = ipr_lane(), wv = __builtin_amdgcn_readfirstlane(threadIdx.x / IPR_WAVE);
         const IP_MESSAGE_TYPE id = ipr_policy::ident();
         for (;;) {
             if (threadIdx.x == 0) tcur = (uint32_t)atomicAdd(ctr, 1ull);
             __syncthreads();
-            const uint32_t t = tcur;
+            const uint32_t t = __builtin_amdgcn_readfirstlane(tcur);
             if (t >= d.g_tn) break;
             const ipr_tblock tb = d.g_tblk[t];
             for (uint32_t r = threadIdx.x; r < tb.nr; r += NT) ts[r] = id;
@@ -1403,35 +1408,37 @@ __global__ void __launch_bounds__(IPR_TILE_THREADS) ipr_k_gather_tiled(ipr_dev d
 #ifdef IP_NEEDS_OUT_NEIGHBOUR_WEIGHTS
                 uint32_t wt[K];
 #endif
-                /* slot group j of this pass (whole groups: tiles start and
-                 * end on them): its base position (one scalar load), then
-                 * each lane's slot; the index is read once per launch, so it
-                 * is kept out of L2, which holds the records */
-                uint32_t h[K], qv[K];
+                /* slot groups j < ng of this pass (whole groups: tiles start
+                 * and end on them): lane j loads group j's base, every lane
+                 * its slot of each group (the index is read once per launch,
+                 * so it is kept out of L2, which holds the records); the
+                 * bases are read back per group with readlane only after all
+                 * the loads are issued */
+                const uint32_t ng = (uint32_t)((a1 - base) >> 6 < K ? (a1 - base) >> 6 : K);
+                const uint32_t hl = lane < ng ? d.g_thead[(base >> 6) + lane] : 0u;
+                uint32_t qv[K];
 #pragma unroll
-                for (int j = 0; j < (int)K; j++) {
-                    const uint64_t gs = base + (uint64_t)j * IPR_WAVE;
-                    h[j] = 0;
-                    qv[j] = 0xFFFFFFFFu;
-                    if (gs < a1) {
-                        h[j] = d.g_thead[gs >> 6];
-                        qv[j] = __builtin_nontemporal_load(d.g_tq + gs + lane);
-                    }
+                for (int j = 0; j < (int)K; j++)
+                    qv[j] = (uint32_t)j < ng ? __builtin_nontemporal_load(d.g_tq + base + (uint64_t)j * IPR_WAVE + lane)
+                                              : 0xFFFFFFFFu;
 #ifdef IP_NEEDS_OUT_NEIGHBOUR_WEIGHTS
-                    wt[j] = 0;
-                    if (gs < a1) {
-                        if (d.g_twsz == 1) wt[j] = __builtin_nontemporal_load((const uint8_t*)d.g_twgt + gs + lane);
-                        else if (d.g_twsz == 2)
-                            wt[j] = __builtin_nontemporal_load((const uint16_t*)d.g_twgt + gs + lane);
-                        else wt[j] = __builtin_nontemporal_load((const uint32_t*)d.g_twgt + gs + lane);
-                    }
-#endif
-                }
 #pragma unroll
                 for (int j = 0; j < (int)K; j++) {
+                    const uint64_t i = base + (uint64_t)j * IPR_WAVE + lane;
+                    wt[j] = 0;
+                    if ((uint32_t)j < ng) {
+                        if (d.g_twsz == 1) wt[j] = __builtin_nontemporal_load((const uint8_t*)d.g_twgt + i);
+                        else if (d.g_twsz == 2) wt[j] = __builtin_nontemporal_load((const uint16_t*)d.g_twgt + i);
+                        else wt[j] = __builtin_nontemporal_load((const uint32_t*)d.g_twgt + i);
+                    }
+                }
+#endif
+#pragma unroll
+                for (int j = 0; j < (int)K; j++) {
+                    const uint32_t h = __builtin_amdgcn_readlane(hl, j);
                     r[j] = qv[j] & 0xFFFFu;
-                    if (h[j] & IPR_TQ_ESC) p[j] = d.g_tesc[(uint64_t)(h[j] & ~IPR_TQ_ESC) * IPR_WAVE + lane];
-                    else p[j] = h[j] + (qv[j] >> 16);
+                    if (h & IPR_TQ_ESC) p[j] = d.g_tesc[(uint64_t)(h & ~IPR_TQ_ESC) * IPR_WAVE + lane];
+                    else p[j] = h + (qv[j] >> 16);
                     if (r[j] == IPR_TQ_NONE) p[j] = 0xFFFFFFFFu;
                 }
                 if (live) {

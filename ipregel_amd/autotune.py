@@ -126,7 +126,7 @@ ENGINE_GRID = {
     "push": [(2048, 2048), (1024, 1024), (4096, 4096), (1024, 4096), (2048, 8192), (512, 2048)],
     "grid_blocks": [0, 1024, 4096],
     # push apps: gather a superstep when its broadcasters' out-arcs * alpha > E
-    "dir_alpha": [4, 2, 8],
+    "dir_alpha": [6, 3, 12],
     # pull (single broadcast): (pull_small_max, pull_medium_max)
     "pull": [(0, 0), (2, 16), (4, 64), (8, 256), (16, 1024)],
 }

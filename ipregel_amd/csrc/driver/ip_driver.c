@@ -356,7 +356,7 @@ static void dev_setup(void)
     /* the gather index marks list ends in bit 31 of the source ids */
     d->dir_opt = dopt == 1 && a->can_gather && (var == IPR_VARIANT_PUSH || var == IPR_VARIANT_SPREAD) &&
                  V < (1ull << 31);
-    d->dir_alpha = c && c->dir_alpha ? c->dir_alpha : env_u32("IPR_DIR_ALPHA", 4);
+    d->dir_alpha = c && c->dir_alpha ? c->dir_alpha : env_u32("IPR_DIR_ALPHA", 6);
     d->large_chunk = c && c->bin_small_max ? c->bin_small_max : env_u32("IPR_LARGE_CHUNK", 2048);
     if (d->large_chunk < 64) d->large_chunk = 64;
     if (!pull && d->dir_opt) {

@@ -20,7 +20,7 @@ PKG = Path(__file__).resolve().parent
 LIB = PKG / "lib"
 # the driver's push-or-gather threshold when none is given: gather when the
 # broadcasters' out-arcs times this exceed E
-DIR_ALPHA_DEFAULT = 4
+DIR_ALPHA_DEFAULT = 6
 BIN = PKG / "bin"
 
 # application name -> (value dtype, takes a source argument, needs weights)
