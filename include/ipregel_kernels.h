@@ -1365,6 +1365,24 @@ __device__ __forceinline__ void ipr_tile_fold(S* p, IP_MESSAGE_TYPE m)
     }
 }
 
+/* The slot groups of a pass at base (whole groups: tiles start and end on
+ * them; none past a1): lane j returns group j's base, and qv[j] gets each
+ * lane's slot of group j (IPR_TQ_NONE past the end). The index is read once
+ * per launch, so it is kept out of L2, which holds the records; the bases are
+ * read back per group with readlane only after all the loads are issued.
+ * (Issuing the next pass's slots before this pass's records measured the
+ * same: the launch is bound by the record lines, not by the index latency.) */
+__device__ __forceinline__ uint32_t ipr_tile_load(const ipr_dev& d, uint64_t base, uint64_t a1, uint32_t lane,
+                                                  uint32_t (&qv)[IPR_TILE_K])
+{
+    const uint32_t ng = base < a1 ? (uint32_t)((a1 - base) >> 6 < IPR_TILE_K ? (a1 - base) >> 6 : IPR_TILE_K) : 0u;
+#pragma unroll
+    for (int j = 0; j < IPR_TILE_K; j++)
+        qv[j] = (uint32_t)j < ng ? __builtin_nontemporal_load(d.g_tq + base + (uint64_t)j * IPR_WAVE + lane)
+                                  : 0xFFFFFFFFu;
+    return lane < ng ? d.g_thead[(base >> 6) + lane] : 0u;
+}
+
 __global__ void __launch_bounds__(IPR_TILE_THREADS) ipr_k_gather_tiled(ipr_dev d)
 {
     if constexpr (ipr_tile_ok) {
@@ -1408,20 +1426,10 @@ __global__ void __launch_bounds__(IPR_TILE_THREADS) ipr_k_gather_tiled(ipr_dev d
 #ifdef IP_NEEDS_OUT_NEIGHBOUR_WEIGHTS
                 uint32_t wt[K];
 #endif
-                /* slot groups j < ng of this pass (whole groups: tiles start
-                 * and end on them): lane j loads group j's base, every lane
-                 * its slot of each group (the index is read once per launch,
-                 * so it is kept out of L2, which holds the records); the
-                 * bases are read back per group with readlane only after all
-                 * the loads are issued */
-                const uint32_t ng = (uint32_t)((a1 - base) >> 6 < K ? (a1 - base) >> 6 : K);
-                const uint32_t hl = lane < ng ? d.g_thead[(base >> 6) + lane] : 0u;
                 uint32_t qv[K];
-#pragma unroll
-                for (int j = 0; j < (int)K; j++)
-                    qv[j] = (uint32_t)j < ng ? __builtin_nontemporal_load(d.g_tq + base + (uint64_t)j * IPR_WAVE + lane)
-                                              : 0xFFFFFFFFu;
+                const uint32_t hl = ipr_tile_load(d, base, a1, lane, qv);
 #ifdef IP_NEEDS_OUT_NEIGHBOUR_WEIGHTS
+                const uint32_t ng = (uint32_t)((a1 - base) >> 6 < K ? (a1 - base) >> 6 : K);
 #pragma unroll
                 for (int j = 0; j < (int)K; j++) {
                     const uint64_t i = base + (uint64_t)j * IPR_WAVE + lane;
