@@ -194,7 +194,7 @@ typedef struct ipr_dev {
      * this superstep, and the gather reads only those records */
     uint64_t* g_live;
     uint32_t live_pct;
-    uint32_t live_pad;
+    uint32_t live_vpct;          /* ... and the vertices computed are below this % of V */
     /* push apps with integral messages that gather: a 4-byte copy of each
      * broadcast record at the same position as in bslot (IPR_REC4_* layout),
      * read by the gather in place of the 8-byte record (NULL: off) */

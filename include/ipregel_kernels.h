@@ -335,13 +335,18 @@ __device__ inline bool ipr_gathers(const ipr_dev& d)
     return d.dir_opt && ipr_bcast_arcs(d) * (uint64_t)d.dir_alpha > d.E;
 }
 
-/* a gathered superstep with few live arcs reads the broadcasters' bitmap
- * before any record (wavefront-uniform) */
+/* a gathered superstep with few live arcs from few computed vertices reads
+ * the broadcasters' bitmap before any record (wavefront-uniform): then most
+ * of the records it skips are cold ones (few broadcasters holding many arcs
+ * are hubs; the others' records are the ones that miss) */
 __device__ inline bool ipr_gathers_live(const ipr_dev& d)
 {
     if (!d.dir_opt || !d.g_live) return false;
     const uint64_t w = ipr_bcast_arcs(d);
-    return w * (uint64_t)d.dir_alpha > d.E && w * 100 < d.E * (uint64_t)d.live_pct;
+    uint64_t n = d.cnt[IPR_C_WORKV * IPR_CSLOTS + ipr_lane()];
+    for (int o = 32; o > 0; o >>= 1) n += __shfl_xor(n, o);
+    return w * (uint64_t)d.dir_alpha > d.E && w * 100 < d.E * (uint64_t)d.live_pct &&
+           n * 100 < d.V * (uint64_t)d.live_vpct;
 }
 
 /* position of vertex v's broadcast record */

@@ -450,10 +450,13 @@ static void dev_setup(void)
         d->g_arc = g_arc;
         d->g_own = g_own;
         d->g_dst = g_dst;
-        /* broadcasters' bitmap for gathers with few live arcs: below this
-         * percentage of E (0, the default, turns it off: with 4-byte records
-         * the bitmap's loads cost more than the records they skip) */
-        d->live_pct = env_u32("IPR_GATHER_LIVE", 0);
+        /* broadcasters' bitmap for gathers with few live arcs (below this
+         * percentage of E) from few computed vertices (below live_vpct % of
+         * V): unit SSSP R-MAT 24 superstep 1 (238K vertices, 52% of the arcs)
+         * 0.54 -> 0.40 ms; with many computed vertices the skipped records
+         * are cheap ones (CC R-MAT 22 superstep 2, 34% live: 2-4% slower) */
+        d->live_pct = env_u32("IPR_GATHER_LIVE", 60);
+        d->live_vpct = env_u32("IPR_GATHER_LIVE_VERTICES", 5);
         if (d->live_pct > 0) d->g_live = dalloc(((vv + 63) / 64) * 8, "gather live bitmap");
         /* 4-byte records for the gather (0 turns them off); padded to 16
          * entries for the window clear of the frontier compaction, cleared by

@@ -150,6 +150,7 @@ def test_gather_every_superstep(blocks, order, live, ranges, record4, monkeypatc
             monkeypatch.delenv(k, raising=False)
     monkeypatch.setenv("IPR_RECORD_ORDER", order)
     monkeypatch.setenv("IPR_GATHER_LIVE", live)
+    monkeypatch.setenv("IPR_GATHER_LIVE_VERTICES", "100")
     g = G.rmat(13, 16, seed=5, symmetrise=True)
     r = EN.run_app("cc_spread", g, dir_alpha=1 << 30)
     assert np.array_equal(r.values, O.min_ancestor(g))
@@ -167,25 +168,27 @@ def test_gather_every_superstep(blocks, order, live, ranges, record4, monkeypatc
     assert np.array_equal(r.values, O.min_ancestor(star))
 
 
-@pytest.mark.parametrize("blocks,recv,arcs,order,record4", [
-    ("", "", "", "1", "1"), ("1", "", "", "1", "1"), ("3", "32", "100", "0", "1"),
-    ("", "64", "1000", "1", "0"), ("7", "16384", "5000", "1", "1"), ("2", "33", "65536", "0", "0"),
-    ("", "64", "1000", "1", "1")])
-def test_tiled_gather_every_superstep(blocks, recv, arcs, order, record4, monkeypatch):
+@pytest.mark.parametrize("blocks,recv,arcs,order,record4,live", [
+    ("", "", "", "1", "1", ""), ("1", "", "", "1", "1", "100"), ("3", "32", "100", "0", "1", ""),
+    ("", "64", "1000", "1", "0", "100"), ("7", "16384", "5000", "1", "1", "0"), ("2", "33", "65536", "0", "0", ""),
+    ("", "64", "1000", "1", "1", "100")])
+def test_tiled_gather_every_superstep(blocks, recv, arcs, order, record4, live, monkeypatch):
     """The receiver-blocked gather (tiles of consecutive receivers, arcs sorted
     by source record, folds in LDS) forced on every superstep, with 1, 2, 3,
     7 or the default number of resident workgroups; blocks of 32, 33, 64,
     16384 or the default number of receivers; tiles of 100 arcs up to the
     default, so that hub in-lists split into many tiles that deliver by
-    atomics; records by id or by degree, 4- or 8-byte; SPREAD and PUSH, with
-    and without weights; a hub star; sparse graphs of 160-200K vertices
+    atomics; records by id or by degree, 4- or 8-byte; the broadcasters'
+    bitmap by default, off, or read in every gathered superstep; SPREAD and
+    PUSH, with and without weights; a hub star; sparse graphs of 160-200K vertices
     whose small blocks have slot groups spanning more than 16 bits of record
     positions (the escape rows: 155-294 groups with 32-64 receivers per block
     and 100-1000 arcs per tile, by a count over the same layout on the CPU)."""
     monkeypatch.setenv("IPR_GATHER_TILED", "1")
     monkeypatch.setenv("IPR_GATHER_RECORD4", record4)
     monkeypatch.setenv("IPR_RECORD_ORDER", order)
-    for k, v in (("IPR_TILE_BLOCKS", blocks), ("IPR_TILE_RECV", recv), ("IPR_TILE_ARCS", arcs)):
+    for k, v in (("IPR_TILE_BLOCKS", blocks), ("IPR_TILE_RECV", recv), ("IPR_TILE_ARCS", arcs),
+                 ("IPR_GATHER_LIVE", live), ("IPR_GATHER_LIVE_VERTICES", "100" if live else "")):
         if v:
             monkeypatch.setenv(k, v)
         else:
