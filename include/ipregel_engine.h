@@ -189,6 +189,9 @@ typedef struct ipr_dev {
      * so that consecutive frontier vertices write nearby records (NULL:
      * bslot[v]) */
     const uint32_t* bpos;
+    /* its inverse, bvert[bpos[v]] = v: a sweep over all records (the SPREAD
+     * expansion of superstep 0) reads them in position order */
+    const uint32_t* bvert;
     /* dense supersteps whose broadcasters' out-arcs are below live_pct % of
      * E: bit p of g_live is set when the record at position p was broadcast
      * this superstep, and the gather reads only those records */

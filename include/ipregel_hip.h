@@ -84,8 +84,10 @@ int ipr_hip_build_pull_bins(ipr_dev* dev, uint32_t small_max, uint32_t medium_ma
                             void* stream);
 
 /* record order: bpos[v] = rank of v by descending out-degree class
- * [2^k, 2^(k+1)), ids ascending within a class; no out-arcs last */
-int ipr_hip_build_record_order(uint64_t V, const uint64_t* out_idx, uint32_t* bpos, void* stream);
+ * [2^k, 2^(k+1)), ids ascending within a class; no out-arcs last. With bvert
+ * (may be NULL), also its inverse: bvert[bpos[v]] = v. */
+int ipr_hip_build_record_order(uint64_t V, const uint64_t* out_idx, uint32_t* bpos, uint32_t* bvert,
+                               void* stream);
 
 /* index of the arc-balanced gather over a device CSC (in-lists):
  * g_arc[E] = the sources (their record positions bpos[u] when bpos is not

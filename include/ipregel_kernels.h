@@ -706,6 +706,10 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_expand(ipr_dev d)
         n = ctl->fsize[s & 1];
         list = d.front[s & 1];
     }
+    /* a sweep over all V with records in out-degree order (SPREAD's superstep
+     * 0) takes i as a record position: the records are read in order, and
+     * only a broadcaster's vertex id (bvert) and offsets after */
+    const bool bypos = !list && d.bvert;
     const uint64_t tile_stride = (uint64_t)gridDim.x * IPR_BLOCK;
     for (uint64_t base = (uint64_t)blockIdx.x * IPR_BLOCK; base < n; base += tile_stride) {
         const uint64_t i = base + threadIdx.x;
@@ -722,7 +726,8 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_expand(ipr_dev d)
             }
             IP_MESSAGE_TYPE bm;
             bool wf;
-            if (ipr_bcast_at(d, ipr_brec(d, v), want, want4, &bm, &wf)) {
+            if (ipr_bcast_at(d, bypos ? (uint32_t)i : ipr_brec(d, v), want, want4, &bm, &wf)) {
+                if (bypos) v = d.bvert[i];
                 if (!list) {
                     off = d.out_idx[v];
                     end = d.out_idx[v + 1];

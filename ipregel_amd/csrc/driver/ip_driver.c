@@ -181,6 +181,7 @@ static void dev_free_all(void)
     ipr_hip_free((void*)d->g_twgt);
     ipr_hip_free((void*)d->g_tblk);
     ipr_hip_free((void*)d->bpos);
+    ipr_hip_free((void*)d->bvert);
     ipr_hip_free(d->g_live);
     ipr_hip_free(d->brec4);
     ipr_hip_free(d->bslot);
@@ -402,8 +403,10 @@ static void dev_setup(void)
          * reordering them by out-degree costs little there (PUSH scans all V) */
         if (var == IPR_VARIANT_SPREAD && env_u32("IPR_RECORD_ORDER", 1) == 1) {
             uint32_t* bpos = dalloc(vv * 4, "record positions");
-            hip_check(ipr_hip_build_record_order(V, d->out_idx, bpos, st), "record order");
+            uint32_t* bvert = dalloc(vv * 4, "record vertices");
+            hip_check(ipr_hip_build_record_order(V, d->out_idx, bpos, bvert, st), "record order");
             d->bpos = bpos;
+            d->bvert = bvert;
         }
         if (tiled) {
             /* the resident workgroups the LDS allows (one per CU with 128 KB

@@ -385,7 +385,8 @@ __global__ void k_scatter_pos(uint64_t V, const uint32_t* order, uint32_t* pos)
         pos[order[i]] = (uint32_t)i;
 }
 
-extern "C" int ipr_hip_build_record_order(uint64_t V, const uint64_t* out_idx, uint32_t* bpos, void* stream)
+extern "C" int ipr_hip_build_record_order(uint64_t V, const uint64_t* out_idx, uint32_t* bpos, uint32_t* bvert,
+                                          void* stream)
 {
     hipStream_t st = (hipStream_t)stream;
     if (V == 0) return 0;
@@ -411,6 +412,10 @@ extern "C" int ipr_hip_build_record_order(uint64_t V, const uint64_t* out_idx, u
     e = rocprim::radix_sort_pairs(tmp, tmp_bytes, kin, kout, vin, vout, (size_t)V, 0, 7, st);
     if (e != hipSuccess) { rc = fail(e, "record order sort"); goto out; }
     hipLaunchKernelGGL(k_scatter_pos, dim3((unsigned)vb), dim3(256), 0, st, V, vout, bpos);
+    if (bvert && (e = hipMemcpyAsync(bvert, vout, V * 4, hipMemcpyDeviceToDevice, st)) != hipSuccess) {
+        rc = fail(e, "record order inverse");
+        goto out;
+    }
     if ((e = hipGetLastError()) != hipSuccess) { rc = fail(e, "record order"); goto out; }
     if ((e = hipStreamSynchronize(st)) != hipSuccess) { rc = fail(e, "record order sync"); goto out; }
 out:
