@@ -41,7 +41,7 @@ __device__ __forceinline__ void ipr_coherent_store(uint64_t* p, uint64_t v)
  * wavefront, lane k owning slot k */
 __device__ inline uint64_t ipr_take(const ipr_dev& d, int id, bool clear)
 {
-    uint64_t* p = &d.cnt[id * IPR_CSLOTS + (threadIdx.x & 63)];
+    uint64_t* p = &d.cnt[IPR_CSLOT(id, threadIdx.x & 63)];
     uint64_t x = ipr_coherent_load(p);
     if (clear) ipr_coherent_store(p, 0);
     for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
@@ -52,7 +52,7 @@ __device__ inline uint64_t ipr_take(const ipr_dev& d, int id, bool clear)
  * 0..63 of one wavefront */
 __device__ inline uint64_t ipr_take_max(const ipr_dev& d, int id, bool clear)
 {
-    uint64_t* p = &d.cnt[id * IPR_CSLOTS + (threadIdx.x & 63)];
+    uint64_t* p = &d.cnt[IPR_CSLOT(id, threadIdx.x & 63)];
     uint64_t x = ipr_coherent_load(p);
     if (clear) ipr_coherent_store(p, 0);
     for (int o = 32; o > 0; o >>= 1) {
@@ -79,8 +79,8 @@ __device__ inline bool ipr_last_workgroup(const ipr_dev& d, uint32_t n)
         const uint32_t slot = blockIdx.x % IPR_CSLOTS;
         const uint32_t share = n / IPR_CSLOTS + (slot < n % IPR_CSLOTS ? 1u : 0u);
         const uint32_t slots = n < IPR_CSLOTS ? n : IPR_CSLOTS;
-        unsigned long long* sc = (unsigned long long*)&d.cnt[IPR_C_ARRIVE * IPR_CSLOTS + slot];
-        unsigned long long* top = (unsigned long long*)&d.cnt[IPR_C_ARRIVE_TOP * IPR_CSLOTS];
+        unsigned long long* sc = (unsigned long long*)&d.cnt[IPR_CSLOT(IPR_C_ARRIVE, slot)];
+        unsigned long long* top = (unsigned long long*)&d.cnt[IPR_CSLOT(IPR_C_ARRIVE_TOP, 0)];
         uint32_t last = 0;
         if (atomicAdd(sc, 1ull) + 1 == (unsigned long long)share) {
             atomicExch(sc, 0ull);
@@ -138,8 +138,9 @@ __device__ inline void ipr_finalize_step(const ipr_dev& d, int variant, int flag
             }
         }
         /* the gather's range counters start from 0 in the next superstep */
-        ipr_coherent_store(&d.cnt[IPR_C_GNEXT * IPR_CSLOTS + (threadIdx.x & 63)], 0);
-        ipr_coherent_store(&d.cnt[(IPR_C_GNEXT + 1) * IPR_CSLOTS + (threadIdx.x & 63)], 0);
+        ipr_coherent_store(&d.cnt[IPR_CSLOT(IPR_C_GNEXT, threadIdx.x & 63)], 0);
+        /* and the large-broadcaster registrations of the next superstep too */
+        ipr_coherent_store(&d.cnt[IPR_CSLOT(IPR_C_LPACK, threadIdx.x & 63)], 0);
     }
     if ((threadIdx.x & 63) != 0) return;
     if (variant == IPR_VARIANT_PULL) {
@@ -160,7 +161,6 @@ __device__ inline void ipr_finalize_step(const ipr_dev& d, int variant, int flag
         c->executed = s + 1;
         c->superstep = s + 1;
     }
-    c->lpack = 0;
     c->step_edges = edges;
     c->work_edges += edges;
     c->work_vertices += computed;

@@ -53,8 +53,12 @@ enum { IPR_BIN_SMALL = 0, IPR_BIN_MEDIUM = 1, IPR_BIN_LARGE = 2, IPR_NBINS = 3 }
 
 /* counters spread over IPR_CSLOTS slots (a workgroup adds its total to slot
  * blockIdx % IPR_CSLOTS) so that no single address takes every atomic; the
- * finalize kernel sums and clears them */
+ * finalize kernel sums and clears them. Device atomics on one cache line are
+ * performed one after another (about 12 ns each, whatever their address in
+ * the line), so every slot has a 128-byte line of its own: IPR_CPAD words. */
 #define IPR_CSLOTS 64
+#define IPR_CPAD 16
+#define IPR_CSLOT(id, slot) ((((uint64_t)(id)) * IPR_CSLOTS + (uint64_t)(slot)) * IPR_CPAD)
 enum {
     IPR_C_ACT0 = 0,      /* ActiveVertexCount, superstep parity 0 */
     IPR_C_ACT1 = 1,      /* ... parity 1 */
@@ -62,8 +66,8 @@ enum {
     IPR_C_EDGES = 3,     /* out-arcs of this superstep's broadcasters (W) */
     IPR_C_BCAST0 = 4,    /* broadcasts, superstep parity 0 */
     IPR_C_BCAST1 = 5,    /* ... parity 1 */
-    IPR_C_GNEXT = 6,     /* gather range counters: IPR_GATHER_CTRS of them, 128 bytes apart
-                          * (spans this id and the next; zeroed at every superstep's end) */
+    IPR_C_GNEXT = 6,     /* gather range counters: IPR_GATHER_CTRS of them, one per slot
+                          * (zeroed at every superstep's end); id 7 is unused */
     IPR_C_ARRIVE = 8,    /* workgroups of the running kernel that have finished, by
                           * blockIdx mod 64, for kernels whose last workgroup ends
                           * the superstep ... */
@@ -72,7 +76,9 @@ enum {
     IPR_C_FMAX = 11,     /* spread: its largest out-degree (per-slot maxima) */
     IPR_C_FNEXT = 12,    /* spread: the next frontier's size, counted by the fused
                           * sparse kernel as its deliveries activate receivers */
-    IPR_NCNT = 13
+    IPR_C_LPACK = 13,    /* large broadcasters registered in slot k: entries << IPR_LPACK_SHIFT
+                          * | arc chunks; slot k's entries are lbins/lbase[k * lcap ...] */
+    IPR_NCNT = 14
 };
 
 /* device-resident control block: every kernel reads the superstep and the
@@ -83,7 +89,6 @@ typedef struct ipr_ctl {
     uint32_t done;               /* set by the finalize kernel when the loop ends */
     uint32_t error;              /* device-side API misuse flags */
     uint64_t fsize[2];           /* frontier list sizes by parity */
-    uint64_t lpack;              /* large broadcasters: entries << 40 | arc chunks */
     uint64_t pull_mode;          /* push apps: 1 when the last finalized superstep gathered */
     uint64_t step_edges;         /* W of the last finalized superstep */
     uint64_t work_edges;         /* W accumulated */
@@ -157,7 +162,7 @@ typedef struct ipr_gather_tiles {
 
 typedef struct ipr_dev {
     ipr_ctl* ctl;
-    uint64_t* cnt;               /* IPR_NCNT x IPR_CSLOTS counters */
+    uint64_t* cnt;               /* IPR_NCNT x IPR_CSLOTS counters, IPR_CPAD words apart */
     uint64_t V, E;
     const uint64_t* out_idx;
     const uint32_t* out_adj;
@@ -237,8 +242,9 @@ typedef struct ipr_dev {
     void* mbox[2];
     uint8_t* recv[2];
     uint32_t* front[2];
-    uint32_t* lbins;             /* large broadcasters of this superstep */
-    uint64_t* lbase;             /* their first arc chunk */
+    uint32_t* lbins;             /* large broadcasters of this superstep, IPR_CSLOTS regions of lcap */
+    uint64_t* lbase;             /* their first arc chunk within their slot */
+    uint64_t lcap;               /* vertices above large_min out-arcs */
     void* bslot;                 /* push variants: {superstep+1 | weighted<<31, message} */
     /* static pull / gather bins over all V (by in-degree), built at load */
     uint32_t* pbins[IPR_NBINS];

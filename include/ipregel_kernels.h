@@ -136,7 +136,7 @@ __device__ inline void ipr_block_count3(const ipr_dev& d, int ia, uint64_t a, in
         for (int w = 0; w < IPR_WAVES; w++) t += red[threadIdx.x][w];
         int id = threadIdx.x == 0 ? ia : (threadIdx.x == 1 ? ib : ic);
         if (t && id >= 0)
-            atomicAdd((unsigned long long*)&d.cnt[id * IPR_CSLOTS + (blockIdx.x % IPR_CSLOTS)],
+            atomicAdd((unsigned long long*)&d.cnt[IPR_CSLOT(id, blockIdx.x % IPR_CSLOTS)],
                       (unsigned long long)t);
     }
     __syncthreads();
@@ -325,7 +325,7 @@ struct ipr_bslot_t {
  * by the same rule. */
 __device__ inline uint64_t ipr_bcast_arcs(const ipr_dev& d)
 {
-    uint64_t x = d.cnt[IPR_C_EDGES * IPR_CSLOTS + ipr_lane()];
+    uint64_t x = d.cnt[IPR_CSLOT(IPR_C_EDGES, ipr_lane())];
     for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
     return x;
 }
@@ -343,7 +343,7 @@ __device__ inline bool ipr_gathers_live(const ipr_dev& d)
 {
     if (!d.dir_opt || !d.g_live) return false;
     const uint64_t w = ipr_bcast_arcs(d);
-    uint64_t n = d.cnt[IPR_C_WORKV * IPR_CSLOTS + ipr_lane()];
+    uint64_t n = d.cnt[IPR_CSLOT(IPR_C_WORKV, ipr_lane())];
     for (int o = 32; o > 0; o >>= 1) n += __shfl_xor(n, o);
     return w * (uint64_t)d.dir_alpha > d.E && w * 100 < d.E * (uint64_t)d.live_pct &&
            n * 100 < d.V * (uint64_t)d.live_vpct;
@@ -534,6 +534,185 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_init(ipr_dev d)
 /* push variants: compute                                                    */
 /* ------------------------------------------------------------------------ */
 
+/* vertices per thread per round of the compute kernel. The sweep is bound by
+ * load latency, not bytes: one vertex at a time, a wavefront waits for its
+ * id, then its state, then its broadcast bookkeeping before the next vertex
+ * starts. U at a time, each of those loads is issued for all U vertices
+ * before the first is used. No branch may separate them: at a join the
+ * compiler waits for every load in flight, so indices past the end are
+ * clamped to the last entry (their results are dropped and their stores
+ * skipped), and the list or full-sweep choice is a template parameter. */
+#ifndef IPR_COMPUTE_U
+#define IPR_COMPUTE_U 4
+#endif
+static_assert(IPR_COMPUTE_U >= 1 && IPR_COMPUTE_U <= 8, "IPR_COMPUTE_U out of range");
+
+/* one round of U vertices per thread: entries base + u*IPR_BLOCK + thread
+ * (coalesced per u) of the frontier list (LIST) or of 0..V-1. SPREAD and
+ * SPREAD_PULL read a list exactly in the supersteps after the first, so
+ * LIST stands for s > 0 there. */
+template <bool LIST>
+__device__ __forceinline__ void ipr_compute_round(const ipr_dev& d, uint64_t s, const uint32_t* list, uint64_t n,
+                                                  uint64_t base, uint64_t& n_active, uint64_t& n_computed,
+                                                  uint64_t& n_edges)
+{
+    constexpr int U = IPR_COMPUTE_U;
+    ipr_slot_t* mcur = (ipr_slot_t*)d.mbox[s & 1];
+    ipr_bslot_t* bs = (ipr_bslot_t*)d.bslot;
+    const uint32_t stamp4 = (uint32_t)(s % IPR_REC4_WINDOW) + 1u;
+    /* the ids */
+    uint32_t vs[U];
+    bool run[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const uint64_t i = base + (uint64_t)u * IPR_BLOCK + threadIdx.x;
+        run[u] = i < n;
+        const uint64_t ic = run[u] ? i : n - 1;   /* base < n, so n >= 1 */
+        vs[u] = LIST ? list[ic] : (uint32_t)ic;
+    }
+    /* the state: message, value, and a listed vertex's broadcast bookkeeping
+     * with it (most frontier vertices broadcast) */
+    struct ip_vertex_t xs[U];
+    bool hm_before[U];
+    uint64_t o0[U], o1[U];
+    uint32_t rp[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const uint32_t v = vs[u];
+        struct ip_vertex_t& x = xs[u];
+        o0[u] = 0;
+        o1[u] = 0;
+        rp[u] = 0;
+        if (IPR_VARIANT == IPR_VARIANT_SPREAD) {
+            /* after the first superstep every listed vertex has a message,
+             * read straight from its mailbox (reset below) */
+            x.active = true;
+            x.has_message = LIST;
+            if (LIST) x.message = ipr_policy::value(mcur[v]);
+        } else if (IPR_VARIANT == IPR_VARIANT_SPREAD_PULL) {
+            /* targets: message fetched at the end of the previous superstep */
+            x.active = true;
+            x.has_message = d.hm[v] != 0;
+            if (x.has_message) x.message = ((const IP_MESSAGE_TYPE*)d.msg)[v];
+        } else {
+            /* PUSH: a full sweep that runs only the vertices with active or
+             * has_message; the others' values are not loaded */
+            x.active = d.act[v] != 0;
+            x.has_message = d.hm[v] != 0;
+            if (x.has_message) x.message = ((const IP_MESSAGE_TYPE*)d.msg)[v];
+            run[u] = run[u] && (x.active || x.has_message);
+            x.active = true;
+            if (!run[u]) continue;
+        }
+        if (LIST) {
+            o0[u] = d.out_idx[v];
+            o1[u] = d.out_idx[v + 1];
+            rp[u] = ipr_brec(d, v);
+        }
+        ipr_fill_vertex(d, &x, v);
+        hm_before[u] = x.has_message;
+    }
+    if (IPR_VARIANT == IPR_VARIANT_SPREAD && LIST) {
+        /* mailboxes are reset for reuse two supersteps on */
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (run[u]) mcur[vs[u]] = ipr_policy::empty();
+    }
+    /* the computes, then their value stores: a store between two computes
+     * would make the compiler reload any application global the next one
+     * reads (a store may alias it), and wait for the store with it */
+    IP_VALUE_TYPE before[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        if (!run[u]) continue;
+        before[u] = xs[u].value;
+        ip_compute(&xs[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        if (!run[u]) continue;
+        const uint32_t v = vs[u];
+        struct ip_vertex_t& x = xs[u];
+        ipr_store_value(d, v, before[u], x.value);
+        n_computed++;
+        if (IPR_VARIANT == IPR_VARIANT_PUSH) {
+            if (!x.active) d.act[v] = 0;
+            else if (!d.act[v]) d.act[v] = 1;
+            if (hm_before[u] != x.has_message) d.hm[v] = x.has_message;
+            n_active += x.active;
+        }
+    }
+    /* a full sweep loads the bookkeeping only for the vertices that
+     * broadcast, all of them before the first record is stored */
+    if (!LIST) {
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (run[u] && xs[u].ipr_bcast) {
+                o0[u] = d.out_idx[vs[u]];
+                o1[u] = d.out_idx[vs[u] + 1];
+                rp[u] = ipr_brec(d, vs[u]);
+            }
+    }
+    /* the records: every figure first, then the stores (a store before the
+     * next vertex's bookkeeping is read would be waited for with it) */
+    uint32_t large[U], chunks[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        large[u] = 0;
+        chunks[u] = 0;
+        if (!run[u] || !xs[u].ipr_bcast) continue;
+        /* recorded even without out-arcs: with directed=false a SPREAD_PULL
+         * in-list may still name this vertex */
+        const uint64_t deg = o1[u] - o0[u];
+        n_edges += deg;
+        if (deg > d.large_min) {
+            large[u] = 1;
+            chunks[u] = (uint32_t)((deg + d.large_chunk - 1) / d.large_chunk);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        if (!run[u] || !xs[u].ipr_bcast) continue;
+        const struct ip_vertex_t& x = xs[u];
+        ipr_bslot_t rec;
+        rec.step = (uint32_t)(s + 1) | (x.ipr_bcast_weighted ? IPR_BSTEP_WEIGHTED : 0u);
+        rec.msg = x.ipr_bcast_msg;
+        if (d.brec4) {
+            const uint32_t r4 = ipr_rec4(stamp4, rec.msg, x.ipr_bcast_weighted);
+            d.brec4[rp[u]] = r4;
+            if ((r4 & IPR_REC4_ESC) == IPR_REC4_ESC) bs[rp[u]] = rec;
+        } else {
+            bs[rp[u]] = rec;
+        }
+    }
+    /* large broadcasters of this wavefront take consecutive entries and
+     * consecutive chunk ranges of one slot (wavefront index mod IPR_CSLOTS)
+     * with one atomic per u: slots on lines of their own keep the atomics
+     * from queueing on one address (no workgroup barrier: a barrier would
+     * wait for this round's stores before the next round) */
+    const uint32_t lslot = (blockIdx.x * IPR_WAVES + threadIdx.x / IPR_WAVE) % IPR_CSLOTS;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const uint64_t lmask = __ballot(large[u]);
+        if (!lmask) continue;
+        const uint32_t lane = ipr_lane();
+        const uint32_t el = (uint32_t)__popcll(lmask & ((1ull << lane) - 1ull));
+        const uint32_t ci = ipr_wave_incl_scan(chunks[u]);
+        const uint32_t ec = ci - chunks[u];
+        const uint32_t tl = (uint32_t)__popcll(lmask), tc = __shfl(ci, IPR_WAVE - 1);
+        unsigned long long lb = 0;
+        if (lane == 0)
+            lb = atomicAdd((unsigned long long*)&d.cnt[IPR_CSLOT(IPR_C_LPACK, lslot)],
+                           ((unsigned long long)tl << IPR_LPACK_SHIFT) | tc);
+        lb = __shfl(lb, 0);
+        if (large[u]) {
+            const uint64_t e = (uint64_t)lslot * d.lcap + (lb >> IPR_LPACK_SHIFT) + el;
+            d.lbins[e] = vs[u];
+            d.lbase[e] = (lb & IPR_LPACK_MASK) + ec;
+        }
+    }
+}
+
 /* SPREAD: src/combiner_spread_postamble.h:227-276 (superstep 0 over all V,
  * then the frontier) and :322-340 (the frontier's mailboxes become messages:
  * read straight from the mailbox, which is reset for reuse two supersteps
@@ -552,108 +731,15 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_compute_push(ipr_dev d)
         n = ctl->fsize[s & 1];
         list = d.front[s & 1];
     }
-    ipr_slot_t* mcur = (ipr_slot_t*)d.mbox[s & 1];
-    ipr_bslot_t* bs = (ipr_bslot_t*)d.bslot;
-    const uint32_t stamp4 = (uint32_t)(s % IPR_REC4_WINDOW) + 1u;
     uint64_t n_active = 0, n_computed = 0, n_edges = 0;
-    const uint64_t tile_stride = (uint64_t)gridDim.x * IPR_BLOCK;
-    for (uint64_t base = (uint64_t)blockIdx.x * IPR_BLOCK; base < n; base += tile_stride) {
-        const uint64_t i = base + threadIdx.x;
-        uint32_t large = 0, chunks = 0, entry = 0;
-        if (i < n) {
-            uint32_t v = list ? list[i] : (uint32_t)i;
-            struct ip_vertex_t x;
-            bool run = true;
-            if (IPR_VARIANT == IPR_VARIANT_SPREAD) {
-                x.active = true;
-                x.has_message = s > 0;
-                if (s > 0) {
-                    x.message = ipr_policy::value(mcur[v]);
-                    mcur[v] = ipr_policy::empty();
-                }
-            } else if (IPR_VARIANT == IPR_VARIANT_SPREAD_PULL) {
-                /* targets: message fetched at the end of the previous superstep */
-                x.active = true;
-                x.has_message = d.hm[v] != 0;
-                if (x.has_message) x.message = ((const IP_MESSAGE_TYPE*)d.msg)[v];
-            } else {
-                x.active = d.act[v] != 0;
-                x.has_message = d.hm[v] != 0;
-                if (x.has_message) x.message = ((const IP_MESSAGE_TYPE*)d.msg)[v];
-                run = x.active || x.has_message;
-                x.active = true;
-            }
-            if (run) {
-                /* a frontier vertex's broadcast bookkeeping is loaded with its
-                 * own state (most frontier vertices broadcast); a full sweep
-                 * loads it only for the vertices that do */
-                uint64_t o0 = 0, o1 = 0;
-                uint32_t rp = 0;
-                if (list) {
-                    o0 = d.out_idx[v];
-                    o1 = d.out_idx[v + 1];
-                    rp = ipr_brec(d, v);
-                }
-                ipr_fill_vertex(d, &x, v);
-                const IP_VALUE_TYPE before = x.value;
-                const bool hm_before = x.has_message;
-                ip_compute(&x);
-                ipr_store_value(d, v, before, x.value);
-                n_computed++;
-                if (IPR_VARIANT == IPR_VARIANT_PUSH) {
-                    if (!x.active) d.act[v] = 0;
-                    else if (!d.act[v]) d.act[v] = 1;
-                    if (hm_before != x.has_message) d.hm[v] = x.has_message;
-                    n_active += x.active;
-                }
-                if (x.ipr_bcast) {
-                    if (!list) {
-                        o0 = d.out_idx[v];
-                        o1 = d.out_idx[v + 1];
-                        rp = ipr_brec(d, v);
-                    }
-                    /* recorded even without out-arcs: with directed=false a
-                     * SPREAD_PULL in-list may still name this vertex */
-                    uint64_t deg = o1 - o0;
-                    n_edges += deg;
-                    ipr_bslot_t rec;
-                    rec.step = (uint32_t)(s + 1) | (x.ipr_bcast_weighted ? IPR_BSTEP_WEIGHTED : 0u);
-                    rec.msg = x.ipr_bcast_msg;
-                    if (d.brec4) {
-                        const uint32_t r4 = ipr_rec4(stamp4, rec.msg, x.ipr_bcast_weighted);
-                        d.brec4[rp] = r4;
-                        if ((r4 & IPR_REC4_ESC) == IPR_REC4_ESC) bs[rp] = rec;
-                    } else {
-                        bs[rp] = rec;
-                    }
-                    if (deg > d.large_min) {
-                        large = 1;
-                        chunks = (uint32_t)((deg + d.large_chunk - 1) / d.large_chunk);
-                        entry = v;
-                    }
-                }
-            }
-        }
-        /* large broadcasters of this wavefront take consecutive list slots
-         * and consecutive chunk ranges with one atomic (no workgroup barrier:
-         * a barrier would wait for this tile's stores before the next tile) */
-        const uint64_t lmask = __ballot(large);
-        if (lmask) {
-            const uint32_t lane = ipr_lane();
-            const uint32_t el = (uint32_t)__popcll(lmask & ((1ull << lane) - 1ull));
-            const uint32_t ci = ipr_wave_incl_scan(chunks);
-            const uint32_t ec = ci - chunks;
-            const uint32_t tl = (uint32_t)__popcll(lmask), tc = __shfl(ci, IPR_WAVE - 1);
-            unsigned long long lb = 0;
-            if (lane == 0)
-                lb = atomicAdd((unsigned long long*)&ctl->lpack, ((unsigned long long)tl << IPR_LPACK_SHIFT) | tc);
-            lb = __shfl(lb, 0);
-            if (large) {
-                const uint64_t slot = (lb >> IPR_LPACK_SHIFT) + el;
-                d.lbins[slot] = entry;
-                d.lbase[slot] = (lb & IPR_LPACK_MASK) + ec;
-            }
-        }
+    const uint64_t per_round = (uint64_t)IPR_BLOCK * IPR_COMPUTE_U;
+    const uint64_t stride = (uint64_t)gridDim.x * per_round;
+    if (list) {
+        for (uint64_t base = (uint64_t)blockIdx.x * per_round; base < n; base += stride)
+            ipr_compute_round<true>(d, s, list, n, base, n_active, n_computed, n_edges);
+    } else {
+        for (uint64_t base = (uint64_t)blockIdx.x * per_round; base < n; base += stride)
+            ipr_compute_round<false>(d, s, list, n, base, n_active, n_computed, n_edges);
     }
     ipr_block_count3(d, IPR_C_ACT0 + (int)(s & 1), n_active, IPR_C_WORKV, n_computed, IPR_C_EDGES, n_edges);
 }
@@ -678,21 +764,36 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_expand(ipr_dev d)
     const uint32_t want = (uint32_t)(s + 1);
     const uint32_t want4 = (uint32_t)(s % IPR_REC4_WINDOW) + 1u;
 
-    /* large broadcasters: one workgroup per chunk of large_chunk arcs */
-    const uint64_t lp = ctl->lpack;
-    const uint64_t nL = lp >> IPR_LPACK_SHIFT, nC = lp & IPR_LPACK_MASK;
+    /* large broadcasters: one workgroup per chunk of large_chunk arcs. They
+     * registered in IPR_CSLOTS slots; the chunks of slot k follow those of
+     * slots 0..k-1, so every wavefront scans the 64 slot totals (lane k
+     * holding slot k) and finds a chunk's slot by ballot */
+    const uint32_t lane = ipr_lane();
+    const uint64_t mine = d.lcap ? d.cnt[IPR_CSLOT(IPR_C_LPACK, lane)] : 0;
+    const uint64_t my_chunks = mine & IPR_LPACK_MASK;
+    uint64_t incl = my_chunks;
+    for (uint32_t o = 1; o < IPR_WAVE; o <<= 1) {
+        const uint64_t y = __shfl_up(incl, o);
+        if (lane >= o) incl += y;
+    }
+    const uint64_t nC = __shfl(incl, IPR_WAVE - 1);
     for (uint64_t c = blockIdx.x; c < nC; c += gridDim.x) {
-        uint64_t lo = 0, hi = nL;            /* last j with lbase[j] <= c */
+        const uint32_t k = (uint32_t)__ffsll((unsigned long long)__ballot(incl > c)) - 1;
+        const uint64_t ck = c - (__shfl(incl, k) - __shfl(my_chunks, k));   /* chunk within slot k */
+        const uint64_t nL = __shfl(mine, k) >> IPR_LPACK_SHIFT;
+        const uint32_t* lbins = d.lbins + (uint64_t)k * d.lcap;
+        const uint64_t* lbase = d.lbase + (uint64_t)k * d.lcap;
+        uint64_t lo = 0, hi = nL;            /* last j with lbase[j] <= ck */
         while (hi - lo > 1) {
             uint64_t mid = (lo + hi) / 2;
-            if (d.lbase[mid] <= c) lo = mid; else hi = mid;
+            if (lbase[mid] <= ck) lo = mid; else hi = mid;
         }
-        uint32_t v = d.lbins[lo];
+        uint32_t v = lbins[lo];
         IP_MESSAGE_TYPE bm;
         bool wf;
         ipr_bcast_at(d, ipr_brec(d, v), want, want4, &bm, &wf);   /* a listed broadcaster */
         uint64_t off = d.out_idx[v], deg = d.out_idx[v + 1] - off;
-        uint64_t beg = (c - d.lbase[lo]) * d.large_chunk;
+        uint64_t beg = (ck - lbase[lo]) * d.large_chunk;
         uint64_t end = beg + d.large_chunk < deg ? beg + d.large_chunk : deg;
         for (uint64_t k = beg + threadIdx.x; k < end; k += IPR_BLOCK)
             ipr_emit(d, s, d.out_adj[off + k], ipr_edge_msg(d.out_wgt, bm, wf, off + k));
@@ -954,8 +1055,8 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_sparse_step(ipr_dev d)
             uint64_t a = 0, m = 0;
             for (int w = 0; w < IPR_WAVES; w++) { a += r_arcs[w]; m = r_max[w] > m ? r_max[w] : m; }
             const uint32_t slot = blockIdx.x % IPR_CSLOTS;
-            if (a) atomicAdd((unsigned long long*)&d.cnt[IPR_C_FARCS * IPR_CSLOTS + slot], (unsigned long long)a);
-            if (m) atomicMax((unsigned long long*)&d.cnt[IPR_C_FMAX * IPR_CSLOTS + slot], (unsigned long long)m);
+            if (a) atomicAdd((unsigned long long*)&d.cnt[IPR_CSLOT(IPR_C_FARCS, slot)], (unsigned long long)a);
+            if (m) atomicMax((unsigned long long*)&d.cnt[IPR_CSLOT(IPR_C_FMAX, slot)], (unsigned long long)m);
         }
     }
     /* the last workgroup to finish ends the superstep; the next frontier is
@@ -1075,12 +1176,23 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_gather_live(ipr_dev d)
     const ipr_bslot_t* bs = (const ipr_bslot_t*)d.bslot;
     const uint64_t words = (d.V + 63) / 64;
     const uint64_t nw = (uint64_t)gridDim.x * IPR_WAVES;
-    for (uint64_t wd = (uint64_t)blockIdx.x * IPR_WAVES + threadIdx.x / IPR_WAVE; wd < words; wd += nw) {
-        const uint64_t p = wd * 64 + ipr_lane();
-        const bool on = p < d.V && (d.brec4 ? (d.brec4[p] >> IPR_REC4_SHIFT) == want4
-                                            : (bs[p].step & ~IPR_BSTEP_WEIGHTED) == want);
-        const uint64_t m = __ballot(on);
-        if (ipr_lane() == 0) d.g_live[wd] = m;
+    /* a wavefront covers many words: the stamps of U words are loaded
+     * before the first ballot, U loads in flight per lane instead of one */
+    constexpr int U = 8;
+    for (uint64_t w0 = (uint64_t)blockIdx.x * IPR_WAVES + threadIdx.x / IPR_WAVE; w0 < words; w0 += nw * U) {
+        bool on[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t p = (w0 + u * nw) * 64 + ipr_lane();
+            on[u] = p < d.V && (d.brec4 ? (d.brec4[p] >> IPR_REC4_SHIFT) == want4
+                                        : (bs[p].step & ~IPR_BSTEP_WEIGHTED) == want);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t wd = w0 + u * nw;
+            const uint64_t m = __ballot(on[u]);
+            if (ipr_lane() == 0 && wd < words) d.g_live[wd] = m;
+        }
     }
 }
 
@@ -1150,7 +1262,7 @@ __global__ void __launch_bounds__(IPR_GATHER_WAVES * IPR_WAVE, ipr_gather_min_wa
      * end) */
     const uint32_t G = gridDim.x < IPR_GATHER_CTRS ? gridDim.x : IPR_GATHER_CTRS;
     const uint32_t gc = blockIdx.x % G;
-    unsigned long long* ctr = (unsigned long long*)&d.cnt[IPR_C_GNEXT * IPR_CSLOTS + gc * 16];
+    unsigned long long* ctr = (unsigned long long*)&d.cnt[IPR_CSLOT(IPR_C_GNEXT, gc)];
     /* arcs split by record line (g_xcd): workgroup b takes the ranges of class
      * b mod classes from that class's counter (workgroups reach the XCDs in
      * turn, so an XCD reads one share of the record lines), then helps with
@@ -1164,7 +1276,7 @@ __global__ void __launch_bounds__(IPR_GATHER_WAVES * IPR_WAVE, ipr_gather_min_wa
             while (ctried < IPR_GATHER_CLASSES) {
                 unsigned long long k = 0;
                 if (lane == 0)
-                    k = atomicAdd((unsigned long long*)&d.cnt[IPR_C_GNEXT * IPR_CSLOTS + cls * 16], 1ull);
+                    k = atomicAdd((unsigned long long*)&d.cnt[IPR_CSLOT(IPR_C_GNEXT, cls)], 1ull);
                 k = __shfl(k, 0);
                 w0 = d.g_cbeg[cls] + (uint64_t)k * per;
                 if (w0 < d.g_cend[cls]) {
@@ -1414,7 +1526,7 @@ __global__ void __launch_bounds__(IPR_TILE_THREADS) ipr_k_gather_tiled(ipr_dev d
         const uint32_t* b4 = d.brec4;
         ipr_slot_t* mb = (ipr_slot_t*)d.mbox[(s + 1) & 1];
         uint8_t* rv = d.recv[(s + 1) & 1];
-        unsigned long long* ctr = (unsigned long long*)&d.cnt[IPR_C_GNEXT * IPR_CSLOTS];
+        unsigned long long* ctr = (unsigned long long*)&d.cnt[IPR_CSLOT(IPR_C_GNEXT, 0)];
         /* the wavefront index and the tile number are uniform: read them
          * through readfirstlane so that the tile's fields, the pass bounds and
          * the group bases live in scalar registers (scalar loads, scalar
@@ -1903,6 +2015,11 @@ static void ipr_bin_blocks(const ipr_dev& d, uint32_t nb[3])
     if (nb[0] + nb[1] + nb[2] == 0) nb[0] = 1;
 }
 
+/* the compute kernel's resident workgroups on this device (0 = unknown): its
+ * grid is capped there, since workgroups past one resident round would run
+ * in a second round after the first has finished its loop */
+static uint32_t ipr_compute_resident = 0;
+
 static int ipr_app_launch(int kernel, const ipr_dev* dp, void* stream)
 {
     hipStream_t st = (hipStream_t)stream;
@@ -1914,10 +2031,12 @@ static int ipr_app_launch(int kernel, const ipr_dev* dp, void* stream)
     case IPR_K_INIT:
         hipLaunchKernelGGL(ipr_k_init, dim3(vblocks), dim3(IPR_BLOCK), 0, st, d);
         break;
-    case IPR_K_COMPUTE:
-        hipLaunchKernelGGL(ipr_k_compute_push, dim3(vblocks < gblocks ? vblocks : gblocks),
-                           dim3(IPR_BLOCK), 0, st, d);
+    case IPR_K_COMPUTE: {
+        uint32_t cb = vblocks < gblocks ? vblocks : gblocks;
+        if (ipr_compute_resident && cb > ipr_compute_resident) cb = ipr_compute_resident;
+        hipLaunchKernelGGL(ipr_k_compute_push, dim3(cb), dim3(IPR_BLOCK), 0, st, d);
         break;
+    }
     case IPR_K_EXPAND:
         hipLaunchKernelGGL(ipr_k_expand, dim3(gblocks), dim3(IPR_BLOCK), 0, st, d);
         break;
@@ -1937,9 +2056,11 @@ static int ipr_app_launch(int kernel, const ipr_dev* dp, void* stream)
                                dim3(IPR_GATHER_WAVES * IPR_WAVE), 0, st, d);
         break;
     case IPR_K_GATHER_LIVE: {
-        /* a small grid: launched every superstep, it usually exits at once */
+        /* launched every superstep, it usually exits at once; when it runs,
+         * its 4-byte loads need about eight wavefronts per SIMD in flight
+         * (2048 workgroups: 512 left it at a third of the read rate) */
         uint64_t wb = ((d.V + 63) / 64 + IPR_WAVES - 1) / IPR_WAVES;
-        hipLaunchKernelGGL(ipr_k_gather_live, dim3((uint32_t)(wb < 1 ? 1 : (wb > 512 ? 512 : wb))),
+        hipLaunchKernelGGL(ipr_k_gather_live, dim3((uint32_t)(wb < 1 ? 1 : (wb > 2048 ? 2048 : wb))),
                            dim3(IPR_BLOCK), 0, st, d);
         break;
     }
@@ -1972,7 +2093,16 @@ static int ipr_app_launch(int kernel, const ipr_dev* dp, void* stream)
 static int ipr_app_bind(const ipr_dev* d)
 {
     hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(ipr_d_dev), d, sizeof(ipr_dev), 0, hipMemcpyHostToDevice);
-    return e == hipSuccess ? 0 : -(int)e;
+    if (e != hipSuccess) return -(int)e;
+    int dev = 0, cus = 0, occ = 0;
+    e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(ipr_k_compute_push),
+                                                         IPR_BLOCK, 0);
+    if (e != hipSuccess) return -(int)e;
+    ipr_compute_resident = occ > 0 && cus > 0 ? (uint32_t)occ * (uint32_t)cus : 0u;
+    return 0;
 }
 
 /* the declared combiner must agree with ip_combine on sample pairs */

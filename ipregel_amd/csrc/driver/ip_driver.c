@@ -326,14 +326,10 @@ static void dev_setup(void)
         if (var == IPR_VARIANT_SPREAD || var == IPR_VARIANT_SPREAD_PULL)
             d->front[i] = dalloc(vv * 4, "frontier");
     }
-    if (records) {
-        d->lbins = dalloc(vv * 4, "large broadcasters");
-        d->bslot = dalloc(vv * a->bslot_size, "broadcast records");
-        d->lbase = dalloc(vv * 8, "large-bin chunk bases");
-    }
+    if (records) d->bslot = dalloc(vv * a->bslot_size, "broadcast records");
     d->ctl = dalloc(sizeof(ipr_ctl), "control block");
-    d->cnt = dalloc(IPR_NCNT * IPR_CSLOTS * 8, "counters");
-    hip_check(ipr_hip_fill(d->cnt, 0, IPR_NCNT * IPR_CSLOTS * 8, st), "counters init");
+    d->cnt = dalloc(IPR_NCNT * IPR_CSLOTS * IPR_CPAD * 8, "counters");
+    hip_check(ipr_hip_fill(d->cnt, 0, IPR_NCNT * IPR_CSLOTS * IPR_CPAD * 8, st), "counters init");
     d->trace_active = dalloc(IPR_TRACE_CAP * 8, "trace");
     d->trace_ts = dalloc((IPR_TRACE_CAP + 1) * 8, "trace");
     d->trace_edges = dalloc(IPR_TRACE_CAP * 8, "trace");
@@ -345,6 +341,18 @@ static void dev_setup(void)
     /* tuning */
     const ipr_harness_cfg* c = D.have_cfg ? &D.cfg : NULL;
     d->large_min = c && c->bin_medium_max ? c->bin_medium_max : env_u32("IPR_LARGE_MIN", 2048);
+    if (records) {
+        /* broadcasters above large_min out-arcs register in one of IPR_CSLOTS
+         * slots; each slot has room for all of them (a vertex computes at most
+         * once per superstep) */
+        uint64_t nl = 0;
+        for (uint64_t v = 0; v < V; v++) nl += D.h_idx[v + 1] - D.h_idx[v] > d->large_min;
+        d->lcap = nl;
+        if (nl) {
+            d->lbins = dalloc(nl * IPR_CSLOTS * 4, "large broadcasters");
+            d->lbase = dalloc(nl * IPR_CSLOTS * 8, "large-bin chunk bases");
+        }
+    }
     d->grid_blocks = c && c->grid_blocks ? c->grid_blocks : env_u32("IPR_GRID_BLOCKS", 2048);
     {
         uint32_t cus = 256;

@@ -1033,7 +1033,7 @@ __global__ void __launch_bounds__(256) k_compact(ipr_dev d, int fin_variant, int
     uint64_t f_arcs = 0, f_max = 0;
     bool want_f = d.fused && !relist;
     if (want_f && d.dir_opt) {
-        uint64_t w = d.cnt[IPR_C_EDGES * IPR_CSLOTS + (threadIdx.x & 63)];
+        uint64_t w = d.cnt[IPR_CSLOT(IPR_C_EDGES, threadIdx.x & 63)];
         for (int o = 32; o > 0; o >>= 1) w += __shfl_xor(w, o);
         want_f = w * (uint64_t)d.dir_alpha <= d.E;
     }
@@ -1136,8 +1136,8 @@ __global__ void __launch_bounds__(256) k_compact(ipr_dev d, int fin_variant, int
             /* spread over the 64 slots; the end of the superstep collapses
              * them into ctl->farcs / fmaxdeg */
             const uint32_t slot = blockIdx.x % IPR_CSLOTS;
-            if (a) atomicAdd((unsigned long long*)&d.cnt[IPR_C_FARCS * IPR_CSLOTS + slot], (unsigned long long)a);
-            if (m) atomicMax((unsigned long long*)&d.cnt[IPR_C_FMAX * IPR_CSLOTS + slot], (unsigned long long)m);
+            if (a) atomicAdd((unsigned long long*)&d.cnt[IPR_CSLOT(IPR_C_FARCS, slot)], (unsigned long long)a);
+            if (m) atomicMax((unsigned long long*)&d.cnt[IPR_CSLOT(IPR_C_FMAX, slot)], (unsigned long long)m);
         }
     }
     /* after a fused sparse superstep: the last workgroup ends the superstep
@@ -1249,7 +1249,7 @@ extern "C" int ipr_hip_equal_noalloc(const void* a, const void* b, uint64_t byte
 
 __global__ void k_begin(ipr_dev d)
 {
-    for (uint32_t i = threadIdx.x; i < IPR_NCNT * IPR_CSLOTS; i += blockDim.x) d.cnt[i] = 0;
+    for (uint32_t i = threadIdx.x; i < IPR_NCNT * IPR_CSLOTS * IPR_CPAD; i += blockDim.x) d.cnt[i] = 0;
     if (threadIdx.x != 0) return;
     ipr_ctl* c = d.ctl;
     uint64_t cap = c->trace_cap;

@@ -119,12 +119,29 @@ def test_weighted_sssp_medium(name):
 
 @pytest.mark.parametrize("tune", [
     dict(use_graph=2), dict(max_chunk=1), dict(bin_small_max=1, bin_medium_max=2),
-    dict(bin_small_max=4096, bin_medium_max=8192), dict(grid_blocks=7), dict(dir_opt=2)])
+    dict(bin_small_max=4096, bin_medium_max=8192), dict(grid_blocks=7), dict(dir_opt=2),
+    dict(dir_opt=2, bin_small_max=1, bin_medium_max=2)])
 def test_cc_tuning_does_not_change_results(tune):
     g = G.rmat(14, 16, seed=9, symmetrise=True)
     ref = O.min_ancestor(g)
     r = EN.run_app("cc_spread", g, **tune)
     assert np.array_equal(r.values, ref)
+
+
+@pytest.mark.parametrize("large_min,chunk", [(2, 64), (40, 100), (300, 64)])
+def test_large_broadcasters_push(large_min, chunk):
+    """Every superstep pushed (no gather), broadcasters above large_min
+    out-arcs split into chunks of `chunk` arcs: they register in the 64
+    counter slots and the expansion finds each chunk's slot and entry.
+    Weighted SSSP, so each chunk also carries its arcs' weights."""
+    g = G.rmat(14, 16, seed=3).with_weights(3)
+    src = int(np.argmax(np.diff(g.idx)))
+    r = EN.run_app("sssp_weighted_spread", g, source=src, dir_opt=2, bin_small_max=chunk,
+                   bin_medium_max=large_min)
+    assert np.array_equal(r.values, O.dijkstra(g, src))
+    gs = G.rmat(14, 16, seed=9, symmetrise=True)
+    r = EN.run_app("cc_spread", gs, dir_opt=2, bin_small_max=chunk, bin_medium_max=large_min)
+    assert np.array_equal(r.values, O.min_ancestor(gs))
 
 
 @pytest.mark.parametrize("blocks,order,live,ranges,record4", [
