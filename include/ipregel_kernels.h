@@ -811,8 +811,9 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_expand(ipr_dev d)
      * 0) takes i as a record position: the records are read in order, and
      * only a broadcaster's vertex id (bvert) and offsets after */
     const bool bypos = !list && d.bvert;
-    const uint64_t tile_stride = (uint64_t)gridDim.x * IPR_BLOCK;
-    for (uint64_t base = (uint64_t)blockIdx.x * IPR_BLOCK; base < n; base += tile_stride) {
+    /* one tile of IPR_BLOCK entries from `base`: every thread of the
+     * workgroup calls it */
+    auto tile = [&](uint64_t base) {
         const uint64_t i = base + threadIdx.x;
         uint32_t deg = 0;
         if (i < n) {
@@ -857,6 +858,36 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_expand(ipr_dev d)
             }
         }
         __syncthreads();
+    };
+    if (bypos && d.brec4) {
+        /* the sweep by record position reads 4 records per thread with one
+         * 16-byte load, a window of 4 tiles per workgroup, and runs the tile
+         * code only on tiles that hold a broadcaster (wavefront w's four
+         * records per lane are tile w's): with few broadcasters (SSSP's
+         * superstep 0) a window costs one load and one barrier. The records
+         * are padded to 16, so the load stays inside the array; records past
+         * V are masked. */
+        __shared__ uint32_t s_any[2][IPR_WAVES];
+        const uint64_t span = (uint64_t)IPR_BLOCK * 4;
+        uint32_t par = 0;
+        for (uint64_t w0 = (uint64_t)blockIdx.x * span; w0 < n; w0 += (uint64_t)gridDim.x * span, par ^= 1u) {
+            const uint64_t p = w0 + (uint64_t)threadIdx.x * 4;
+            bool hit = false;
+            if (p < n) {
+                const uint4 q = *(const uint4*)(d.brec4 + p);
+                const uint32_t r[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+                for (int k = 0; k < 4; k++) hit |= p + k < n && (r[k] >> IPR_REC4_SHIFT) == want4;
+            }
+            const bool any = __ballot(hit) != 0;
+            if (ipr_lane() == 0) s_any[par][threadIdx.x / IPR_WAVE] = any;
+            __syncthreads();
+            for (uint32_t w = 0; w < IPR_WAVES; w++)
+                if (s_any[par][w]) tile(w0 + (uint64_t)w * IPR_BLOCK);
+        }
+    } else {
+        const uint64_t tile_stride = (uint64_t)gridDim.x * IPR_BLOCK;
+        for (uint64_t base = (uint64_t)blockIdx.x * IPR_BLOCK; base < n; base += tile_stride) tile(base);
     }
 }
 
