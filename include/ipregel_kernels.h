@@ -391,12 +391,16 @@ __device__ __forceinline__ bool ipr_bcast_at(const ipr_dev& d, uint32_t p, uint3
     return (b.step & ~IPR_BSTEP_WEIGHTED) == want;
 }
 
-/* raise the received flag of u by atomic OR on its 32-bit word (plain byte
- * read first: most deliveries find it already raised) */
+/* raise the received flag of u: a plain byte store. Every writer stores the
+ * same 1 and no kernel clears flags while deliveries run, so concurrent
+ * stores need no atomic (the L2s write back dirty bytes only, so stores to
+ * neighbouring flags from other XCDs are kept). A device atomic here was the
+ * second of a delivery's two: random device atomics run at about 26G/s on
+ * MI355X, plain random stores at about 84G/s. Callers that must learn
+ * whether their delivery raised the flag use ipr_deliver_activates. */
 __device__ inline void ipr_mark_recv(uint8_t* recv, uint32_t u)
 {
-    if (__atomic_load_n(recv + u, __ATOMIC_RELAXED)) return;
-    atomicOr((uint32_t*)(recv + (u & ~3u)), 1u << (8 * (u & 3u)));
+    recv[u] = 1;
 }
 
 /* deliver one message to u in superstep s (push: concurrent senders) */

@@ -254,14 +254,14 @@ __global__ void k_push_begin(PushArgs a, int cur)
         a.ctl->nlarge[nxt] = 0;
     }
     if (t < 64) {
-        a.ctl->orm[t] = 0;
-        a.ctl->relax[t] = 0;
+        a.ctl->orm[t].v = 0;
+        a.ctl->relax[t].v = 0;
         for (int k = 0; k < SPREAD; ++k) {
-            a.ctl->fmin[k][t] = INF;
-            a.ctl->pdmin[k][t] = INF;
+            a.ctl->fmin[k][t].v = INF;
+            a.ctl->pdmin[k][t].v = INF;
         }
     }
-    if (t < SPREAD) a.ctl->pdef[t] = 0;
+    if (t < SPREAD) a.ctl->pdef[t].v = 0;
 }
 
 /* a new state: no parked entries, no thresholds */
@@ -271,8 +271,8 @@ __global__ void k_ctl_init(PushArgs a)
     a.ctl->thr[t] = INF;
     a.ctl->pmin[t] = INF;
     for (int k = 0; k < SPREAD; ++k) {
-        a.ctl->pdmin[k][t] = INF;
-        a.ctl->pscan[k][t] = INF;
+        a.ctl->pdmin[k][t].v = INF;
+        a.ctl->pscan[k][t].v = INF;
     }
 }
 
@@ -359,8 +359,8 @@ __global__ __launch_bounds__(256) void k_relax_small_s(PushArgs a, int cur)
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) pmn = min(pmn, (uint32_t)__shfl_xor(pmn, o, 64));
         if ((t & 63) == 0 && pmn != INF) {
-            atomicOr(&a.ctl->pdef[blockIdx.x % SPREAD], 1ull);
-            atomicMin(&a.ctl->pdmin[blockIdx.x % SPREAD][0], pmn);
+            atomicOr(&a.ctl->pdef[blockIdx.x % SPREAD].v, 1ull);
+            atomicMin(&a.ctl->pdmin[blockIdx.x % SPREAD][0].v, pmn);
         }
     }
 }
@@ -533,8 +533,8 @@ __global__ __launch_bounds__(256) void k_relax_small_b(PushArgs a, int cur)
         }
         __syncthreads();
     }
-    if (lane == 0 && parked) atomicOr(&a.ctl->pdef[blockIdx.x % SPREAD], parked);
-    if (pmn != INF) atomicMin(&a.ctl->pdmin[blockIdx.x % SPREAD][lane], pmn);
+    if (lane == 0 && parked) atomicOr(&a.ctl->pdef[blockIdx.x % SPREAD].v, parked);
+    if (pmn != INF) atomicMin(&a.ctl->pdmin[blockIdx.x % SPREAD][lane].v, pmn);
 }
 
 /* large frontier vertices: chunk c of the large list goes to workgroup c mod grid */
@@ -563,8 +563,8 @@ __global__ __launch_bounds__(256) void k_relax_large(PushArgs a, int cur)
             if (du >= thrl) {
                 if (c == lp[i] && t == 0) {
                     ((uint8_t*)a.pend)[u] = 1;
-                    atomicOr(&a.ctl->pdef[blockIdx.x % SPREAD], 1ull);
-                    atomicMin(&a.ctl->pdmin[blockIdx.x % SPREAD][0], du);
+                    atomicOr(&a.ctl->pdef[blockIdx.x % SPREAD].v, 1ull);
+                    atomicMin(&a.ctl->pdmin[blockIdx.x % SPREAD][0].v, du);
                 }
                 continue;
             }
@@ -584,9 +584,9 @@ __global__ __launch_bounds__(256) void k_relax_large(PushArgs a, int cur)
             const unsigned long long dm = __ballot(act_l && !on);
             if (dm && c == lp[i] && t == 0) {
                 ((unsigned long long*)a.pend)[u] |= dm;
-                atomicOr(&a.ctl->pdef[blockIdx.x % SPREAD], dm);
+                atomicOr(&a.ctl->pdef[blockIdx.x % SPREAD].v, dm);
             }
-            if (act_l && !on && c == lp[i] && wave == 0) atomicMin(&a.ctl->pdmin[blockIdx.x % SPREAD][lane], du);
+            if (act_l && !on && c == lp[i] && wave == 0) atomicMin(&a.ctl->pdmin[blockIdx.x % SPREAD][lane].v, du);
             if (!__ballot(on)) continue;
             for (uint64_t base = e0; base < e1; base += 256) {
                 const uint64_t e = base + t;
@@ -699,10 +699,10 @@ __global__ __launch_bounds__(256) void k_apply_s(PushArgs a, int cur)
     __syncthreads();
     apply_flush(a, nxt, S, Lg, buf_s, buf_l, cnt_s, cnt_l, base);
     rel = wave_sum_u64(rel);
-    if ((t & 63) == 0 && rel) atomicAdd(&a.ctl->relax[blockIdx.x & 63], rel);
+    if ((t & 63) == 0 && rel) atomicAdd(&a.ctl->relax[blockIdx.x & 63].v, rel);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) fmn = min(fmn, (uint32_t)__shfl_xor(fmn, o, 64));
-    if ((t & 63) == 0 && fmn != INF) atomicMin(&a.ctl->fmin[blockIdx.x % SPREAD][0], fmn);
+    if ((t & 63) == 0 && fmn != INF) atomicMin(&a.ctl->fmin[blockIdx.x % SPREAD][0].v, fmn);
 }
 
 /* batched apply: a workgroup takes 256-vertex granules; each wavefront scans
@@ -815,10 +815,28 @@ __global__ __launch_bounds__(256) void k_apply_b(PushArgs a, int cur)
     }
     __syncthreads();
     apply_flush(a, nxt, S, Lg, buf_s, buf_l, cnt_s, cnt_l, base);
-    if (lane == 0 && orm) atomicOr(&a.ctl->orm[blockIdx.x & 63], orm);
-    if (lane == 0 && rel) atomicAdd(&a.ctl->relax[blockIdx.x & 63], rel);
-    if (dig) atomicAdd(&a.ctl->dig[blockIdx.x % SPREAD][lane], dig);
-    if (fmn != INF) atomicMin(&a.ctl->fmin[blockIdx.x % SPREAD][lane], fmn);
+    /* the four wavefronts' figures are combined in LDS first: one wavefront
+     * issues the workgroup's atomics (orm and rel are wavefront-uniform;
+     * dig and fmn are per query, lane q holding query q) */
+    __shared__ unsigned long long s_dig[4][64], s_orm[4], s_rel[4];
+    __shared__ uint32_t s_fmn[4][64];
+    s_dig[wave][lane] = dig;
+    s_fmn[wave][lane] = fmn;
+    if (lane == 0) {
+        s_orm[wave] = orm;
+        s_rel[wave] = rel;
+    }
+    __syncthreads();
+    if (wave == 0) {
+        const unsigned long long dsum = s_dig[0][lane] + s_dig[1][lane] + s_dig[2][lane] + s_dig[3][lane];
+        const uint32_t fm = min(min(s_fmn[0][lane], s_fmn[1][lane]), min(s_fmn[2][lane], s_fmn[3][lane]));
+        const unsigned long long om = s_orm[0] | s_orm[1] | s_orm[2] | s_orm[3];
+        const unsigned long long rs = s_rel[0] + s_rel[1] + s_rel[2] + s_rel[3];
+        if (lane == 0 && om) atomicOr(&a.ctl->orm[blockIdx.x & 63].v, om);
+        if (lane == 0 && rs) atomicAdd(&a.ctl->relax[blockIdx.x & 63].v, rs);
+        if (dsum) atomicAdd(&a.ctl->dig[blockIdx.x % SPREAD][lane].v, dsum);
+        if (fm != INF) atomicMin(&a.ctl->fmin[blockIdx.x % SPREAD][lane].v, fm);
+    }
 }
 
 /* after the apply pass: per query, if no entry of the next frontier lies below
@@ -828,13 +846,13 @@ __global__ void k_push_thr(PushArgs a)
 {
     const int t = threadIdx.x;   /* 64 threads, one per slot */
     unsigned long long pd = 0;
-    for (int k = 0; k < SPREAD; ++k) pd |= a.ctl->pdef[k];
+    for (int k = 0; k < SPREAD; ++k) pd |= a.ctl->pdef[k].v;
     const unsigned long long pleft = a.ctl->pleft | pd;
     uint32_t fm = INF;
-    for (int k = 0; k < SPREAD; ++k) fm = min(fm, a.ctl->fmin[k][t]);
+    for (int k = 0; k < SPREAD; ++k) fm = min(fm, a.ctl->fmin[k][t].v);
     const uint32_t th = a.ctl->thr[t];
     uint32_t pm = a.ctl->pmin[t];
-    for (int k = 0; k < SPREAD; ++k) pm = min(pm, a.ctl->pdmin[k][t]);
+    for (int k = 0; k < SPREAD; ++k) pm = min(pm, a.ctl->pdmin[k][t].v);
     a.ctl->pmin[t] = pm;
     const bool parked = (pleft >> t) & 1ull;
     bool wake = false;
@@ -937,11 +955,11 @@ __global__ __launch_bounds__(256) void k_push_react(PushArgs a, int nxt)
     }
     rel = wave_sum_u64(rel);
     __syncthreads();
-    if (t < 64 && s_pmin[t] != INF) atomicMin(&a.ctl->pscan[blockIdx.x % SPREAD][t], s_pmin[t]);
+    if (t < 64 && s_pmin[t] != INF) atomicMin(&a.ctl->pscan[blockIdx.x % SPREAD][t].v, s_pmin[t]);
     if ((t & 63) == 0) {
-        if (rem) atomicOr(&a.ctl->prem[blockIdx.x % SPREAD], rem);
-        if (orm) atomicOr(&a.ctl->orm[blockIdx.x & 63], orm);
-        if (rel) atomicAdd(&a.ctl->relax[blockIdx.x & 63], rel);
+        if (rem) atomicOr(&a.ctl->prem[blockIdx.x % SPREAD].v, rem);
+        if (orm) atomicOr(&a.ctl->orm[blockIdx.x & 63].v, orm);
+        if (rel) atomicAdd(&a.ctl->relax[blockIdx.x & 63].v, rel);
     }
 }
 
@@ -949,8 +967,8 @@ __global__ __launch_bounds__(256) void k_push_react(PushArgs a, int nxt)
 __global__ void k_push_out(PushArgs a, int nxt, int batched)
 {
     const int t = threadIdx.x;   /* 64 threads */
-    unsigned long long o = a.ctl->orm[t], r = a.ctl->relax[t], d = 0;
-    for (int k = 0; k < SPREAD; ++k) d += a.ctl->dig[k][t];
+    unsigned long long o = a.ctl->orm[t].v, r = a.ctl->relax[t].v, d = 0;
+    for (int k = 0; k < SPREAD; ++k) d += a.ctl->dig[k][t].v;
 #pragma unroll
     for (int s = 32; s > 0; s >>= 1) {
         uint32_t lo = __shfl_xor((uint32_t)o, s, 64), hi = __shfl_xor((uint32_t)(o >> 32), s, 64);
@@ -962,8 +980,8 @@ __global__ void k_push_out(PushArgs a, int nxt, int batched)
     if ((scanned >> t) & 1ull) {
         uint32_t pm = INF;
         for (int k = 0; k < SPREAD; ++k) {
-            pm = min(pm, a.ctl->pscan[k][t]);
-            a.ctl->pscan[k][t] = INF;
+            pm = min(pm, a.ctl->pscan[k][t].v);
+            a.ctl->pscan[k][t].v = INF;
         }
         a.ctl->pmin[t] = pm;
     }
@@ -973,8 +991,8 @@ __global__ void k_push_out(PushArgs a, int nxt, int batched)
         if (a.ctl->react) {
             pleft = 0;
             for (int k = 0; k < SPREAD; ++k) {
-                pleft |= a.ctl->prem[k];
-                a.ctl->prem[k] = 0;
+                pleft |= a.ctl->prem[k].v;
+                a.ctl->prem[k].v = 0;
             }
             a.ctl->pleft = pleft;
             a.ctl->react = 0;
@@ -1070,8 +1088,8 @@ __global__ void k_admit_b(PushArgs a, AdmitArgs ad, unsigned long long base)
     mark(a, s);
     a.ctl->thr[q] = a.delta;
     a.ctl->pmin[q] = INF;
-    a.ctl->dig[0][q] = base;
-    for (int k = 1; k < SPREAD; ++k) a.ctl->dig[k][q] = 0;
+    a.ctl->dig[0][q].v = base;
+    for (int k = 1; k < SPREAD; ++k) a.ctl->dig[k][q].v = 0;
 }
 
 __global__ void k_release_b(PushArgs a, unsigned long long mask)

@@ -101,20 +101,28 @@ struct TickOut {
     unsigned long long dig[64];   /* batched: per-slot digest of dist (telescoped) */
 };
 
+/* one accumulator on a 128-byte line of its own: device atomics on one line
+ * are performed one after another (about 12 ns each on MI355X, whatever the
+ * word), so the copies that spread an accumulator's atomics must not share
+ * lines */
+template <typename T> struct alignas(128) Lined {
+    T v;
+};
+
 struct PushCtl {
     uint32_t nsmall[2], nlarge[2];
-    unsigned long long orm[64];
-    unsigned long long relax[64];
-    unsigned long long dig[SPREAD][64];
+    Lined<unsigned long long> orm[64];
+    Lined<unsigned long long> relax[64];
+    Lined<unsigned long long> dig[SPREAD][64];
     uint32_t thr[64];                 /* near/far threshold per query */
-    uint32_t fmin[SPREAD][64];        /* smallest distance in the next frontier, per query */
-    unsigned long long pdef[SPREAD];  /* queries that parked (far) entries this superstep */
-    unsigned long long prem[SPREAD];  /* queries with parked entries left after a wake-up scan */
+    Lined<uint32_t> fmin[SPREAD][64]; /* smallest distance in the next frontier, per query */
+    Lined<unsigned long long> pdef[SPREAD];  /* queries that parked (far) entries this superstep */
+    Lined<unsigned long long> prem[SPREAD];  /* queries with parked entries left after a wake-up scan */
     unsigned long long pleft;         /* queries that may have parked entries */
     unsigned long long react;         /* queries whose threshold moved: wake parked entries below it */
     uint32_t pmin[64];                /* smallest parked distance per query (lower bound) */
-    uint32_t pdmin[SPREAD][64];       /* smallest distance parked this superstep */
-    uint32_t pscan[SPREAD][64];       /* smallest distance left parked after a wake-up scan */
+    Lined<uint32_t> pdmin[SPREAD][64];  /* smallest distance parked this superstep */
+    Lined<uint32_t> pscan[SPREAD][64];  /* smallest distance left parked after a wake-up scan */
     uint32_t step, pad;               /* superstep counter (frontier stamps) */
     unsigned long long relax_total;   /* cumulative over the run */
     unsigned long long busy_total;    /* supersteps of the run that left work for another */
