@@ -37,37 +37,14 @@ __device__ __forceinline__ void ipr_coherent_store(uint64_t* p, uint64_t v)
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-/* sum (and optionally clear) one spread counter: lanes 0..63 of one
- * wavefront, lane k owning slot k */
-__device__ inline uint64_t ipr_take(const ipr_dev& d, int id, bool clear)
-{
-    uint64_t* p = &d.cnt[IPR_CSLOT(id, threadIdx.x & 63)];
-    uint64_t x = ipr_coherent_load(p);
-    if (clear) ipr_coherent_store(p, 0);
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
-    return x;
-}
-
-/* the largest of one spread counter's slots (optionally cleared): lanes
- * 0..63 of one wavefront */
-__device__ inline uint64_t ipr_take_max(const ipr_dev& d, int id, bool clear)
-{
-    uint64_t* p = &d.cnt[IPR_CSLOT(id, threadIdx.x & 63)];
-    uint64_t x = ipr_coherent_load(p);
-    if (clear) ipr_coherent_store(p, 0);
-    for (int o = 32; o > 0; o >>= 1) {
-        const uint64_t y = __shfl_xor(x, o);
-        x = y > x ? y : x;
-    }
-    return x;
-}
-
 /* Called by every thread of workgroups 0..n-1 after their last store and
  * atomic of the kernel; true in the workgroup that finished last. Arrivals are
  * counted in two levels, so that no address takes more than n/64 + 64 of
  * them: workgroup b counts in slot b mod 64, and the workgroup that
- * completes a slot counts the slot in the top counter. Counters are left at
- * 0 for the next kernel. */
+ * completes a slot counts the slot in the top counter. The last workgroup
+ * (every slot complete by then) sets the counters back to 0 for the next
+ * kernel, with stores issued after its two atomics rather than an exchange
+ * after each. */
 __device__ inline bool ipr_last_workgroup(const ipr_dev& d, uint32_t n)
 {
     __shared__ uint32_t s_last;
@@ -82,17 +59,18 @@ __device__ inline bool ipr_last_workgroup(const ipr_dev& d, uint32_t n)
         unsigned long long* sc = (unsigned long long*)&d.cnt[IPR_CSLOT(IPR_C_ARRIVE, slot)];
         unsigned long long* top = (unsigned long long*)&d.cnt[IPR_CSLOT(IPR_C_ARRIVE_TOP, 0)];
         uint32_t last = 0;
-        if (atomicAdd(sc, 1ull) + 1 == (unsigned long long)share) {
-            atomicExch(sc, 0ull);
-            if (atomicAdd(top, 1ull) + 1 == (unsigned long long)slots) {
-                atomicExch(top, 0ull);
-                last = 1;
-            }
-        }
+        if (atomicAdd(sc, 1ull) + 1 == (unsigned long long)share &&
+            atomicAdd(top, 1ull) + 1 == (unsigned long long)slots)
+            last = 1;
         s_last = last;
     }
     __syncthreads();
-    return s_last != 0;
+    const bool last = s_last != 0;
+    if (last && threadIdx.x < IPR_CSLOTS) {
+        ipr_coherent_store(&d.cnt[IPR_CSLOT(IPR_C_ARRIVE, threadIdx.x)], 0);
+        if (threadIdx.x == 0) ipr_coherent_store(&d.cnt[IPR_CSLOT(IPR_C_ARRIVE_TOP, 0)], 0);
+    }
+    return last;
 }
 
 /* End of superstep s (PULL: of s-1, after the fused fetch(s-1)+compute(s)),
@@ -105,44 +83,84 @@ __device__ inline void ipr_finalize_step(const ipr_dev& d, int variant, int flag
     if (c->done) return;
     const uint64_t s = c->superstep;
     uint64_t fin, active, edges, computed;
+    const uint32_t lane = threadIdx.x & 63;
     if (variant == IPR_VARIANT_PULL) {
         if (s == 0) {
-            if ((threadIdx.x & 63) == 0) c->superstep = 1;
+            if (lane == 0) c->superstep = 1;
             return;
         }
         fin = s - 1;
-        active = ipr_take(d, IPR_C_ACT0 + (int)(fin & 1), true);
-        const uint64_t bc = ipr_take(d, IPR_C_BCAST0 + (int)(fin & 1), true);
-        computed = ipr_take(d, IPR_C_WORKV, true);
-        edges = bc ? d.in_idx[d.V] : 0;
+        /* every figure loaded before any is cleared or reduced: one round
+         * trip (a load waited for after a store waits for the store too) */
+        uint64_t* pa = &d.cnt[IPR_CSLOT(IPR_C_ACT0 + (int)(fin & 1), lane)];
+        uint64_t* pb = &d.cnt[IPR_CSLOT(IPR_C_BCAST0 + (int)(fin & 1), lane)];
+        uint64_t* pw = &d.cnt[IPR_CSLOT(IPR_C_WORKV, lane)];
+        uint64_t xa = ipr_coherent_load(pa), xb = ipr_coherent_load(pb), xw = ipr_coherent_load(pw);
+        const uint64_t in_arcs = d.in_idx[d.V];
+        ipr_coherent_store(pa, 0);
+        ipr_coherent_store(pb, 0);
+        ipr_coherent_store(pw, 0);
+        for (int o = 32; o > 0; o >>= 1) {
+            xa += __shfl_xor(xa, o);
+            xb += __shfl_xor(xb, o);
+            xw += __shfl_xor(xw, o);
+        }
+        active = xa;
+        computed = xw;
+        edges = xb ? in_arcs : 0;
     } else {
         fin = s;
-        /* spread: the next frontier's size, from the compaction's list counter
-         * or from the fused kernel's count of activated receivers */
-        active = variant == IPR_VARIANT_PUSH ? ipr_take(d, IPR_C_ACT0 + (int)(s & 1), true)
-                                             : ipr_coherent_load(&c->fsize[(s + 1) & 1]) +
-                                                   ipr_take(d, IPR_C_FNEXT, true);
-        edges = ipr_take(d, IPR_C_EDGES, true);
-        computed = ipr_take(d, IPR_C_WORKV, true);
-        if (variant != IPR_VARIANT_PUSH) {
-            /* the out-arcs of the frontier just compacted, for the next
-             * superstep's fused-or-general choice */
-            const uint64_t fa = ipr_take(d, IPR_C_FARCS, true);
-            const uint64_t fm = ipr_take_max(d, IPR_C_FMAX, true);
-            /* after a superstep that gathered the compaction does not add
-             * them up: unknown, so the next superstep is not fused */
+        const bool push = variant == IPR_VARIANT_PUSH;
+        /* the figures, loaded together: ActiveVertexCount (push) or the
+         * fused kernel's count of activated receivers (spread, added to the
+         * compaction's list counter), W, computed vertices and, for spread,
+         * the out-arcs and largest out-degree of the frontier just compacted */
+        uint64_t* pa = &d.cnt[IPR_CSLOT(push ? IPR_C_ACT0 + (int)(s & 1) : IPR_C_FNEXT, lane)];
+        uint64_t* pe = &d.cnt[IPR_CSLOT(IPR_C_EDGES, lane)];
+        uint64_t* pw = &d.cnt[IPR_CSLOT(IPR_C_WORKV, lane)];
+        uint64_t* pf = &d.cnt[IPR_CSLOT(IPR_C_FARCS, lane)];
+        uint64_t* pm = &d.cnt[IPR_CSLOT(IPR_C_FMAX, lane)];
+        uint64_t xa = ipr_coherent_load(pa), xe = ipr_coherent_load(pe), xw = ipr_coherent_load(pw);
+        uint64_t xf = 0, xm = 0, fsz = 0;
+        if (!push) {
+            xf = ipr_coherent_load(pf);
+            xm = ipr_coherent_load(pm);
+            fsz = ipr_coherent_load(&c->fsize[(s + 1) & 1]);
+        }
+        ipr_coherent_store(pa, 0);
+        ipr_coherent_store(pe, 0);
+        ipr_coherent_store(pw, 0);
+        if (!push) {
+            ipr_coherent_store(pf, 0);
+            ipr_coherent_store(pm, 0);
+        }
+        /* the gather's range counters and the large-broadcaster
+         * registrations start from 0 in the next superstep */
+        ipr_coherent_store(&d.cnt[IPR_CSLOT(IPR_C_GNEXT, lane)], 0);
+        ipr_coherent_store(&d.cnt[IPR_CSLOT(IPR_C_LPACK, lane)], 0);
+        for (int o = 32; o > 0; o >>= 1) {
+            xa += __shfl_xor(xa, o);
+            xe += __shfl_xor(xe, o);
+            xw += __shfl_xor(xw, o);
+            xf += __shfl_xor(xf, o);
+            const uint64_t y = __shfl_xor(xm, o);
+            xm = y > xm ? y : xm;
+        }
+        active = push ? xa : fsz + xa;
+        edges = xe;
+        computed = xw;
+        if (!push) {
+            /* after a superstep that gathered the compaction does not add up
+             * the frontier's out-arcs: unknown, so the next superstep is not
+             * fused */
             const bool dense = d.dir_opt && edges * (uint64_t)d.dir_alpha > d.E;
-            if ((threadIdx.x & 63) == 0) {
-                c->farcs[(s + 1) & 1] = dense ? ~0ull : fa;
-                c->fmaxdeg[(s + 1) & 1] = dense ? ~0ull : fm;
+            if (lane == 0) {
+                c->farcs[(s + 1) & 1] = dense ? ~0ull : xf;
+                c->fmaxdeg[(s + 1) & 1] = dense ? ~0ull : xm;
             }
         }
-        /* the gather's range counters start from 0 in the next superstep */
-        ipr_coherent_store(&d.cnt[IPR_CSLOT(IPR_C_GNEXT, threadIdx.x & 63)], 0);
-        /* and the large-broadcaster registrations of the next superstep too */
-        ipr_coherent_store(&d.cnt[IPR_CSLOT(IPR_C_LPACK, threadIdx.x & 63)], 0);
     }
-    if ((threadIdx.x & 63) != 0) return;
+    if (lane != 0) return;
     if (variant == IPR_VARIANT_PULL) {
         c->done = active == 0;
         c->executed = fin + 1;
