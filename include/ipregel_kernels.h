@@ -1497,8 +1497,16 @@ __global__ void __launch_bounds__(IPR_GATHER_WAVES * IPR_WAVE, ipr_gather_min_wa
 #ifndef IPR_TILE_LDS_BYTES
 #define IPR_TILE_LDS_BYTES 131072
 #endif
+/* slot groups per wavefront pass: as many as fit in the 128 VGPRs that four
+ * wavefronts per SIMD leave (the LDS allows no more): 20 for 4-byte messages
+ * without weights (125 VGPRs; 1-2% faster gathers than 16 on CC), 16 with
+ * weights or 8-byte messages, where 20 spills */
 #ifndef IPR_TILE_K
+#if defined(IP_NEEDS_OUT_NEIGHBOUR_WEIGHTS)
 #define IPR_TILE_K 16
+#else
+#define IPR_TILE_K (sizeof(IP_MESSAGE_TYPE) > 4 ? 16 : 20)
+#endif
 #endif
 constexpr bool ipr_tile_ok = ipr_policy::exact_gather && IPR_COMBINER != IPR_COMBINER_GENERIC &&
                              std::is_integral<IP_MESSAGE_TYPE>::value &&

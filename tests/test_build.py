@@ -105,6 +105,7 @@ def _resource_usage(src, defines, tmp_path):
     ("sssp.c", ["-DIP_USE_SPREAD", "-DIP_COMBINER_MIN"], 5),
     ("cc.c", ["-DIP_COMBINER_MIN"], 5),
     ("sssp_weighted.c", ["-DIP_USE_SPREAD", "-DIP_COMBINER_MIN"], 4),
+    ("cc.c", ["-DIP_USE_SPREAD", "-DIP_COMBINER_MIN", "-UIP_VERTEX_ID_TYPE", "-DIP_VERTEX_ID_TYPE=uint64_t"], 4),
     ("pagerank.c", ["-DIP_USE_SINGLE_BROADCAST", "-DIP_COMBINER_SUM"], 4)])
 def test_engine_kernels_spill_free(app, defines, gather_waves, tmp_path):
     """The engine's kernels compile without scratch or spills, and the
