@@ -146,10 +146,10 @@ WORKLOADS = {
 # factors a plain 1 GiB device copy gives; KiB, mean over the launches that do
 # work: the dense gathers, the pull launches of supersteps 0-11, every fused
 # sparse launch); commands and per-launch figures are in profiles/
-TRAFFIC = {("cc22", "ipr_k_gather_tiled"): (384_288 * 2 + 17_428) * 1024,
-           ("cc24", "ipr_k_gather_tiled"): (2_609_166 * 2 + 67_260) * 1024,
-           ("ss24a", "ipr_k_gather_tiled"): (1_063_381 * 2 + 58_900) * 1024,
-           ("ss24b", "ipr_k_gather_tiled"): (1_469_543 * 2 + 59_040) * 1024,
+TRAFFIC = {("cc22", "ipr_k_gather_tiled"): (384_700 * 2 + 17_429) * 1024,
+           ("cc24", "ipr_k_gather_tiled"): (2_598_107 * 2 + 67_267) * 1024,
+           ("ss24a", "ipr_k_gather_tiled"): (1_092_595 * 2 + 58_909) * 1024,
+           ("ss24b", "ipr_k_gather_tiled"): (1_460_152 * 2 + 59_062) * 1024,
            ("cc22", "ipr_k_gather"): (767_103 * 2 + 56_781) * 1024,
            ("cc24", "ipr_k_gather"): (7_919_949 * 2 + 245_533) * 1024,
            ("ss24a", "ipr_k_gather"): (4_066_499 * 2 + 181_607) * 1024,
