@@ -526,7 +526,10 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_init(ipr_dev d)
 #else
         ((ipr_slot_t*)d.mbox[0])[v] = ipr_policy::empty();
         ((ipr_slot_t*)d.mbox[1])[v] = ipr_policy::empty();
-        ((ipr_bslot_t*)d.bslot)[v].step = 0;
+        /* with 4-byte records the stamp lives there (cleared below): the
+         * 8-byte record is read only for an escaping message, never for
+         * its step */
+        if (!d.brec4) ((ipr_bslot_t*)d.bslot)[v].step = 0;
         d.recv[0][v] = 0;
         d.recv[1][v] = 0;
 #endif

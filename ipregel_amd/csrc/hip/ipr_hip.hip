@@ -1249,7 +1249,8 @@ extern "C" int ipr_hip_equal_noalloc(const void* a, const void* b, uint64_t byte
 
 __global__ void k_begin(ipr_dev d)
 {
-    for (uint32_t i = threadIdx.x; i < IPR_NCNT * IPR_CSLOTS * IPR_CPAD; i += blockDim.x) d.cnt[i] = 0;
+    /* the counters' slot words (the rest of each slot's line is never used) */
+    for (uint32_t i = threadIdx.x; i < IPR_NCNT * IPR_CSLOTS; i += blockDim.x) d.cnt[(uint64_t)i * IPR_CPAD] = 0;
     if (threadIdx.x != 0) return;
     ipr_ctl* c = d.ctl;
     uint64_t cap = c->trace_cap;
@@ -1267,7 +1268,7 @@ __global__ void k_finalize(ipr_dev d, int variant)
 
 extern "C" int ipr_hip_begin(const ipr_dev* dev, void* stream)
 {
-    hipLaunchKernelGGL(k_begin, dim3(1), dim3(64), 0, (hipStream_t)stream, *dev);
+    hipLaunchKernelGGL(k_begin, dim3(1), dim3(256), 0, (hipStream_t)stream, *dev);
     CK(hipGetLastError());
     return 0;
 }
