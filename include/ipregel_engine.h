@@ -113,6 +113,12 @@ typedef struct ipr_ctl {
      * receive flags (a fused sparse kernel built it, no list), 0 when it is
      * the compacted list front[superstep & 1] */
     uint64_t front_flags;
+    /* the broadcasters' bitmap: superstep + 1 it was last built for, bitmaps
+     * built this run, and whether a gather wanted one that was not built
+     * (the host launches the bitmap kernel only while runs use it) */
+    uint64_t live_step;
+    uint32_t live_built;
+    uint32_t live_missed;
 } ipr_ctl;
 
 #define IPR_LPACK_SHIFT 40

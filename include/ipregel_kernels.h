@@ -349,6 +349,19 @@ __device__ inline bool ipr_gathers_live(const ipr_dev& d)
            n * 100 < d.V * (uint64_t)d.live_vpct;
 }
 
+/* a gather reads the broadcasters' bitmap when the superstep asks for it
+ * and the bitmap kernel built it for this superstep (the host may leave that
+ * launch out while runs do not use it); otherwise it reads every record, with
+ * the same result, and tells the host (uniform across the launch) */
+__device__ inline bool ipr_gather_bitmap(const ipr_dev& d)
+{
+    if (!ipr_gathers_live(d)) return false;
+    ipr_ctl* ctl = d.ctl;
+    if (ctl->live_step == ctl->superstep + 1) return true;
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->live_missed = 1;
+    return false;
+}
+
 /* position of vertex v's broadcast record */
 __device__ __forceinline__ uint32_t ipr_brec(const ipr_dev& d, uint32_t v)
 {
@@ -1209,6 +1222,12 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_gather_live(ipr_dev d)
     ipr_ctl* ctl = d.ctl;
     if (ctl->done) return;
     if (!ipr_gathers_live(d)) return;
+    /* the gather reads the bitmap only for the superstep stamped here (the
+     * whole kernel ends before the gather starts) */
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->live_step = ctl->superstep + 1;
+        ctl->live_built++;
+    }
     const uint32_t want = (uint32_t)(ctl->superstep + 1);
     const uint32_t want4 = (uint32_t)(ctl->superstep % IPR_REC4_WINDOW) + 1u;
     const ipr_bslot_t* bs = (const ipr_bslot_t*)d.bslot;
@@ -1272,7 +1291,7 @@ __global__ void __launch_bounds__(IPR_GATHER_WAVES * IPR_WAVE, ipr_gather_min_wa
     ipr_ctl* ctl = d.ctl;
     if (ctl->done) return;
     if (!ipr_gathers(d)) return;
-    const bool live = ipr_gathers_live(d);
+    const bool live = ipr_gather_bitmap(d);
     const uint64_t s = ctl->superstep;
     const uint32_t want = (uint32_t)(s + 1);
     const uint32_t want4 = (uint32_t)(s % IPR_REC4_WINDOW) + 1u;
@@ -1564,7 +1583,7 @@ __global__ void __launch_bounds__(IPR_TILE_THREADS) ipr_k_gather_tiled(ipr_dev d
         /* few live arcs: the broadcasters' bitmap (built by
          * ipr_k_gather_live) is read first, and records only of sources
          * whose bit is set */
-        const bool live = ipr_gathers_live(d);
+        const bool live = ipr_gather_bitmap(d);
         const uint64_t s = ctl->superstep;
         const uint32_t want = (uint32_t)(s + 1);
         const uint32_t want4 = (uint32_t)(s % IPR_REC4_WINDOW) + 1u;

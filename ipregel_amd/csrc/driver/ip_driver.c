@@ -61,6 +61,13 @@ static struct {
     int adapt_chunk;
     uint32_t prev_launches;
     int prev_fused;
+    /* the broadcasters' bitmap kernel is launched while runs use it
+     * (IPR_GATHER_LIVE_LAUNCH: 2 adaptive, the default; 1 always; 0 never):
+     * a run that built no bitmap and missed none turns it off, a gather that
+     * wanted one and found none turns it back on; a gather without the
+     * bitmap reads every record, with the same result */
+    int live_launch;
+    uint32_t live_mode;
     int quiet;
     int kev_on;              /* events around the dominant kernel */
     void* kev[3][64];        /* start / split / stop per superstep slot of a chunk */
@@ -515,6 +522,8 @@ static void dev_setup(void)
     d->fin_in_kernel = (pull || var == IPR_VARIANT_SPREAD) && env_u32("IPR_FIN_IN_KERNEL", 1) == 1;
     uint32_t ug = c && c->use_graph ? (uint32_t)c->use_graph : env_u32("IPR_GRAPH", 1);
     D.use_graph = ug == 1;
+    D.live_mode = env_u32("IPR_GATHER_LIVE_LAUNCH", 2);
+    D.live_launch = D.live_mode != 0;
     D.max_chunk = c && c->max_chunk ? c->max_chunk : env_u32("IPR_MAX_CHUNK", 64);
     /* a power of two; the pull kernel's no-op launches are cheap */
     {
@@ -848,7 +857,7 @@ static void enqueue_superstep(int fused, uint32_t step_in_chunk)
     case IPR_VARIANT_PUSH:
         launch_app(IPR_K_COMPUTE);
         pmark(1);
-        if (D.dev.dir_opt && D.dev.g_live) launch_app(IPR_K_GATHER_LIVE);
+        if (D.dev.dir_opt && D.dev.g_live && D.live_launch) launch_app(IPR_K_GATHER_LIVE);
         kev_mark(0);
         if (D.dev.dir_opt) launch_app(IPR_K_GATHER);
         kev_mark(1);
@@ -1020,6 +1029,17 @@ static void run_once(void)
     /* pull: launch s finalizes superstep s-1, so one launch more than supersteps */
     D.prev_launches = (uint32_t)(D.h_ctl->executed + (D.app.variant == IPR_VARIANT_PULL ? 1 : 0));
     D.prev_fused = D.h_ctl->fused_steps > 0;
+    if (D.live_mode == 2 && D.dev.g_live) {
+        const int want = D.h_ctl->live_built > 0 || D.h_ctl->live_missed != 0;
+        if (want != D.live_launch) {
+            /* the cached chunk graphs hold the old launch list */
+            D.live_launch = want;
+            for (int i = 0; i < IPR_MAX_GRAPHS; i++) {
+                if (D.graphs[i]) ipr_hip_graph_destroy(D.graphs[i]);
+                D.graphs[i] = NULL;
+            }
+        }
+    }
 }
 
 static int run_and_report(int reps);

@@ -128,6 +128,36 @@ def test_cc_tuning_does_not_change_results(tune):
     assert np.array_equal(r.values, ref)
 
 
+@pytest.mark.parametrize("mode", ["0", "1", "2"])
+def test_bitmap_launch_modes(mode, monkeypatch):
+    """The broadcasters' bitmap kernel launched never (0: every gather that
+    asks for the bitmap finds none built for its superstep and reads every
+    record), always (1) or while runs use it (2, the default: CC's runs
+    build none, so the later runs of its session leave the launch out).
+    Three runs per session, each checked."""
+    monkeypatch.setenv("IPR_GATHER_LIVE_LAUNCH", mode)
+    monkeypatch.setenv("IPR_GATHER_LIVE", "100")
+    monkeypatch.setenv("IPR_GATHER_LIVE_VERTICES", "100")
+    g = G.rmat(14, 16, seed=3)
+    src = int(np.argmax(np.diff(g.idx)))
+    ref = O.bfs(g, src)
+    s = EN.Session("sssp_spread", g, source=src, dir_alpha=1 << 30)
+    for _ in range(3):
+        st = s.run(1)
+        assert np.array_equal(s.values(), ref)
+    assert st.gathered > 0
+    s.close()
+    monkeypatch.setenv("IPR_GATHER_LIVE", "60")
+    monkeypatch.setenv("IPR_GATHER_LIVE_VERTICES", "5")
+    gs = G.rmat(14, 16, seed=9, symmetrise=True)
+    ref = O.min_ancestor(gs)
+    s = EN.Session("cc_spread", gs)
+    for _ in range(3):
+        s.run(1)
+        assert np.array_equal(s.values(), ref)
+    s.close()
+
+
 @pytest.mark.parametrize("large_min,chunk", [(2, 64), (40, 100), (300, 64)])
 def test_large_broadcasters_push(large_min, chunk):
     """Every superstep pushed (no gather), broadcasters above large_min
