@@ -184,7 +184,10 @@ __device__ inline void ipr_finalize_step(const ipr_dev& d, int variant, int flag
     c->work_vertices += computed;
     /* the delivery kernels gathered this superstep by the same rule */
     c->pull_mode = variant != IPR_VARIANT_PULL && d.dir_opt && edges * (uint64_t)d.dir_alpha > d.E;
-    if (c->pull_mode) c->gathered++;
+    if (c->pull_mode) {
+        c->gathered++;
+        if (fin < 64) c->gathered_mask |= 1ull << fin;
+    }
     c->last_active = active;
     if (fin < c->trace_cap) {
         d.trace_active[fin] = active;

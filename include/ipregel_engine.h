@@ -119,6 +119,12 @@ typedef struct ipr_ctl {
     uint64_t live_step;
     uint32_t live_built;
     uint32_t live_missed;
+    /* push apps: bit s set when superstep s < 64 gathered; set when a
+     * delivery kernel launched alone (a launch plan) found the superstep
+     * needs the other one (the run stops; the host runs it again unplanned) */
+    uint64_t gathered_mask;
+    uint32_t plan_miss;
+    uint32_t plan_pad;
 } ipr_ctl;
 
 #define IPR_LPACK_SHIFT 40
@@ -279,6 +285,10 @@ typedef struct ipr_dev {
     uint32_t fin_in_kernel;
     /* position of a fused sparse kernel in its chunk (set per launch) */
     uint32_t step_in_chunk;
+    /* set in the copy passed to a gather or expansion launched without the
+     * other in its superstep (a launch plan): it flags ctl->plan_miss if the
+     * superstep needs the other */
+    uint32_t plan_alone;
 } ipr_dev;
 
 /* 4-byte broadcast record: bits 31-26 the superstep's stamp (superstep mod

@@ -335,6 +335,17 @@ __device__ inline bool ipr_gathers(const ipr_dev& d)
     return d.dir_opt && ipr_bcast_arcs(d) * (uint64_t)d.dir_alpha > d.E;
 }
 
+/* a delivery kernel that finds its superstep is the other kernel's: when it
+ * was launched alone (d.plan_alone), the superstep cannot be delivered, so
+ * the run stops here and the host runs it again without a launch plan */
+__device__ inline void ipr_plan_miss(const ipr_dev& d)
+{
+    if (d.plan_alone && blockIdx.x == 0 && threadIdx.x == 0) {
+        d.ctl->plan_miss = 1;
+        d.ctl->done = 1;
+    }
+}
+
 /* a gathered superstep with few live arcs from few computed vertices reads
  * the broadcasters' bitmap before any record (wavefront-uniform): then most
  * of the records it skips are cold ones (few broadcasters holding many arcs
@@ -777,7 +788,10 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_expand(ipr_dev d)
     __shared__ uint32_t scan[IPR_WAVES];
     ipr_ctl* ctl = d.ctl;
     if (ctl->done) return;
-    if (ipr_gathers(d)) return;
+    if (ipr_gathers(d)) {
+        ipr_plan_miss(d);
+        return;
+    }
     const uint64_t s = ctl->superstep;
     ipr_sm_superstep = s;
     ipr_sm_counted = 0;
@@ -1290,7 +1304,7 @@ __global__ void __launch_bounds__(IPR_GATHER_WAVES * IPR_WAVE, ipr_gather_min_wa
     uint32_t* hot4 = (uint32_t*)hot;
     ipr_ctl* ctl = d.ctl;
     if (ctl->done) return;
-    if (!ipr_gathers(d)) return;
+    if (!ipr_gathers(d)) return;   /* never launched alone: plans need the tiled gather */
     const bool live = ipr_gather_bitmap(d);
     const uint64_t s = ctl->superstep;
     const uint32_t want = (uint32_t)(s + 1);
@@ -1579,7 +1593,10 @@ __global__ void __launch_bounds__(IPR_TILE_THREADS) ipr_k_gather_tiled(ipr_dev d
         __shared__ uint32_t tcur;
         ipr_ctl* ctl = d.ctl;
         if (ctl->done) return;
-        if (!ipr_gathers(d)) return;
+        if (!ipr_gathers(d)) {
+            ipr_plan_miss(d);
+            return;
+        }
         /* few live arcs: the broadcasters' bitmap (built by
          * ipr_k_gather_live) is read first, and records only of sources
          * whose bit is set */

@@ -68,6 +68,21 @@ static struct {
      * bitmap reads every record, with the same result */
     int live_launch;
     uint32_t live_mode;
+    /* launch plan (push apps, IPR_LAUNCH_PLAN=1 by default): a run that
+     * repeats the previous one (same launches, no fused sparse supersteps)
+     * launches in each of its first 64 supersteps only the delivery kernel
+     * that superstep used, gather or expansion. The kernel launched alone
+     * stops the run if the superstep needs the other one, and the run is
+     * done again without the plan. The plan is a chunk starting at superstep
+     * 0, captured in a graph of its own; it needs the receiver-blocked
+     * gather (the arc-balanced one has no room for the check). */
+    uint32_t plan_mode;
+    int plan_on;
+    uint64_t plan_mask;
+    uint32_t plan_steps;
+    int planning;            /* the chunk being enqueued follows the plan */
+    uint32_t chunk_no;       /* chunks enqueued so far in this run */
+    void* plan_graph;
     int quiet;
     int kev_on;              /* events around the dominant kernel */
     void* kev[3][64];        /* start / split / stop per superstep slot of a chunk */
@@ -172,6 +187,8 @@ static void dev_free_all(void)
         if (D.fgraphs[i]) ipr_hip_graph_destroy(D.fgraphs[i]);
         D.graphs[i] = D.fgraphs[i] = NULL;
     }
+    if (D.plan_graph) ipr_hip_graph_destroy(D.plan_graph);
+    D.plan_graph = NULL;
     ipr_dev* d = &D.dev;
     if (d->in_ids && d->in_ids != d->in_adj && d->in_ids != d->out_ids) ipr_hip_free((void*)d->in_ids);
     if (d->out_ids && d->out_ids != d->out_adj) ipr_hip_free((void*)d->out_ids);
@@ -524,6 +541,8 @@ static void dev_setup(void)
     D.use_graph = ug == 1;
     D.live_mode = env_u32("IPR_GATHER_LIVE_LAUNCH", 2);
     D.live_launch = D.live_mode != 0;
+    D.plan_mode = env_u32("IPR_LAUNCH_PLAN", 1);
+    D.plan_on = 0;
     D.max_chunk = c && c->max_chunk ? c->max_chunk : env_u32("IPR_MAX_CHUNK", 64);
     /* a power of two; the pull kernel's no-op launches are cheap */
     {
@@ -854,14 +873,28 @@ static void enqueue_superstep(int fused, uint32_t step_in_chunk)
         pmark(2);
         break;
     case IPR_VARIANT_SPREAD:
-    case IPR_VARIANT_PUSH:
+    case IPR_VARIANT_PUSH: {
+        /* under a launch plan only the delivery kernel the superstep used */
+        int gather = D.dev.dir_opt, expand = 1;
+        if (D.planning && step_in_chunk < 64 && D.dev.dir_opt) {
+            gather = (int)((D.plan_mask >> step_in_chunk) & 1u);
+            expand = !gather;
+        }
         launch_app(IPR_K_COMPUTE);
         pmark(1);
-        if (D.dev.dir_opt && D.dev.g_live && D.live_launch) launch_app(IPR_K_GATHER_LIVE);
+        if (gather && D.dev.g_live && D.live_launch) launch_app(IPR_K_GATHER_LIVE);
         kev_mark(0);
-        if (D.dev.dir_opt) launch_app(IPR_K_GATHER);
+        if (gather) {
+            D.dev.plan_alone = !expand;
+            launch_app(IPR_K_GATHER);
+            D.dev.plan_alone = 0;
+        }
         kev_mark(1);
-        launch_app(IPR_K_EXPAND);
+        if (expand) {
+            D.dev.plan_alone = !gather && D.dev.dir_opt;
+            launch_app(IPR_K_EXPAND);
+            D.dev.plan_alone = 0;
+        }
         kev_mark(2);
         pmark(2);
         /* spread: the compaction's last workgroup ends the superstep */
@@ -871,6 +904,7 @@ static void enqueue_superstep(int fused, uint32_t step_in_chunk)
         else
             launch_app(IPR_K_SWAP);
         break;
+    }
     default:
         die("Engine variant %d is not available in this build.\n", D.app.variant);
     }
@@ -891,17 +925,22 @@ static void enqueue_chunk(uint32_t n, int fused, uint64_t base)
     const uint32_t slot = n;
     D.kev_slot = 0;
     if (fused) hip_check(ipr_hip_set_u64(&D.dev.ctl->chunk_base, base, D.stream), "chunk base");
+    /* the first chunk of a run that repeats the planned one */
+    D.planning = !fused && D.plan_on && D.chunk_no == 0 && n == D.plan_steps;
     void** cache = fused ? D.fgraphs : D.graphs;
-    if (D.use_graph && !D.kev_on && !D.prof_on && slot < IPR_MAX_GRAPHS) {
-        if (!cache[slot]) {
+    void** g = D.planning ? &D.plan_graph : (slot < IPR_MAX_GRAPHS ? &cache[slot] : NULL);
+    if (D.use_graph && !D.kev_on && !D.prof_on && g) {
+        if (!*g) {
             hip_check(ipr_hip_graph_begin(D.stream), "graph capture");
             for (uint32_t i = 0; i < n; i++) enqueue_superstep(fused, i);
-            hip_check(ipr_hip_graph_end(D.stream, &cache[slot]), "graph capture");
+            hip_check(ipr_hip_graph_end(D.stream, g), "graph capture");
         }
-        hip_check(ipr_hip_graph_launch(cache[slot], D.stream), "graph launch");
+        hip_check(ipr_hip_graph_launch(*g, D.stream), "graph launch");
     } else {
         for (uint32_t i = 0; i < n; i++) enqueue_superstep(fused, i);
     }
+    D.planning = 0;
+    D.chunk_no++;
 }
 
 /* the superstep about to run can be a fused sparse one: its frontier's
@@ -938,6 +977,7 @@ static void report_device_error(uint32_t err)
 static void run_once(void)
 {
     void* st = D.stream;
+    D.chunk_no = 0;
     hip_check(ipr_hip_begin(&D.dev, st), "begin");
     launch_app(IPR_K_INIT);
     hip_check(ipr_hip_event_record(D.ev0, st), "event");
@@ -1012,6 +1052,16 @@ static void run_once(void)
         if (chunk < D.max_chunk) chunk *= 2;
         if (chunk > D.max_chunk) chunk = D.max_chunk;
     }
+    if (D.h_ctl->plan_miss) {
+        /* a superstep left its delivery kernel out: the run stopped there.
+         * Drop the plan and run again (the run starts from its init). */
+        D.plan_on = 0;
+        D.plan_steps = 0;
+        if (D.plan_graph) ipr_hip_graph_destroy(D.plan_graph);
+        D.plan_graph = NULL;
+        run_once();
+        return;
+    }
     hip_check(ipr_hip_event_record(D.ev1, st), "event");
     float ms = 0;
     hip_check(ipr_hip_event_elapsed_ms(D.ev0, D.ev1, &ms), "event");
@@ -1029,6 +1079,23 @@ static void run_once(void)
     /* pull: launch s finalizes superstep s-1, so one launch more than supersteps */
     D.prev_launches = (uint32_t)(D.h_ctl->executed + (D.app.variant == IPR_VARIANT_PULL ? 1 : 0));
     D.prev_fused = D.h_ctl->fused_steps > 0;
+    /* the next run's launch plan: this run's gathered supersteps, for a run
+     * of at most 64 supersteps without fused sparse ones (IPR_LAUNCH_PLAN=2
+     * inverts it, which makes every planned run stop in its first superstep
+     * and run again without the plan: a test of that path) */
+    if (D.plan_mode >= 1 && D.dev.dir_opt && D.dev.g_tn &&
+        (D.app.variant == IPR_VARIANT_SPREAD || D.app.variant == IPR_VARIANT_PUSH)) {
+        const int ok = D.h_ctl->fused_steps == 0 && D.h_ctl->executed > 0 && D.h_ctl->executed <= 64;
+        const uint64_t mask = D.plan_mode == 2 ? ~D.h_ctl->gathered_mask : D.h_ctl->gathered_mask;
+        const uint32_t steps = ok ? (uint32_t)D.h_ctl->executed : 0u;
+        if (ok != D.plan_on || mask != D.plan_mask || steps != D.plan_steps) {
+            if (D.plan_graph) ipr_hip_graph_destroy(D.plan_graph);
+            D.plan_graph = NULL;
+        }
+        D.plan_on = ok;
+        D.plan_mask = mask;
+        D.plan_steps = steps;
+    }
     if (D.live_mode == 2 && D.dev.g_live) {
         const int want = D.h_ctl->live_built > 0 || D.h_ctl->live_missed != 0;
         if (want != D.live_launch) {
@@ -1038,6 +1105,8 @@ static void run_once(void)
                 if (D.graphs[i]) ipr_hip_graph_destroy(D.graphs[i]);
                 D.graphs[i] = NULL;
             }
+            if (D.plan_graph) ipr_hip_graph_destroy(D.plan_graph);
+            D.plan_graph = NULL;
         }
     }
 }

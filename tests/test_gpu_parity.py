@@ -158,6 +158,32 @@ def test_bitmap_launch_modes(mode, monkeypatch):
     s.close()
 
 
+@pytest.mark.parametrize("plan", ["1", "2", "0"])
+def test_launch_plan(plan, monkeypatch):
+    """A run that repeats the previous one launches in each superstep only
+    the delivery kernel that superstep used, gather or expansion (1, the
+    default); 2 inverts the recorded plan, so every planned run stops at its
+    first superstep and runs again without it; 0 launches both. Three runs
+    per session, each checked, with gathered and pushed supersteps mixed."""
+    monkeypatch.setenv("IPR_LAUNCH_PLAN", plan)
+    gs = G.rmat(14, 16, seed=9, symmetrise=True)
+    ref = O.min_ancestor(gs)
+    s = EN.Session("cc_spread", gs)
+    for _ in range(3):
+        st = s.run(1)
+        assert np.array_equal(s.values(), ref)
+    assert 0 < st.gathered < st.supersteps
+    s.close()
+    g = G.rmat(14, 16, seed=3).with_weights(3)
+    src = int(np.argmax(np.diff(g.idx)))
+    ref = O.dijkstra(g, src)
+    s = EN.Session("sssp_weighted_spread", g, source=src)
+    for _ in range(3):
+        s.run(1)
+        assert np.array_equal(s.values(), ref)
+    s.close()
+
+
 @pytest.mark.parametrize("large_min,chunk", [(2, 64), (40, 100), (300, 64)])
 def test_large_broadcasters_push(large_min, chunk):
     """Every superstep pushed (no gather), broadcasters above large_min
