@@ -289,6 +289,9 @@ typedef struct ipr_dev {
      * other in its superstep (a launch plan): it flags ctl->plan_miss if the
      * superstep needs the other */
     uint32_t plan_alone;
+    /* SPREAD: after a superstep that gathered, the compaction only counts
+     * the raised flags and the next superstep reads its frontier from them */
+    uint32_t dense_flags;
 } ipr_dev;
 
 /* 4-byte broadcast record: bits 31-26 the superstep's stamp (superstep mod

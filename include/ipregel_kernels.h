@@ -578,16 +578,21 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_init(ipr_dev d)
 #endif
 static_assert(IPR_COMPUTE_U >= 1 && IPR_COMPUTE_U <= 8, "IPR_COMPUTE_U out of range");
 
+/* how the compute kernel finds its vertices: all V (superstep 0, PUSH), a
+ * frontier list, or all V filtered by the frontier's raised receive flags
+ * (SPREAD, after a superstep that gathered: the compaction only counted
+ * them); in the last two every vertex run has a message */
+enum { IPR_CR_SWEEP = 0, IPR_CR_LIST = 1, IPR_CR_FLAGS = 2 };
+
 /* one round of U vertices per thread: entries base + u*IPR_BLOCK + thread
- * (coalesced per u) of the frontier list (LIST) or of 0..V-1. SPREAD and
- * SPREAD_PULL read a list exactly in the supersteps after the first, so
- * LIST stands for s > 0 there. */
-template <bool LIST>
-__device__ __forceinline__ void ipr_compute_round(const ipr_dev& d, uint64_t s, const uint32_t* list, uint64_t n,
-                                                  uint64_t base, uint64_t& n_active, uint64_t& n_computed,
+ * (coalesced per u) of the frontier list or of 0..V-1 */
+template <int MODE>
+__device__ __forceinline__ void ipr_compute_round(const ipr_dev& d, uint64_t s, const uint32_t* list, uint8_t* flags,
+                                                  uint64_t n, uint64_t base, uint64_t& n_active, uint64_t& n_computed,
                                                   uint64_t& n_edges)
 {
     constexpr int U = IPR_COMPUTE_U;
+    constexpr bool LIST = MODE == IPR_CR_LIST, FLAGS = MODE == IPR_CR_FLAGS, FRONT = LIST || FLAGS;
     ipr_slot_t* mcur = (ipr_slot_t*)d.mbox[s & 1];
     ipr_bslot_t* bs = (ipr_bslot_t*)d.bslot;
     const uint32_t stamp4 = (uint32_t)(s % IPR_REC4_WINDOW) + 1u;
@@ -600,6 +605,15 @@ __device__ __forceinline__ void ipr_compute_round(const ipr_dev& d, uint64_t s, 
         run[u] = i < n;
         const uint64_t ic = run[u] ? i : n - 1;   /* base < n, so n >= 1 */
         vs[u] = LIST ? list[ic] : (uint32_t)ic;
+    }
+    if (FLAGS) {
+        /* the frontier's flags, all U loaded before the first is used; the
+         * others' state is not loaded */
+        uint8_t fl[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) fl[u] = flags[vs[u]];
+#pragma unroll
+        for (int u = 0; u < U; ++u) run[u] = run[u] && fl[u] != 0;
     }
     /* the state: message, value, and a listed vertex's broadcast bookkeeping
      * with it (most frontier vertices broadcast) */
@@ -614,12 +628,14 @@ __device__ __forceinline__ void ipr_compute_round(const ipr_dev& d, uint64_t s, 
         o0[u] = 0;
         o1[u] = 0;
         rp[u] = 0;
+        hm_before[u] = false;
+        if (FLAGS && !run[u]) continue;
         if (IPR_VARIANT == IPR_VARIANT_SPREAD) {
-            /* after the first superstep every listed vertex has a message,
+            /* after the first superstep every frontier vertex has a message,
              * read straight from its mailbox (reset below) */
             x.active = true;
-            x.has_message = LIST;
-            if (LIST) x.message = ipr_policy::value(mcur[v]);
+            x.has_message = FRONT;
+            if (FRONT) x.message = ipr_policy::value(mcur[v]);
         } else if (IPR_VARIANT == IPR_VARIANT_SPREAD_PULL) {
             /* targets: message fetched at the end of the previous superstep */
             x.active = true;
@@ -635,7 +651,7 @@ __device__ __forceinline__ void ipr_compute_round(const ipr_dev& d, uint64_t s, 
             x.active = true;
             if (!run[u]) continue;
         }
-        if (LIST) {
+        if (FRONT) {
             o0[u] = d.out_idx[v];
             o1[u] = d.out_idx[v + 1];
             rp[u] = ipr_brec(d, v);
@@ -643,11 +659,15 @@ __device__ __forceinline__ void ipr_compute_round(const ipr_dev& d, uint64_t s, 
         ipr_fill_vertex(d, &x, v);
         hm_before[u] = x.has_message;
     }
-    if (IPR_VARIANT == IPR_VARIANT_SPREAD && LIST) {
-        /* mailboxes are reset for reuse two supersteps on */
+    if (IPR_VARIANT == IPR_VARIANT_SPREAD && FRONT) {
+        /* mailboxes are reset for reuse two supersteps on, and so are the
+         * flags of a frontier left as flags */
 #pragma unroll
         for (int u = 0; u < U; ++u)
-            if (run[u]) mcur[vs[u]] = ipr_policy::empty();
+            if (run[u]) {
+                mcur[vs[u]] = ipr_policy::empty();
+                if (FLAGS) flags[vs[u]] = 0;
+            }
     }
     /* the computes, then their value stores: a store between two computes
      * would make the compiler reload any application global the next one
@@ -675,7 +695,7 @@ __device__ __forceinline__ void ipr_compute_round(const ipr_dev& d, uint64_t s, 
     }
     /* a full sweep loads the bookkeeping only for the vertices that
      * broadcast, all of them before the first record is stored */
-    if (!LIST) {
+    if (!FRONT) {
 #pragma unroll
         for (int u = 0; u < U; ++u)
             if (run[u] && xs[u].ipr_bcast) {
@@ -758,19 +778,27 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_compute_push(ipr_dev d)
     ipr_sm_counted = 0;
     uint64_t n = d.V;
     const uint32_t* list = nullptr;
+    uint8_t* flags = nullptr;
     if ((IPR_VARIANT == IPR_VARIANT_SPREAD || IPR_VARIANT == IPR_VARIANT_SPREAD_PULL) && s > 0) {
-        n = ctl->fsize[s & 1];
-        list = d.front[s & 1];
+        if (IPR_VARIANT == IPR_VARIANT_SPREAD && ctl->front_flags) {
+            flags = d.recv[s & 1];
+        } else {
+            n = ctl->fsize[s & 1];
+            list = d.front[s & 1];
+        }
     }
     uint64_t n_active = 0, n_computed = 0, n_edges = 0;
     const uint64_t per_round = (uint64_t)IPR_BLOCK * IPR_COMPUTE_U;
     const uint64_t stride = (uint64_t)gridDim.x * per_round;
     if (list) {
         for (uint64_t base = (uint64_t)blockIdx.x * per_round; base < n; base += stride)
-            ipr_compute_round<true>(d, s, list, n, base, n_active, n_computed, n_edges);
+            ipr_compute_round<IPR_CR_LIST>(d, s, list, flags, n, base, n_active, n_computed, n_edges);
+    } else if (flags) {
+        for (uint64_t base = (uint64_t)blockIdx.x * per_round; base < n; base += stride)
+            ipr_compute_round<IPR_CR_FLAGS>(d, s, list, flags, n, base, n_active, n_computed, n_edges);
     } else {
         for (uint64_t base = (uint64_t)blockIdx.x * per_round; base < n; base += stride)
-            ipr_compute_round<false>(d, s, list, n, base, n_active, n_computed, n_edges);
+            ipr_compute_round<IPR_CR_SWEEP>(d, s, list, flags, n, base, n_active, n_computed, n_edges);
     }
     ipr_block_count3(d, IPR_C_ACT0 + (int)(s & 1), n_active, IPR_C_WORKV, n_computed, IPR_C_EDGES, n_edges);
 }
@@ -837,13 +865,18 @@ __global__ void __launch_bounds__(IPR_BLOCK) ipr_k_expand(ipr_dev d)
      * end to end by a prefix sum and taken 256 at a time */
     uint64_t n = d.V;
     const uint32_t* list = nullptr;
-    if ((IPR_VARIANT == IPR_VARIANT_SPREAD || IPR_VARIANT == IPR_VARIANT_SPREAD_PULL) && s > 0) {
+    /* a frontier left as flags (after a superstep that gathered) has no
+     * list: the broadcasters are found by a sweep of the records, as in
+     * superstep 0 */
+    if ((IPR_VARIANT == IPR_VARIANT_SPREAD || IPR_VARIANT == IPR_VARIANT_SPREAD_PULL) && s > 0 &&
+        !(IPR_VARIANT == IPR_VARIANT_SPREAD && ctl->front_flags)) {
         n = ctl->fsize[s & 1];
         list = d.front[s & 1];
     }
     /* a sweep over all V with records in out-degree order (SPREAD's superstep
-     * 0) takes i as a record position: the records are read in order, and
-     * only a broadcaster's vertex id (bvert) and offsets after */
+     * 0, or a frontier left as flags) takes i as a record position: the
+     * records are read in order, and only a broadcaster's vertex id (bvert)
+     * and offsets after */
     const bool bypos = !list && d.bvert;
     /* one tile of IPR_BLOCK entries from `base`: every thread of the
      * workgroup calls it */

@@ -537,6 +537,9 @@ static void dev_setup(void)
     /* pull and spread: the last workgroup of the fused pull kernel or of the
      * frontier compaction ends the superstep (no finalize launch) */
     d->fin_in_kernel = (pull || var == IPR_VARIANT_SPREAD) && env_u32("IPR_FIN_IN_KERNEL", 1) == 1;
+    /* the compaction's last workgroup must end the superstep to mark the
+     * frontier as flags */
+    d->dense_flags = var == IPR_VARIANT_SPREAD && d->fin_in_kernel && env_u32("IPR_DENSE_FLAGS", 1) == 1;
     uint32_t ug = c && c->use_graph ? (uint32_t)c->use_graph : env_u32("IPR_GRAPH", 1);
     D.use_graph = ug == 1;
     D.live_mode = env_u32("IPR_GATHER_LIVE_LAUNCH", 2);

@@ -1042,6 +1042,47 @@ __global__ void __launch_bounds__(256) k_compact(ipr_dev d, int fin_variant, int
     /* the last superstep of a stamp window clears the 4-byte gather records
      * (padded to 16 like the flags) */
     uint4* clr = !relist && d.brec4 && (s + 1) % IPR_REC4_WINDOW == 0 ? (uint4*)d.brec4 : nullptr;
+    if (relist && blockIdx.x == 0 && threadIdx.x == 0) c->front_flags = 0;   /* the list is the frontier */
+    /* after a superstep that gathered (SPREAD): count the raised flags and
+     * leave them for the next superstep's kernels, which read the frontier
+     * from them (no list, no staging, no list-counter atomic) */
+    if (d.dense_flags && !relist) {
+        uint64_t w = d.cnt[IPR_CSLOT(IPR_C_EDGES, threadIdx.x & 63)];
+        for (int o = 32; o > 0; o >>= 1) w += __shfl_xor(w, o);
+        if (d.dir_opt && w * (uint64_t)d.dir_alpha > d.E) {
+            uint32_t cnt = 0;
+            for (uint64_t b0 = (uint64_t)blockIdx.x * span; b0 < groups; b0 += (uint64_t)gridDim.x * span) {
+                uint4 q[T];
+#pragma unroll
+                for (int t = 0; t < T; t++) {
+                    const uint64_t g = b0 + (uint64_t)t * 256 + threadIdx.x;
+                    q[t] = make_uint4(0, 0, 0, 0);
+                    if (g < groups) q[t] = flags[g];
+                    if (clr && g < groups) {
+                        const uint4 z = make_uint4(0, 0, 0, 0);
+                        clr[4 * g] = z;
+                        clr[4 * g + 1] = z;
+                        clr[4 * g + 2] = z;
+                        clr[4 * g + 3] = z;
+                    }
+                }
+#pragma unroll
+                for (int t = 0; t < T; t++) cnt += __popc(q[t].x) + __popc(q[t].y) + __popc(q[t].z) + __popc(q[t].w);
+            }
+            __shared__ uint32_t s_cnt[4];
+            for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+            if (lane == 0) s_cnt[wave] = cnt;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                const uint64_t tot = (uint64_t)s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+                if (tot) atomicAdd((unsigned long long*)&d.cnt[IPR_CSLOT(IPR_C_FNEXT, blockIdx.x % IPR_CSLOTS)],
+                                   (unsigned long long)tot);
+            }
+            if (fin_variant >= 0 && ipr_last_workgroup(d, gridDim.x) && threadIdx.x < 64)
+                ipr_finalize_step(d, fin_variant, 1);
+            return;
+        }
+    }
     for (uint64_t b0 = (uint64_t)blockIdx.x * span; b0 < groups; b0 += (uint64_t)gridDim.x * span) {
         uint4 q[T];
         uint32_t n[T], incl[T];

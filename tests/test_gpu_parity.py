@@ -158,6 +158,31 @@ def test_bitmap_launch_modes(mode, monkeypatch):
     s.close()
 
 
+@pytest.mark.parametrize("dense_flags,order", [("1", "1"), ("1", "0"), ("0", "1")])
+def test_frontier_left_as_flags(dense_flags, order, monkeypatch):
+    """After a superstep that gathered, the compaction only counts the raised
+    flags, and the next superstep computes all V filtered by them and finds
+    its broadcasters by a sweep of the records (by position with records in
+    out-degree order, by vertex id without): CC, unit and weighted SSSP,
+    where gathered and pushed supersteps alternate; the list-only mode (0)
+    beside them."""
+    monkeypatch.setenv("IPR_DENSE_FLAGS", dense_flags)
+    monkeypatch.setenv("IPR_RECORD_ORDER", order)
+    gs = G.rmat(14, 16, seed=9, symmetrise=True)
+    r = EN.run_app("cc_spread", gs)
+    assert np.array_equal(r.values, O.min_ancestor(gs))
+    ref = O.run(gs, O.CC, O.VARIANTS["_spread"])
+    assert list(r.active) == list(ref.active[: ref.supersteps])
+    assert 0 < r.gathered < r.supersteps
+    g = G.rmat(14, 16, seed=3)
+    src = int(np.argmax(np.diff(g.idx)))
+    r = EN.run_app("sssp_spread", g, source=src)
+    assert np.array_equal(r.values, O.bfs(g, src))
+    gw = g.with_weights(3)
+    r = EN.run_app("sssp_weighted_spread", gw, source=src)
+    assert np.array_equal(r.values, O.dijkstra(gw, src))
+
+
 @pytest.mark.parametrize("plan", ["1", "2", "0"])
 def test_launch_plan(plan, monkeypatch):
     """A run that repeats the previous one launches in each superstep only
