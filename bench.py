@@ -146,16 +146,16 @@ WORKLOADS = {
 # factors a plain 1 GiB device copy gives; KiB, mean over the launches that do
 # work: the dense gathers, the pull launches of supersteps 0-11, every fused
 # sparse launch); commands and per-launch figures are in profiles/
-TRAFFIC = {("cc22", "ipr_k_gather_tiled"): (384_700 * 2 + 17_429) * 1024,
-           ("cc24", "ipr_k_gather_tiled"): (2_598_107 * 2 + 67_267) * 1024,
-           ("ss24a", "ipr_k_gather_tiled"): (1_092_595 * 2 + 58_909) * 1024,
-           ("ss24b", "ipr_k_gather_tiled"): (1_460_152 * 2 + 59_062) * 1024,
+TRAFFIC = {("cc22", "ipr_k_gather_tiled"): (382_816 * 2 + 17_428) * 1024,
+           ("cc24", "ipr_k_gather_tiled"): (2_617_108 * 2 + 67_282) * 1024,
+           ("ss24a", "ipr_k_gather_tiled"): (1_088_746 * 2 + 58_912) * 1024,
+           ("ss24b", "ipr_k_gather_tiled"): (1_459_258 * 2 + 59_058) * 1024,
            ("cc22", "ipr_k_gather"): (767_103 * 2 + 56_781) * 1024,
            ("cc24", "ipr_k_gather"): (7_919_949 * 2 + 245_533) * 1024,
            ("ss24a", "ipr_k_gather"): (4_066_499 * 2 + 181_607) * 1024,
            ("ss24b", "ipr_k_gather"): (4_645_812 * 2 + 205_246) * 1024,
            ("pr16", "ipr_k_pull_step"): (5_975 * 2 + 1_385) * 1024,
-           ("grid", "ipr_k_sparse_step"): (34_089 * 2 + 31_473) * 1024}
+           ("grid", "ipr_k_sparse_step"): (34_113 * 2 + 31_488) * 1024}
 
 
 def model_bytes(app: str, g: G.Graph, W: int, F: int, supersteps_with_bcast: int) -> int:
